@@ -1,0 +1,167 @@
+"""Benchmark of the gfx950 residual / SSP-RK3 path on the transonic workload.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--limiter venkat|bj]
+
+Workload: NACA 2412 at Mach 0.85, AoA 2.5 deg, Re 5000, Pr 0.72, FP64,
+Venkatakrishnan limiter K = 5 (or Barth-Jespersen), SSP-RK3 at CFL 0.5, on a
+199,680-triangle / 100,352-vertex O-mesh (wall spacing 5e-4, far field at 20
+chords), freestream cold start (synthetic data: the mesh is generated, the
+state starts uniform).  One step = dt reduction + 3 x (residual + stage
+update), all on the device.  W untimed steps, then K timed steps between
+device synchronisations.  Prints one JSON line:
+  value     = N_vertices * K / seconds (vertex-steps/s, one GPU)
+  roofline  = the dominant kernel's compulsory bytes per launch / its mean
+              launch time (HIP events on the solver's stream), against
+              8 TB/s HBM3E
+  cpu_baseline = the CPU restatement of the reference (oracle/liboracle.so,
+              OpenMP over the host cores) on a bounded sample of the same
+              workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+KERNEL_NAMES = {"vertex_gather": "o2f::k_vertex_gather<4>", "cell_reconstruct": "o2f::k_cell_reconstruct",
+                "vertex_viscous_gather": "o2f::k_vertex_gather<5>", "flux_update": "o2f::k_cell_flux_update"}
+METRIC = "vertex-steps/s"
+
+
+def workload_params(limiter):
+    mach = 0.85
+    mu = 1.225 * mach * np.sqrt(1.4 * 101325.0 / 1.225) / 5000.0
+    return dict(viscosity=mu, Pr=0.72, cfl=0.5, mach=mach, venkat_k=5.0,
+                limiter=0 if limiter == "venkat" else 1, interp_mode=1)
+
+
+def kernel_bytes(desc, nv, ne, nc):
+    """Compulsory HBM bytes per launch of each per-stage kernel: every array
+    the kernel reads or writes counted once (int32 ids, FP64 values)."""
+    nnz = int(np.sum(desc["cell_nv"]))
+    stencil = nv * (1 + 4 + 1) + nnz * (4 + 8)
+    geom_edge = ne * (8 + 1 + 5 * 8)          # 2 vertex ids, tag, len n_x n_y m_x m_y
+    cell_topo = nc * (1 + 16 + 16 + 24)       # info, 4 edge ids, 4 neighbour ids, V c_x c_y
+    return {
+        "vertex_gather": stencil + 32 * nc + 32 * nv,
+        "cell_reconstruct": cell_topo + geom_edge + 32 * nc + 32 * nv + 64 * ne + 40 * nc,
+        "vertex_viscous_gather": stencil + 40 * nc + 40 * nv,
+        # own U read + write, U_old (read in 2 stages, written in 1), face
+        # states, edge normal/length/ids/tag, vertex stresses
+        "flux_update": nc * (1 + 16 + 8 + 32 + 32 + 32) + ne * (64 + 24 + 8 + 1) + 40 * nv,
+    }
+
+
+def step_algorithmic_bytes(nv, ne, nc, k=3):
+    """Algorithmic bytes of one SSP-RK3 step: B_step = 256 N_c + 3 G,
+    G = (24 + 4k) N_c + 56 N_e + (20 + 4 vbar) N_v (state once per stage,
+    geometry once per stage, intermediates excluded)."""
+    vbar = k * nc / nv
+    G = (24 + 4 * k) * nc + 56 * ne + (20 + 4 * vbar) * nv
+    return 256 * nc + 3 * G
+
+
+def cpu_baseline(desc, flow, nv, budget_s=12.0):
+    from oracle import oracle as O
+    p = O.params(**flow)
+    m = O.Mesh(desc)
+    m.init_freestream(p)
+    t0 = time.perf_counter()
+    m.run_steps(p, 2)
+    per = (time.perf_counter() - t0) / 2
+    n = int(max(3, min(1000, budget_s / max(per, 1e-6))))
+    t0 = time.perf_counter()
+    m.run_steps(p, n)
+    el = time.perf_counter() - t0
+    return {"value": nv * n / el, "unit": METRIC, "cores": O.num_threads(), "kind": "port",
+            "sample": "%d SSP-RK3 steps of the same 199,680-triangle workload after 2 warm-up steps, "
+                      "oracle/liboracle.so (C restatement of the reference's OpenMP loop), %.1f s"
+                      % (n, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--limiter", default="venkat", choices=["venkat", "bj"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-steps", type=int, default=200,
+                    help="steps of the per-kernel event-timed pass")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build measures one GPU only (the per-mesh path runs on a single "
+              "MI355X; multi-GPU sharding is described, not implemented); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from optimisation2forme_amd import capi, mesher
+
+    flow = workload_params(a.limiter)
+    desc = mesher.transonic_200k()
+    kw = dict(flow)
+    kw["cfl_number"] = kw.pop("cfl")
+    kw["inflow_velocity"] = kw.pop("mach")
+    s = capi.Solver(desc, capi.make_params(**kw))
+    nv, ne, nc = s.nv, s.ne, s.nc
+    s.init_freestream()
+    if a.warmup > 0:
+        s.run_steps(a.warmup)
+    t0 = time.perf_counter()
+    total_ms, _ = s.time_steps(a.steps)        # synchronises before and after
+    wall = time.perf_counter() - t0
+    value = nv * a.steps / wall
+    U = s.state()
+    finite = bool(np.isfinite(U).all())
+
+    # per-kernel pass: every stage kernel between its own events
+    _, kms = s.time_steps(a.profile_steps, per_kernel=True)
+    names = ["vertex_gather", "cell_reconstruct", "vertex_viscous_gather", "flux_update"]
+    launches = 3 * a.profile_steps
+    avg_us = {n: 1e3 * kms[i] / launches for i, n in enumerate(names)}
+    kb = kernel_bytes(desc, nv, ne, nc)
+    dom = max(names, key=lambda n: avg_us[n])
+    achieved = kb[dom] / (avg_us[dom] * 1e-6) / 1e9
+    step_bytes = step_algorithmic_bytes(nv, ne, nc)
+    ms_step = total_ms / a.steps
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": METRIC,
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 * wall / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (generated NACA 2412 O-mesh, freestream cold start)",
+        "config": {"workload": "naca2412_transonic_200k_triangles", "cells": nc, "vertices": nv,
+                   "edges": ne, "mach": 0.85, "aoa_deg": 2.5, "re": 5000, "cfl": 0.5,
+                   "limiter": "venkatakrishnan K=5" if a.limiter == "venkat" else "barth_jespersen",
+                   "interp_mode": "mvc", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": KERNEL_NAMES[dom],
+                     "bytes_per_launch": kb[dom], "avg_launch_us": avg_us[dom]},
+        "kernels_avg_us": avg_us,
+        "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
+                          "event_ms_per_step": ms_step,
+                          "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,
+                          "frac": step_bytes / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "state_finite": finite,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(desc, flow, nv)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,205 @@
+/*
+ * o2f.h -- C ABI of the MI355X (gfx950) residual / SSP-RK3 / adjoint path.
+ *
+ * A drop-in for the OpenMP hot loop of ThibaultMaurelOujia/Optimisation2Forme
+ * (namespace navier_stokes).  Plain C types only: pointers, sizes, int status.
+ * Every array that crosses this boundary is float64 SoA in the reference's
+ * own order: cells in Gmsh element order, edges in the order of the
+ * reference's std::map keyed by (min vertex, max vertex), vertices in node
+ * order (src/mesh.cpp:14-229).  Cell state is [rho | rho_u | rho_v | E],
+ * each block n_cells long.  Calls are synchronous (they return with results
+ * in host memory) unless documented otherwise.  A context is not
+ * thread-safe: one context per host thread.
+ *
+ * Errors: every function returns O2F_OK (0) or a negative code; the message
+ * is in o2f_last_error(ctx) (or o2f_global_error() when no context exists).
+ * The reference throws C++ exceptions with the same messages (cited).
+ */
+#ifndef O2F_H
+#define O2F_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define O2F_OK 0
+#define O2F_ERR_ARG -1      /* bad argument / size mismatch               */
+#define O2F_ERR_BC -2       /* std::invalid_argument, src/boundary.cpp:90 */
+#define O2F_ERR_HIP -3      /* HIP runtime failure / no GPU               */
+#define O2F_ERR_IO -4       /* std::runtime_error on file I/O             */
+#define O2F_ERR_DIVERGED -5 /* "BiCGSTAB diverged", shape_optimization.cpp:725        */
+#define O2F_ERR_NOCONV -6   /* "BiCGSTAB did not converge", shape_optimization.cpp:730 */
+
+#define O2F_LIMITER_VENKAT 0 /* computeVenkatAlpha, src/advection.cpp:209-230 */
+#define O2F_LIMITER_BJ 1     /* computeBarthJespersenAlpha, :195-207 */
+
+#define O2F_INTERP_REF 0 /* src/advection.cpp:26-131 exactly: valence >= 5 never written (stays 0) */
+#define O2F_INTERP_MVC 1 /* same, with the valence-4 mean-value weights generalised to any valence */
+
+typedef struct o2f_ctx o2f_ctx;
+
+/* Physical and numerical parameters: the members of SimulationParams that
+ * the hot path reads (include/params.hpp:15-73), plus the limiter and
+ * interpolation choices the reference hard-codes. */
+typedef struct o2f_params {
+    double gamma;           /* SimulationParams::gamma            (1.4)      */
+    double viscosity;       /* ::viscosity, dynamic mu            (1e-6)     */
+    double Pr;              /* ::Pr                               (0.71)     */
+    double cp;              /* ::cp                               (1005)     */
+    double cfl_number;      /* ::cfl_number                       (0.1)      */
+    double rho_ref;         /* ::rho_ref                          (1.225)    */
+    double p_ref;           /* ::p_ref                            (101325)   */
+    double inflow_velocity; /* ::inflow_velocity (a Mach number)  (1.0)      */
+    double venkat_k;        /* k_eps of computeVenkatAlpha        (1e-2)     */
+    int limiter;            /* O2F_LIMITER_*                      (VENKAT)   */
+    int interp_mode;        /* O2F_INTERP_*                       (REF)      */
+    int bc_inflow_outflow_x;/* 1 iff boundary_conditions == "inflow_outflow_x"; 0 makes
+                               every operator fail like apply_boundary_conditions */
+} o2f_params;
+
+/* Fills *p with the defaults of SimulationParams (include/params.hpp). */
+void o2f_params_default(o2f_params *p);
+
+/* Mesh as the Gmsh API hands it to Mesh::Mesh (src/mesh.cpp:7-179):
+ *  xy[2*n_vertices]        node coordinates in node order;
+ *  cell_nv[n_cells]        3 (triangle) or 4 (quad), in element order;
+ *  cell_vtx[4*n_cells]     vertex indices (0-based), unused slots -1;
+ *  bnd_vtx[2*n_bnd]        boundary line elements, in physical-group order;
+ *  bnd_tag[n_bnd]          'i','o','f','w' (Inlet, Outlet, FreeStream,
+ *                          Wall/Airfoil); the last line touching a vertex
+ *                          or an edge sets its tag (src/mesh.cpp:76-110). */
+typedef struct o2f_mesh_desc {
+    int n_vertices;
+    const double *xy;
+    int n_cells;
+    const int *cell_nv;
+    const int *cell_vtx;
+    int n_bnd;
+    const int *bnd_vtx;
+    const char *bnd_tag;
+} o2f_mesh_desc;
+
+/* ---- mesh files ---------------------------------------------------- */
+
+/* Reads a Gmsh MSH 2.2 ASCII file into library-owned arrays with the
+ * element order of the Gmsh API as used by Mesh::Mesh (physical groups
+ * ascending, entities ascending, file order).  Replaces gmsh::open +
+ * getNodes/getElements (src/mesh.cpp:9-111).  Free with o2f_msh2_free. */
+int o2f_msh2_read(const char *path, o2f_mesh_desc *out);
+void o2f_msh2_free(o2f_mesh_desc *desc);
+
+/* ---- context -------------------------------------------------------- */
+
+/* Builds topology and geometry (Mesh::Mesh, src/mesh.cpp:112-248) on the
+ * host, uploads it to the current HIP device, allocates state.  Replaces the
+ * function-local static buffers of src/advection.cpp:9-23,422-430 and
+ * src/diffusion.cpp:9-25,212-221 with per-context device buffers. */
+int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx **out);
+void o2f_ctx_destroy(o2f_ctx *ctx);
+const char *o2f_last_error(const o2f_ctx *ctx);
+const char *o2f_global_error(void);
+int o2f_counts(const o2f_ctx *ctx, int *n_vertices, int *n_edges, int *n_cells);
+/* Hot reload of parameters (src/simulation.cpp:118-119). */
+int o2f_set_params(o2f_ctx *ctx, const o2f_params *params);
+
+/* ---- state ---------------------------------------------------------- */
+
+int o2f_set_state(o2f_ctx *ctx, const double *U);   /* 4*n_cells */
+int o2f_get_state(o2f_ctx *ctx, double *U);         /* 4*n_cells */
+/* init_one_x: uniform freestream cold start (src/init.cpp:59-81). */
+int o2f_init_freestream(o2f_ctx *ctx);
+/* loadCellStates / saveCellStates text format (src/utils.cpp:29-107). */
+int o2f_load_state_file(o2f_ctx *ctx, const char *path);
+int o2f_save_state_file(o2f_ctx *ctx, const char *path);
+
+/* ---- residual (computeAdvectionTerm, computeDiffusionTerm) ---------- */
+
+/* computeAdvectionTerm (src/advection.cpp:415-466): cell gradients, limited
+ * face states with boundary ghosts, HLLC, R = -(1/V) sum s F L.  Overwrites
+ * the device residual.  R_out (4*n_cells) may be NULL. */
+int o2f_advection(o2f_ctx *ctx, double *R_out);
+/* computeDiffusionTerm (src/diffusion.cpp:206-243): adds the viscous
+ * divergence to the device residual, using the gradients of the last
+ * o2f_advection.  R_out may be NULL. */
+int o2f_diffusion(o2f_ctx *ctx, double *R_out);
+/* computeResidual (src/shape_optimization.cpp:171-189) = both of the above. */
+int o2f_residual(o2f_ctx *ctx, double *R_out);
+/* Side effects the reference leaves in Cell / Edge after the residual:
+ * cell gradients [rho_x|rho_u_x|rho_v_x|E_x|rho_y|rho_u_y|rho_v_y|E_y]
+ * (8*n_cells), face states L and R (4*n_edges each, src/advection.cpp:302-317). */
+int o2f_get_gradients(o2f_ctx *ctx, double *G);
+int o2f_get_face_states(o2f_ctx *ctx, double *face_L, double *face_R);
+int o2f_set_face_states(o2f_ctx *ctx, const double *face_L, const double *face_R);
+
+/* compute_hllc_flux (src/advection.cpp:335-412) on a batch of n faces:
+ * qL, qR are 4*n SoA conservative states, normal is 2*n SoA (the stored
+ * edge normal, pointing into the left cell), F is 4*n.  Runs on the GPU. */
+int o2f_hllc_flux_batch(const double *qL, const double *qR, const double *normal, double gamma,
+                        double *F, int n);
+
+/* ---- time integration (Simulator::time_loop / step_SSP_RK3) --------- */
+
+/* dt = min(CFL min_h / max_wave, 0.5 min_h^2 / mu) (src/simulation.cpp:77-110). */
+int o2f_compute_dt(o2f_ctx *ctx, double *dt);
+/* step_SSP_RK3 with a given dt (src/simulation.cpp:130-171). */
+int o2f_rk3_step(o2f_ctx *ctx, double dt);
+/* nsteps x { dt reduction; SSP-RK3 step; t += dt }, device resident, no host
+ * synchronisation inside.  dts (nsteps) may be NULL; t_inout (may be NULL)
+ * carries the simulated time. */
+int o2f_run_steps(o2f_ctx *ctx, int nsteps, double *dts, double *t_inout);
+/* while (t < final_time) { ... } of src/simulation.cpp:75-124 without the
+ * output / reload side branches; returns steps taken and final t. */
+int o2f_time_loop(o2f_ctx *ctx, double final_time, int max_steps, int *steps, double *t);
+/* sqrt(mean_c R_rho^2) / (rho_inf U_inf / c), c = 1: the density residual
+ * norm of the last residual evaluation (convergence criterion, not in the
+ * reference, which has none). */
+int o2f_residual_norm(o2f_ctx *ctx, double *rnorm);
+
+/* ---- forces (computeLiftDragCoefficients, computeDirectObjectiveTerm) - */
+
+/* src/compute.cpp:92-132: wall pressure + viscous forces from the face L
+ * states of the last residual. */
+int o2f_lift_drag(o2f_ctx *ctx, double *C_L, double *C_D);
+/* src/shape_optimization.cpp:893-939: J = -C_L + beta C_D over wall edges
+ * whose centre has x <= 1.3. */
+int o2f_objective(o2f_ctx *ctx, double beta, double *J);
+
+/* ---- discrete adjoint ----------------------------------------------- */
+
+/* assembleAdjointRHS (src/shape_optimization.cpp:491-554); b is 4*n_cells. */
+int o2f_adjoint_rhs(o2f_ctx *ctx, double beta, double *b);
+/* applyJacobianTranspose (src/shape_optimization.cpp:237-484): y = J^T x
+ * with central finite-difference (absolute step eps) HLLC blocks at the
+ * current face states.  x, y are 4*n_cells. */
+int o2f_apply_jt(o2f_ctx *ctx, const double *x, double *y, double eps);
+/* solveAdjointSystem (src/shape_optimization.cpp:658-740): BiCGSTAB with an
+ * identity preconditioner on J^T lambda = b, restarted every `chunk`
+ * iterations up to `max_total`, true residual checked after each chunk.
+ * Reference values: eps 1e-6, tol 1e-6, chunk 500, max_total 100000.
+ * lambda (4*n_cells) out; iters, relres out.  Returns O2F_ERR_DIVERGED or
+ * O2F_ERR_NOCONV like the reference's throws (lambda still written). */
+int o2f_bicgstab(o2f_ctx *ctx, double beta, double eps, double tol, int chunk, int max_total,
+                 double *lambda, int *iters, double *relres);
+/* computeShapeGradient (src/shape_optimization.cpp:942-1016) with the
+ * NACA 4-digit wall perturbation of applyShapePerturbationNACA12 (:828-890).
+ * names_all/values_all: every key of the shape-parameter file (pos_x, c, m,
+ * p, t, beta ...); opt_names: the keys marked "*=", in file order.
+ * grad[n_opt] out.  Leaves the face states at the current state. */
+int o2f_shape_gradient(o2f_ctx *ctx, const char *const *names_all, const double *values_all,
+                       int n_all, const char *const *opt_names, int n_opt, const double *lambda,
+                       double beta, double eps, double *grad);
+
+/* ---- measurement ----------------------------------------------------- */
+
+/* Runs nsteps device-resident SSP-RK3 steps (as o2f_run_steps) between HIP
+ * events on the context's stream and returns the elapsed milliseconds.
+ * kernel_ms[4] (may be NULL) receives the summed durations of the four
+ * per-stage kernels (vertex gather, cell reconstruction, vertex viscous
+ * gather, flux + update), each bracketed by its own events. */
+int o2f_time_steps(o2f_ctx *ctx, int nsteps, double *total_ms, double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* O2F_H */

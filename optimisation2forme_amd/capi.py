@@ -1,0 +1,330 @@
+"""ctypes binding of the C ABI in include/o2f.h (libo2f.so, built in-tree).
+
+The product path: every call goes to the gfx950 library.  There is no CPU
+fallback -- loading fails loudly when libo2f.so is missing, and creating a
+context fails loudly without a GPU.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libo2f.so")
+
+D = C.POINTER(C.c_double)
+I = C.POINTER(C.c_int)
+
+O2F_OK = 0
+ERRORS = {-1: "O2F_ERR_ARG", -2: "O2F_ERR_BC", -3: "O2F_ERR_HIP", -4: "O2F_ERR_IO",
+          -5: "O2F_ERR_DIVERGED", -6: "O2F_ERR_NOCONV"}
+
+LIMITERS = {"venkat": 0, "venkatakrishnan": 0, "barth_jespersen": 1, "bj": 1}
+INTERP_MODES = {"ref": 0, "mvc": 1}
+
+# Every symbol include/o2f.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "o2f_params_default", "o2f_msh2_read", "o2f_msh2_free", "o2f_ctx_create", "o2f_ctx_destroy",
+    "o2f_last_error", "o2f_global_error", "o2f_counts", "o2f_set_params", "o2f_set_state",
+    "o2f_get_state", "o2f_init_freestream", "o2f_load_state_file", "o2f_save_state_file",
+    "o2f_advection", "o2f_diffusion", "o2f_residual", "o2f_get_gradients", "o2f_get_face_states",
+    "o2f_set_face_states", "o2f_hllc_flux_batch", "o2f_compute_dt", "o2f_rk3_step",
+    "o2f_run_steps", "o2f_time_loop", "o2f_residual_norm", "o2f_lift_drag", "o2f_objective",
+    "o2f_adjoint_rhs", "o2f_apply_jt", "o2f_bicgstab", "o2f_shape_gradient", "o2f_time_steps",
+]
+
+
+class Params(C.Structure):
+    """o2f_params (include/o2f.h)."""
+    _fields_ = [
+        ("gamma", C.c_double), ("viscosity", C.c_double), ("Pr", C.c_double), ("cp", C.c_double),
+        ("cfl_number", C.c_double), ("rho_ref", C.c_double), ("p_ref", C.c_double),
+        ("inflow_velocity", C.c_double), ("venkat_k", C.c_double), ("limiter", C.c_int),
+        ("interp_mode", C.c_int), ("bc_inflow_outflow_x", C.c_int),
+    ]
+
+
+class MeshDesc(C.Structure):
+    """o2f_mesh_desc (include/o2f.h)."""
+    _fields_ = [
+        ("n_vertices", C.c_int), ("xy", D), ("n_cells", C.c_int), ("cell_nv", I),
+        ("cell_vtx", I), ("n_bnd", C.c_int), ("bnd_vtx", I), ("bnd_tag", C.c_char_p),
+    ]
+
+
+class O2FError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s: %s" % (ERRORS.get(code, code), msg))
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libo2f.so is not built: run `python -m optimisation2forme_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        P = C.POINTER(Params)
+        sig = {
+            "o2f_params_default": (None, [P]),
+            "o2f_msh2_read": (C.c_int, [C.c_char_p, C.POINTER(MeshDesc)]),
+            "o2f_msh2_free": (None, [C.POINTER(MeshDesc)]),
+            "o2f_ctx_create": (C.c_int, [C.POINTER(MeshDesc), P, C.POINTER(vp)]),
+            "o2f_ctx_destroy": (None, [vp]),
+            "o2f_last_error": (C.c_char_p, [vp]),
+            "o2f_global_error": (C.c_char_p, []),
+            "o2f_counts": (C.c_int, [vp, I, I, I]),
+            "o2f_set_params": (C.c_int, [vp, P]),
+            "o2f_set_state": (C.c_int, [vp, D]),
+            "o2f_get_state": (C.c_int, [vp, D]),
+            "o2f_init_freestream": (C.c_int, [vp]),
+            "o2f_load_state_file": (C.c_int, [vp, C.c_char_p]),
+            "o2f_save_state_file": (C.c_int, [vp, C.c_char_p]),
+            "o2f_advection": (C.c_int, [vp, D]),
+            "o2f_diffusion": (C.c_int, [vp, D]),
+            "o2f_residual": (C.c_int, [vp, D]),
+            "o2f_get_gradients": (C.c_int, [vp, D]),
+            "o2f_get_face_states": (C.c_int, [vp, D, D]),
+            "o2f_set_face_states": (C.c_int, [vp, D, D]),
+            "o2f_hllc_flux_batch": (C.c_int, [D, D, D, C.c_double, D, C.c_int]),
+            "o2f_compute_dt": (C.c_int, [vp, D]),
+            "o2f_rk3_step": (C.c_int, [vp, C.c_double]),
+            "o2f_run_steps": (C.c_int, [vp, C.c_int, D, D]),
+            "o2f_time_loop": (C.c_int, [vp, C.c_double, C.c_int, I, D]),
+            "o2f_residual_norm": (C.c_int, [vp, D]),
+            "o2f_lift_drag": (C.c_int, [vp, D, D]),
+            "o2f_objective": (C.c_int, [vp, C.c_double, D]),
+            "o2f_adjoint_rhs": (C.c_int, [vp, C.c_double, D]),
+            "o2f_apply_jt": (C.c_int, [vp, D, D, C.c_double]),
+            "o2f_bicgstab": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D]),
+            "o2f_shape_gradient": (C.c_int, [vp, C.POINTER(C.c_char_p), D, C.c_int,
+                                             C.POINTER(C.c_char_p), C.c_int, D, C.c_double,
+                                             C.c_double, D]),
+            "o2f_time_steps": (C.c_int, [vp, C.c_int, D, D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(D)
+
+
+def _i(a):
+    return a.ctypes.data_as(I)
+
+
+def make_params(gamma=1.4, viscosity=1e-6, Pr=0.71, cp=1005.0, cfl_number=0.1, rho_ref=1.225,
+                p_ref=101325.0, inflow_velocity=1.0, venkat_k=1e-2, limiter="venkat",
+                interp_mode="ref", bc_inflow_outflow_x=True):
+    """o2f_params; defaults are SimulationParams' (include/params.hpp:15-73)."""
+    if isinstance(limiter, str):
+        limiter = LIMITERS[limiter]
+    if isinstance(interp_mode, str):
+        interp_mode = INTERP_MODES[interp_mode]
+    return Params(gamma, viscosity, Pr, cp, cfl_number, rho_ref, p_ref, inflow_velocity, venkat_k,
+                  int(limiter), int(interp_mode), int(bool(bc_inflow_outflow_x)))
+
+
+def read_msh2(path):
+    """o2f_msh2_read -> dict of numpy arrays (the Gmsh API element order)."""
+    L = lib()
+    d = MeshDesc()
+    st = L.o2f_msh2_read(path.encode(), C.byref(d))
+    if st != O2F_OK:
+        raise O2FError(st, L.o2f_global_error().decode())
+    try:
+        nv, nc, nb = d.n_vertices, d.n_cells, d.n_bnd
+        out = {
+            "xy": np.ctypeslib.as_array(d.xy, (nv * 2,)).reshape(nv, 2).copy(),
+            "cell_nv": np.ctypeslib.as_array(d.cell_nv, (nc,)).copy(),
+            "cell_vtx": np.ctypeslib.as_array(d.cell_vtx, (nc * 4,)).reshape(nc, 4).copy(),
+            "bnd_vtx": (np.ctypeslib.as_array(d.bnd_vtx, (nb * 2,)).reshape(nb, 2).copy()
+                        if nb else np.zeros((0, 2), np.int32)),
+            "bnd_tag": np.frombuffer(C.string_at(d.bnd_tag, nb), dtype=np.uint8).copy(),
+        }
+    finally:
+        L.o2f_msh2_free(C.byref(d))
+    return out
+
+
+class Solver:
+    """One o2f context: a mesh resident on the GPU plus its state.
+
+    Arrays in and out are float64 SoA in the reference's cell / edge order."""
+
+    def __init__(self, desc, params):
+        L = lib()
+        self._keep = []
+        xy = np.ascontiguousarray(desc["xy"], dtype=np.float64)
+        cnv = np.ascontiguousarray(desc["cell_nv"], dtype=np.int32)
+        cv = np.ascontiguousarray(desc["cell_vtx"], dtype=np.int32)
+        bv = np.ascontiguousarray(desc["bnd_vtx"], dtype=np.int32)
+        bt = bytes(np.asarray(desc["bnd_tag"], dtype=np.uint8))
+        md = MeshDesc(xy.shape[0], _d(xy), cnv.shape[0], _i(cnv), _i(cv), bv.shape[0], _i(bv), bt)
+        self.params = params
+        h = C.c_void_p()
+        st = L.o2f_ctx_create(C.byref(md), C.byref(params), C.byref(h))
+        if st != O2F_OK:
+            raise O2FError(st, L.o2f_global_error().decode())
+        self.h = h
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        L.o2f_counts(self.h, C.byref(a), C.byref(b), C.byref(c))
+        self.nv, self.ne, self.nc = a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().o2f_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != O2F_OK:
+            raise O2FError(st, lib().o2f_last_error(self.h).decode())
+
+    def set_params(self, params):
+        self.params = params
+        self._check(lib().o2f_set_params(self.h, C.byref(params)))
+
+    def set_state(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(4, self.nc)
+        self._check(lib().o2f_set_state(self.h, _d(U)))
+
+    def state(self):
+        U = np.zeros((4, self.nc))
+        self._check(lib().o2f_get_state(self.h, _d(U)))
+        return U
+
+    def init_freestream(self):
+        self._check(lib().o2f_init_freestream(self.h))
+
+    def load_state_file(self, path):
+        self._check(lib().o2f_load_state_file(self.h, path.encode()))
+
+    def save_state_file(self, path):
+        self._check(lib().o2f_save_state_file(self.h, path.encode()))
+
+    def advection(self):
+        R = np.zeros((4, self.nc))
+        self._check(lib().o2f_advection(self.h, _d(R)))
+        return R
+
+    def diffusion(self):
+        R = np.zeros((4, self.nc))
+        self._check(lib().o2f_diffusion(self.h, _d(R)))
+        return R
+
+    def residual(self):
+        R = np.zeros((4, self.nc))
+        self._check(lib().o2f_residual(self.h, _d(R)))
+        return R
+
+    def gradients(self):
+        G = np.zeros((8, self.nc))
+        self._check(lib().o2f_get_gradients(self.h, _d(G)))
+        return G
+
+    def faces(self):
+        fl, fr = np.zeros((4, self.ne)), np.zeros((4, self.ne))
+        self._check(lib().o2f_get_face_states(self.h, _d(fl), _d(fr)))
+        return fl, fr
+
+    def set_faces(self, fl, fr):
+        fl = np.ascontiguousarray(fl, dtype=np.float64)
+        fr = np.ascontiguousarray(fr, dtype=np.float64)
+        self._check(lib().o2f_set_face_states(self.h, _d(fl), _d(fr)))
+
+    def dt(self):
+        v = C.c_double()
+        self._check(lib().o2f_compute_dt(self.h, C.byref(v)))
+        return v.value
+
+    def rk3_step(self, dt):
+        self._check(lib().o2f_rk3_step(self.h, dt))
+
+    def run_steps(self, n, t=0.0):
+        dts = np.zeros(max(n, 1))
+        tt = C.c_double(t)
+        self._check(lib().o2f_run_steps(self.h, n, _d(dts), C.byref(tt)))
+        return dts[:n], tt.value
+
+    def time_loop(self, final_time, max_steps=-1):
+        n, t = C.c_int(), C.c_double()
+        self._check(lib().o2f_time_loop(self.h, final_time, max_steps, C.byref(n), C.byref(t)))
+        return n.value, t.value
+
+    def residual_norm(self):
+        v = C.c_double()
+        self._check(lib().o2f_residual_norm(self.h, C.byref(v)))
+        return v.value
+
+    def lift_drag(self):
+        a, b = C.c_double(), C.c_double()
+        self._check(lib().o2f_lift_drag(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def objective(self, beta):
+        v = C.c_double()
+        self._check(lib().o2f_objective(self.h, beta, C.byref(v)))
+        return v.value
+
+    def adjoint_rhs(self, beta):
+        b = np.zeros((4, self.nc))
+        self._check(lib().o2f_adjoint_rhs(self.h, beta, _d(b)))
+        return b
+
+    def apply_jt(self, x, eps=1e-6):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(4, self.nc)
+        y = np.zeros((4, self.nc))
+        self._check(lib().o2f_apply_jt(self.h, _d(x), _d(y), eps))
+        return y
+
+    def bicgstab(self, beta, eps=1e-6, tol=1e-6, chunk=500, max_total=100000, check=True):
+        lam = np.zeros((4, self.nc))
+        it, rr = C.c_int(), C.c_double()
+        st = lib().o2f_bicgstab(self.h, beta, eps, tol, chunk, max_total, _d(lam), C.byref(it),
+                                C.byref(rr))
+        if check:
+            self._check(st)
+        return lam, it.value, rr.value, st
+
+    def shape_gradient(self, names_all, values_all, opt_names, lam, beta, eps=1e-6):
+        na = (C.c_char_p * len(names_all))(*[n.encode() for n in names_all])
+        va = np.ascontiguousarray(values_all, dtype=np.float64)
+        no = (C.c_char_p * max(len(opt_names), 1))(*[n.encode() for n in opt_names])
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        g = np.zeros(max(len(opt_names), 1))
+        self._check(lib().o2f_shape_gradient(self.h, na, _d(va), len(names_all), no, len(opt_names),
+                                             _d(lam), beta, eps, _d(g)))
+        return g[:len(opt_names)]
+
+    def time_steps(self, n, per_kernel=False):
+        tot = C.c_double()
+        km = np.zeros(4)
+        self._check(lib().o2f_time_steps(self.h, n, C.byref(tot), _d(km) if per_kernel else None))
+        return tot.value, (km if per_kernel else None)
+
+
+def hllc_flux_batch(qL, qR, normal, gamma=1.4):
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    normal = np.ascontiguousarray(normal, dtype=np.float64)
+    n = qL.shape[1]
+    F = np.zeros((4, n))
+    st = lib().o2f_hllc_flux_batch(_d(qL), _d(qR), _d(normal), gamma, _d(F), n)
+    if st != O2F_OK:
+        raise O2FError(st, lib().o2f_global_error().decode())
+    return F

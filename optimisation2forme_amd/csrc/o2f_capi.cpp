@@ -1,0 +1,939 @@
+// C ABI of the gfx950 residual / SSP-RK3 / adjoint path (include/o2f.h).
+//
+// One context owns one mesh: its topology and geometry (uploaded once), the
+// SoA FP64 state, the per-stage intermediates, the adjoint work vectors and
+// one HIP stream.  Everything between o2f_set_state and o2f_get_state stays
+// on the device; host round trips happen only where the API hands results
+// back.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "host_mesh.hpp"
+#include "o2f.h"
+#include "o2f_kernels.hpp"
+
+using namespace o2f;
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class T>
+hipError_t dalloc(T *&p, size_t n) {
+    p = nullptr;
+    return hipMalloc(reinterpret_cast<void **>(&p), (n ? n : 1) * sizeof(T));
+}
+
+template <class T>
+hipError_t upload(T *&p, const std::vector<T> &v) {
+    hipError_t e = dalloc(p, v.size());
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+struct GeomSet {
+    double *cvol = nullptr, *ccx = nullptr, *ccy = nullptr;
+    double *elen = nullptr, *enx = nullptr, *eny = nullptr, *emx = nullptr, *emy = nullptr;
+    double *sw = nullptr;
+    Geom view() const { return Geom{cvol, ccx, ccy, elen, enx, eny, emx, emy, sw}; }
+    hipError_t alloc(const HostMesh &hm) {
+        hipError_t e;
+        if ((e = dalloc(cvol, hm.nc)) || (e = dalloc(ccx, hm.nc)) || (e = dalloc(ccy, hm.nc)) ||
+            (e = dalloc(elen, hm.ne)) || (e = dalloc(enx, hm.ne)) || (e = dalloc(eny, hm.ne)) ||
+            (e = dalloc(emx, hm.ne)) || (e = dalloc(emy, hm.ne)) || (e = dalloc(sw, hm.sw.size())))
+            return e;
+        return hipSuccess;
+    }
+    hipError_t fill(const HostMesh &hm, hipStream_t s) {
+        auto cp = [&](double *d, const std::vector<double> &v) {
+            return v.empty() ? hipSuccess
+                             : hipMemcpyAsync(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s);
+        };
+        hipError_t e;
+        if ((e = cp(cvol, hm.cvol)) || (e = cp(ccx, hm.ccx)) || (e = cp(ccy, hm.ccy)) || (e = cp(elen, hm.elen)) ||
+            (e = cp(enx, hm.enx)) || (e = cp(eny, hm.eny)) || (e = cp(emx, hm.emx)) || (e = cp(emy, hm.emy)) ||
+            (e = cp(sw, hm.sw)))
+            return e;
+        return hipStreamSynchronize(s);
+    }
+    void release() {
+        for (double *p : {cvol, ccx, ccy, elen, enx, eny, emx, emy, sw})
+            if (p) (void)hipFree(p);
+        *this = GeomSet();
+    }
+};
+
+}  // namespace
+
+struct o2f_ctx {
+    HostMesh hm;
+    o2f_params prm{};
+    Phys phys{};
+    double q_inf = 0.0;  // 0.5 rho_inf U_inf^2 (src/compute.cpp:96-100)
+    hipStream_t stream = nullptr;
+    std::string err;
+    // topology
+    int *cedge = nullptr, *cnbr = nullptr, *eva = nullptr, *evb = nullptr, *ecl = nullptr, *ecr = nullptr;
+    int *sptr = nullptr, *scell = nullptr, *wall = nullptr;
+    uint8_t *cinfo = nullptr, *skind = nullptr;
+    char *ebc = nullptr, *vbc = nullptr;
+    GeomSet geo[3];  // 0: the mesh, 1/2: +/- shape perturbations
+    // state and per-stage intermediates
+    double *U = nullptr, *Uold = nullptr, *R = nullptr, *Gr = nullptr, *tq = nullptr;
+    double *vU = nullptr, *vtq = nullptr, *fL = nullptr, *fR = nullptr;
+    double *scal = nullptr, *partials = nullptr, *wall_out = nullptr;
+    unsigned *counter = nullptr;
+    bool have_grad = false;
+    // adjoint
+    double *blk = nullptr;
+    long faces_version = 0, blk_version = -1;
+    double blk_eps = 0.0;
+    double *kv[10] = {};  // BiCGSTAB / gradient work vectors (4 nc each)
+    double *Rp = nullptr, *Rm = nullptr;
+
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+    Topo topo() const {
+        return Topo{hm.nv, hm.ne, hm.nc, cedge, cnbr, cinfo, eva, evb, ecl, ecr, ebc, sptr, scell, skind, vbc};
+    }
+};
+
+#define HIPCHK(ctx, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return (ctx)->fail(O2F_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+namespace {
+
+// Freestream constants of apply_inflow_outflow_bc (src/boundary.cpp:10-17)
+// and the other per-run scalars, with the reference's expression order.
+Phys make_phys(const o2f_params &p, double *q_inf) {
+    Phys P{};
+    P.gamma = p.gamma;
+    P.mu = p.viscosity;
+    P.kappa = p.viscosity * p.cp / p.Pr;
+    P.cfl = p.cfl_number;
+    const double rho0 = p.rho_ref, p0 = p.p_ref, M = p.inflow_velocity;
+    const double a0 = std::sqrt(p.gamma * p0 / rho0);
+    const double u0 = M * a0, v0 = 0.0;
+    const double kinetic = 0.5 * rho0 * (u0 * u0 + v0 * v0);
+    P.fs_rho = rho0;
+    P.fs_ru = rho0 * u0;
+    P.fs_rv = rho0 * v0;
+    P.fs_E = p.p_ref / (p.gamma - 1.0) + kinetic;
+    P.venkat_k = p.venkat_k;
+    P.limiter = p.limiter;
+    const double a_inf = std::sqrt(p.gamma * p.p_ref / p.rho_ref);
+    const double U_inf = M * a_inf;
+    *q_inf = 0.5 * (p.rho_ref * U_inf * U_inf);
+    return P;
+}
+
+bool any_tagged(const HostMesh &hm) {
+    for (char b : hm.vbc)
+        if (b == 'i' || b == 'o' || b == 'f' || b == 'w') return true;
+    for (char b : hm.ebc)
+        if (b == 'i' || b == 'o' || b == 'f' || b == 'w') return true;
+    return false;
+}
+
+// apply_boundary_conditions throws for any boundary_conditions other than
+// "inflow_outflow_x" on the first tagged vertex (src/boundary.cpp:84-92).
+int check_bc(o2f_ctx *c) {
+    if (!c->prm.bc_inflow_outflow_x && any_tagged(c->hm))
+        return c->fail(O2F_ERR_BC, "Unknown boundary condition: boundary_conditions is not inflow_outflow_x");
+    return O2F_OK;
+}
+
+const long kNone = -1;
+
+// One residual evaluation (optionally fused with a stage update) on
+// geometry set g.  Four kernels: vertex gather, cell reconstruction,
+// viscous vertex gather, flux + update.
+int run_stage(o2f_ctx *c, int g, int flags, const double *dtp, double c0, double c1, bool store_grad) {
+    const Topo T = c->topo();
+    const Geom G = c->geo[g].view();
+    hipStream_t s = c->stream;
+    HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, s));
+    HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, store_grad ? c->Gr : nullptr, c->phys, s));
+    HIPCHK(c, launch_vertex_visc(T, G, c->tq, c->vtq, c->phys, s));
+    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, dtp, c0, c1,
+                                 flags | kFlagAdv | kFlagVisc, c->phys, s));
+    ++c->faces_version;
+    if (store_grad) c->have_grad = true;
+    return O2F_OK;
+}
+
+int residual_on(o2f_ctx *c, int g) {
+    int st = check_bc(c);
+    if (st) return st;
+    return run_stage(c, g, kFlagStoreR, nullptr, 0.0, 0.0, true);
+}
+
+int copy_out(o2f_ctx *c, double *host, const double *dev, size_t n) {
+    if (!host) return O2F_OK;
+    HIPCHK(c, hipMemcpyAsync(host, dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+// One SSP-RK3 step (src/simulation.cpp:130-142).  The dt in scal[kScalDt]
+// is read by the stage kernels on the device.
+int rk3_stages(o2f_ctx *c, bool last_outputs) {
+    const double *dtp = c->scal + kScalDt;
+    int st;
+    if ((st = run_stage(c, 0, kFlagRK | kFlagSaveOld, dtp, 0.0, 1.0, false))) return st;
+    if ((st = run_stage(c, 0, kFlagRK, dtp, 3.0 / 4.0, 1.0 / 4.0, false))) return st;
+    return run_stage(c, 0, kFlagRK | (last_outputs ? kFlagStoreR : 0), dtp, 1.0 / 3.0, 2.0 / 3.0, last_outputs);
+}
+
+std::string fmt_double(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%g", v);
+    return b;
+}
+
+}  // namespace
+
+extern "C" {
+
+void o2f_params_default(o2f_params *p) {
+    p->gamma = 1.4;
+    p->viscosity = 1e-6;
+    p->Pr = 0.71;
+    p->cp = 1005.0;
+    p->cfl_number = 0.1;
+    p->rho_ref = 1.225;
+    p->p_ref = 101325.0;
+    p->inflow_velocity = 1.0;
+    p->venkat_k = 1e-2;
+    p->limiter = O2F_LIMITER_VENKAT;
+    p->interp_mode = O2F_INTERP_REF;
+    p->bc_inflow_outflow_x = 1;
+}
+
+const char *o2f_global_error(void) { return g_error.c_str(); }
+const char *o2f_last_error(const o2f_ctx *ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+int o2f_msh2_read(const char *path, o2f_mesh_desc *out) {
+    if (!path || !out) {
+        g_error = "o2f_msh2_read: null argument";
+        return O2F_ERR_ARG;
+    }
+    std::vector<double> xy;
+    std::vector<int> cnv, cvtx, bvtx;
+    std::vector<char> btag;
+    std::string err;
+    if (!read_msh2(path, xy, cnv, cvtx, bvtx, btag, err)) {
+        g_error = err;
+        return O2F_ERR_IO;
+    }
+    auto *pxy = new double[xy.size()];
+    auto *pcnv = new int[cnv.size() + 1];
+    auto *pcv = new int[cvtx.size() + 1];
+    auto *pbv = new int[bvtx.size() + 1];
+    auto *pbt = new char[btag.size() + 1];
+    std::copy(xy.begin(), xy.end(), pxy);
+    std::copy(cnv.begin(), cnv.end(), pcnv);
+    std::copy(cvtx.begin(), cvtx.end(), pcv);
+    std::copy(bvtx.begin(), bvtx.end(), pbv);
+    std::copy(btag.begin(), btag.end(), pbt);
+    out->n_vertices = static_cast<int>(xy.size() / 2);
+    out->xy = pxy;
+    out->n_cells = static_cast<int>(cnv.size());
+    out->cell_nv = pcnv;
+    out->cell_vtx = pcv;
+    out->n_bnd = static_cast<int>(btag.size());
+    out->bnd_vtx = pbv;
+    out->bnd_tag = pbt;
+    return O2F_OK;
+}
+
+void o2f_msh2_free(o2f_mesh_desc *d) {
+    if (!d) return;
+    delete[] d->xy;
+    delete[] d->cell_nv;
+    delete[] d->cell_vtx;
+    delete[] d->bnd_vtx;
+    delete[] d->bnd_tag;
+    std::memset(d, 0, sizeof(*d));
+}
+
+int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx **out) {
+    if (!mesh || !params || !out) {
+        g_error = "o2f_ctx_create: null argument";
+        return O2F_ERR_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        g_error = "o2f_ctx_create: no HIP device (this library runs only on a GPU)";
+        return O2F_ERR_HIP;
+    }
+    std::unique_ptr<o2f_ctx> c(new o2f_ctx());
+    c->prm = *params;
+    std::string err;
+    if (!build_host_mesh(*mesh, params->interp_mode, c->hm, err)) {
+        g_error = err;
+        return O2F_ERR_ARG;
+    }
+    c->phys = make_phys(c->prm, &c->q_inf);
+    const HostMesh &hm = c->hm;
+    const size_t nc = hm.nc, ne = hm.ne, nv = hm.nv;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    std::vector<int> sptr(hm.vptr.begin(), hm.vptr.end());
+    if (!e) e = upload(c->cedge, hm.cedge);
+    if (!e) e = upload(c->cnbr, hm.cnbr);
+    if (!e) e = upload(c->cinfo, hm.cinfo);
+    if (!e) e = upload(c->eva, hm.eva);
+    if (!e) e = upload(c->evb, hm.evb);
+    if (!e) e = upload(c->ecl, hm.ecl);
+    if (!e) e = upload(c->ecr, hm.ecr);
+    if (!e) e = upload(c->ebc, hm.ebc);
+    if (!e) e = upload(c->vbc, hm.vbc);
+    if (!e) e = upload(c->sptr, sptr);
+    if (!e) e = upload(c->scell, hm.scell);
+    if (!e) e = upload(c->skind, hm.skind);
+    if (!e) e = upload(c->wall, hm.wall_edges);
+    if (!e) e = c->geo[0].alloc(hm);
+    if (!e) e = c->geo[0].fill(hm, c->stream);
+    if (!e) e = dalloc(c->U, 4 * nc);
+    if (!e) e = dalloc(c->Uold, 4 * nc);
+    if (!e) e = dalloc(c->R, 4 * nc);
+    if (!e) e = dalloc(c->Gr, 8 * nc);
+    if (!e) e = dalloc(c->tq, 5 * nc);
+    if (!e) e = dalloc(c->vU, 4 * nv);
+    if (!e) e = dalloc(c->vtq, 5 * nv);
+    if (!e) e = dalloc(c->fL, 4 * ne);
+    if (!e) e = dalloc(c->fR, 4 * ne);
+    if (!e) e = dalloc(c->scal, kScalCount);
+    if (!e) e = dalloc(c->partials, 2 * kRedGrid);
+    if (!e) e = dalloc(c->wall_out, 2);
+    if (!e) e = dalloc(c->counter, 1);
+    if (!e) e = hipMemset(c->counter, 0, sizeof(unsigned));
+    if (!e) e = hipMemset(c->scal, 0, kScalCount * sizeof(double));
+    if (!e) e = hipMemset(c->U, 0, 4 * nc * sizeof(double));
+    if (!e) e = hipMemset(c->fL, 0, 4 * ne * sizeof(double));
+    if (!e) e = hipMemset(c->fR, 0, 4 * ne * sizeof(double));
+    if (!e) e = hipDeviceSynchronize();
+    if (e) {
+        g_error = std::string("o2f_ctx_create: ") + hipGetErrorString(e);
+        o2f_ctx_destroy(c.release());
+        return O2F_ERR_HIP;
+    }
+    *out = c.release();
+    return O2F_OK;
+}
+
+void o2f_ctx_destroy(o2f_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->cinfo,
+                    c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
+                    c->scal, c->partials, c->wall_out, c->counter, c->blk, c->Rp, c->Rm};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    for (double *p : c->kv)
+        if (p) (void)hipFree(p);
+    for (auto &g : c->geo) g.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int o2f_counts(const o2f_ctx *c, int *nv, int *ne, int *nc) {
+    if (!c) return O2F_ERR_ARG;
+    if (nv) *nv = c->hm.nv;
+    if (ne) *ne = c->hm.ne;
+    if (nc) *nc = c->hm.nc;
+    return O2F_OK;
+}
+
+int o2f_set_params(o2f_ctx *c, const o2f_params *p) {
+    if (!c || !p) return O2F_ERR_ARG;
+    const bool restencil = p->interp_mode != c->prm.interp_mode;
+    c->prm = *p;
+    c->phys = make_phys(c->prm, &c->q_inf);
+    if (restencil) {
+        compute_stencils(c->hm, p->interp_mode);
+        HIPCHK(c, hipMemcpyAsync(c->skind, c->hm.skind.data(), c->hm.skind.size(), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->scell, c->hm.scell.data(), c->hm.scell.size() * sizeof(int),
+                                 hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, c->geo[0].fill(c->hm, c->stream));
+    }
+    return O2F_OK;
+}
+
+int o2f_set_state(o2f_ctx *c, const double *U) {
+    if (!c || !U) return O2F_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(c->U, U, 4 * (size_t)c->hm.nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+int o2f_get_state(o2f_ctx *c, double *U) {
+    if (!c || !U) return O2F_ERR_ARG;
+    return copy_out(c, U, c->U, 4 * (size_t)c->hm.nc);
+}
+
+int o2f_init_freestream(o2f_ctx *c) {
+    if (!c) return O2F_ERR_ARG;
+    // init_one_x + primitives_to_conserved (src/init.cpp:8-16, :59-81)
+    const o2f_params &p = c->prm;
+    const double rho0 = p.rho_ref, p0 = p.p_ref;
+    const double a0 = std::sqrt(p.gamma * p0 / rho0);
+    const double u0 = p.inflow_velocity * a0, v0 = 0.0;
+    const double vals[4] = {rho0, rho0 * u0, rho0 * v0, p0 / (p.gamma - 1.0) + 0.5 * rho0 * (u0 * u0 + v0 * v0)};
+    const long nc = c->hm.nc;
+    for (int q = 0; q < 4; ++q) HIPCHK(c, launch_fill(nc, c->U + q * nc, vals[q], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+int o2f_load_state_file(o2f_ctx *c, const char *path) {
+    if (!c || !path) return O2F_ERR_ARG;
+    // loadCellStates, src/utils.cpp:29-67
+    std::ifstream in(path);
+    if (!in) return c->fail(O2F_ERR_IO, std::string("Impossible d'ouvrir ") + path + " en lecture.");
+    size_t n = 0;
+    in >> n;
+    if (!in)
+        return c->fail(O2F_ERR_IO, std::string("Fichier corrompu (") + path +
+                                       "): impossible de lire le nombre de cellules.");
+    const size_t nc = c->hm.nc;
+    if (n != nc) {
+        std::ostringstream oss;
+        oss << "Taille incohérente (" << path << "): fichier=" << n << " vs mesh=" << nc
+            << " (vérifier l'ordre/la topologie).";
+        return c->fail(O2F_ERR_IO, oss.str());
+    }
+    std::vector<double> U(4 * nc);
+    for (size_t i = 0; i < nc; ++i) {
+        double v[4];
+        in >> v[0] >> v[1] >> v[2] >> v[3];
+        if (!in) {
+            std::ostringstream oss;
+            oss << "Lecture interrompue à la cellule " << i << " dans " << path << ".";
+            return c->fail(O2F_ERR_IO, oss.str());
+        }
+        for (int q = 0; q < 4; ++q) U[q * nc + i] = v[q];
+    }
+    return o2f_set_state(c, U.data());
+}
+
+int o2f_save_state_file(o2f_ctx *c, const char *path) {
+    if (!c || !path) return O2F_ERR_ARG;
+    // saveCellStates, src/utils.cpp:70-107
+    const size_t nc = c->hm.nc;
+    std::vector<double> U(4 * nc);
+    int st = o2f_get_state(c, U.data());
+    if (st) return st;
+    FILE *f = std::fopen(path, "w");
+    if (!f) return c->fail(O2F_ERR_IO, std::string("Impossible d'ouvrir ") + path + " en écriture.");
+    std::fprintf(f, "%zu\n", nc);
+    for (size_t i = 0; i < nc; ++i)
+        std::fprintf(f, "%.17e %.17e %.17e %.17e\n", U[i], U[nc + i], U[2 * nc + i], U[3 * nc + i]);
+    if (std::fclose(f) != 0) return c->fail(O2F_ERR_IO, std::string("Erreur d'écriture dans ") + path);
+    return O2F_OK;
+}
+
+int o2f_advection(o2f_ctx *c, double *R_out) {
+    if (!c) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, c->stream));
+    HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, c->Gr, c->phys, c->stream));
+    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, nullptr, 0.0, 0.0,
+                                 kFlagAdv | kFlagStoreR, c->phys, c->stream));
+    ++c->faces_version;
+    c->have_grad = true;
+    if (!R_out) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return O2F_OK;
+    }
+    return copy_out(c, R_out, c->R, 4 * (size_t)c->hm.nc);
+}
+
+int o2f_diffusion(o2f_ctx *c, double *R_out) {
+    if (!c) return O2F_ERR_ARG;
+    if (!c->have_grad) return c->fail(O2F_ERR_ARG, "o2f_diffusion: no cell gradients yet (call o2f_advection first)");
+    int st = check_bc(c);
+    if (st) return st;
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    HIPCHK(c, launch_stress(T, c->U, c->Gr, c->tq, c->phys, c->stream));
+    HIPCHK(c, launch_vertex_visc(T, G, c->tq, c->vtq, c->phys, c->stream));
+    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, nullptr, 0.0, 0.0,
+                                 kFlagVisc | kFlagStoreR, c->phys, c->stream));
+    if (!R_out) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return O2F_OK;
+    }
+    return copy_out(c, R_out, c->R, 4 * (size_t)c->hm.nc);
+}
+
+int o2f_residual(o2f_ctx *c, double *R_out) {
+    if (!c) return O2F_ERR_ARG;
+    int st = residual_on(c, 0);
+    if (st) return st;
+    if (!R_out) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        return O2F_OK;
+    }
+    return copy_out(c, R_out, c->R, 4 * (size_t)c->hm.nc);
+}
+
+int o2f_get_gradients(o2f_ctx *c, double *G) {
+    if (!c || !G) return O2F_ERR_ARG;
+    return copy_out(c, G, c->Gr, 8 * (size_t)c->hm.nc);
+}
+
+int o2f_get_face_states(o2f_ctx *c, double *L, double *R) {
+    if (!c) return O2F_ERR_ARG;
+    int st = copy_out(c, L, c->fL, 4 * (size_t)c->hm.ne);
+    if (st) return st;
+    return copy_out(c, R, c->fR, 4 * (size_t)c->hm.ne);
+}
+
+int o2f_set_face_states(o2f_ctx *c, const double *L, const double *R) {
+    if (!c || !L || !R) return O2F_ERR_ARG;
+    const size_t n = 4 * (size_t)c->hm.ne * sizeof(double);
+    HIPCHK(c, hipMemcpyAsync(c->fL, L, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->fR, R, n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    ++c->faces_version;
+    return O2F_OK;
+}
+
+int o2f_hllc_flux_batch(const double *qL, const double *qR, const double *normal, double gamma, double *F, int n) {
+    if (!qL || !qR || !normal || !F || n < 0) {
+        g_error = "o2f_hllc_flux_batch: bad argument";
+        return O2F_ERR_ARG;
+    }
+    if (n == 0) return O2F_OK;
+    double *d = nullptr;
+    const size_t bytes = (size_t)n * sizeof(double);
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&d), 14 * bytes);
+    if (!e) e = hipMemcpy(d, qL, 4 * bytes, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(d + 4 * (size_t)n, qR, 4 * bytes, hipMemcpyHostToDevice);
+    if (!e) e = hipMemcpy(d + 8 * (size_t)n, normal, 2 * bytes, hipMemcpyHostToDevice);
+    if (!e) e = launch_hllc_batch(d, d + 4 * (size_t)n, d + 8 * (size_t)n, gamma, d + 10 * (size_t)n, n, nullptr);
+    if (!e) e = hipMemcpy(F, d + 10 * (size_t)n, 4 * bytes, hipMemcpyDeviceToHost);
+    if (d) (void)hipFree(d);
+    if (e) {
+        g_error = std::string("o2f_hllc_flux_batch: ") + hipGetErrorString(e);
+        return O2F_ERR_HIP;
+    }
+    return O2F_OK;
+}
+
+int o2f_compute_dt(o2f_ctx *c, double *dt) {
+    if (!c || !dt) return O2F_ERR_ARG;
+    HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr,
+                        c->stream, 0));
+    HIPCHK(c, hipMemcpyAsync(dt, c->scal + kScalDt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+int o2f_rk3_step(o2f_ctx *c, double dt) {
+    if (!c) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalDt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((st = rk3_stages(c, true))) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+int o2f_run_steps(o2f_ctx *c, int nsteps, double *dts, double *t_inout) {
+    if (!c || nsteps < 0) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    double t0 = t_inout ? *t_inout : 0.0;
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    double *hist = nullptr;
+    if (dts && nsteps > 0) HIPCHK(c, hipMalloc(reinterpret_cast<void **>(&hist), nsteps * sizeof(double)));
+    for (int s = 0; s < nsteps; ++s) {
+        HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal,
+                            hist ? hist + s : nullptr, c->stream));
+        if ((st = rk3_stages(c, s == nsteps - 1))) return st;
+    }
+    if (hist) {
+        HIPCHK(c, hipMemcpyAsync(dts, hist, nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        HIPCHK(c, hipFree(hist));
+    }
+    double t1 = t0;
+    HIPCHK(c, hipMemcpyAsync(&t1, c->scal + kScalT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (t_inout) *t_inout = t1;
+    return O2F_OK;
+}
+
+int o2f_time_loop(o2f_ctx *c, double final_time, int max_steps, int *steps, double *t) {
+    if (!c) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    // while (t < final_time): the step count is unknown in advance, so t is
+    // read back after each step (the reference checks it on the host too).
+    double tt = 0.0;
+    int n = 0;
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &tt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    while (tt < final_time && (max_steps < 0 || n < max_steps)) {
+        HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr, c->stream));
+        if ((st = rk3_stages(c, true))) return st;
+        HIPCHK(c, hipMemcpyAsync(&tt, c->scal + kScalT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        ++n;
+    }
+    if (steps) *steps = n;
+    if (t) *t = tt;
+    return O2F_OK;
+}
+
+int o2f_residual_norm(o2f_ctx *c, double *rnorm) {
+    if (!c || !rnorm) return O2F_ERR_ARG;
+    const long nc = c->hm.nc;
+    HIPCHK(c, launch_dot2(c->R, c->R, nullptr, nullptr, nc, c->partials, c->counter, c->scal, kOpStore, c->stream));
+    double s = 0.0;
+    HIPCHK(c, hipMemcpyAsync(&s, c->scal + kScalDotA, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const double a_inf = std::sqrt(c->prm.gamma * c->prm.p_ref / c->prm.rho_ref);
+    const double scale = c->prm.rho_ref * (c->prm.inflow_velocity * a_inf) / 1.0;
+    *rnorm = std::sqrt(s / nc) / scale;
+    return O2F_OK;
+}
+
+static int wall_sums(o2f_ctx *c, int g, int skip_x, double *L, double *D) {
+    const int nw = static_cast<int>(c->hm.wall_edges.size());
+    HIPCHK(c, launch_wall_forces(c->geo[g].view(), c->wall, nw, c->fL, c->hm.ne, c->prm.gamma, skip_x, c->wall_out,
+                                 c->stream));
+    double out[2];
+    HIPCHK(c, hipMemcpyAsync(out, c->wall_out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *L = out[0];
+    *D = out[1];
+    return O2F_OK;
+}
+
+int o2f_lift_drag(o2f_ctx *c, double *C_L, double *C_D) {
+    if (!c || !C_L || !C_D) return O2F_ERR_ARG;
+    double L, D;
+    int st = wall_sums(c, 0, 0, &L, &D);
+    if (st) return st;
+    const double corde = 1;
+    *C_L = L / (c->q_inf * corde);
+    *C_D = D / (c->q_inf * corde);
+    return O2F_OK;
+}
+
+static int objective_on(o2f_ctx *c, int g, double beta, double *J) {
+    double L, D;
+    int st = wall_sums(c, g, 1, &L, &D);
+    if (st) return st;
+    const double corde = 1;
+    const double CL = L / (c->q_inf * corde), CD = D / (c->q_inf * corde);
+    *J = -CL + beta * CD;
+    return O2F_OK;
+}
+
+int o2f_objective(o2f_ctx *c, double beta, double *J) {
+    if (!c || !J) return O2F_ERR_ARG;
+    return objective_on(c, 0, beta, J);
+}
+
+static int ensure_work(o2f_ctx *c) {
+    const size_t n = 4 * (size_t)c->hm.nc;
+    for (double *&p : c->kv)
+        if (!p) HIPCHK(c, dalloc(p, n));
+    if (!c->blk) HIPCHK(c, dalloc(c->blk, 32 * (size_t)c->hm.ne));
+    return O2F_OK;
+}
+
+static int rhs_device(o2f_ctx *c, double beta, double *b) {
+    const long n = 4L * c->hm.nc;
+    HIPCHK(c, launch_fill(n, b, 0.0, c->stream));
+    HIPCHK(c, launch_adjoint_rhs(c->topo(), c->geo[0].view(), c->wall, (int)c->hm.wall_edges.size(), c->fL,
+                                 c->prm.gamma, c->q_inf, beta, b, c->stream));
+    return O2F_OK;
+}
+
+int o2f_adjoint_rhs(o2f_ctx *c, double beta, double *b) {
+    if (!c || !b) return O2F_ERR_ARG;
+    int st = ensure_work(c);
+    if (st) return st;
+    if ((st = rhs_device(c, beta, c->kv[0]))) return st;
+    return copy_out(c, b, c->kv[0], 4 * (size_t)c->hm.nc);
+}
+
+// FD blocks at the current face states, rebuilt only when the face states
+// or eps changed.
+static int ensure_blocks(o2f_ctx *c, double eps) {
+    int st = ensure_work(c);
+    if (st) return st;
+    if (c->blk_version == c->faces_version && c->blk_eps == eps) return O2F_OK;
+    HIPCHK(c, launch_jt_blocks(c->topo(), c->geo[0].view(), c->fL, c->fR, eps, c->phys, c->blk, c->stream));
+    c->blk_version = c->faces_version;
+    c->blk_eps = eps;
+    return O2F_OK;
+}
+
+int o2f_apply_jt(o2f_ctx *c, const double *x, double *y, double eps) {
+    if (!c || !x || !y) return O2F_ERR_ARG;
+    int st = ensure_blocks(c, eps);
+    if (st) return st;
+    const size_t n = 4 * (size_t)c->hm.nc;
+    HIPCHK(c, hipMemcpyAsync(c->kv[0], x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_jt_apply(c->topo(), c->geo[0].view(), c->blk, c->kv[0], c->kv[1], c->stream));
+    return copy_out(c, y, c->kv[1], n);
+}
+
+namespace {
+struct HostScal {
+    double v[kScalCount];
+};
+int read_scal(o2f_ctx *c, HostScal &h) {
+    HIPCHK(c, hipMemcpyAsync(h.v, c->scal, sizeof(h.v), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+int set_scal(o2f_ctx *c, int slot, double v) {
+    HIPCHK(c, hipMemcpyAsync(c->scal + slot, &v, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+// One call of Eigen's internal::bicgstab (identity preconditioner) starting
+// from x: returns the number of iterations it ran.
+int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol, int *iters) {
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    hipStream_t s = c->stream;
+    const long n = 4L * c->hm.nc;
+    double *r = c->kv[1], *r0 = c->kv[2], *v = c->kv[3], *p = c->kv[4], *sv = c->kv[5], *t = c->kv[6];
+    double *tmp = c->kv[7];
+    HostScal h;
+    int st;
+    // r = b - A x; r0 = r
+    HIPCHK(c, launch_jt_apply(T, G, c->blk, x, tmp, s));
+    HIPCHK(c, launch_sub(n, b, tmp, r, s));
+    HIPCHK(c, launch_sub(n, r, nullptr, r0, s));
+    HIPCHK(c, launch_dot2(r0, r0, b, b, n, c->partials, c->counter, c->scal, kOpStore, s));
+    if ((st = read_scal(c, h))) return st;
+    const double r0_sq = h.v[kScalDotA], rhs_sq = h.v[kScalDotB];
+    if (rhs_sq == 0) {
+        HIPCHK(c, launch_fill(n, x, 0.0, s));
+        *iters = 0;
+        return O2F_OK;
+    }
+    double r0_sqnorm = r0_sq;
+    // rho = alpha = w = 1; v = p = 0
+    const double init[4] = {1.0, 1.0, 1.0, 1.0};
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalRho, init, 4 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(c, launch_fill(n, v, 0.0, s));
+    HIPCHK(c, launch_fill(n, p, 0.0, s));
+    const double tol2 = tol * tol * rhs_sq;
+    const double eps2 = DBL_EPSILON * DBL_EPSILON;
+    int i = 0, restarts = 0;
+    // rho_old = rho; rho = r0.r, together with r.r for the loop test
+    HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->counter, c->scal, kOpRho, s));
+    if ((st = read_scal(c, h))) return st;
+    while (h.v[kScalRR] > tol2 && i < maxit) {
+        if (std::fabs(h.v[kScalRho]) < eps2 * r0_sqnorm) {
+            // r became too orthogonal to r0: restart with r0 = r = b - A x
+            HIPCHK(c, launch_jt_apply(T, G, c->blk, x, tmp, s));
+            HIPCHK(c, launch_sub(n, b, tmp, r, s));
+            HIPCHK(c, launch_sub(n, r, nullptr, r0, s));
+            HIPCHK(c, launch_dot2(r, r, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpRestart, s));
+            if ((st = read_scal(c, h))) return st;
+            r0_sqnorm = h.v[kScalR0Sq];
+            if (restarts++ == 0) i = 0;
+        }
+        HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
+        HIPCHK(c, launch_jt_apply(T, G, c->blk, p, v, s));
+        HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpAlpha, s));
+        HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, s));
+        HIPCHK(c, launch_jt_apply(T, G, c->blk, sv, t, s));
+        HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->counter, c->scal, kOpOmega, s));
+        HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s));
+        ++i;
+        HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->counter, c->scal, kOpRho, s));
+        if ((st = read_scal(c, h))) return st;
+    }
+    *iters = i;
+    return O2F_OK;
+}
+}  // namespace
+
+int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int max_total, double *lambda,
+                 int *iters, double *relres) {
+    if (!c || !lambda || chunk <= 0) return O2F_ERR_ARG;
+    int st = ensure_blocks(c, eps);
+    if (st) return st;
+    const long n = 4L * c->hm.nc;
+    double *b = c->kv[0], *x = c->kv[8], *tmp = c->kv[9];
+    if ((st = rhs_device(c, beta, b))) return st;
+    HIPCHK(c, launch_fill(n, x, 0.0, c->stream));
+    HostScal h;
+    HIPCHK(c, launch_dot2(b, b, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpStore, c->stream));
+    if ((st = read_scal(c, h))) return st;
+    const double bnorm = std::sqrt(h.v[kScalDotA]);
+    double rr = INFINITY;
+    int total = 0, status = O2F_OK;
+    while (total < max_total) {
+        int it = 0;
+        if ((st = bicgstab_chunk(c, b, x, chunk, tol, &it))) return st;
+        total += it;
+        // true residual b - J^T x (src/shape_optimization.cpp:715-717)
+        HIPCHK(c, launch_jt_apply(c->topo(), c->geo[0].view(), c->blk, x, tmp, c->stream));
+        HIPCHK(c, launch_sub(n, b, tmp, tmp, c->stream));
+        HIPCHK(c, launch_dot2(tmp, tmp, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpStore, c->stream));
+        if ((st = read_scal(c, h))) return st;
+        rr = std::sqrt(h.v[kScalDotA]) / (bnorm > 0 ? bnorm : 1.0);
+        if (rr <= tol) break;
+        if (!std::isfinite(rr)) {
+            status = c->fail(O2F_ERR_DIVERGED, "BiCGSTAB diverged");
+            break;
+        }
+    }
+    if (status == O2F_OK && rr > tol) status = c->fail(O2F_ERR_NOCONV, "BiCGSTAB did not converge");
+    if (iters) *iters = total;
+    if (relres) *relres = rr;
+    st = copy_out(c, lambda, x, n);
+    return st ? st : status;
+}
+
+int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *values_all, int n_all,
+                       const char *const *opt_names, int n_opt, const double *lambda, double beta, double eps,
+                       double *grad) {
+    if (!c || (n_all > 0 && (!names_all || !values_all)) || (n_opt > 0 && (!opt_names || !grad)) || !lambda)
+        return O2F_ERR_ARG;
+    // getParamOrDefault (src/shape_optimization.cpp:101-109, defaults :830-834)
+    auto param = [&](const char *key, double def) {
+        for (int i = 0; i < n_all; ++i)
+            if (std::strcmp(names_all[i], key) == 0) return values_all[i];
+        return def;
+    };
+    const double pos_x = param("pos_x", 0.355), chord = param("c", 1.0), m0 = param("m", 0.02);
+    const double p0 = param("p", 0.4), t = param("t", 0.12);
+    int st = ensure_work(c);
+    if (st) return st;
+    const long n = 4L * c->hm.nc;
+    double *lam = c->kv[0], *dR = c->kv[1];
+    HIPCHK(c, hipMemcpyAsync(lam, lambda, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (!c->Rp) HIPCHK(c, dalloc(c->Rp, n));
+    if (!c->Rm) HIPCHK(c, dalloc(c->Rm, n));
+    // computeResidual(mesh): face states at the current state
+    if ((st = residual_on(c, 0))) return st;
+    for (int k = 0; k < n_opt; ++k) {
+        const std::string name = opt_names[k];
+        const int which = (name == "m" || name == "cambrure") ? 0 : (name == "p" || name == "position_cambrure") ? 1 : -1;
+        if (which < 0) {  // applyShapePerturbationNACA12 leaves the mesh unchanged
+            grad[k] = 0.0;
+            continue;
+        }
+        double J[2];
+        double *Rpm[2] = {c->Rp, c->Rm};
+        for (int side = 0; side < 2; ++side) {
+            HostMesh hmp = c->hm;
+            perturb_naca(hmp, which, pos_x, chord, m0, p0, t, side == 0 ? eps : -eps);
+            compute_stencils(hmp, c->prm.interp_mode);
+            GeomSet &gs = c->geo[1 + side];
+            if (!gs.cvol) HIPCHK(c, gs.alloc(hmp));
+            HIPCHK(c, gs.fill(hmp, c->stream));
+            if ((st = residual_on(c, 1 + side))) return st;
+            HIPCHK(c, hipMemcpyAsync(Rpm[side], c->R, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+            if ((st = objective_on(c, 1 + side, beta, &J[side]))) return st;
+        }
+        HIPCHK(c, launch_fd_diff(n, c->Rp, c->Rm, eps, dR, c->stream));
+        HIPCHK(c, launch_dot2(lam, dR, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpStore, c->stream));
+        HostScal h;
+        if ((st = read_scal(c, h))) return st;
+        const double dJ_direct = (J[0] - J[1]) / (2.0 * eps);
+        grad[k] = dJ_direct - h.v[kScalDotA];
+    }
+    // leave the context as computeResidual(mesh) left the reference's mesh
+    if ((st = residual_on(c, 0))) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) {
+    if (!c || nsteps <= 0 || !total_ms) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    hipStream_t s = c->stream;
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipStreamSynchronize(s));
+    HIPCHK(c, hipEventRecord(e0, s));
+    for (int k = 0; k < nsteps; ++k) {
+        HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr, s));
+        if ((st = rk3_stages(c, k == nsteps - 1))) return st;
+    }
+    HIPCHK(c, hipEventRecord(e1, s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    *total_ms = ms;
+    if (kernel_ms) {
+        // Second pass: every stage kernel bracketed by its own events.
+        const int nk = 4;
+        std::vector<hipEvent_t> ev(2 * nk * 3 * nsteps);
+        for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+        const Topo T = c->topo();
+        const Geom G = c->geo[0].view();
+        const double *dtp = c->scal + kScalDt;
+        const double cs[3][2] = {{0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0}};
+        int q = 0;
+        for (int k = 0; k < nsteps; ++k) {
+            HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr, s));
+            for (int stg = 0; stg < 3; ++stg) {
+                const int fl = kFlagAdv | kFlagVisc | kFlagRK | (stg == 0 ? kFlagSaveOld : 0);
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+                HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, s));
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+                HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, nullptr, c->phys, s));
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+                HIPCHK(c, launch_vertex_visc(T, G, c->tq, c->vtq, c->phys, s));
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+                HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, dtp, cs[stg][0],
+                                             cs[stg][1], fl, c->phys, s));
+                HIPCHK(c, hipEventRecord(ev[q++], s));
+            }
+        }
+        HIPCHK(c, hipStreamSynchronize(s));
+        for (int j = 0; j < nk; ++j) kernel_ms[j] = 0.0;
+        for (int j = 0; j < q; j += 2) {
+            float m = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&m, ev[j], ev[j + 1]));
+            kernel_ms[(j / 2) % nk] += m;
+        }
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        ++c->faces_version;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return O2F_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,895 @@
+// gfx950 kernels of the residual / SSP-RK3 / adjoint path.
+//
+// FP64 throughout, one work-item per vertex, cell or edge, no MFMA (the
+// flux path is gather-bound, HBM / Infinity-Cache bound).  Compiled with
+// -ffp-contract=off: every expression rounds exactly as the reference's C++,
+// so results match the CPU restatement bit for bit wherever the summation
+// order is the same.  Scatter-free: every cell gathers its own sums in
+// ascending edge id, which makes the results deterministic and independent
+// of scheduling (the reference's `omp atomic` scatter is not).
+#include "o2f_kernels.hpp"
+
+#include <cfloat>
+
+namespace o2f {
+
+// std::min / std::max / std::clamp semantics (NaN handling included).
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double sclamp(double v, double lo, double hi) {
+    return (v < lo) ? lo : (hi < v) ? hi : v;
+}
+__device__ __forceinline__ bool tagged(char b) {
+    return b == 'i' || b == 'o' || b == 'f' || b == 'w';
+}
+
+// apply_inflow_outflow_bc, src/boundary.cpp:7-45 (freestream state precomputed
+// on the host with the same expression order).
+__device__ __forceinline__ void bc_state(const Phys &P, char bc, double &r, double &ru, double &rv,
+                                         double &E) {
+    if (bc == 'i' || bc == 'f') {
+        r = P.fs_rho;
+        ru = P.fs_ru;
+        rv = P.fs_rv;
+        E = P.fs_E;
+    } else if (bc == 'w') {
+        const double kinetic_old = 0.5 * (ru * ru + rv * rv) / r;
+        const double e_internal = E - kinetic_old;
+        ru = 0.0;
+        rv = 0.0;
+        E = e_internal;
+    }
+}
+
+// apply_InterfaceStates_boundary_conditions, src/boundary.cpp:48-80.
+__device__ __forceinline__ void bc_ghost(const Phys &P, char bc, const double *L, double *R) {
+    R[0] = L[0];
+    R[1] = L[1];
+    R[2] = L[2];
+    R[3] = L[3];
+    if (bc == 'i' || bc == 'o' || bc == 'f') {
+        bc_state(P, bc, R[0], R[1], R[2], R[3]);
+    } else if (bc == 'w') {
+        R[1] = -L[1];
+        R[2] = -L[2];
+    }
+}
+
+// compute_hllc_flux, src/advection.cpp:325-412.  (enx, eny) is the stored
+// normal (into the left cell); the flux is taken along -n.
+__device__ __forceinline__ void hllc(const double *qL, const double *qR, double enx, double eny,
+                                     double gamma, double *F) {
+    const double nx = -enx, ny = -eny;
+    const double rho_L = qL[0];
+    const double u_L = (qL[1] * (nx) + qL[2] * ny) / rho_L;
+    const double v_L = (qL[1] * (-ny) + qL[2] * nx) / rho_L;
+    const double E_L = qL[3];
+    const double p_L = (gamma - 1.0) * (E_L - 0.5 * rho_L * (u_L * u_L + v_L * v_L));
+    const double c_L = sqrt(gamma * p_L / rho_L);
+    const double rho_R = qR[0];
+    const double u_R = (qR[1] * (nx) + qR[2] * ny) / rho_R;
+    const double v_R = (qR[1] * (-ny) + qR[2] * nx) / rho_R;
+    const double E_R = qR[3];
+    const double p_R = (gamma - 1.0) * (E_R - 0.5 * rho_R * (u_R * u_R + v_R * v_R));
+    const double c_R = sqrt(gamma * p_R / rho_R);
+
+    const double S_L = smin(u_L - c_L, u_R - c_R);
+    const double S_R = smax(u_L + c_L, u_R + c_R);
+    const double numer = p_R - p_L + rho_L * u_L * (S_L - u_L) - rho_R * u_R * (S_R - u_R);
+    const double denom = rho_L * (S_L - u_L) - rho_R * (S_R - u_R);
+    const double S_star = numer / denom;
+    const double p_star = p_L + rho_L * (S_L - u_L) * (S_star - u_L);
+    double f0, fu, fv, fE;
+    if (S_L >= 0.0) {
+        f0 = rho_L * u_L;
+        fu = rho_L * u_L * u_L + p_L;
+        fv = rho_L * u_L * v_L;
+        fE = (E_L + p_L) * u_L;
+    } else if (S_L <= 0.0 && S_star >= 0.0) {
+        const double rs = rho_L * (S_L - u_L) / (S_L - S_star);
+        const double rvs = rs * v_L;
+        const double Es = ((S_L - u_L) * E_L - p_L * u_L + p_star * S_star) / (S_L - S_star);
+        const double rus = rs * S_star;
+        f0 = rus;
+        fu = rus * rus / rs + p_star;
+        fv = rvs * rus / rs;
+        fE = (Es + p_star) * rus / rs;
+    } else if (S_star <= 0.0 && S_R >= 0.0) {
+        const double rs = rho_R * (S_R - u_R) / (S_R - S_star);
+        const double rvs = rs * v_R;
+        const double Es = ((S_R - u_R) * E_R - p_R * u_R + p_star * S_star) / (S_R - S_star);
+        const double rus = rs * S_star;
+        f0 = rus;
+        fu = rus * rus / rs + p_star;
+        fv = rvs * rus / rs;
+        fE = (Es + p_star) * rus / rs;
+    } else {
+        f0 = rho_R * u_R;
+        fu = rho_R * u_R * u_R + p_R;
+        fv = rho_R * u_R * v_R;
+        fE = (E_R + p_R) * u_R;
+    }
+    F[0] = f0;
+    F[1] = (fu * nx + fv * (-ny));
+    F[2] = (fu * ny + fv * (nx));
+    F[3] = fE;
+}
+
+// ------------------------------------------------------------------------
+// Cell -> vertex gather with precomputed stencils
+// (src/advection.cpp:26-131 for the state, src/diffusion.cpp:28-140 for the
+// viscous quantities).  NV = 4 state variables with the state BC, or NV = 5
+// viscous quantities with apply_diffusion_bc (src/boundary.cpp:101-127).
+// ------------------------------------------------------------------------
+template <int NV>
+__global__ __launch_bounds__(256) void k_vertex_gather(Topo T, Geom G, const double *__restrict__ src,
+                                                       double *__restrict__ dst, Phys P) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= T.nv) return;
+    const int nc = T.nc;
+    const uint8_t kind = T.skind[v];
+    const int b = T.sptr[v], e = T.sptr[v + 1];
+    double out[NV];
+    if (kind == kStencilWsum) {
+        int c = T.scell[b];
+        double w = G.sw[b];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = w * src[q * nc + c];
+        for (int k = b + 1; k < e; ++k) {
+            c = T.scell[k];
+            w = G.sw[k];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] = out[q] + w * src[q * nc + c];
+        }
+    } else if (kind == kStencilAvg2) {
+        const int c0 = T.scell[b], c1 = T.scell[b + 1];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = 0.5 * (src[q * nc + c0] + src[q * nc + c1]);
+    } else if (kind == kStencilCopy) {
+        const int c0 = T.scell[b];
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = src[q * nc + c0];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = 0.0;
+    }
+    const char bc = T.vbc[v];
+    if (NV == 4) {
+        if (tagged(bc)) bc_state(P, bc, out[0], out[1], out[2], out[3]);
+    } else {
+        if (bc == 'f' || bc == 'w') {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NV; ++q) dst[q * T.nv + v] = out[q];
+}
+
+// ------------------------------------------------------------------------
+// Cell reconstruction: edge values + Green-Gauss gradient
+// (src/advection.cpp:136-188), neighbour min/max, limiter and face states
+// with boundary ghosts (src/advection.cpp:232-321), and the cell stress /
+// heat flux of computeDiffusionTerm (src/diffusion.cpp:223-241), fused.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double alpha_venkat(double phi_i, double phi_pred, double phi_min,
+                                               double phi_max, double cellArea, double k_eps) {
+    const double delta = phi_pred - phi_i;
+    if (fabs(delta) < DBL_EPSILON) return 1.0;
+    const double h = sqrt(cellArea / M_PI);
+    const double eps = k_eps * k_eps * k_eps * h * h * h;
+    const double a = (delta > 0.0) ? (phi_max - phi_i) : (phi_min - phi_i);
+    const double alpha = (a * a + eps + 2.0 * a * delta) / (a * a + eps + 2.0 * delta * delta);
+    return sclamp(alpha, 0.0, 1.0);
+}
+
+__device__ __forceinline__ double alpha_bj(double phi_i, double phi_pred, double phi_min,
+                                           double phi_max) {
+    double alpha = 1.0;
+    if (fabs(phi_pred - phi_i) < DBL_EPSILON) return alpha;
+    if (phi_pred > phi_i)
+        alpha = smin(1.0, (phi_max - phi_i) / (phi_pred - phi_i));
+    else if (phi_pred < phi_i)
+        alpha = smin(1.0, (phi_min - phi_i) / (phi_pred - phi_i));
+    return alpha;
+}
+
+__global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const double *__restrict__ U,
+                                                          const double *__restrict__ vU,
+                                                          double *__restrict__ fL, double *__restrict__ fR,
+                                                          double *__restrict__ tq, double *__restrict__ Gout,
+                                                          Phys P) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= T.nc) return;
+    const int nc = T.nc, nv = T.nv, ne = T.ne;
+    const unsigned info = T.cinfo[c];
+    const int nk = info >> 4;
+    double u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) u[q] = U[q * nc + c];
+    const double cv = G.cvol[c], cx = G.ccx[c], cy = G.ccy[c];
+    int ed[4];
+    double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nk; ++k) {
+        const int e = T.cedge[4 * c + k];
+        ed[k] = e;
+        const int a = T.eva[e], b = T.evb[e];
+        double val[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) val[q] = 0.5 * (vU[q * nv + a] + vU[q * nv + b]);
+        const char bc = T.ebc[e];
+        if (tagged(bc)) bc_state(P, bc, val[0], val[1], val[2], val[3]);
+        const double el = G.elen[e];
+        double enx = G.enx[e], eny = G.eny[e];
+        if ((info >> k) & 1u) {
+            enx = -enx;
+            eny = -eny;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            gx[q] += val[q] * el * enx / cv;
+            gy[q] += val[q] * el * eny / cv;
+        }
+    }
+    if (Gout) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Gout[q * nc + c] = gx[q];
+            Gout[(4 + q) * nc + c] = gy[q];
+        }
+    }
+    double mn[4], mx[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) mn[q] = mx[q] = u[q];
+    for (int k = 0; k < 4; ++k) {
+        const int nb = T.cnbr[4 * c + k];
+        if (nb < 0) break;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double v = U[q * nc + nb];
+            if (v < mn[q]) mn[q] = v;
+            if (v > mx[q]) mx[q] = v;
+        }
+    }
+    double al[4] = {1.0, 1.0, 1.0, 1.0};
+    double dxs[4], dys[4];
+    for (int k = 0; k < nk; ++k) {
+        const int e = ed[k];
+        const double dx = G.emx[e] - cx, dy = G.emy[e] - cy;
+        dxs[k] = dx;
+        dys[k] = dy;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double pred = u[q] + gx[q] * dx + gy[q] * dy;
+            const double a = (P.limiter == 1) ? alpha_bj(u[q], pred, mn[q], mx[q])
+                                              : alpha_venkat(u[q], pred, mn[q], mx[q], cv, P.venkat_k);
+            al[q] = smin(al[q], a);
+        }
+    }
+    for (int k = 0; k < nk; ++k) {
+        const int e = ed[k];
+        double f[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[q] = u[q] + al[q] * (gx[q] * dxs[k] + gy[q] * dys[k]);
+        if ((info >> k) & 1u) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fL[q * ne + e] = f[q];
+            const char bc = T.ebc[e];
+            if (tagged(bc)) {
+                double g[4];
+                bc_ghost(P, bc, f, g);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) fR[q * ne + e] = g[q];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) fR[q * ne + e] = f[q];
+        }
+    }
+    // viscous stress and heat flux of the cell (src/diffusion.cpp:227-240)
+    const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
+    const double rho = u[0], E = u[3];
+    const double uu = u[1] / rho, vv = u[2] / rho;
+    const double u_x = (gx[1] - uu * gx[0]) / rho;
+    const double v_y = (gy[2] - vv * gy[0]) / rho;
+    const double u_y = (gy[1] - uu * gy[0]) / rho;
+    const double v_x = (gx[2] - vv * gx[0]) / rho;
+    tq[c] = 2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y);
+    tq[nc + c] = 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y);
+    tq[2 * nc + c] = mu * (u_y + v_x);
+    const double T_x = (gamma - 1.0) * ((gx[3] * rho - E * gx[0]) / (rho * rho) - (uu * u_x + vv * v_x));
+    const double T_y = (gamma - 1.0) * ((gy[3] * rho - E * gy[0]) / (rho * rho) - (uu * u_y + vv * v_y));
+    tq[3 * nc + c] = -kappa * T_x;
+    tq[4 * nc + c] = -kappa * T_y;
+}
+
+// Stress / heat flux from stored gradients (computeDiffusionTerm called on
+// its own, src/diffusion.cpp:223-241).
+__global__ __launch_bounds__(256) void k_cell_stress(Topo T, const double *__restrict__ U,
+                                                     const double *__restrict__ Gs,
+                                                     double *__restrict__ tq, Phys P) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= T.nc) return;
+    const int nc = T.nc;
+    const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
+    const double rho = U[c], E = U[3 * nc + c];
+    const double rx = Gs[c], rux = Gs[nc + c], rvx = Gs[2 * nc + c], Ex = Gs[3 * nc + c];
+    const double ry = Gs[4 * nc + c], ruy = Gs[5 * nc + c], rvy = Gs[6 * nc + c], Ey = Gs[7 * nc + c];
+    const double uu = U[nc + c] / rho, vv = U[2 * nc + c] / rho;
+    const double u_x = (rux - uu * rx) / rho;
+    const double v_y = (rvy - vv * ry) / rho;
+    const double u_y = (ruy - uu * ry) / rho;
+    const double v_x = (rvx - vv * rx) / rho;
+    tq[c] = 2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y);
+    tq[nc + c] = 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y);
+    tq[2 * nc + c] = mu * (u_y + v_x);
+    const double T_x = (gamma - 1.0) * ((Ex * rho - E * rx) / (rho * rho) - (uu * u_x + vv * v_x));
+    const double T_y = (gamma - 1.0) * ((Ey * rho - E * ry) / (rho * rho) - (uu * u_y + vv * v_y));
+    tq[3 * nc + c] = -kappa * T_x;
+    tq[4 * nc + c] = -kappa * T_y;
+}
+
+// ------------------------------------------------------------------------
+// Flux + update: HLLC on each of the cell's edges and the convective sum
+// (src/advection.cpp:433-464), the viscous divergence
+// (src/diffusion.cpp:145-201), and the Shu-Osher stage update
+// (src/simulation.cpp:159-167), fused.  Each interior edge's HLLC is
+// evaluated by both of its cells (bit-identical), trading 2x flux flops for
+// no flux array and no scatter.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double *__restrict__ U,
+                                                          double *__restrict__ Uold,
+                                                          const double *__restrict__ fL,
+                                                          const double *__restrict__ fR,
+                                                          const double *__restrict__ vtq,
+                                                          double *__restrict__ R,
+                                                          const double *__restrict__ dtp, double c0,
+                                                          double c1, int flags, Phys P) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= T.nc) return;
+    const int nc = T.nc, nv = T.nv, ne = T.ne;
+    const unsigned info = T.cinfo[c];
+    const int nk = info >> 4;
+    const double cv = G.cvol[c];
+    double r[4];
+    if (flags & kFlagAdv) {
+        r[0] = r[1] = r[2] = r[3] = 0.0;
+        for (int k = 0; k < nk; ++k) {
+            const int e = T.cedge[4 * c + k];
+            double qL[4], qR[4], F[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                qL[q] = fL[q * ne + e];
+                qR[q] = fR[q * ne + e];
+            }
+            hllc(qL, qR, G.enx[e], G.eny[e], P.gamma, F);
+            const double sign = ((info >> k) & 1u) ? +1.0 : -1.0;
+            const double L = G.elen[e];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] += sign * F[q] * L;
+        }
+        const double invV = 1.0 / cv;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = r[q] * -invV;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) r[q] = R[q * nc + c];
+    }
+    if (flags & kFlagVisc) {
+        const double uu = U[nc + c] / U[c];
+        const double vv = U[2 * nc + c] / U[c];
+        for (int k = 0; k < nk; ++k) {
+            const int e = T.cedge[4 * c + k];
+            const double el = G.elen[e];
+            double enx = G.enx[e], eny = G.eny[e];
+            if ((info >> k) & 1u) {
+                enx = -enx;
+                eny = -eny;
+            }
+            const int a = T.eva[e], b = T.evb[e];
+            double t[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) t[q] = 0.5 * (vtq[q * nv + a] + vtq[q * nv + b]);
+            const char bc = T.ebc[e];
+            if (bc == 'f' || bc == 'w') {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) t[q] = 0.0;
+            }
+            const double txx = t[0], tyy = t[1], txy = t[2], qx = t[3], qy = t[4];
+            r[1] += txx * el * enx / cv + txy * el * eny / cv;
+            r[2] += txy * el * enx / cv + tyy * el * eny / cv;
+            r[3] += (uu * txx + vv * txy - qx) * el * enx / cv + (uu * txy + vv * tyy - qy) * el * eny / cv;
+        }
+    }
+    if (flags & kFlagStoreR) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R[q * nc + c] = r[q];
+    }
+    if (flags & kFlagRK) {
+        const double dt = *dtp;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double cur = U[q * nc + c];
+            double old;
+            if (flags & kFlagSaveOld) {
+                old = cur;
+                Uold[q * nc + c] = cur;
+            } else {
+                old = Uold[q * nc + c];
+            }
+            U[q * nc + c] = c0 * old + c1 * (cur + dt * (r[q]));
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
+// Deterministic reductions: fixed grid, fixed in-block tree, the last block
+// to finish combines the per-block partials in block order and runs the
+// epilogue.  The arrival counter is re-armed by the last block, so no memset
+// is ever needed (graph-capture safe).
+// ------------------------------------------------------------------------
+__device__ __forceinline__ bool last_block_arrived(unsigned *counter) {
+    __shared__ bool last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == gridDim.x - 1);
+        if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    return last;
+}
+
+__device__ __forceinline__ double load_partial(const double *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Time step, src/simulation.cpp:77-110: max over cells of |u|+c (starting
+// from 1e-8); min_h is static and precomputed on the host.  Also advances
+// the simulated time (src/simulation.cpp:121) and records dt.
+__global__ __launch_bounds__(kRedBlock) void k_dt(const double *__restrict__ U, int nc, Phys P,
+                                                 double min_h, double *__restrict__ partials,
+                                                 unsigned *counter, double *__restrict__ scal,
+                                                 double *__restrict__ hist, int advance_t) {
+    __shared__ double sm[kRedBlock];
+    double m = 1e-8;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        const double rho = U[c], ru = U[nc + c], rv = U[2 * nc + c], E = U[3 * nc + c];
+        const double u = ru / rho, v = rv / rho;
+        const double kinetic = 0.5 * (ru * ru + rv * rv) / rho;
+        const double p = (P.gamma - 1.0) * (E - kinetic);
+        const double cs = sqrt(P.gamma * p / rho);
+        m = smax(m, hypot(u, v) + cs);
+    }
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(&partials[blockIdx.x], sm[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (!last_block_arrived(counter)) return;
+    double mm = 1e-8;
+    for (int b = threadIdx.x; b < gridDim.x; b += blockDim.x) mm = smax(mm, load_partial(&partials[b]));
+    sm[threadIdx.x] = mm;
+    __syncthreads();
+    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double max_wave = sm[0];
+        const double dt_adv = P.cfl * min_h / max_wave;
+        const double dt_diff = 0.5 * (min_h * min_h) / P.mu;
+        const double dt = smin(dt_adv, dt_diff);
+        scal[kScalDt] = dt;
+        if (advance_t) scal[kScalT] = scal[kScalT] + dt;
+        if (hist) *hist = dt;
+    }
+}
+
+// Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i) over n
+// entries, combined in a fixed order; the last block applies `op`
+// to the BiCGSTAB scalars or just stores the sums.
+__global__ __launch_bounds__(kRedBlock) void k_dot2(const double *__restrict__ a, const double *__restrict__ b,
+                                                   const double *__restrict__ c, const double *__restrict__ d,
+                                                   long n, double *__restrict__ partials, unsigned *counter,
+                                                   double *__restrict__ scal, int op) {
+    __shared__ double s1[kRedBlock], s2[kRedBlock];
+    double x = 0.0, y = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        x += a[i] * b[i];
+        if (c) y += c[i] * d[i];
+    }
+    s1[threadIdx.x] = x;
+    s2[threadIdx.x] = y;
+    __syncthreads();
+    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+            s2[threadIdx.x] += s2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[2 * blockIdx.x], s1[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[2 * blockIdx.x + 1], s2[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (!last_block_arrived(counter)) return;
+    x = 0.0;
+    y = 0.0;
+    for (int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x) {
+        x += load_partial(&partials[2 * bb]);
+        y += load_partial(&partials[2 * bb + 1]);
+    }
+    s1[threadIdx.x] = x;
+    s2[threadIdx.x] = y;
+    __syncthreads();
+    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+            s2[threadIdx.x] += s2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double ab = s1[0], cd = s2[0];
+    switch (op) {
+        case kOpStore:  // raw sums
+            scal[kScalDotA] = ab;
+            scal[kScalDotB] = cd;
+            break;
+        case kOpRho:  // (r0.r, r.r): rho_old = rho; rho = r0.r
+            scal[kScalRhoOld] = scal[kScalRho];
+            scal[kScalRho] = ab;
+            scal[kScalRR] = cd;
+            break;
+        case kOpRestart:  // (r.r): rho = r0_sqnorm = r.r
+            scal[kScalRho] = ab;
+            scal[kScalR0Sq] = ab;
+            scal[kScalRR] = ab;
+            break;
+        case kOpAlpha:  // (r0.v): alpha = rho / r0.v
+            scal[kScalAlpha] = scal[kScalRho] / ab;
+            break;
+        case kOpOmega:  // (t.t, t.s): w = t.s / t.t or 0
+            scal[kScalW] = (ab > 0) ? cd / ab : 0.0;
+            break;
+    }
+}
+
+// BiCGSTAB vector updates (Eigen's internal::bicgstab recurrence).
+__global__ __launch_bounds__(256) void k_bicg_p(long n, const double *__restrict__ r, double *__restrict__ p,
+                                                const double *__restrict__ v, const double *__restrict__ scal) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double beta = (scal[kScalRho] / scal[kScalRhoOld]) * (scal[kScalAlpha] / scal[kScalW]);
+    const double w = scal[kScalW];
+    p[i] = r[i] + beta * (p[i] - w * v[i]);
+}
+
+__global__ __launch_bounds__(256) void k_bicg_s(long n, const double *__restrict__ r, const double *__restrict__ v,
+                                                double *__restrict__ s, const double *__restrict__ scal) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s[i] = r[i] - scal[kScalAlpha] * v[i];
+}
+
+__global__ __launch_bounds__(256) void k_bicg_xr(long n, double *__restrict__ x, double *__restrict__ r,
+                                                 const double *__restrict__ p, const double *__restrict__ s,
+                                                 const double *__restrict__ t, const double *__restrict__ scal) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double alpha = scal[kScalAlpha], w = scal[kScalW];
+    x[i] += alpha * p[i] + w * s[i];
+    r[i] = s[i] - w * t[i];
+}
+
+// out = a - b (r = b - A x); also used to copy (b == nullptr).
+__global__ __launch_bounds__(256) void k_sub(long n, const double *__restrict__ a, const double *__restrict__ b,
+                                             double *__restrict__ out) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = b ? a[i] - b[i] : a[i];
+}
+
+__global__ __launch_bounds__(256) void k_fill(long n, double *__restrict__ out, double v) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = v;
+}
+
+// (R+ - R-) / (2 eps), the shape derivative of the residual
+// (src/shape_optimization.cpp:1004-1006).
+__global__ __launch_bounds__(256) void k_fd_diff(long n, const double *__restrict__ Rp,
+                                                 const double *__restrict__ Rm, double eps,
+                                                 double *__restrict__ out) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (Rp[i] - Rm[i]) / (2.0 * eps);
+}
+
+// ------------------------------------------------------------------------
+// Wall integrals.  One block: the per-edge terms are formed in parallel and
+// summed by one lane in ascending edge order (the order of the reference's
+// serial loops), so C_L, C_D and the adjoint RHS match it bit for bit.
+// ------------------------------------------------------------------------
+// computeLiftDragCoefficients (src/compute.cpp:92-132) and, with
+// skip_x = 1, computeDirectObjectiveTerm (src/shape_optimization.cpp:893-939).
+// The wall stresses the reference stores are identically zero
+// (apply_diffusion_bc zeroes them, src/diffusion.cpp:155-162); the terms are
+// kept so the sums round the same way.
+__global__ __launch_bounds__(kRedBlock) void k_wall_forces(Geom G, const int *__restrict__ wall, int nw,
+                                                          const double *__restrict__ fL, int ne,
+                                                          double gamma, int skip_x,
+                                                          double *__restrict__ out) {
+    __shared__ double tl[kRedBlock], td[kRedBlock];
+    __shared__ int flag[kRedBlock];
+    double L = 0.0, D = 0.0;
+    for (int base = 0; base < nw; base += kRedBlock) {
+        const int k = base + threadIdx.x;
+        double l = 0.0, d = 0.0;
+        bool use = false;
+        if (k < nw) {
+            const int e = wall[k];
+            use = !(skip_x && G.emx[e] > 1.3);
+            const double nx = G.enx[e], ny = G.eny[e], A = G.elen[e];
+            const double rl = fL[e], rul = fL[ne + e], rvl = fL[2 * ne + e], El = fL[3 * ne + e];
+            const double p = (gamma - 1.0) * (El - 0.5 * (rul * rul + rvl * rvl) / rl);
+            const double txx = 0.0, txy = 0.0, tyy = 0.0;
+            l = (-p * ny + txy * nx + tyy * ny) * A;
+            d = (-p * nx + txx * nx + txy * ny) * A;
+        }
+        tl[threadIdx.x] = use ? l : 0.0;
+        td[threadIdx.x] = use ? d : 0.0;
+        flag[threadIdx.x] = use;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(kRedBlock, nw - base);
+            for (int j = 0; j < m; ++j)
+                if (flag[j]) {
+                    L += tl[j];
+                    D += td[j];
+                }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = L;
+        out[1] = D;
+    }
+}
+
+// assembleAdjointRHS (src/shape_optimization.cpp:491-554); b must be zero.
+__global__ __launch_bounds__(64) void k_adjoint_rhs(Topo T, Geom G, const int *__restrict__ wall, int nw,
+                                                   const double *__restrict__ fL, double gamma, double q_inf,
+                                                   double beta, double *__restrict__ b) {
+    if (threadIdx.x != 0) return;
+    const int nc = T.nc, ne = T.ne;
+    const double corde = 1;
+    for (int k = 0; k < nw; ++k) {
+        const int e = wall[k];
+        const int iL = T.ecl[e];
+        const double A_f = G.elen[e], nx = G.enx[e], ny = G.eny[e];
+        const double rL = fL[e], ruL = fL[ne + e], rvL = fL[2 * ne + e];
+        double dp[4];
+        dp[0] = 0.5 * (gamma - 1.0) * ((ruL * ruL + rvL * rvL) / (rL * rL));
+        dp[1] = -(gamma - 1.0) * (ruL / rL);
+        dp[2] = -(gamma - 1.0) * (rvL / rL);
+        dp[3] = (gamma - 1.0);
+        const double coeff_CL = +ny * A_f / (q_inf * corde);
+        for (int q = 0; q < 4; ++q) b[q * nc + iL] += coeff_CL * dp[q];
+        const double coeff_CD = -nx * A_f / (q_inf * corde) * beta;
+        for (int q = 0; q < 4; ++q) b[q * nc + iL] += coeff_CD * dp[q];
+    }
+}
+
+// ------------------------------------------------------------------------
+// Adjoint J^T x (src/shape_optimization.cpp:237-484).
+// ------------------------------------------------------------------------
+// FD flux-Jacobian blocks per edge at the frozen face states:
+// blk[(r*4+c)*ne + e] = dF_r/dU_L,c, blk[(16 + r*4+c)*ne + e] = dF_r/dU_R,c.
+// A boundary edge rebuilds its ghost from each perturbed left state; its
+// right-state block is never read by the scatter and is stored as zero.
+__global__ __launch_bounds__(256) void k_jt_blocks(Topo T, Geom G, const double *__restrict__ fL,
+                                                   const double *__restrict__ fR, double eps, Phys P,
+                                                   double *__restrict__ blk) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= T.ne) return;
+    const int ne = T.ne;
+    const char bc = T.ebc[e];
+    const double enx = G.enx[e], eny = G.eny[e];
+    double bL[4], bR[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        bL[q] = fL[q * ne + e];
+        bR[q] = fR[q * ne + e];
+    }
+    for (int c = 0; c < 4; ++c) {
+        double Lp[4], Lm[4], Rp[4], Rm[4], Fp[4], Fm[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            Lp[q] = Lm[q] = bL[q];
+            Rp[q] = Rm[q] = bR[q];
+        }
+        Lp[c] = bL[c] + eps;
+        Lm[c] = bL[c] - eps;
+        if (bc != '\0') {
+            bc_ghost(P, bc, Lp, Rp);
+            bc_ghost(P, bc, Lm, Rm);
+        }
+        hllc(Lp, Rp, enx, eny, P.gamma, Fp);
+        hllc(Lm, Rm, enx, eny, P.gamma, Fm);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) blk[(r * 4 + c) * ne + e] = (Fp[r] - Fm[r]) / (2 * eps);
+    }
+    for (int c = 0; c < 4; ++c) {
+        if (bc != '\0') {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) blk[(16 + r * 4 + c) * ne + e] = 0.0;
+            continue;
+        }
+        double Rp[4], Rm[4], Fp[4], Fm[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Rp[q] = Rm[q] = bR[q];
+        Rp[c] = bR[c] + eps;
+        Rm[c] = bR[c] - eps;
+        hllc(bL, Rp, enx, eny, P.gamma, Fp);
+        hllc(bL, Rm, enx, eny, P.gamma, Fm);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) blk[(16 + r * 4 + c) * ne + e] = (Fp[r] - Fm[r]) / (2 * eps);
+    }
+}
+
+// y = J^T x as a per-cell gather in ascending edge id: the serialisation of
+// the reference's atomic scatter in edge order, without atomics.
+__global__ __launch_bounds__(256) void k_jt_apply(Topo T, Geom G, const double *__restrict__ blk,
+                                                  const double *__restrict__ x, double *__restrict__ y) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= T.nc) return;
+    const int nc = T.nc, ne = T.ne;
+    const int nk = T.cinfo[c] >> 4;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nk; ++k) {
+        const int e = T.cedge[4 * c + k];
+        const double L = G.elen[e];
+        const int iL = T.ecl[e];
+        const double invV_L = 1.0 / G.cvol[iL];
+        double D[4];
+        int off;
+        if (T.ebc[e] != '\0') {
+            if (c != iL) continue;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) D[q] = (-x[q * nc + iL] * invV_L) * L;
+            off = 0;
+        } else {
+            const int iR = T.ecr[e];
+            const double invV_R = 1.0 / G.cvol[iR];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) D[q] = (-x[q * nc + iL] * invV_L + x[q * nc + iR] * invV_R) * L;
+            off = (c == iL) ? 0 : 16;
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const double *F = blk + (size_t)(off + kk) * ne + e;  // column kk: rows r at stride 4*ne
+            acc[kk] += (F[0] * D[0] + F[(size_t)4 * ne] * D[1] + F[(size_t)8 * ne] * D[2] +
+                        F[(size_t)12 * ne] * D[3]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) y[q * nc + c] = acc[q];
+}
+
+// Standalone HLLC on a batch of faces (known-answer surface).
+__global__ __launch_bounds__(256) void k_hllc_batch(const double *__restrict__ qL, const double *__restrict__ qR,
+                                                    const double *__restrict__ nrm, double gamma,
+                                                    double *__restrict__ F, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a[4], b[4], f[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        a[q] = qL[q * n + i];
+        b[q] = qR[q * n + i];
+    }
+    hllc(a, b, nrm[i], nrm[n + i], gamma, f);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) F[q * n + i] = f[q];
+}
+
+// ------------------------------------------------------------------------
+// Launch helpers
+// ------------------------------------------------------------------------
+static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_vertex_state(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_vertex_gather<4>, dim3(nblk(T.nv)), dim3(256), 0, s, T, G, U, vU, P);
+    return hipGetLastError();
+}
+hipError_t launch_vertex_visc(const Topo &T, const Geom &G, const double *tq, double *vtq, const Phys &P,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_vertex_gather<5>, dim3(nblk(T.nv)), dim3(256), 0, s, T, G, tq, vtq, P);
+    return hipGetLastError();
+}
+hipError_t launch_reconstruct(const Topo &T, const Geom &G, const double *U, const double *vU, double *fL,
+                              double *fR, double *tq, double *Gout, const Phys &P, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_reconstruct, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, U, vU, fL, fR, tq, Gout, P);
+    return hipGetLastError();
+}
+hipError_t launch_stress(const Topo &T, const double *U, const double *Gs, double *tq, const Phys &P,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_stress, dim3(nblk(T.nc)), dim3(256), 0, s, T, U, Gs, tq, P);
+    return hipGetLastError();
+}
+hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *fL,
+                              const double *fR, const double *vtq, double *R, const double *dtp, double c0,
+                              double c1, int flags, const Phys &P, hipStream_t s) {
+    hipLaunchKernelGGL(k_cell_flux_update, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, U, Uold, fL, fR, vtq, R,
+                       dtp, c0, c1, flags, P);
+    return hipGetLastError();
+}
+hipError_t launch_dt(const double *U, int nc, const Phys &P, double min_h, double *partials, unsigned *counter,
+                     double *scal, double *hist, hipStream_t s, int advance_t) {
+    const unsigned g = std::min<unsigned>(kRedGrid, nblk(nc, kRedBlock));
+    hipLaunchKernelGGL(k_dt, dim3(g), dim3(kRedBlock), 0, s, U, nc, P, min_h, partials, counter, scal, hist,
+                       advance_t);
+    return hipGetLastError();
+}
+hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
+                       double *partials, unsigned *counter, double *scal, int op, hipStream_t s) {
+    const unsigned g = (unsigned)std::min<long>(kRedGrid, (long)nblk(n, kRedBlock));
+    hipLaunchKernelGGL(k_dot2, dim3(g), dim3(kRedBlock), 0, s, a, b, c, d, n, partials, counter, scal, op);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n)), dim3(256), 0, s, n, r, p, v, scal);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n)), dim3(256), 0, s, n, r, v, sv, scal);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_xr(long n, double *x, double *r, const double *p, const double *sv, const double *t,
+                          const double *scal, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_xr, dim3(nblk(n)), dim3(256), 0, s, n, x, r, p, sv, t, scal);
+    return hipGetLastError();
+}
+hipError_t launch_sub(long n, const double *a, const double *b, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sub, dim3(nblk(n)), dim3(256), 0, s, n, a, b, out);
+    return hipGetLastError();
+}
+hipError_t launch_fill(long n, double *out, double v, hipStream_t s) {
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(256), 0, s, n, out, v);
+    return hipGetLastError();
+}
+hipError_t launch_fd_diff(long n, const double *Rp, const double *Rm, double eps, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_fd_diff, dim3(nblk(n)), dim3(256), 0, s, n, Rp, Rm, eps, out);
+    return hipGetLastError();
+}
+hipError_t launch_wall_forces(const Geom &G, const int *wall, int nw, const double *fL, int ne, double gamma,
+                              int skip_x, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_wall_forces, dim3(1), dim3(kRedBlock), 0, s, G, wall, nw, fL, ne, gamma, skip_x, out);
+    return hipGetLastError();
+}
+hipError_t launch_adjoint_rhs(const Topo &T, const Geom &G, const int *wall, int nw, const double *fL,
+                              double gamma, double q_inf, double beta, double *b, hipStream_t s) {
+    hipLaunchKernelGGL(k_adjoint_rhs, dim3(1), dim3(64), 0, s, T, G, wall, nw, fL, gamma, q_inf, beta, b);
+    return hipGetLastError();
+}
+hipError_t launch_jt_blocks(const Topo &T, const Geom &G, const double *fL, const double *fR, double eps,
+                            const Phys &P, double *blk, hipStream_t s) {
+    hipLaunchKernelGGL(k_jt_blocks, dim3(nblk(T.ne)), dim3(256), 0, s, T, G, fL, fR, eps, P, blk);
+    return hipGetLastError();
+}
+hipError_t launch_jt_apply(const Topo &T, const Geom &G, const double *blk, const double *x, double *y,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_jt_apply, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, blk, x, y);
+    return hipGetLastError();
+}
+hipError_t launch_hllc_batch(const double *qL, const double *qR, const double *nrm, double gamma, double *F,
+                             int n, hipStream_t s) {
+    hipLaunchKernelGGL(k_hllc_batch, dim3(nblk(n)), dim3(256), 0, s, qL, qR, nrm, gamma, F, n);
+    return hipGetLastError();
+}
+
+}  // namespace o2f

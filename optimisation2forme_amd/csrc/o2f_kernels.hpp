@@ -1,0 +1,105 @@
+// Device data views and kernel launchers of the gfx950 residual / adjoint
+// path (kernels in o2f_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "host_mesh.hpp"
+
+namespace o2f {
+
+constexpr int kRedBlock = 256;  // reduction block size
+constexpr int kRedGrid = 512;   // max blocks of a reduction (fixed => deterministic order)
+
+// Flags of k_cell_flux_update.
+enum : int {
+    kFlagAdv = 1,       // convective HLLC sum (else start from the stored R)
+    kFlagVisc = 2,      // add the viscous divergence
+    kFlagStoreR = 4,    // write R
+    kFlagRK = 8,        // stage update U = c0 Uold + c1 (U + dt R)
+    kFlagSaveOld = 16,  // first stage: Uold = U before the update
+};
+
+// Slots of the device scalar block.
+enum : int {
+    kScalDt = 0,
+    kScalT = 1,
+    kScalRho = 2,
+    kScalRhoOld = 3,
+    kScalAlpha = 4,
+    kScalW = 5,
+    kScalRR = 6,
+    kScalR0Sq = 7,
+    kScalDotA = 8,
+    kScalDotB = 9,
+    kScalCount = 16,
+};
+
+// Epilogues of k_dot2.
+enum : int { kOpStore = 0, kOpRho = 1, kOpRestart = 2, kOpAlpha = 3, kOpOmega = 4 };
+
+// Constants every kernel needs (host-computed with the reference's
+// expression order, see Context::make_phys).
+struct Phys {
+    double gamma, mu, kappa, cfl;
+    double fs_rho, fs_ru, fs_rv, fs_E;  // apply_inflow_outflow_bc freestream state
+    double venkat_k;
+    int limiter;
+};
+
+// Read-only topology (device pointers).
+struct Topo {
+    int nv, ne, nc;
+    const int *cedge, *cnbr;
+    const uint8_t *cinfo;
+    const int *eva, *evb, *ecl, *ecr;
+    const char *ebc;
+    const int *sptr, *scell;
+    const uint8_t *skind;
+    const char *vbc;
+};
+
+// Geometry (device pointers); several sets exist for shape perturbations.
+struct Geom {
+    const double *cvol, *ccx, *ccy;
+    const double *elen, *enx, *eny, *emx, *emy;
+    const double *sw;
+};
+
+hipError_t launch_vertex_state(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
+                               hipStream_t s);
+hipError_t launch_vertex_visc(const Topo &T, const Geom &G, const double *tq, double *vtq, const Phys &P,
+                              hipStream_t s);
+hipError_t launch_reconstruct(const Topo &T, const Geom &G, const double *U, const double *vU, double *fL,
+                              double *fR, double *tq, double *Gout, const Phys &P, hipStream_t s);
+hipError_t launch_stress(const Topo &T, const double *U, const double *Gs, double *tq, const Phys &P,
+                         hipStream_t s);
+hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *fL,
+                              const double *fR, const double *vtq, double *R, const double *dtp, double c0,
+                              double c1, int flags, const Phys &P, hipStream_t s);
+hipError_t launch_dt(const double *U, int nc, const Phys &P, double min_h, double *partials, unsigned *counter,
+                     double *scal, double *hist, hipStream_t s, int advance_t = 1);
+hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
+                       double *partials, unsigned *counter, double *scal, int op, hipStream_t s);
+hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s);
+hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s);
+hipError_t launch_bicg_xr(long n, double *x, double *r, const double *p, const double *sv, const double *t,
+                          const double *scal, hipStream_t s);
+hipError_t launch_sub(long n, const double *a, const double *b, double *out, hipStream_t s);
+hipError_t launch_fill(long n, double *out, double v, hipStream_t s);
+hipError_t launch_fd_diff(long n, const double *Rp, const double *Rm, double eps, double *out, hipStream_t s);
+hipError_t launch_wall_forces(const Geom &G, const int *wall, int nw, const double *fL, int ne, double gamma,
+                              int skip_x, double *out, hipStream_t s);
+hipError_t launch_adjoint_rhs(const Topo &T, const Geom &G, const int *wall, int nw, const double *fL,
+                              double gamma, double q_inf, double beta, double *b, hipStream_t s);
+hipError_t launch_jt_blocks(const Topo &T, const Geom &G, const double *fL, const double *fR, double eps,
+                            const Phys &P, double *blk, hipStream_t s);
+hipError_t launch_jt_apply(const Topo &T, const Geom &G, const double *blk, const double *x, double *y,
+                           hipStream_t s);
+hipError_t launch_hllc_batch(const double *qL, const double *qR, const double *nrm, double gamma, double *F,
+                             int n, hipStream_t s);
+
+}  // namespace o2f

@@ -1,0 +1,47 @@
+"""Writes the data fixtures of this directory from the reference's shipped
+data (run in a container where the reference is mounted; the fixtures are
+committed, the GPU machines never read the reference).
+
+- naca_2D_shape_0.msh.gz      mesh_opti/naca_2D_shape_0.msh (25,274 quads)
+- naca_2D_flow_1.gz           mesh_opti/naca_2D_flow_1 (state produced on shape 0)
+- naca_2D_shape_params_0.txt  mesh_opti/naca_2D_shape_params_0.txt
+- trajectory.json             (m, p) of mesh_opti/naca_2D_shape_params_{0..30}.txt:
+                              the sign of each step is minus the sign of the
+                              reference's shape gradient (run_opti.sh:91-120)
+"""
+import gzip
+import json
+import os
+import re
+import shutil
+import sys
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def gz(src, dst):
+    with open(src, "rb") as f, gzip.GzipFile(dst, "wb", compresslevel=9, mtime=0) as g:
+        shutil.copyfileobj(f, g)
+
+
+def main():
+    mo = os.path.join(REF, "mesh_opti")
+    gz(os.path.join(mo, "naca_2D_shape_0.msh"), os.path.join(HERE, "naca_2D_shape_0.msh.gz"))
+    gz(os.path.join(mo, "naca_2D_flow_1"), os.path.join(HERE, "naca_2D_flow_1.gz"))
+    shutil.copy(os.path.join(mo, "naca_2D_shape_params_0.txt"), os.path.join(HERE, "naca_2D_shape_params_0.txt"))
+    traj = []
+    for k in range(31):
+        vals = {}
+        with open(os.path.join(mo, "naca_2D_shape_params_%d.txt" % k)) as f:
+            for line in f:
+                mm = re.match(r"\s*(\w+)\s*\*?=\s*([-+0-9.eE]+)", line)
+                if mm:
+                    vals[mm.group(1)] = float(mm.group(2))
+        traj.append({"m": vals["m"], "p": vals["p"], "t": vals.get("t"), "beta": vals.get("beta")})
+    with open(os.path.join(HERE, "trajectory.json"), "w") as f:
+        json.dump(traj, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

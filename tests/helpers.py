@@ -1,0 +1,113 @@
+"""Shared fixtures of the test suite: meshes, states and flow conditions."""
+import functools
+import gzip
+import os
+import tempfile
+
+import numpy as np
+
+from oracle import msh2
+from oracle import oracle as O
+from optimisation2forme_amd import mesher
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# Flow of the shipped optimisation run (config.txt:14-75): Mach 0.3,
+# mu = 1.8e-5, Pr = 0.71, CFL 0.5, other SimulationParams defaults.
+SHIPPED = dict(viscosity=1.8e-5, Pr=0.71, cfl=0.5, mach=0.3)
+
+
+def re_viscosity(mach, re=5000.0, rho=1.225, p=101325.0, gamma=1.4, chord=1.0):
+    """mu = rho_inf U_inf c / Re."""
+    return rho * mach * np.sqrt(gamma * p / rho) * chord / re
+
+
+def transonic(limiter=0, interp_mode=1, venkat_k=5.0):
+    """NACA 2412 transonic flow: Mach 0.85, Re 5000, Pr 0.72, CFL 0.5, K = 5."""
+    return dict(viscosity=re_viscosity(0.85), Pr=0.72, cfl=0.5, mach=0.85, venkat_k=venkat_k,
+                limiter=limiter, interp_mode=interp_mode)
+
+
+def subsonic_small(limiter=0, interp_mode=1):
+    """NACA 0012: Mach 0.6, Re 5000, Pr 0.72, CFL 0.5, K = 5."""
+    return dict(viscosity=re_viscosity(0.6), Pr=0.72, cfl=0.5, mach=0.6, venkat_k=5.0,
+                limiter=limiter, interp_mode=interp_mode)
+
+
+def oracle_params(d):
+    return O.params(**d)
+
+
+def capi_params(d):
+    from optimisation2forme_amd import capi
+    kw = dict(d)
+    kw["cfl_number"] = kw.pop("cfl")
+    kw["inflow_velocity"] = kw.pop("mach")
+    return capi.make_params(**kw)
+
+
+@functools.lru_cache(maxsize=None)
+def _gunzip(name):
+    src = os.path.join(GOLDEN, name)
+    fd, path = tempfile.mkstemp(suffix="_" + name.replace(".gz", ""))
+    with gzip.open(src, "rb") as f, os.fdopen(fd, "wb") as g:
+        g.write(f.read())
+    return path
+
+
+@functools.lru_cache(maxsize=None)
+def shipped_mesh():
+    """The reference's shipped quad mesh (25,274 cells)."""
+    return msh2.read_msh2(_gunzip("naca_2D_shape_0.msh.gz"))
+
+
+def shipped_mesh_path():
+    return _gunzip("naca_2D_shape_0.msh.gz")
+
+
+@functools.lru_cache(maxsize=None)
+def shipped_flow():
+    """State naca_2D_flow_1, produced by the reference on shape 0."""
+    return msh2.read_state(_gunzip("naca_2D_flow_1.gz"))
+
+
+def shipped_flow_path():
+    return _gunzip("naca_2D_flow_1.gz")
+
+
+@functools.lru_cache(maxsize=None)
+def tiny_mesh():
+    """160-triangle O-mesh (brute-force sizes)."""
+    return mesher.naca_mesh(m=0.02, p=0.4, t=0.12, n_around=20, n_radial=5, wall_spacing=0.05,
+                            radius=6.0)
+
+
+@functools.lru_cache(maxsize=None)
+def small_mesh():
+    return mesher.small_10k()
+
+
+@functools.lru_cache(maxsize=None)
+def big_mesh():
+    return mesher.transonic_200k()
+
+
+def developed_state(desc, flow, steps=20, seed=None):
+    """Freestream start advanced `steps` SSP-RK3 steps by the oracle: a state
+    with realistic limiter / HLLC branch mixes."""
+    m = O.Mesh(desc)
+    p = oracle_params(flow)
+    m.init_freestream(p)
+    m.run_steps(p, steps)
+    U = m.state()
+    if seed is not None:
+        rng = np.random.default_rng(seed)
+        U = U * (1.0 + 1e-3 * rng.standard_normal(U.shape))
+    return U
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    den = np.maximum(np.abs(b), 1e-300)
+    scale = np.max(np.abs(b)) if b.size else 1.0
+    return float(np.max(np.abs(a - b) / np.maximum(den, 1e-12 * scale))) if a.size else 0.0
