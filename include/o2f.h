@@ -88,6 +88,19 @@ typedef struct o2f_mesh_desc {
 int o2f_msh2_read(const char *path, o2f_mesh_desc *out);
 void o2f_msh2_free(o2f_mesh_desc *desc);
 
+/* Topology and geometry the library derives from a mesh description,
+ * exactly as o2f_ctx_create does (Mesh::Mesh, src/mesh.cpp:112-229), computed
+ * on the host only (no GPU needed).  Call once with every array NULL to get
+ * *n_edges, then again with arrays of that size; any array may be NULL.
+ *   edge_vtx[2*ne]    (min, max) vertex of each edge, edges in std::map order
+ *   edge_cells[2*ne]  left cell (first recorder), right cell or -1
+ *   edge_bc[ne]       '\0', 'i', 'o', 'f' or 'w'
+ *   edge_geom[5*ne]   length | n_x | n_y | mid_x | mid_y (n into the left cell)
+ *   cell_geom[3*nc]   area | centre_x | centre_y
+ *   cell_edges[4*nc]  the cell's edges in ascending id, -1 pad */
+int o2f_mesh_topology(const o2f_mesh_desc *mesh, int *n_edges, int *edge_vtx, int *edge_cells,
+                      char *edge_bc, double *edge_geom, double *cell_geom, int *cell_edges);
+
 /* ---- context -------------------------------------------------------- */
 
 /* Builds topology and geometry (Mesh::Mesh, src/mesh.cpp:112-248) on the
@@ -150,6 +163,13 @@ int o2f_run_steps(o2f_ctx *ctx, int nsteps, double *dts, double *t_inout);
 /* while (t < final_time) { ... } of src/simulation.cpp:75-124 without the
  * output / reload side branches; returns steps taken and final t. */
 int o2f_time_loop(o2f_ctx *ctx, double final_time, int max_steps, int *steps, double *t);
+/* The same loop, device resident: steps are launched in batches and a
+ * device-side flag stops them once t >= final_time, so the host synchronises
+ * once per batch, not once per step.  *t_inout carries t in and out;
+ * *t_last_start (may be NULL) is t at the start of the last step and *dt_last
+ * (may be NULL) its dt (the values output_step prints, src/simulation.cpp:115-116). */
+int o2f_run_until(o2f_ctx *ctx, double final_time, int max_steps, int *steps, double *t_inout,
+                  double *t_last_start, double *dt_last);
 /* sqrt(mean_c R_rho^2) / (rho_inf U_inf / c), c = 1: the density residual
  * norm of the last residual evaluation (convergence criterion, not in the
  * reference, which has none). */

@@ -40,6 +40,28 @@ def _compile(src):
     return out
 
 
+HOST = os.path.join(HERE, "host")
+DRIVER = os.path.join(HERE, "bin", "navier_stokes")
+
+
+def build_driver(verbose=False):
+    """The reference's `navier_stokes [config]` driver and operator headers
+    (host/include) over the C ABI, linked against libo2f.so."""
+    srcs = sorted(os.path.join(HOST, "src", f) for f in os.listdir(os.path.join(HOST, "src"))
+                  if f.endswith(".cpp"))
+    os.makedirs(os.path.dirname(DRIVER), exist_ok=True)
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           "-I" + os.path.join(HOST, "include")] + srcs + [
+           "-L" + HERE, "-lo2f", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
+
+
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -52,6 +74,7 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_driver(verbose)
     return LIB
 
 

@@ -31,6 +31,7 @@ EXPORTS = [
     "o2f_set_face_states", "o2f_hllc_flux_batch", "o2f_compute_dt", "o2f_rk3_step",
     "o2f_run_steps", "o2f_time_loop", "o2f_residual_norm", "o2f_lift_drag", "o2f_objective",
     "o2f_adjoint_rhs", "o2f_apply_jt", "o2f_bicgstab", "o2f_shape_gradient", "o2f_time_steps",
+    "o2f_mesh_topology", "o2f_run_until",
 ]
 
 
@@ -105,6 +106,8 @@ def lib():
                                              C.POINTER(C.c_char_p), C.c_int, D, C.c_double,
                                              C.c_double, D]),
             "o2f_time_steps": (C.c_int, [vp, C.c_int, D, D]),
+            "o2f_mesh_topology": (C.c_int, [C.POINTER(MeshDesc), I, I, I, C.c_char_p, D, D, I]),
+            "o2f_run_until": (C.c_int, [vp, C.c_double, C.c_int, I, D, D, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -132,6 +135,39 @@ def make_params(gamma=1.4, viscosity=1e-6, Pr=0.71, cp=1005.0, cfl_number=0.1, r
         interp_mode = INTERP_MODES[interp_mode]
     return Params(gamma, viscosity, Pr, cp, cfl_number, rho_ref, p_ref, inflow_velocity, venkat_k,
                   int(limiter), int(interp_mode), int(bool(bc_inflow_outflow_x)))
+
+
+def _mesh_desc(desc):
+    xy = np.ascontiguousarray(desc["xy"], dtype=np.float64)
+    cnv = np.ascontiguousarray(desc["cell_nv"], dtype=np.int32)
+    cv = np.ascontiguousarray(desc["cell_vtx"], dtype=np.int32)
+    bv = np.ascontiguousarray(desc["bnd_vtx"], dtype=np.int32)
+    bt = bytes(np.asarray(desc["bnd_tag"], dtype=np.uint8))
+    keep = (xy, cnv, cv, bv, bt)
+    return MeshDesc(xy.shape[0], _d(xy), cnv.shape[0], _i(cnv), _i(cv), bv.shape[0], _i(bv), bt), keep
+
+
+def mesh_topology(desc):
+    """o2f_mesh_topology: the library's host-side topology and geometry."""
+    L = lib()
+    md, keep = _mesh_desc(desc)
+    ne = C.c_int()
+    st = L.o2f_mesh_topology(C.byref(md), C.byref(ne), None, None, None, None, None, None)
+    if st != O2F_OK:
+        raise O2FError(st, L.o2f_global_error().decode())
+    n, nc = ne.value, md.n_cells
+    ev, ec = np.zeros(2 * n, np.int32), np.zeros(2 * n, np.int32)
+    bc = C.create_string_buffer(max(n, 1))
+    eg, cg = np.zeros(5 * n), np.zeros(3 * nc)
+    ce = np.zeros(4 * nc, np.int32)
+    st = L.o2f_mesh_topology(C.byref(md), C.byref(ne), _i(ev), _i(ec), bc, _d(eg), _d(cg), _i(ce))
+    if st != O2F_OK:
+        raise O2FError(st, L.o2f_global_error().decode())
+    eg, cg = eg.reshape(5, n), cg.reshape(3, nc)
+    return {"va": ev[0::2], "vb": ev[1::2], "cl": ec[0::2], "cr": ec[1::2],
+            "bc": np.frombuffer(bc.raw[:n], dtype=np.uint8).copy(), "len": eg[0], "nx": eg[1],
+            "ny": eg[2], "mx": eg[3], "my": eg[4], "vol": cg[0], "cx": cg[1], "cy": cg[2],
+            "edges": ce.reshape(nc, 4)}
 
 
 def read_msh2(path):
@@ -260,6 +296,12 @@ class Solver:
         tt = C.c_double(t)
         self._check(lib().o2f_run_steps(self.h, n, _d(dts), C.byref(tt)))
         return dts[:n], tt.value
+
+    def run_until(self, final_time, max_steps=-1, t=0.0):
+        n, tt, ts, dl = C.c_int(), C.c_double(t), C.c_double(), C.c_double()
+        self._check(lib().o2f_run_until(self.h, final_time, max_steps, C.byref(n), C.byref(tt),
+                                        C.byref(ts), C.byref(dl)))
+        return n.value, tt.value, ts.value, dl.value
 
     def time_loop(self, final_time, max_steps=-1):
         n, t = C.c_int(), C.c_double()

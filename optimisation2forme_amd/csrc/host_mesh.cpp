@@ -187,6 +187,13 @@ bool build_host_mesh(const o2f_mesh_desc &d, int interp_mode, HostMesh &hm, std:
         }
     }
     for (int c = 0; c < hm.nc; ++c) hm.cinfo[c] |= static_cast<uint8_t>(n_edges[c] << 4);
+    hm.cevx.assign(8 * static_cast<size_t>(hm.nc), -1);
+    for (int c = 0; c < hm.nc; ++c)
+        for (int k = 0; k < n_edges[c]; ++k) {
+            const int e = hm.cedge[4 * c + k];
+            hm.cevx[8 * c + 2 * k] = hm.eva[e];
+            hm.cevx[8 * c + 2 * k + 1] = hm.evb[e];
+        }
     // Vertex::cellIDsOrdered: cells by increasing atan2 angle around the
     // vertex (src/mesh.cpp:231-248); equal angles keep ascending cell id.
     hm.vcell_ang = hm.vcell_asc;

@@ -37,6 +37,7 @@ struct HostMesh {
     std::vector<int> cedge;       // 4 per cell, ascending edge id, -1 pad
     std::vector<int> cnbr;        // 4 per cell, distinct face neighbours, -1 pad
     std::vector<uint8_t> cinfo;   // bits 0-3: cell is left cell of cedge[k]; bits 4-6: edge count
+    std::vector<int> cevx;        // 8 per cell: (min, max) vertex of each edge slot, -1 pad
     // edges (std::map order of the sorted vertex pair)
     std::vector<int> eva, evb, ecl, ecr;
     std::vector<char> ebc;

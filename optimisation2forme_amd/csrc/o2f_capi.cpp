@@ -83,7 +83,8 @@ struct o2f_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     // topology
-    int *cedge = nullptr, *cnbr = nullptr, *eva = nullptr, *evb = nullptr, *ecl = nullptr, *ecr = nullptr;
+    int *cedge = nullptr, *cnbr = nullptr, *cevx = nullptr, *eva = nullptr, *evb = nullptr, *ecl = nullptr,
+        *ecr = nullptr;
     int *sptr = nullptr, *scell = nullptr, *wall = nullptr;
     uint8_t *cinfo = nullptr, *skind = nullptr;
     char *ebc = nullptr, *vbc = nullptr;
@@ -92,7 +93,8 @@ struct o2f_ctx {
     double *U = nullptr, *Uold = nullptr, *R = nullptr, *Gr = nullptr, *tq = nullptr;
     double *vU = nullptr, *vtq = nullptr, *fL = nullptr, *fR = nullptr;
     double *scal = nullptr, *partials = nullptr, *wall_out = nullptr;
-    unsigned *counter = nullptr;
+    unsigned long long *speed = nullptr;  // max_c(|u|+c) word of the next step's dt
+    bool speed_valid = false;             // it holds the speed of the current U
     bool have_grad = false;
     // adjoint
     double *blk = nullptr;
@@ -106,7 +108,7 @@ struct o2f_ctx {
         return code;
     }
     Topo topo() const {
-        return Topo{hm.nv, hm.ne, hm.nc, cedge, cnbr, cinfo, eva, evb, ecl, ecr, ebc, sptr, scell, skind, vbc};
+        return Topo{hm.nv, hm.ne, hm.nc, cedge, cnbr, cevx, cinfo, eva, evb, ecl, ecr, ebc, sptr, scell, skind, vbc};
     }
 };
 
@@ -171,7 +173,7 @@ int run_stage(o2f_ctx *c, int g, int flags, const double *dtp, double c0, double
     HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, store_grad ? c->Gr : nullptr, c->phys, s));
     HIPCHK(c, launch_vertex_visc(T, G, c->tq, c->vtq, c->phys, s));
     HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, dtp, c0, c1,
-                                 flags | kFlagAdv | kFlagVisc, c->phys, s));
+                                 flags | kFlagAdv | kFlagVisc, c->phys, s, c->speed));
     ++c->faces_version;
     if (store_grad) c->have_grad = true;
     return O2F_OK;
@@ -192,12 +194,28 @@ int copy_out(o2f_ctx *c, double *host, const double *dev, size_t n) {
 
 // One SSP-RK3 step (src/simulation.cpp:130-142).  The dt in scal[kScalDt]
 // is read by the stage kernels on the device.
-int rk3_stages(o2f_ctx *c, bool last_outputs) {
+int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed) {
     const double *dtp = c->scal + kScalDt;
     int st;
     if ((st = run_stage(c, 0, kFlagRK | kFlagSaveOld, dtp, 0.0, 1.0, false))) return st;
     if ((st = run_stage(c, 0, kFlagRK, dtp, 3.0 / 4.0, 1.0 / 4.0, false))) return st;
-    return run_stage(c, 0, kFlagRK | (last_outputs ? kFlagStoreR : 0), dtp, 1.0 / 3.0, 2.0 / 3.0, last_outputs);
+    return run_stage(c, 0, kFlagRK | (last_outputs ? kFlagStoreR : 0) | (next_speed ? kFlagSpeed : 0), dtp,
+                     1.0 / 3.0, 2.0 / 3.0, last_outputs);
+}
+
+// One step of the time loop body: dt from the speed word, the three stages,
+// and (fused into the last stage) the speed of the new state for the next
+// step's dt.
+int one_step(o2f_ctx *c, bool last_outputs, double *hist) {
+    hipStream_t s = c->stream;
+    if (!c->speed_valid) {
+        HIPCHK(c, launch_speed_reset(c->speed, s));
+        HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
+    }
+    HIPCHK(c, launch_step_begin(c->speed, c->phys, c->hm.min_h, c->scal, hist, 1, s));
+    int st = rk3_stages(c, last_outputs, true);
+    c->speed_valid = (st == O2F_OK);
+    return st;
 }
 
 std::string fmt_double(double v) {
@@ -272,6 +290,50 @@ void o2f_msh2_free(o2f_mesh_desc *d) {
     std::memset(d, 0, sizeof(*d));
 }
 
+int o2f_mesh_topology(const o2f_mesh_desc *mesh, int *n_edges, int *edge_vtx, int *edge_cells, char *edge_bc,
+                      double *edge_geom, double *cell_geom, int *cell_edges) {
+    if (!mesh) {
+        g_error = "o2f_mesh_topology: null mesh";
+        return O2F_ERR_ARG;
+    }
+    HostMesh hm;
+    std::string err;
+    if (!build_host_mesh(*mesh, O2F_INTERP_REF, hm, err)) {
+        g_error = err;
+        return O2F_ERR_ARG;
+    }
+    const int ne = hm.ne, nc = hm.nc;
+    if (n_edges) *n_edges = ne;
+    for (int e = 0; e < ne; ++e) {
+        if (edge_vtx) {
+            edge_vtx[2 * e] = hm.eva[e];
+            edge_vtx[2 * e + 1] = hm.evb[e];
+        }
+        if (edge_cells) {
+            edge_cells[2 * e] = hm.ecl[e];
+            edge_cells[2 * e + 1] = hm.ecr[e];
+        }
+        if (edge_bc) edge_bc[e] = hm.ebc[e];
+        if (edge_geom) {
+            edge_geom[e] = hm.elen[e];
+            edge_geom[ne + e] = hm.enx[e];
+            edge_geom[2 * ne + e] = hm.eny[e];
+            edge_geom[3 * ne + e] = hm.emx[e];
+            edge_geom[4 * ne + e] = hm.emy[e];
+        }
+    }
+    for (int c = 0; c < nc; ++c) {
+        if (cell_geom) {
+            cell_geom[c] = hm.cvol[c];
+            cell_geom[nc + c] = hm.ccx[c];
+            cell_geom[2 * nc + c] = hm.ccy[c];
+        }
+        if (cell_edges)
+            for (int k = 0; k < 4; ++k) cell_edges[4 * c + k] = hm.cedge[4 * c + k];
+    }
+    return O2F_OK;
+}
+
 int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx **out) {
     if (!mesh || !params || !out) {
         g_error = "o2f_ctx_create: null argument";
@@ -297,6 +359,7 @@ int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx 
     std::vector<int> sptr(hm.vptr.begin(), hm.vptr.end());
     if (!e) e = upload(c->cedge, hm.cedge);
     if (!e) e = upload(c->cnbr, hm.cnbr);
+    if (!e) e = upload(c->cevx, hm.cevx);
     if (!e) e = upload(c->cinfo, hm.cinfo);
     if (!e) e = upload(c->eva, hm.eva);
     if (!e) e = upload(c->evb, hm.evb);
@@ -322,8 +385,7 @@ int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx 
     if (!e) e = dalloc(c->scal, kScalCount);
     if (!e) e = dalloc(c->partials, 2 * kRedGrid);
     if (!e) e = dalloc(c->wall_out, 2);
-    if (!e) e = dalloc(c->counter, 1);
-    if (!e) e = hipMemset(c->counter, 0, sizeof(unsigned));
+    if (!e) e = dalloc(c->speed, 1);
     if (!e) e = hipMemset(c->scal, 0, kScalCount * sizeof(double));
     if (!e) e = hipMemset(c->U, 0, 4 * nc * sizeof(double));
     if (!e) e = hipMemset(c->fL, 0, 4 * ne * sizeof(double));
@@ -343,7 +405,7 @@ void o2f_ctx_destroy(o2f_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->cinfo,
                     c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
-                    c->scal, c->partials, c->wall_out, c->counter, c->blk, c->Rp, c->Rm};
+                    c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c->kv)
@@ -380,6 +442,7 @@ int o2f_set_state(o2f_ctx *c, const double *U) {
     if (!c || !U) return O2F_ERR_ARG;
     HIPCHK(c, hipMemcpyAsync(c->U, U, 4 * (size_t)c->hm.nc * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->speed_valid = false;
     return O2F_OK;
 }
 
@@ -399,6 +462,7 @@ int o2f_init_freestream(o2f_ctx *c) {
     const long nc = c->hm.nc;
     for (int q = 0; q < 4; ++q) HIPCHK(c, launch_fill(nc, c->U + q * nc, vals[q], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->speed_valid = false;
     return O2F_OK;
 }
 
@@ -543,10 +607,12 @@ int o2f_hllc_flux_batch(const double *qL, const double *qR, const double *normal
 
 int o2f_compute_dt(o2f_ctx *c, double *dt) {
     if (!c || !dt) return O2F_ERR_ARG;
-    HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr,
-                        c->stream, 0));
+    HIPCHK(c, launch_speed_reset(c->speed, c->stream));
+    HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, c->stream));
+    HIPCHK(c, launch_step_begin(c->speed, c->phys, c->hm.min_h, c->scal, nullptr, 0, c->stream));
     HIPCHK(c, hipMemcpyAsync(dt, c->scal + kScalDt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->speed_valid = false;
     return O2F_OK;
 }
 
@@ -555,7 +621,8 @@ int o2f_rk3_step(o2f_ctx *c, double dt) {
     int st = check_bc(c);
     if (st) return st;
     HIPCHK(c, hipMemcpyAsync(c->scal + kScalDt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    if ((st = rk3_stages(c, true))) return st;
+    c->speed_valid = false;
+    if ((st = rk3_stages(c, true, false))) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return O2F_OK;
 }
@@ -568,11 +635,8 @@ int o2f_run_steps(o2f_ctx *c, int nsteps, double *dts, double *t_inout) {
     HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
     double *hist = nullptr;
     if (dts && nsteps > 0) HIPCHK(c, hipMalloc(reinterpret_cast<void **>(&hist), nsteps * sizeof(double)));
-    for (int s = 0; s < nsteps; ++s) {
-        HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal,
-                            hist ? hist + s : nullptr, c->stream));
-        if ((st = rk3_stages(c, s == nsteps - 1))) return st;
-    }
+    for (int s = 0; s < nsteps; ++s)
+        if ((st = one_step(c, s == nsteps - 1, hist ? hist + s : nullptr))) return st;
     if (hist) {
         HIPCHK(c, hipMemcpyAsync(dts, hist, nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -585,31 +649,59 @@ int o2f_run_steps(o2f_ctx *c, int nsteps, double *dts, double *t_inout) {
     return O2F_OK;
 }
 
-int o2f_time_loop(o2f_ctx *c, double final_time, int max_steps, int *steps, double *t) {
+int o2f_run_until(o2f_ctx *c, double final_time, int max_steps, int *steps, double *t_inout, double *t_last_start,
+                  double *dt_last) {
     if (!c) return O2F_ERR_ARG;
     int st = check_bc(c);
     if (st) return st;
-    // while (t < final_time): the step count is unknown in advance, so t is
-    // read back after each step (the reference checks it on the host too).
-    double tt = 0.0;
-    int n = 0;
-    HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &tt, sizeof(double), hipMemcpyHostToDevice, c->stream));
-    while (tt < final_time && (max_steps < 0 || n < max_steps)) {
-        HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr, c->stream));
-        if ((st = rk3_stages(c, true))) return st;
-        HIPCHK(c, hipMemcpyAsync(&tt, c->scal + kScalT, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
-        ++n;
+    hipStream_t s = c->stream;
+    const double t0 = t_inout ? *t_inout : 0.0;
+    const double init[4] = {1.0, 0.0, t0, 0.0};  // active, steps, t_prev, (unused)
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &t0, sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalActive, init, 3 * sizeof(double), hipMemcpyHostToDevice, s));
+    const Phys saved = c->phys;
+    c->phys.gate = c->scal + kScalActive;
+    c->phys.final_t = final_time;
+    double h[kScalCount];
+    int launched = 0;
+    for (;;) {
+        int batch = 64;
+        if (max_steps >= 0) batch = std::min(batch, max_steps - launched);
+        if (batch <= 0) break;
+        for (int k = 0; k < batch && st == 0; ++k) st = one_step(c, true, nullptr);
+        launched += batch;
+        if (st) break;
+        hipError_t e = hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            st = c->fail(O2F_ERR_HIP, std::string("o2f_run_until: ") + hipGetErrorString(e));
+            break;
+        }
+        if (h[kScalActive] == 0.0) break;
     }
-    if (steps) *steps = n;
-    if (t) *t = tt;
+    c->phys = saved;
+    c->speed_valid = false;  // gated steps may have skipped the speed fold
+    if (st) return st;
+    HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    if (steps) *steps = static_cast<int>(h[kScalSteps]);
+    if (t_inout) *t_inout = h[kScalT];
+    if (t_last_start) *t_last_start = h[kScalTPrev];
+    if (dt_last) *dt_last = h[kScalDt];
     return O2F_OK;
+}
+
+int o2f_time_loop(o2f_ctx *c, double final_time, int max_steps, int *steps, double *t) {
+    double tt = 0.0;
+    int st = o2f_run_until(c, final_time, max_steps, steps, &tt, nullptr, nullptr);
+    if (st == O2F_OK && t) *t = tt;
+    return st;
 }
 
 int o2f_residual_norm(o2f_ctx *c, double *rnorm) {
     if (!c || !rnorm) return O2F_ERR_ARG;
     const long nc = c->hm.nc;
-    HIPCHK(c, launch_dot2(c->R, c->R, nullptr, nullptr, nc, c->partials, c->counter, c->scal, kOpStore, c->stream));
+    HIPCHK(c, launch_dot2(c->R, c->R, nullptr, nullptr, nc, c->partials, c->scal, kOpStore, c->stream));
     double s = 0.0;
     HIPCHK(c, hipMemcpyAsync(&s, c->scal + kScalDotA, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -733,7 +825,7 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     HIPCHK(c, launch_jt_apply(T, G, c->blk, x, tmp, s));
     HIPCHK(c, launch_sub(n, b, tmp, r, s));
     HIPCHK(c, launch_sub(n, r, nullptr, r0, s));
-    HIPCHK(c, launch_dot2(r0, r0, b, b, n, c->partials, c->counter, c->scal, kOpStore, s));
+    HIPCHK(c, launch_dot2(r0, r0, b, b, n, c->partials, c->scal, kOpStore, s));
     if ((st = read_scal(c, h))) return st;
     const double r0_sq = h.v[kScalDotA], rhs_sq = h.v[kScalDotB];
     if (rhs_sq == 0) {
@@ -751,7 +843,7 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     const double eps2 = DBL_EPSILON * DBL_EPSILON;
     int i = 0, restarts = 0;
     // rho_old = rho; rho = r0.r, together with r.r for the loop test
-    HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->counter, c->scal, kOpRho, s));
+    HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s));
     if ((st = read_scal(c, h))) return st;
     while (h.v[kScalRR] > tol2 && i < maxit) {
         if (std::fabs(h.v[kScalRho]) < eps2 * r0_sqnorm) {
@@ -759,20 +851,20 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
             HIPCHK(c, launch_jt_apply(T, G, c->blk, x, tmp, s));
             HIPCHK(c, launch_sub(n, b, tmp, r, s));
             HIPCHK(c, launch_sub(n, r, nullptr, r0, s));
-            HIPCHK(c, launch_dot2(r, r, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpRestart, s));
+            HIPCHK(c, launch_dot2(r, r, nullptr, nullptr, n, c->partials, c->scal, kOpRestart, s));
             if ((st = read_scal(c, h))) return st;
             r0_sqnorm = h.v[kScalR0Sq];
             if (restarts++ == 0) i = 0;
         }
         HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
         HIPCHK(c, launch_jt_apply(T, G, c->blk, p, v, s));
-        HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpAlpha, s));
+        HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->scal, kOpAlpha, s));
         HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, s));
         HIPCHK(c, launch_jt_apply(T, G, c->blk, sv, t, s));
-        HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->counter, c->scal, kOpOmega, s));
+        HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->scal, kOpOmega, s));
         HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s));
         ++i;
-        HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->counter, c->scal, kOpRho, s));
+        HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s));
         if ((st = read_scal(c, h))) return st;
     }
     *iters = i;
@@ -790,7 +882,7 @@ int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int
     if ((st = rhs_device(c, beta, b))) return st;
     HIPCHK(c, launch_fill(n, x, 0.0, c->stream));
     HostScal h;
-    HIPCHK(c, launch_dot2(b, b, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpStore, c->stream));
+    HIPCHK(c, launch_dot2(b, b, nullptr, nullptr, n, c->partials, c->scal, kOpStore, c->stream));
     if ((st = read_scal(c, h))) return st;
     const double bnorm = std::sqrt(h.v[kScalDotA]);
     double rr = INFINITY;
@@ -802,7 +894,7 @@ int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int
         // true residual b - J^T x (src/shape_optimization.cpp:715-717)
         HIPCHK(c, launch_jt_apply(c->topo(), c->geo[0].view(), c->blk, x, tmp, c->stream));
         HIPCHK(c, launch_sub(n, b, tmp, tmp, c->stream));
-        HIPCHK(c, launch_dot2(tmp, tmp, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpStore, c->stream));
+        HIPCHK(c, launch_dot2(tmp, tmp, nullptr, nullptr, n, c->partials, c->scal, kOpStore, c->stream));
         if ((st = read_scal(c, h))) return st;
         rr = std::sqrt(h.v[kScalDotA]) / (bnorm > 0 ? bnorm : 1.0);
         if (rr <= tol) break;
@@ -861,7 +953,7 @@ int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *v
             if ((st = objective_on(c, 1 + side, beta, &J[side]))) return st;
         }
         HIPCHK(c, launch_fd_diff(n, c->Rp, c->Rm, eps, dR, c->stream));
-        HIPCHK(c, launch_dot2(lam, dR, nullptr, nullptr, n, c->partials, c->counter, c->scal, kOpStore, c->stream));
+        HIPCHK(c, launch_dot2(lam, dR, nullptr, nullptr, n, c->partials, c->scal, kOpStore, c->stream));
         HostScal h;
         if ((st = read_scal(c, h))) return st;
         const double dJ_direct = (J[0] - J[1]) / (2.0 * eps);
@@ -883,10 +975,8 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipStreamSynchronize(s));
     HIPCHK(c, hipEventRecord(e0, s));
-    for (int k = 0; k < nsteps; ++k) {
-        HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr, s));
-        if ((st = rk3_stages(c, k == nsteps - 1))) return st;
-    }
+    for (int k = 0; k < nsteps; ++k)
+        if ((st = one_step(c, k == nsteps - 1, nullptr))) return st;
     HIPCHK(c, hipEventRecord(e1, s));
     HIPCHK(c, hipEventSynchronize(e1));
     float ms = 0.f;
@@ -903,9 +993,15 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
         const double cs[3][2] = {{0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0}};
         int q = 0;
         for (int k = 0; k < nsteps; ++k) {
-            HIPCHK(c, launch_dt(c->U, c->hm.nc, c->phys, c->hm.min_h, c->partials, c->counter, c->scal, nullptr, s));
+            if (!c->speed_valid) {
+                HIPCHK(c, launch_speed_reset(c->speed, s));
+                HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
+                c->speed_valid = true;
+            }
+            HIPCHK(c, launch_step_begin(c->speed, c->phys, c->hm.min_h, c->scal, nullptr, 1, s));
             for (int stg = 0; stg < 3; ++stg) {
-                const int fl = kFlagAdv | kFlagVisc | kFlagRK | (stg == 0 ? kFlagSaveOld : 0);
+                const int fl = kFlagAdv | kFlagVisc | kFlagRK | (stg == 0 ? kFlagSaveOld : 0) |
+                               (stg == 2 ? kFlagSpeed : 0);
                 HIPCHK(c, hipEventRecord(ev[q++], s));
                 HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, s));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
@@ -917,7 +1013,7 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
                 HIPCHK(c, hipEventRecord(ev[q++], s));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
                 HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, dtp, cs[stg][0],
-                                             cs[stg][1], fl, c->phys, s));
+                                             cs[stg][1], fl, c->phys, s, c->speed));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
             }
         }

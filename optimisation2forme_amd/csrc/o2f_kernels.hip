@@ -121,34 +121,54 @@ __device__ __forceinline__ void hllc(const double *qL, const double *qR, double 
 // viscous quantities).  NV = 4 state variables with the state BC, or NV = 5
 // viscous quantities with apply_diffusion_bc (src/boundary.cpp:101-127).
 // ------------------------------------------------------------------------
+constexpr int kStencilSlots = 8;  // stencil entries loaded together (longer ones loop)
+
 template <int NV>
 __global__ __launch_bounds__(256) void k_vertex_gather(Topo T, Geom G, const double *__restrict__ src,
                                                        double *__restrict__ dst, Phys P) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= T.nv) return;
+    if (v >= T.nv || (P.gate && *P.gate == 0.0)) return;
     const int nc = T.nc;
     const uint8_t kind = T.skind[v];
-    const int b = T.sptr[v], e = T.sptr[v + 1];
+    const int b = T.sptr[v], n = T.sptr[v + 1] - b;
+    // All stencil ids and weights first, then all values: two dependent
+    // load levels instead of one per stencil entry.
+    int cid[kStencilSlots];
+    double w[kStencilSlots], val[kStencilSlots][NV];
+#pragma unroll
+    for (int k = 0; k < kStencilSlots; ++k) {
+        cid[k] = (k < n) ? T.scell[b + k] : 0;
+        w[k] = (k < n) ? G.sw[b + k] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kStencilSlots; ++k)
+        if (k < n) {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) val[k][q] = src[q * nc + cid[k]];
+        }
     double out[NV];
     if (kind == kStencilWsum) {
-        int c = T.scell[b];
-        double w = G.sw[b];
+        // w0*a0 + w1*a1 + ... left to right (src/advection.cpp:47-50, :101-104)
 #pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = w * src[q * nc + c];
-        for (int k = b + 1; k < e; ++k) {
-            c = T.scell[k];
-            w = G.sw[k];
+        for (int q = 0; q < NV; ++q) out[q] = w[0] * val[0][q];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) out[q] = out[q] + w * src[q * nc + c];
+        for (int k = 1; k < kStencilSlots; ++k)
+            if (k < n) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) out[q] = out[q] + w[k] * val[k][q];
+            }
+        for (int k = kStencilSlots; k < n; ++k) {
+            const int c = T.scell[b + k];
+            const double wk = G.sw[b + k];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] = out[q] + wk * src[q * nc + c];
         }
     } else if (kind == kStencilAvg2) {
-        const int c0 = T.scell[b], c1 = T.scell[b + 1];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = 0.5 * (src[q * nc + c0] + src[q * nc + c1]);
+        for (int q = 0; q < NV; ++q) out[q] = 0.5 * (val[0][q] + val[1][q]);
     } else if (kind == kStencilCopy) {
-        const int c0 = T.scell[b];
 #pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = src[q * nc + c0];
+        for (int q = 0; q < NV; ++q) out[q] = val[0][q];
     } else {
 #pragma unroll
         for (int q = 0; q < NV; ++q) out[q] = 0.0;
@@ -200,37 +220,73 @@ __global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const 
                                                           double *__restrict__ tq, double *__restrict__ Gout,
                                                           Phys P) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= T.nc) return;
+    if (c >= T.nc || (P.gate && *P.gate == 0.0)) return;
     const int nc = T.nc, nv = T.nv, ne = T.ne;
+    // level 1: the cell's own rows
     const unsigned info = T.cinfo[c];
     const int nk = info >> 4;
+    const int4 ed4 = reinterpret_cast<const int4 *>(T.cedge)[c];
+    const int4 nb4 = reinterpret_cast<const int4 *>(T.cnbr)[c];
+    const int4 ea4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c];
+    const int4 eb4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c + 1];
+    const int ed[4] = {ed4.x, ed4.y, ed4.z, ed4.w};
+    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+    const int va[4] = {ea4.x, ea4.z, eb4.x, eb4.z}, vb[4] = {ea4.y, ea4.w, eb4.y, eb4.w};
     double u[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) u[q] = U[q * nc + c];
-    const double cv = G.cvol[c], cx = G.ccx[c], cy = G.ccy[c];
-    int ed[4];
+    const double vol = G.cvol[c], cx = G.ccx[c], cy = G.ccy[c];
+    // level 2: edge end-point values, edge geometry, neighbour states, all at once
+    double xa[4][4], xb[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < nk) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                xa[k][q] = vU[q * nv + va[k]];
+                xb[k][q] = vU[q * nv + vb[k]];
+            }
+        }
+    double el[4], enx[4], eny[4], dxs[4], dys[4];
+    char ebc[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < nk) {
+            const int e = ed[k];
+            el[k] = G.elen[e];
+            enx[k] = G.enx[e];
+            eny[k] = G.eny[e];
+            dxs[k] = G.emx[e] - cx;
+            dys[k] = G.emy[e] - cy;
+            ebc[k] = T.ebc[e];
+        }
+    double un[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (nb[k] >= 0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) un[k][q] = U[q * nc + nb[k]];
+        }
+    // edge values (+ edge BC) and the Green-Gauss sum in ascending edge id
     double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < nk; ++k) {
-        const int e = T.cedge[4 * c + k];
-        ed[k] = e;
-        const int a = T.eva[e], b = T.evb[e];
-        double val[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) val[q] = 0.5 * (vU[q * nv + a] + vU[q * nv + b]);
-        const char bc = T.ebc[e];
-        if (tagged(bc)) bc_state(P, bc, val[0], val[1], val[2], val[3]);
-        const double el = G.elen[e];
-        double enx = G.enx[e], eny = G.eny[e];
-        if ((info >> k) & 1u) {
-            enx = -enx;
-            eny = -eny;
-        }
+    for (int k = 0; k < 4; ++k)
+        if (k < nk) {
+            double val[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            gx[q] += val[q] * el * enx / cv;
-            gy[q] += val[q] * el * eny / cv;
+            for (int q = 0; q < 4; ++q) val[q] = 0.5 * (xa[k][q] + xb[k][q]);
+            if (tagged(ebc[k])) bc_state(P, ebc[k], val[0], val[1], val[2], val[3]);
+            double nx = enx[k], ny = eny[k];
+            if ((info >> k) & 1u) {
+                nx = -nx;
+                ny = -ny;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                gx[q] += val[q] * el[k] * nx / vol;
+                gy[q] += val[q] * el[k] * ny / vol;
+            }
         }
-    }
     if (Gout) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -241,64 +297,61 @@ __global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const 
     double mn[4], mx[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) mn[q] = mx[q] = u[q];
-    for (int k = 0; k < 4; ++k) {
-        const int nb = T.cnbr[4 * c + k];
-        if (nb < 0) break;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double v = U[q * nc + nb];
-            if (v < mn[q]) mn[q] = v;
-            if (v > mx[q]) mx[q] = v;
-        }
-    }
-    double al[4] = {1.0, 1.0, 1.0, 1.0};
-    double dxs[4], dys[4];
-    for (int k = 0; k < nk; ++k) {
-        const int e = ed[k];
-        const double dx = G.emx[e] - cx, dy = G.emy[e] - cy;
-        dxs[k] = dx;
-        dys[k] = dy;
+    for (int k = 0; k < 4; ++k)
+        if (nb[k] >= 0) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double pred = u[q] + gx[q] * dx + gy[q] * dy;
-            const double a = (P.limiter == 1) ? alpha_bj(u[q], pred, mn[q], mx[q])
-                                              : alpha_venkat(u[q], pred, mn[q], mx[q], cv, P.venkat_k);
-            al[q] = smin(al[q], a);
-        }
-    }
-    for (int k = 0; k < nk; ++k) {
-        const int e = ed[k];
-        double f[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) f[q] = u[q] + al[q] * (gx[q] * dxs[k] + gy[q] * dys[k]);
-        if ((info >> k) & 1u) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) fL[q * ne + e] = f[q];
-            const char bc = T.ebc[e];
-            if (tagged(bc)) {
-                double g[4];
-                bc_ghost(P, bc, f, g);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) fR[q * ne + e] = g[q];
+            for (int q = 0; q < 4; ++q) {
+                if (un[k][q] < mn[q]) mn[q] = un[k][q];
+                if (un[k][q] > mx[q]) mx[q] = un[k][q];
             }
-        } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) fR[q * ne + e] = f[q];
         }
-    }
+    double al[4] = {1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < nk) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double pred = u[q] + gx[q] * dxs[k] + gy[q] * dys[k];
+                const double a = (P.limiter == 1) ? alpha_bj(u[q], pred, mn[q], mx[q])
+                                                  : alpha_venkat(u[q], pred, mn[q], mx[q], vol, P.venkat_k);
+                al[q] = smin(al[q], a);
+            }
+        }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (k < nk) {
+            const int e = ed[k];
+            double f[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) f[q] = u[q] + al[q] * (gx[q] * dxs[k] + gy[q] * dys[k]);
+            if ((info >> k) & 1u) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) fL[q * ne + e] = f[q];
+                if (tagged(ebc[k])) {
+                    double g[4];
+                    bc_ghost(P, ebc[k], f, g);
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) fR[q * ne + e] = g[q];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) fR[q * ne + e] = f[q];
+            }
+        }
     // viscous stress and heat flux of the cell (src/diffusion.cpp:227-240)
     const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
     const double rho = u[0], E = u[3];
-    const double uu = u[1] / rho, vv = u[2] / rho;
+    const double uu = u[1] / rho, vvel = u[2] / rho;
     const double u_x = (gx[1] - uu * gx[0]) / rho;
-    const double v_y = (gy[2] - vv * gy[0]) / rho;
+    const double v_y = (gy[2] - vvel * gy[0]) / rho;
     const double u_y = (gy[1] - uu * gy[0]) / rho;
-    const double v_x = (gx[2] - vv * gx[0]) / rho;
+    const double v_x = (gx[2] - vvel * gx[0]) / rho;
     tq[c] = 2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y);
     tq[nc + c] = 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y);
     tq[2 * nc + c] = mu * (u_y + v_x);
-    const double T_x = (gamma - 1.0) * ((gx[3] * rho - E * gx[0]) / (rho * rho) - (uu * u_x + vv * v_x));
-    const double T_y = (gamma - 1.0) * ((gy[3] * rho - E * gy[0]) / (rho * rho) - (uu * u_y + vv * v_y));
+    const double T_x = (gamma - 1.0) * ((gx[3] * rho - E * gx[0]) / (rho * rho) - (uu * u_x + vvel * v_x));
+    const double T_y = (gamma - 1.0) * ((gy[3] * rho - E * gy[0]) / (rho * rho) - (uu * u_y + vvel * v_y));
     tq[3 * nc + c] = -kappa * T_x;
     tq[4 * nc + c] = -kappa * T_y;
 }
@@ -337,6 +390,31 @@ __global__ __launch_bounds__(256) void k_cell_stress(Topo T, const double *__res
 // evaluated by both of its cells (bit-identical), trading 2x flux flops for
 // no flux array and no scatter.
 // ------------------------------------------------------------------------
+// |u| + c of one state (src/simulation.cpp:84-95)
+__device__ __forceinline__ double wave_speed(double rho, double ru, double rv, double E, double gamma) {
+    const double u = ru / rho, v = rv / rho;
+    const double kinetic = 0.5 * (ru * ru + rv * rv) / rho;
+    const double p = (gamma - 1.0) * (E - kinetic);
+    const double cs = sqrt(gamma * p / rho);
+    return hypot(u, v) + cs;
+}
+
+// Block max of m (NaN ignored like std::max) folded into the speed word.
+// Positive doubles order like their bit patterns, so an integer atomic max is
+// exact and independent of arrival order.
+__device__ __forceinline__ void speed_fold(double m, unsigned long long *word) {
+    __shared__ double sm[256];
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        __hip_atomic_fetch_max(word, static_cast<unsigned long long>(__double_as_longlong(sm[0])), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double *__restrict__ U,
                                                           double *__restrict__ Uold,
                                                           const double *__restrict__ fL,
@@ -344,158 +422,171 @@ __global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double
                                                           const double *__restrict__ vtq,
                                                           double *__restrict__ R,
                                                           const double *__restrict__ dtp, double c0,
-                                                          double c1, int flags, Phys P) {
+                                                          double c1, int flags, Phys P,
+                                                          unsigned long long *__restrict__ speed) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= T.nc) return;
-    const int nc = T.nc, nv = T.nv, ne = T.ne;
-    const unsigned info = T.cinfo[c];
-    const int nk = info >> 4;
-    const double cv = G.cvol[c];
-    double r[4];
-    if (flags & kFlagAdv) {
-        r[0] = r[1] = r[2] = r[3] = 0.0;
-        for (int k = 0; k < nk; ++k) {
-            const int e = T.cedge[4 * c + k];
-            double qL[4], qR[4], F[4];
+    const bool gated = P.gate && *P.gate == 0.0;
+    double m = 1e-8;
+    if (c < T.nc && !gated) {
+        const int nc = T.nc, nv = T.nv, ne = T.ne;
+        const unsigned info = T.cinfo[c];
+        const int nk = info >> 4;
+        const int4 ed4 = reinterpret_cast<const int4 *>(T.cedge)[c];
+        const int ed[4] = {ed4.x, ed4.y, ed4.z, ed4.w};
+        const double vol = G.cvol[c];
+        double u[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) u[q] = U[q * nc + c];
+        // level 2: face states, edge geometry, vertex stresses, all at once
+        double qL[4][4], qR[4][4], el[4], enx[4], eny[4];
+        char ebc[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (k < nk) {
+                const int e = ed[k];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    qL[k][q] = fL[q * ne + e];
+                    qR[k][q] = fR[q * ne + e];
+                }
+                el[k] = G.elen[e];
+                enx[k] = G.enx[e];
+                eny[k] = G.eny[e];
+                ebc[k] = T.ebc[e];
+            }
+        double ta[4][5], tb[4][5];
+        if (flags & kFlagVisc) {
+            const int4 ea4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c];
+            const int4 eb4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c + 1];
+            const int va[4] = {ea4.x, ea4.z, eb4.x, eb4.z}, vb[4] = {ea4.y, ea4.w, eb4.y, eb4.w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < nk) {
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) {
+                        ta[k][q] = vtq[q * nv + va[k]];
+                        tb[k][q] = vtq[q * nv + vb[k]];
+                    }
+                }
+        }
+        double r[4];
+        if (flags & kFlagAdv) {
+            r[0] = r[1] = r[2] = r[3] = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < nk) {
+                    double F[4];
+                    hllc(qL[k], qR[k], enx[k], eny[k], P.gamma, F);
+                    const double sign = ((info >> k) & 1u) ? +1.0 : -1.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) r[q] += sign * F[q] * el[k];
+                }
+            const double invV = 1.0 / vol;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] = r[q] * -invV;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) r[q] = R[q * nc + c];
+        }
+        if (flags & kFlagVisc) {
+            const double uu = u[1] / u[0];
+            const double vv = u[2] / u[0];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (k < nk) {
+                    double nx = enx[k], ny = eny[k];
+                    if ((info >> k) & 1u) {
+                        nx = -nx;
+                        ny = -ny;
+                    }
+                    double t[5];
+#pragma unroll
+                    for (int q = 0; q < 5; ++q) t[q] = 0.5 * (ta[k][q] + tb[k][q]);
+                    if (ebc[k] == 'f' || ebc[k] == 'w') {
+#pragma unroll
+                        for (int q = 0; q < 5; ++q) t[q] = 0.0;
+                    }
+                    const double txx = t[0], tyy = t[1], txy = t[2], qx = t[3], qy = t[4];
+                    const double L = el[k];
+                    r[1] += txx * L * nx / vol + txy * L * ny / vol;
+                    r[2] += txy * L * nx / vol + tyy * L * ny / vol;
+                    r[3] += (uu * txx + vv * txy - qx) * L * nx / vol + (uu * txy + vv * tyy - qy) * L * ny / vol;
+                }
+        }
+        if (flags & kFlagStoreR) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) R[q * nc + c] = r[q];
+        }
+        if (flags & kFlagRK) {
+            const double dt = *dtp;
+            double un[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                qL[q] = fL[q * ne + e];
-                qR[q] = fR[q * ne + e];
+                double old;
+                if (flags & kFlagSaveOld) {
+                    old = u[q];
+                    Uold[q * nc + c] = u[q];
+                } else {
+                    old = Uold[q * nc + c];
+                }
+                un[q] = c0 * old + c1 * (u[q] + dt * (r[q]));
+                U[q * nc + c] = un[q];
             }
-            hllc(qL, qR, G.enx[e], G.eny[e], P.gamma, F);
-            const double sign = ((info >> k) & 1u) ? +1.0 : -1.0;
-            const double L = G.elen[e];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) r[q] += sign * F[q] * L;
-        }
-        const double invV = 1.0 / cv;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] = r[q] * -invV;
-    } else {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) r[q] = R[q * nc + c];
-    }
-    if (flags & kFlagVisc) {
-        const double uu = U[nc + c] / U[c];
-        const double vv = U[2 * nc + c] / U[c];
-        for (int k = 0; k < nk; ++k) {
-            const int e = T.cedge[4 * c + k];
-            const double el = G.elen[e];
-            double enx = G.enx[e], eny = G.eny[e];
-            if ((info >> k) & 1u) {
-                enx = -enx;
-                eny = -eny;
-            }
-            const int a = T.eva[e], b = T.evb[e];
-            double t[5];
-#pragma unroll
-            for (int q = 0; q < 5; ++q) t[q] = 0.5 * (vtq[q * nv + a] + vtq[q * nv + b]);
-            const char bc = T.ebc[e];
-            if (bc == 'f' || bc == 'w') {
-#pragma unroll
-                for (int q = 0; q < 5; ++q) t[q] = 0.0;
-            }
-            const double txx = t[0], tyy = t[1], txy = t[2], qx = t[3], qy = t[4];
-            r[1] += txx * el * enx / cv + txy * el * eny / cv;
-            r[2] += txy * el * enx / cv + tyy * el * eny / cv;
-            r[3] += (uu * txx + vv * txy - qx) * el * enx / cv + (uu * txy + vv * tyy - qy) * el * eny / cv;
+            if (flags & kFlagSpeed) m = smax(m, wave_speed(un[0], un[1], un[2], un[3], P.gamma));
         }
     }
-    if (flags & kFlagStoreR) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) R[q * nc + c] = r[q];
-    }
-    if (flags & kFlagRK) {
-        const double dt = *dtp;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double cur = U[q * nc + c];
-            double old;
-            if (flags & kFlagSaveOld) {
-                old = cur;
-                Uold[q * nc + c] = cur;
-            } else {
-                old = Uold[q * nc + c];
-            }
-            U[q * nc + c] = c0 * old + c1 * (cur + dt * (r[q]));
-        }
-    }
+    if ((flags & kFlagSpeed) && !gated) speed_fold(m, speed);
 }
 
-// ------------------------------------------------------------------------
-// Deterministic reductions: fixed grid, fixed in-block tree, the last block
-// to finish combines the per-block partials in block order and runs the
-// epilogue.  The arrival counter is re-armed by the last block, so no memset
-// is ever needed (graph-capture safe).
-// ------------------------------------------------------------------------
-__device__ __forceinline__ bool last_block_arrived(unsigned *counter) {
-    __shared__ bool last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == gridDim.x - 1);
-        if (last) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __syncthreads();
-    return last;
+// Speed word: max_c(|u|+c) of the current state (src/simulation.cpp:80-96),
+// for the first step of a run or after the state was set from the host.
+__global__ __launch_bounds__(256) void k_speed_reset(unsigned long long *word) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *word = static_cast<unsigned long long>(__double_as_longlong(1e-8));
 }
 
-__device__ __forceinline__ double load_partial(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Time step, src/simulation.cpp:77-110: max over cells of |u|+c (starting
-// from 1e-8); min_h is static and precomputed on the host.  Also advances
-// the simulated time (src/simulation.cpp:121) and records dt.
-__global__ __launch_bounds__(kRedBlock) void k_dt(const double *__restrict__ U, int nc, Phys P,
-                                                 double min_h, double *__restrict__ partials,
-                                                 unsigned *counter, double *__restrict__ scal,
-                                                 double *__restrict__ hist, int advance_t) {
-    __shared__ double sm[kRedBlock];
+__global__ __launch_bounds__(256) void k_speed_max(const double *__restrict__ U, int nc, Phys P,
+                                                   unsigned long long *word) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     double m = 1e-8;
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        const double rho = U[c], ru = U[nc + c], rv = U[2 * nc + c], E = U[3 * nc + c];
-        const double u = ru / rho, v = rv / rho;
-        const double kinetic = 0.5 * (ru * ru + rv * rv) / rho;
-        const double p = (P.gamma - 1.0) * (E - kinetic);
-        const double cs = sqrt(P.gamma * p / rho);
-        m = smax(m, hypot(u, v) + cs);
-    }
-    sm[threadIdx.x] = m;
-    __syncthreads();
-    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) __hip_atomic_store(&partials[blockIdx.x], sm[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (!last_block_arrived(counter)) return;
-    double mm = 1e-8;
-    for (int b = threadIdx.x; b < gridDim.x; b += blockDim.x) mm = smax(mm, load_partial(&partials[b]));
-    sm[threadIdx.x] = mm;
-    __syncthreads();
-    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double max_wave = sm[0];
-        const double dt_adv = P.cfl * min_h / max_wave;
-        const double dt_diff = 0.5 * (min_h * min_h) / P.mu;
-        const double dt = smin(dt_adv, dt_diff);
-        scal[kScalDt] = dt;
-        if (advance_t) scal[kScalT] = scal[kScalT] + dt;
-        if (hist) *hist = dt;
-    }
+    if (c < nc) m = smax(m, wave_speed(U[c], U[nc + c], U[2 * nc + c], U[3 * nc + c], P.gamma));
+    speed_fold(m, word);
 }
 
-// Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i) over n
-// entries, combined in a fixed order; the last block applies `op`
-// to the BiCGSTAB scalars or just stores the sums.
-__global__ __launch_bounds__(kRedBlock) void k_dot2(const double *__restrict__ a, const double *__restrict__ b,
-                                                   const double *__restrict__ c, const double *__restrict__ d,
-                                                   long n, double *__restrict__ partials, unsigned *counter,
-                                                   double *__restrict__ scal, int op) {
+// Start of a step (one thread): dt = min(CFL min_h / max_wave, 0.5 min_h^2 / mu)
+// (src/simulation.cpp:108-110; min_h is static, precomputed on the host),
+// the time advance of :121, the while (t < final_time) test of :75 when
+// gated, and the re-arm of the speed word for the step's last stage.
+__global__ __launch_bounds__(64) void k_step_begin(unsigned long long *word, Phys P, double min_h,
+                                                   double *__restrict__ scal, double *__restrict__ hist,
+                                                   int advance_t) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (P.gate && !(scal[kScalT] < P.final_t)) {
+        scal[kScalActive] = 0.0;
+        return;
+    }
+    const double max_wave = __longlong_as_double(static_cast<long long>(*word));
+    const double dt_adv = P.cfl * min_h / max_wave;
+    const double dt_diff = 0.5 * (min_h * min_h) / P.mu;
+    const double dt = smin(dt_adv, dt_diff);
+    scal[kScalDt] = dt;
+    if (advance_t) {
+        scal[kScalTPrev] = scal[kScalT];
+        scal[kScalT] = scal[kScalT] + dt;
+    }
+    if (P.gate) scal[kScalSteps] = scal[kScalSteps] + 1.0;
+    if (hist) *hist = dt;
+    *word = static_cast<unsigned long long>(__double_as_longlong(1e-8));
+}
+
+// Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i): per-block
+// partials in a fixed tree, then one block combines them in block order and
+// applies `op` (two launches: the kernel boundary orders the partials, no
+// device-wide fence per block).
+__global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__restrict__ a,
+                                                           const double *__restrict__ b,
+                                                           const double *__restrict__ c,
+                                                           const double *__restrict__ d, long n,
+                                                           double *__restrict__ partials) {
     __shared__ double s1[kRedBlock], s2[kRedBlock];
     double x = 0.0, y = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -513,15 +604,18 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2(const double *__restrict__ a
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[2 * blockIdx.x], s1[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[2 * blockIdx.x + 1], s2[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        partials[2 * blockIdx.x] = s1[0];
+        partials[2 * blockIdx.x + 1] = s2[0];
     }
-    if (!last_block_arrived(counter)) return;
-    x = 0.0;
-    y = 0.0;
-    for (int bb = threadIdx.x; bb < gridDim.x; bb += blockDim.x) {
-        x += load_partial(&partials[2 * bb]);
-        y += load_partial(&partials[2 * bb + 1]);
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restrict__ partials, int nparts,
+                                                         double *__restrict__ scal, int op) {
+    __shared__ double s1[kRedBlock], s2[kRedBlock];
+    double x = 0.0, y = 0.0;
+    for (int bb = threadIdx.x; bb < nparts; bb += blockDim.x) {
+        x += partials[2 * bb];
+        y += partials[2 * bb + 1];
     }
     s1[threadIdx.x] = x;
     s2[threadIdx.x] = y;
@@ -823,22 +917,31 @@ hipError_t launch_stress(const Topo &T, const double *U, const double *Gs, doubl
 }
 hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *fL,
                               const double *fR, const double *vtq, double *R, const double *dtp, double c0,
-                              double c1, int flags, const Phys &P, hipStream_t s) {
+                              double c1, int flags, const Phys &P, hipStream_t s, unsigned long long *speed) {
     hipLaunchKernelGGL(k_cell_flux_update, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, U, Uold, fL, fR, vtq, R,
-                       dtp, c0, c1, flags, P);
+                       dtp, c0, c1, flags, P, speed);
     return hipGetLastError();
 }
-hipError_t launch_dt(const double *U, int nc, const Phys &P, double min_h, double *partials, unsigned *counter,
-                     double *scal, double *hist, hipStream_t s, int advance_t) {
-    const unsigned g = std::min<unsigned>(kRedGrid, nblk(nc, kRedBlock));
-    hipLaunchKernelGGL(k_dt, dim3(g), dim3(kRedBlock), 0, s, U, nc, P, min_h, partials, counter, scal, hist,
-                       advance_t);
+hipError_t launch_speed_reset(unsigned long long *word, hipStream_t s) {
+    hipLaunchKernelGGL(k_speed_reset, dim3(1), dim3(64), 0, s, word);
+    return hipGetLastError();
+}
+hipError_t launch_speed_max(const double *U, int nc, const Phys &P, unsigned long long *word, hipStream_t s) {
+    hipLaunchKernelGGL(k_speed_max, dim3(nblk(nc)), dim3(256), 0, s, U, nc, P, word);
+    return hipGetLastError();
+}
+hipError_t launch_step_begin(unsigned long long *word, const Phys &P, double min_h, double *scal, double *hist,
+                             int advance_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, word, P, min_h, scal, hist, advance_t);
     return hipGetLastError();
 }
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
-                       double *partials, unsigned *counter, double *scal, int op, hipStream_t s) {
-    const unsigned g = (unsigned)std::min<long>(kRedGrid, (long)nblk(n, kRedBlock));
-    hipLaunchKernelGGL(k_dot2, dim3(g), dim3(kRedBlock), 0, s, a, b, c, d, n, partials, counter, scal, op);
+                       double *partials, double *scal, int op, hipStream_t s) {
+    const unsigned g = (unsigned)std::min<long>(kRedGrid, std::max<long>(1, (long)nblk(n, kRedBlock)));
+    hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kRedBlock), 0, s, a, b, c, d, n, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedBlock), 0, s, partials, (int)g, scal, op);
     return hipGetLastError();
 }
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s) {

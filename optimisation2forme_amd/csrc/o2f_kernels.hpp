@@ -21,6 +21,7 @@ enum : int {
     kFlagStoreR = 4,    // write R
     kFlagRK = 8,        // stage update U = c0 Uold + c1 (U + dt R)
     kFlagSaveOld = 16,  // first stage: Uold = U before the update
+    kFlagSpeed = 32,    // after the update: max_c(|u|+c) of the new state into the speed word
 };
 
 // Slots of the device scalar block.
@@ -35,6 +36,9 @@ enum : int {
     kScalR0Sq = 7,
     kScalDotA = 8,
     kScalDotB = 9,
+    kScalActive = 10,  // 1 while t < final_time (gated time loop), else 0
+    kScalSteps = 11,   // steps taken by the gated time loop
+    kScalTPrev = 12,   // t at the start of the last step
     kScalCount = 16,
 };
 
@@ -48,12 +52,17 @@ struct Phys {
     double fs_rho, fs_ru, fs_rv, fs_E;  // apply_inflow_outflow_bc freestream state
     double venkat_k;
     int limiter;
+    // Gated time loop: when non-null, every stage kernel returns at once
+    // while *gate == 0, and the dt kernel clears it once t >= final_t.
+    const double *gate;
+    double final_t;
 };
 
 // Read-only topology (device pointers).
 struct Topo {
     int nv, ne, nc;
-    const int *cedge, *cnbr;
+    const int *cedge, *cnbr;  // 4 per cell, 16-byte aligned rows
+    const int *cevx;          // 8 per cell: edge slot endpoints (min, max vertex)
     const uint8_t *cinfo;
     const int *eva, *evb, *ecl, *ecr;
     const char *ebc;
@@ -79,11 +88,15 @@ hipError_t launch_stress(const Topo &T, const double *U, const double *Gs, doubl
                          hipStream_t s);
 hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *fL,
                               const double *fR, const double *vtq, double *R, const double *dtp, double c0,
-                              double c1, int flags, const Phys &P, hipStream_t s);
-hipError_t launch_dt(const double *U, int nc, const Phys &P, double min_h, double *partials, unsigned *counter,
-                     double *scal, double *hist, hipStream_t s, int advance_t = 1);
+                              double c1, int flags, const Phys &P, hipStream_t s, unsigned long long *speed = nullptr);
+// max_c(|u|+c) accumulation word (the bits of a positive double; 1e-8 when armed)
+hipError_t launch_speed_reset(unsigned long long *word, hipStream_t s);
+hipError_t launch_speed_max(const double *U, int nc, const Phys &P, unsigned long long *word, hipStream_t s);
+// dt from the speed word, time advance, gate, re-arm (one thread)
+hipError_t launch_step_begin(unsigned long long *word, const Phys &P, double min_h, double *scal, double *hist,
+                             int advance_t, hipStream_t s);
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
-                       double *partials, unsigned *counter, double *scal, int op, hipStream_t s);
+                       double *partials, double *scal, int op, hipStream_t s);
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s);
 hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s);
 hipError_t launch_bicg_xr(long n, double *x, double *r, const double *p, const double *sv, const double *t,
