@@ -30,7 +30,8 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 KERNEL_NAMES = {"vertex_gather": "o2f::k_vertex_gather<4>", "cell_reconstruct": "o2f::k_cell_reconstruct",
-                "vertex_viscous_gather": "o2f::k_vertex_gather<5>", "flux_update": "o2f::k_cell_flux_update"}
+                "viscous_gather_edge_flux": "o2f::k_viscous_gather_edge_flux",
+                "flux_update": "o2f::k_cell_flux_update"}
 METRIC = "vertex-steps/s"
 
 
@@ -51,10 +52,12 @@ def kernel_bytes(desc, nv, ne, nc):
     return {
         "vertex_gather": stencil + 32 * nc + 32 * nv,
         "cell_reconstruct": cell_topo + geom_edge + 32 * nc + 32 * nv + 64 * ne + 40 * nc,
-        "vertex_viscous_gather": stencil + 40 * nc + 40 * nv,
-        # own U read + write, U_old (read in 2 stages, written in 1), face
-        # states, edge normal/length/ids/tag, vertex stresses
-        "flux_update": nc * (1 + 16 + 8 + 32 + 32 + 32) + ne * (64 + 24 + 8 + 1) + 40 * nv,
+        # vertex stresses (stencil of tau q) and HLLC per edge (face states,
+        # normal -> flux)
+        "viscous_gather_edge_flux": stencil + 40 * nc + 40 * nv + ne * (64 + 16 + 32),
+        # own U read + write, U_old (read in 2 stages, written in 1), edge
+        # fluxes, edge length/normal/tag, endpoint ids, vertex stresses
+        "flux_update": nc * (1 + 16 + 32 + 8 + 32 + 32 + 32) + ne * (32 + 24 + 1) + 40 * nv,
     }
 
 
@@ -122,7 +125,7 @@ def main():
 
     # per-kernel pass: every stage kernel between its own events
     _, kms = s.time_steps(a.profile_steps, per_kernel=True)
-    names = ["vertex_gather", "cell_reconstruct", "vertex_viscous_gather", "flux_update"]
+    names = ["vertex_gather", "cell_reconstruct", "viscous_gather_edge_flux", "flux_update"]
     launches = 3 * a.profile_steps
     avg_us = {n: 1e3 * kms[i] / launches for i, n in enumerate(names)}
     kb = kernel_bytes(desc, nv, ne, nc)

@@ -91,9 +91,9 @@ struct o2f_ctx {
     GeomSet geo[3];  // 0: the mesh, 1/2: +/- shape perturbations
     // state and per-stage intermediates
     double *U = nullptr, *Uold = nullptr, *R = nullptr, *Gr = nullptr, *tq = nullptr;
-    double *vU = nullptr, *vtq = nullptr, *fL = nullptr, *fR = nullptr;
+    double *vU = nullptr, *vtq = nullptr, *fL = nullptr, *fR = nullptr, *Fe = nullptr;
     double *scal = nullptr, *partials = nullptr, *wall_out = nullptr;
-    unsigned long long *speed = nullptr;  // max_c(|u|+c) word of the next step's dt
+    double *speed = nullptr;              // per-block max_c(|u|+c) partials of the next step's dt
     bool speed_valid = false;             // it holds the speed of the current U
     bool have_grad = false;
     // adjoint
@@ -171,8 +171,8 @@ int run_stage(o2f_ctx *c, int g, int flags, const double *dtp, double c0, double
     hipStream_t s = c->stream;
     HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, s));
     HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, store_grad ? c->Gr : nullptr, c->phys, s));
-    HIPCHK(c, launch_vertex_visc(T, G, c->tq, c->vtq, c->phys, s));
-    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, dtp, c0, c1,
+    HIPCHK(c, launch_visc_and_flux(T, G, c->tq, c->vtq, c->fL, c->fR, c->Fe, c->phys, 1, s));
+    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, dtp, c0, c1,
                                  flags | kFlagAdv | kFlagVisc, c->phys, s, c->speed));
     ++c->faces_version;
     if (store_grad) c->have_grad = true;
@@ -209,10 +209,9 @@ int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed) {
 int one_step(o2f_ctx *c, bool last_outputs, double *hist) {
     hipStream_t s = c->stream;
     if (!c->speed_valid) {
-        HIPCHK(c, launch_speed_reset(c->speed, s));
         HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
     }
-    HIPCHK(c, launch_step_begin(c->speed, c->phys, c->hm.min_h, c->scal, hist, 1, s));
+    HIPCHK(c, launch_step_begin(c->speed, c->hm.nc, c->phys, c->hm.min_h, c->scal, hist, 1, s));
     int st = rk3_stages(c, last_outputs, true);
     c->speed_valid = (st == O2F_OK);
     return st;
@@ -382,10 +381,11 @@ int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx 
     if (!e) e = dalloc(c->vtq, 5 * nv);
     if (!e) e = dalloc(c->fL, 4 * ne);
     if (!e) e = dalloc(c->fR, 4 * ne);
+    if (!e) e = dalloc(c->Fe, 4 * ne);
     if (!e) e = dalloc(c->scal, kScalCount);
     if (!e) e = dalloc(c->partials, 2 * kRedGrid);
     if (!e) e = dalloc(c->wall_out, 2);
-    if (!e) e = dalloc(c->speed, 1);
+    if (!e) e = dalloc(c->speed, speed_parts(hm.nc));
     if (!e) e = hipMemset(c->scal, 0, kScalCount * sizeof(double));
     if (!e) e = hipMemset(c->U, 0, 4 * nc * sizeof(double));
     if (!e) e = hipMemset(c->fL, 0, 4 * ne * sizeof(double));
@@ -405,7 +405,7 @@ void o2f_ctx_destroy(o2f_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->cinfo,
                     c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
-                    c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx};
+                    c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx, c->Fe};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c->kv)
@@ -521,7 +521,8 @@ int o2f_advection(o2f_ctx *c, double *R_out) {
     const Geom G = c->geo[0].view();
     HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, c->stream));
     HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, c->Gr, c->phys, c->stream));
-    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, nullptr, 0.0, 0.0,
+    HIPCHK(c, launch_visc_and_flux(T, G, c->tq, c->vtq, c->fL, c->fR, c->Fe, c->phys, 0, c->stream));
+    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, nullptr, 0.0, 0.0,
                                  kFlagAdv | kFlagStoreR, c->phys, c->stream));
     ++c->faces_version;
     c->have_grad = true;
@@ -541,7 +542,7 @@ int o2f_diffusion(o2f_ctx *c, double *R_out) {
     const Geom G = c->geo[0].view();
     HIPCHK(c, launch_stress(T, c->U, c->Gr, c->tq, c->phys, c->stream));
     HIPCHK(c, launch_vertex_visc(T, G, c->tq, c->vtq, c->phys, c->stream));
-    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, nullptr, 0.0, 0.0,
+    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, nullptr, 0.0, 0.0,
                                  kFlagVisc | kFlagStoreR, c->phys, c->stream));
     if (!R_out) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -566,18 +567,36 @@ int o2f_get_gradients(o2f_ctx *c, double *G) {
     return copy_out(c, G, c->Gr, 8 * (size_t)c->hm.nc);
 }
 
+// Face states live AoS on the device (4 doubles per edge); the ABI is SoA.
+static int faces_to_host(o2f_ctx *c, const double *dev, double *host) {
+    if (!host) return O2F_OK;
+    const size_t ne = c->hm.ne;
+    std::vector<double> a(4 * ne);
+    int st = copy_out(c, a.data(), dev, 4 * ne);
+    if (st) return st;
+    for (size_t e = 0; e < ne; ++e)
+        for (int q = 0; q < 4; ++q) host[q * ne + e] = a[4 * e + q];
+    return O2F_OK;
+}
+
 int o2f_get_face_states(o2f_ctx *c, double *L, double *R) {
     if (!c) return O2F_ERR_ARG;
-    int st = copy_out(c, L, c->fL, 4 * (size_t)c->hm.ne);
+    int st = faces_to_host(c, c->fL, L);
     if (st) return st;
-    return copy_out(c, R, c->fR, 4 * (size_t)c->hm.ne);
+    return faces_to_host(c, c->fR, R);
 }
 
 int o2f_set_face_states(o2f_ctx *c, const double *L, const double *R) {
     if (!c || !L || !R) return O2F_ERR_ARG;
-    const size_t n = 4 * (size_t)c->hm.ne * sizeof(double);
-    HIPCHK(c, hipMemcpyAsync(c->fL, L, n, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->fR, R, n, hipMemcpyHostToDevice, c->stream));
+    const size_t ne = c->hm.ne;
+    std::vector<double> a(4 * ne), b(4 * ne);
+    for (size_t e = 0; e < ne; ++e)
+        for (int q = 0; q < 4; ++q) {
+            a[4 * e + q] = L[q * ne + e];
+            b[4 * e + q] = R[q * ne + e];
+        }
+    HIPCHK(c, hipMemcpyAsync(c->fL, a.data(), 4 * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->fR, b.data(), 4 * ne * sizeof(double), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     ++c->faces_version;
     return O2F_OK;
@@ -607,9 +626,8 @@ int o2f_hllc_flux_batch(const double *qL, const double *qR, const double *normal
 
 int o2f_compute_dt(o2f_ctx *c, double *dt) {
     if (!c || !dt) return O2F_ERR_ARG;
-    HIPCHK(c, launch_speed_reset(c->speed, c->stream));
     HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, c->stream));
-    HIPCHK(c, launch_step_begin(c->speed, c->phys, c->hm.min_h, c->scal, nullptr, 0, c->stream));
+    HIPCHK(c, launch_step_begin(c->speed, c->hm.nc, c->phys, c->hm.min_h, c->scal, nullptr, 0, c->stream));
     HIPCHK(c, hipMemcpyAsync(dt, c->scal + kScalDt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->speed_valid = false;
@@ -994,11 +1012,10 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
         int q = 0;
         for (int k = 0; k < nsteps; ++k) {
             if (!c->speed_valid) {
-                HIPCHK(c, launch_speed_reset(c->speed, s));
                 HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
                 c->speed_valid = true;
             }
-            HIPCHK(c, launch_step_begin(c->speed, c->phys, c->hm.min_h, c->scal, nullptr, 1, s));
+            HIPCHK(c, launch_step_begin(c->speed, c->hm.nc, c->phys, c->hm.min_h, c->scal, nullptr, 1, s));
             for (int stg = 0; stg < 3; ++stg) {
                 const int fl = kFlagAdv | kFlagVisc | kFlagRK | (stg == 0 ? kFlagSaveOld : 0) |
                                (stg == 2 ? kFlagSpeed : 0);
@@ -1009,10 +1026,10 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
                 HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, nullptr, c->phys, s));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
-                HIPCHK(c, launch_vertex_visc(T, G, c->tq, c->vtq, c->phys, s));
+                HIPCHK(c, launch_visc_and_flux(T, G, c->tq, c->vtq, c->fL, c->fR, c->Fe, c->phys, 1, s));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
-                HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->fL, c->fR, c->vtq, c->R, dtp, cs[stg][0],
+                HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, dtp, cs[stg][0],
                                              cs[stg][1], fl, c->phys, s, c->speed));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
             }

@@ -23,6 +23,24 @@ __device__ __forceinline__ bool tagged(char b) {
     return b == 'i' || b == 'o' || b == 'f' || b == 'w';
 }
 
+// Per-stage intermediates (vertex values, face states, edge fluxes, cell and
+// vertex stresses) are stored AoS, four doubles per entity (a fifth
+// component, q_y, sits in a trailing SoA block): one gathered entity is one
+// 32-byte run of a cache line instead of four scattered 8-byte words.
+__device__ __forceinline__ void ld4(const double *__restrict__ p, long i, double (&v)[4]) {
+    const double2 *q = reinterpret_cast<const double2 *>(p) + 2 * i;
+    const double2 a = q[0], b = q[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+}
+__device__ __forceinline__ void st4(double *__restrict__ p, long i, const double (&v)[4]) {
+    double2 *q = reinterpret_cast<double2 *>(p) + 2 * i;
+    q[0] = make_double2(v[0], v[1]);
+    q[1] = make_double2(v[2], v[3]);
+}
+
 // apply_inflow_outflow_bc, src/boundary.cpp:7-45 (freestream state precomputed
 // on the host with the same expression order).
 __device__ __forceinline__ void bc_state(const Phys &P, char bc, double &r, double &ru, double &rv,
@@ -123,11 +141,16 @@ __device__ __forceinline__ void hllc(const double *qL, const double *qR, double 
 // ------------------------------------------------------------------------
 constexpr int kStencilSlots = 8;  // stencil entries loaded together (longer ones loop)
 
-template <int NV>
-__global__ __launch_bounds__(256) void k_vertex_gather(Topo T, Geom G, const double *__restrict__ src,
-                                                       double *__restrict__ dst, Phys P) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= T.nv || (P.gate && *P.gate == 0.0)) return;
+// SRC_AOS: the source is AoS (4 + trailing block), else SoA [q][nc].
+// The destination is always AoS (4 + trailing block).
+template <int NV, bool SRC_AOS>
+__device__ __forceinline__ double src_val(const double *__restrict__ src, int nc, int c, int q) {
+    return SRC_AOS ? (q < 4 ? src[4 * (long)c + q] : src[4 * (long)nc + c]) : src[(long)q * nc + c];
+}
+
+template <int NV, bool SRC_AOS>
+__device__ __forceinline__ void gather_vertex(const Topo &T, const Geom &G, const double *__restrict__ src,
+                                              double *__restrict__ dst, const Phys &P, int v) {
     const int nc = T.nc;
     const uint8_t kind = T.skind[v];
     const int b = T.sptr[v], n = T.sptr[v + 1] - b;
@@ -143,8 +166,16 @@ __global__ __launch_bounds__(256) void k_vertex_gather(Topo T, Geom G, const dou
 #pragma unroll
     for (int k = 0; k < kStencilSlots; ++k)
         if (k < n) {
+            if (SRC_AOS) {
+                double t4[4];
+                ld4(src, cid[k], t4);
 #pragma unroll
-            for (int q = 0; q < NV; ++q) val[k][q] = src[q * nc + cid[k]];
+                for (int q = 0; q < 4; ++q) val[k][q] = t4[q];
+                if (NV == 5) val[k][NV - 1] = src[4 * (long)nc + cid[k]];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) val[k][q] = src[(long)q * nc + cid[k]];
+            }
         }
     double out[NV];
     if (kind == kStencilWsum) {
@@ -161,7 +192,7 @@ __global__ __launch_bounds__(256) void k_vertex_gather(Topo T, Geom G, const dou
             const int c = T.scell[b + k];
             const double wk = G.sw[b + k];
 #pragma unroll
-            for (int q = 0; q < NV; ++q) out[q] = out[q] + wk * src[q * nc + c];
+            for (int q = 0; q < NV; ++q) out[q] = out[q] + wk * src_val<NV, SRC_AOS>(src, nc, c, q);
         }
     } else if (kind == kStencilAvg2) {
 #pragma unroll
@@ -182,8 +213,47 @@ __global__ __launch_bounds__(256) void k_vertex_gather(Topo T, Geom G, const dou
             for (int q = 0; q < NV; ++q) out[q] = 0.0;
         }
     }
-#pragma unroll
-    for (int q = 0; q < NV; ++q) dst[q * T.nv + v] = out[q];
+    double o4[4] = {out[0], out[1], out[2], out[3]};
+    st4(dst, v, o4);
+    if (NV == 5) dst[4 * (long)T.nv + v] = out[NV - 1];
+}
+
+template <int NV, bool SRC_AOS>
+__global__ __launch_bounds__(256) void k_vertex_gather(Topo T, Geom G, const double *__restrict__ src,
+                                                       double *__restrict__ dst, Phys P) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= T.nv || (P.gate && *P.gate == 0.0)) return;
+    gather_vertex<NV, SRC_AOS>(T, G, src, dst, P, v);
+}
+
+// HLLC flux of one edge from its face states (src/advection.cpp:433-437).
+__device__ __forceinline__ void edge_flux(const Topo &T, const Geom &G, const double *__restrict__ fL,
+                                          const double *__restrict__ fR, double *__restrict__ F, const Phys &P,
+                                          int e) {
+    double qL[4], qR[4], f[4];
+    ld4(fL, e, qL);
+    ld4(fR, e, qR);
+    hllc(qL, qR, G.enx[e], G.eny[e], P.gamma, f);
+    st4(F, e, f);
+}
+
+// One launch, two independent jobs that both follow the reconstruction:
+// blocks [0, nbv) gather the viscous quantities at the vertices
+// (src/diffusion.cpp:28-140), the other blocks evaluate HLLC once per edge.
+__global__ __launch_bounds__(256) void k_viscous_gather_edge_flux(Topo T, Geom G, const double *__restrict__ tq,
+                                                                  double *__restrict__ vtq,
+                                                                  const double *__restrict__ fL,
+                                                                  const double *__restrict__ fR,
+                                                                  double *__restrict__ F, Phys P, int nbv,
+                                                                  int do_visc) {
+    if (P.gate && *P.gate == 0.0) return;
+    if ((int)blockIdx.x < nbv) {
+        const int v = blockIdx.x * blockDim.x + threadIdx.x;
+        if (do_visc && v < T.nv) gather_vertex<5, true>(T, G, tq, vtq, P, v);
+    } else {
+        const int e = (blockIdx.x - nbv) * blockDim.x + threadIdx.x;
+        if (e < T.ne) edge_flux(T, G, fL, fR, F, P, e);
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -214,79 +284,137 @@ __device__ __forceinline__ double alpha_bj(double phi_i, double phi_pred, double
     return alpha;
 }
 
+// Cell kernels run four lanes per cell, one per edge slot (triangles leave
+// one idle): a cell's sums over its edges are formed by every lane of the
+// group from the lanes' terms in ascending slot order, i.e. exactly the
+// sequential order of the reference loops, so results stay bit-identical.
+constexpr int kLanesPerCell = 4;
+
+// Value of lane J of this lane's group of four: a DPP quad_perm broadcast
+// (register to register, no LDS round trip).
+template <int J>
+__device__ __forceinline__ double from_lane(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), J * 0x55, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), J * 0x55, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// f(j, value of lane j) for j = 0, 1, 2, 3 in order
+template <class F>
+__device__ __forceinline__ void for_lanes(double v, F &&f) {
+    f(0, from_lane<0>(v));
+    f(1, from_lane<1>(v));
+    f(2, from_lane<2>(v));
+    f(3, from_lane<3>(v));
+}
+
 __global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const double *__restrict__ U,
                                                           const double *__restrict__ vU,
                                                           double *__restrict__ fL, double *__restrict__ fR,
                                                           double *__restrict__ tq, double *__restrict__ Gout,
                                                           Phys P) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= T.nc || (P.gate && *P.gate == 0.0)) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = gid / kLanesPerCell, k = gid % kLanesPerCell;
+    if (c >= T.nc || (P.gate && *P.gate == 0.0)) return;  // whole lane groups only
     const int nc = T.nc, nv = T.nv, ne = T.ne;
-    // level 1: the cell's own rows
     const unsigned info = T.cinfo[c];
     const int nk = info >> 4;
-    const int4 ed4 = reinterpret_cast<const int4 *>(T.cedge)[c];
-    const int4 nb4 = reinterpret_cast<const int4 *>(T.cnbr)[c];
-    const int4 ea4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c];
-    const int4 eb4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c + 1];
-    const int ed[4] = {ed4.x, ed4.y, ed4.z, ed4.w};
-    const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
-    const int va[4] = {ea4.x, ea4.z, eb4.x, eb4.z}, vb[4] = {ea4.y, ea4.w, eb4.y, eb4.w};
+    const bool has_edge = k < nk;
+    const bool is_left = (info >> k) & 1u;
     double u[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) u[q] = U[q * nc + c];
     const double vol = G.cvol[c], cx = G.ccx[c], cy = G.ccy[c];
-    // level 2: edge end-point values, edge geometry, neighbour states, all at once
-    double xa[4][4], xb[4][4];
+    // this lane's edge and face neighbour
+    const int e = has_edge ? T.cedge[4 * c + k] : 0;
+    const int nb = T.cnbr[4 * c + k];
+    double el = 0.0, enx = 0.0, eny = 0.0, dx = 0.0, dy = 0.0, xa[4], xb[4], un[4];
+    char ebc = 0;
+    if (has_edge) {
+        const int va = T.cevx[8 * c + 2 * k], vb = T.cevx[8 * c + 2 * k + 1];
+        el = G.elen[e];
+        enx = G.enx[e];
+        eny = G.eny[e];
+        dx = G.emx[e] - cx;
+        dy = G.emy[e] - cy;
+        ebc = T.ebc[e];
+        ld4(vU, va, xa);
+        ld4(vU, vb, xb);
+    }
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k < nk) {
+    for (int q = 0; q < 4; ++q) un[q] = (nb >= 0) ? U[q * nc + nb] : u[q];
+    // edge value (+ edge BC) and this edge's Green-Gauss terms
+    double tx[4] = {0.0, 0.0, 0.0, 0.0}, ty[4] = {0.0, 0.0, 0.0, 0.0};
+    if (has_edge) {
+        double val[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                xa[k][q] = vU[q * nv + va[k]];
-                xb[k][q] = vU[q * nv + vb[k]];
-            }
+        for (int q = 0; q < 4; ++q) val[q] = 0.5 * (xa[q] + xb[q]);
+        if (tagged(ebc)) bc_state(P, ebc, val[0], val[1], val[2], val[3]);
+        double nx = enx, ny = eny;
+        if (is_left) {
+            nx = -nx;
+            ny = -ny;
         }
-    double el[4], enx[4], eny[4], dxs[4], dys[4];
-    char ebc[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k < nk) {
-            const int e = ed[k];
-            el[k] = G.elen[e];
-            enx[k] = G.enx[e];
-            eny[k] = G.eny[e];
-            dxs[k] = G.emx[e] - cx;
-            dys[k] = G.emy[e] - cy;
-            ebc[k] = T.ebc[e];
+        for (int q = 0; q < 4; ++q) {
+            tx[q] = val[q] * el * nx / vol;
+            ty[q] = val[q] * el * ny / vol;
         }
-    double un[4][4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (nb[k] >= 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) un[k][q] = U[q * nc + nb[k]];
-        }
-    // edge values (+ edge BC) and the Green-Gauss sum in ascending edge id
+    }
     double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k < nk) {
-            double val[4];
+    for (int q = 0; q < 4; ++q) {
+        for_lanes(tx[q], [&](int j, double a) {
+            if (j < nk) gx[q] += a;
+        });
+        for_lanes(ty[q], [&](int j, double b) {
+            if (j < nk) gy[q] += b;
+        });
+    }
+    // neighbour min / max in neighbour-list order (a lane without a
+    // neighbour offers the cell's own value, which changes nothing)
+    double mn[4], mx[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) val[q] = 0.5 * (xa[k][q] + xb[k][q]);
-            if (tagged(ebc[k])) bc_state(P, ebc[k], val[0], val[1], val[2], val[3]);
-            double nx = enx[k], ny = eny[k];
-            if ((info >> k) & 1u) {
-                nx = -nx;
-                ny = -ny;
-            }
+    for (int q = 0; q < 4; ++q) mn[q] = mx[q] = u[q];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                gx[q] += val[q] * el[k] * nx / vol;
-                gy[q] += val[q] * el[k] * ny / vol;
-            }
+    for (int q = 0; q < 4; ++q)
+        for_lanes(un[q], [&](int, double v) {
+            if (v < mn[q]) mn[q] = v;
+            if (v > mx[q]) mx[q] = v;
+        });
+    // limiter: this edge's alpha, then the min over edges in slot order
+    double a_k[4] = {1.0, 1.0, 1.0, 1.0};
+    if (has_edge) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double pred = u[q] + gx[q] * dx + gy[q] * dy;
+            a_k[q] = (P.limiter == 1) ? alpha_bj(u[q], pred, mn[q], mx[q])
+                                      : alpha_venkat(u[q], pred, mn[q], mx[q], vol, P.venkat_k);
         }
+    }
+    double al[4] = {1.0, 1.0, 1.0, 1.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        for_lanes(a_k[q], [&](int j, double a) {
+            if (j < nk) al[q] = smin(al[q], a);
+        });
+    // this edge's face state (+ boundary ghost)
+    if (has_edge) {
+        double f[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[q] = u[q] + al[q] * (gx[q] * dx + gy[q] * dy);
+        if (is_left) {
+            st4(fL, e, f);
+            if (tagged(ebc)) {
+                double g[4];
+                bc_ghost(P, ebc, f, g);
+                st4(fR, e, g);
+            }
+        } else {
+            st4(fR, e, f);
+        }
+    }
+    if (k != 0) return;
     if (Gout) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -294,51 +422,6 @@ __global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const 
             Gout[(4 + q) * nc + c] = gy[q];
         }
     }
-    double mn[4], mx[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) mn[q] = mx[q] = u[q];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (nb[k] >= 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (un[k][q] < mn[q]) mn[q] = un[k][q];
-                if (un[k][q] > mx[q]) mx[q] = un[k][q];
-            }
-        }
-    double al[4] = {1.0, 1.0, 1.0, 1.0};
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k < nk) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const double pred = u[q] + gx[q] * dxs[k] + gy[q] * dys[k];
-                const double a = (P.limiter == 1) ? alpha_bj(u[q], pred, mn[q], mx[q])
-                                                  : alpha_venkat(u[q], pred, mn[q], mx[q], vol, P.venkat_k);
-                al[q] = smin(al[q], a);
-            }
-        }
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-        if (k < nk) {
-            const int e = ed[k];
-            double f[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) f[q] = u[q] + al[q] * (gx[q] * dxs[k] + gy[q] * dys[k]);
-            if ((info >> k) & 1u) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) fL[q * ne + e] = f[q];
-                if (tagged(ebc[k])) {
-                    double g[4];
-                    bc_ghost(P, ebc[k], f, g);
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) fR[q * ne + e] = g[q];
-                }
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) fR[q * ne + e] = f[q];
-            }
-        }
     // viscous stress and heat flux of the cell (src/diffusion.cpp:227-240)
     const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
     const double rho = u[0], E = u[3];
@@ -347,13 +430,12 @@ __global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const 
     const double v_y = (gy[2] - vvel * gy[0]) / rho;
     const double u_y = (gy[1] - uu * gy[0]) / rho;
     const double v_x = (gx[2] - vvel * gx[0]) / rho;
-    tq[c] = 2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y);
-    tq[nc + c] = 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y);
-    tq[2 * nc + c] = mu * (u_y + v_x);
     const double T_x = (gamma - 1.0) * ((gx[3] * rho - E * gx[0]) / (rho * rho) - (uu * u_x + vvel * v_x));
     const double T_y = (gamma - 1.0) * ((gy[3] * rho - E * gy[0]) / (rho * rho) - (uu * u_y + vvel * v_y));
-    tq[3 * nc + c] = -kappa * T_x;
-    tq[4 * nc + c] = -kappa * T_y;
+    const double t4[4] = {2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y), 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y),
+                          mu * (u_y + v_x), -kappa * T_x};
+    st4(tq, c, t4);
+    tq[4 * (long)nc + c] = -kappa * T_y;
 }
 
 // Stress / heat flux from stored gradients (computeDiffusionTerm called on
@@ -373,13 +455,12 @@ __global__ __launch_bounds__(256) void k_cell_stress(Topo T, const double *__res
     const double v_y = (rvy - vv * ry) / rho;
     const double u_y = (ruy - uu * ry) / rho;
     const double v_x = (rvx - vv * rx) / rho;
-    tq[c] = 2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y);
-    tq[nc + c] = 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y);
-    tq[2 * nc + c] = mu * (u_y + v_x);
     const double T_x = (gamma - 1.0) * ((Ex * rho - E * rx) / (rho * rho) - (uu * u_x + vv * v_x));
     const double T_y = (gamma - 1.0) * ((Ey * rho - E * ry) / (rho * rho) - (uu * u_y + vv * v_y));
-    tq[3 * nc + c] = -kappa * T_x;
-    tq[4 * nc + c] = -kappa * T_y;
+    const double t4[4] = {2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y), 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y),
+                          mu * (u_y + v_x), -kappa * T_x};
+    st4(tq, c, t4);
+    tq[4 * (long)nc + c] = -kappa * T_y;
 }
 
 // ------------------------------------------------------------------------
@@ -399,10 +480,9 @@ __device__ __forceinline__ double wave_speed(double rho, double ru, double rv, d
     return hypot(u, v) + cs;
 }
 
-// Block max of m (NaN ignored like std::max) folded into the speed word.
-// Positive doubles order like their bit patterns, so an integer atomic max is
-// exact and independent of arrival order.
-__device__ __forceinline__ void speed_fold(double m, unsigned long long *word) {
+// Block max of m (NaN ignored like std::max) stored as this block's partial;
+// the step-begin block takes the max of the partials (exact, order-free).
+__device__ __forceinline__ void speed_fold(double m, double *parts) {
     __shared__ double sm[256];
     sm[threadIdx.x] = m;
     __syncthreads();
@@ -410,77 +490,93 @@ __device__ __forceinline__ void speed_fold(double m, unsigned long long *word) {
         if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        __hip_atomic_fetch_max(word, static_cast<unsigned long long>(__double_as_longlong(sm[0])), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) parts[blockIdx.x] = sm[0];
 }
 
 __global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double *__restrict__ U,
                                                           double *__restrict__ Uold,
-                                                          const double *__restrict__ fL,
-                                                          const double *__restrict__ fR,
+                                                          const double *__restrict__ Fe,
                                                           const double *__restrict__ vtq,
                                                           double *__restrict__ R,
                                                           const double *__restrict__ dtp, double c0,
                                                           double c1, int flags, Phys P,
-                                                          unsigned long long *__restrict__ speed) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          double *__restrict__ speed) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = gid / kLanesPerCell, k = gid % kLanesPerCell;
     const bool gated = P.gate && *P.gate == 0.0;
     double m = 1e-8;
     if (c < T.nc && !gated) {
         const int nc = T.nc, nv = T.nv, ne = T.ne;
         const unsigned info = T.cinfo[c];
         const int nk = info >> 4;
-        const int4 ed4 = reinterpret_cast<const int4 *>(T.cedge)[c];
-        const int ed[4] = {ed4.x, ed4.y, ed4.z, ed4.w};
+        const bool has_edge = k < nk;
         const double vol = G.cvol[c];
         double u[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) u[q] = U[q * nc + c];
-        // level 2: face states, edge geometry, vertex stresses, all at once
-        double qL[4][4], qR[4][4], el[4], enx[4], eny[4];
-        char ebc[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (k < nk) {
-                const int e = ed[k];
+        // this lane's edge: flux, geometry, end-point stresses
+        double Fk[4] = {0.0, 0.0, 0.0, 0.0}, el = 0.0, enx = 0.0, eny = 0.0, ta[5], tb[5];
+        char ebc = 0;
+        if (has_edge) {
+            const int e = T.cedge[4 * c + k];
+            if (flags & kFlagAdv) ld4(Fe, e, Fk);
+            el = G.elen[e];
+            enx = G.enx[e];
+            eny = G.eny[e];
+            ebc = T.ebc[e];
+            if (flags & kFlagVisc) {
+                const int va = T.cevx[8 * c + 2 * k], vb = T.cevx[8 * c + 2 * k + 1];
+                double a4[4], b4[4];
+                ld4(vtq, va, a4);
+                ld4(vtq, vb, b4);
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    qL[k][q] = fL[q * ne + e];
-                    qR[k][q] = fR[q * ne + e];
+                    ta[q] = a4[q];
+                    tb[q] = b4[q];
                 }
-                el[k] = G.elen[e];
-                enx[k] = G.enx[e];
-                eny[k] = G.eny[e];
-                ebc[k] = T.ebc[e];
+                ta[4] = vtq[4 * (long)nv + va];
+                tb[4] = vtq[4 * (long)nv + vb];
             }
-        double ta[4][5], tb[4][5];
-        if (flags & kFlagVisc) {
-            const int4 ea4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c];
-            const int4 eb4 = reinterpret_cast<const int4 *>(T.cevx)[2 * c + 1];
-            const int va[4] = {ea4.x, ea4.z, eb4.x, eb4.z}, vb[4] = {ea4.y, ea4.w, eb4.y, eb4.w};
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (k < nk) {
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) {
-                        ta[k][q] = vtq[q * nv + va[k]];
-                        tb[k][q] = vtq[q * nv + vb[k]];
-                    }
-                }
         }
+        const bool is_left = (info >> k) & 1u;
+        // convective term of this edge, src/advection.cpp:453-456
+        double adv[4] = {0.0, 0.0, 0.0, 0.0};
+        if (has_edge) {
+            const double sign = is_left ? +1.0 : -1.0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) adv[q] = sign * Fk[q] * el;
+        }
+        // viscous terms of this edge, src/diffusion.cpp:183-200
+        double w1 = 0.0, w2 = 0.0, w3 = 0.0;
+        if ((flags & kFlagVisc) && has_edge) {
+            const double uu = u[1] / u[0];
+            const double vv = u[2] / u[0];
+            double nx = enx, ny = eny;
+            if (is_left) {
+                nx = -nx;
+                ny = -ny;
+            }
+            double t[5];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) t[q] = 0.5 * (ta[q] + tb[q]);
+            if (ebc == 'f' || ebc == 'w') {
+#pragma unroll
+                for (int q = 0; q < 5; ++q) t[q] = 0.0;
+            }
+            const double txx = t[0], tyy = t[1], txy = t[2], qx = t[3], qy = t[4];
+            w1 = txx * el * nx / vol + txy * el * ny / vol;
+            w2 = txy * el * nx / vol + tyy * el * ny / vol;
+            w3 = (uu * txx + vv * txy - qx) * el * nx / vol + (uu * txy + vv * tyy - qy) * el * ny / vol;
+        }
+        // ordered sums over the cell's edges
         double r[4];
         if (flags & kFlagAdv) {
             r[0] = r[1] = r[2] = r[3] = 0.0;
 #pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (k < nk) {
-                    double F[4];
-                    hllc(qL[k], qR[k], enx[k], eny[k], P.gamma, F);
-                    const double sign = ((info >> k) & 1u) ? +1.0 : -1.0;
-#pragma unroll
-                    for (int q = 0; q < 4; ++q) r[q] += sign * F[q] * el[k];
-                }
+            for (int q = 0; q < 4; ++q)
+                for_lanes(adv[q], [&](int j, double a) {
+                    if (j < nk) r[q] += a;
+                });
             const double invV = 1.0 / vol;
 #pragma unroll
             for (int q = 0; q < 4; ++q) r[q] = r[q] * -invV;
@@ -489,82 +585,79 @@ __global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double
             for (int q = 0; q < 4; ++q) r[q] = R[q * nc + c];
         }
         if (flags & kFlagVisc) {
-            const double uu = u[1] / u[0];
-            const double vv = u[2] / u[0];
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-                if (k < nk) {
-                    double nx = enx[k], ny = eny[k];
-                    if ((info >> k) & 1u) {
-                        nx = -nx;
-                        ny = -ny;
-                    }
-                    double t[5];
-#pragma unroll
-                    for (int q = 0; q < 5; ++q) t[q] = 0.5 * (ta[k][q] + tb[k][q]);
-                    if (ebc[k] == 'f' || ebc[k] == 'w') {
-#pragma unroll
-                        for (int q = 0; q < 5; ++q) t[q] = 0.0;
-                    }
-                    const double txx = t[0], tyy = t[1], txy = t[2], qx = t[3], qy = t[4];
-                    const double L = el[k];
-                    r[1] += txx * L * nx / vol + txy * L * ny / vol;
-                    r[2] += txy * L * nx / vol + tyy * L * ny / vol;
-                    r[3] += (uu * txx + vv * txy - qx) * L * nx / vol + (uu * txy + vv * tyy - qy) * L * ny / vol;
-                }
+            // per edge, in slot order: R_rho_u, R_rho_v, R_E each += that edge's term
+            const double a0 = from_lane<0>(w1), b0 = from_lane<0>(w2), d0 = from_lane<0>(w3);
+            const double a1 = from_lane<1>(w1), b1 = from_lane<1>(w2), d1 = from_lane<1>(w3);
+            const double a2 = from_lane<2>(w1), b2 = from_lane<2>(w2), d2 = from_lane<2>(w3);
+            const double a3 = from_lane<3>(w1), b3 = from_lane<3>(w2), d3 = from_lane<3>(w3);
+            if (0 < nk) { r[1] += a0; r[2] += b0; r[3] += d0; }
+            if (1 < nk) { r[1] += a1; r[2] += b1; r[3] += d1; }
+            if (2 < nk) { r[1] += a2; r[2] += b2; r[3] += d2; }
+            if (3 < nk) { r[1] += a3; r[2] += b3; r[3] += d3; }
         }
-        if (flags & kFlagStoreR) {
+        if (k == 0) {
+            if (flags & kFlagStoreR) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) R[q * nc + c] = r[q];
-        }
-        if (flags & kFlagRK) {
-            const double dt = *dtp;
-            double un[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                double old;
-                if (flags & kFlagSaveOld) {
-                    old = u[q];
-                    Uold[q * nc + c] = u[q];
-                } else {
-                    old = Uold[q * nc + c];
-                }
-                un[q] = c0 * old + c1 * (u[q] + dt * (r[q]));
-                U[q * nc + c] = un[q];
+                for (int q = 0; q < 4; ++q) R[q * nc + c] = r[q];
             }
-            if (flags & kFlagSpeed) m = smax(m, wave_speed(un[0], un[1], un[2], un[3], P.gamma));
+            if (flags & kFlagRK) {
+                const double dt = *dtp;
+                double unew[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    double old;
+                    if (flags & kFlagSaveOld) {
+                        old = u[q];
+                        Uold[q * nc + c] = u[q];
+                    } else {
+                        old = Uold[q * nc + c];
+                    }
+                    unew[q] = c0 * old + c1 * (u[q] + dt * (r[q]));
+                    U[q * nc + c] = unew[q];
+                }
+                if (flags & kFlagSpeed) m = smax(m, wave_speed(unew[0], unew[1], unew[2], unew[3], P.gamma));
+            }
         }
     }
     if ((flags & kFlagSpeed) && !gated) speed_fold(m, speed);
 }
 
-// Speed word: max_c(|u|+c) of the current state (src/simulation.cpp:80-96),
-// for the first step of a run or after the state was set from the host.
-__global__ __launch_bounds__(256) void k_speed_reset(unsigned long long *word) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *word = static_cast<unsigned long long>(__double_as_longlong(1e-8));
-}
-
+// Speed partials: per-block max_c(|u|+c) of the current state
+// (src/simulation.cpp:80-96), for the first step of a run or after the state
+// was set from the host (later steps get them from the last stage).
 __global__ __launch_bounds__(256) void k_speed_max(const double *__restrict__ U, int nc, Phys P,
-                                                   unsigned long long *word) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+                                                   double *__restrict__ parts) {
+    // same grid as the 4-lane cell kernels, so both leave the same partials
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = gid / kLanesPerCell;
     double m = 1e-8;
-    if (c < nc) m = smax(m, wave_speed(U[c], U[nc + c], U[2 * nc + c], U[3 * nc + c], P.gamma));
-    speed_fold(m, word);
+    if (c < nc && gid % kLanesPerCell == 0)
+        m = smax(m, wave_speed(U[c], U[nc + c], U[2 * nc + c], U[3 * nc + c], P.gamma));
+    speed_fold(m, parts);
 }
 
-// Start of a step (one thread): dt = min(CFL min_h / max_wave, 0.5 min_h^2 / mu)
-// (src/simulation.cpp:108-110; min_h is static, precomputed on the host),
-// the time advance of :121, the while (t < final_time) test of :75 when
-// gated, and the re-arm of the speed word for the step's last stage.
-__global__ __launch_bounds__(64) void k_step_begin(unsigned long long *word, Phys P, double min_h,
-                                                   double *__restrict__ scal, double *__restrict__ hist,
-                                                   int advance_t) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Start of a step (one block): max_wave = max of the speed partials, then
+// dt = min(CFL min_h / max_wave, 0.5 min_h^2 / mu) (src/simulation.cpp:108-110;
+// min_h is static, precomputed on the host), the time advance of :121 and,
+// when gated, the while (t < final_time) test of :75.
+__global__ __launch_bounds__(256) void k_step_begin(const double *__restrict__ parts, int nparts, Phys P,
+                                                    double min_h, double *__restrict__ scal,
+                                                    double *__restrict__ hist, int advance_t) {
+    __shared__ double sm[256];
+    double m = 1e-8;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = smax(m, parts[b]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     if (P.gate && !(scal[kScalT] < P.final_t)) {
         scal[kScalActive] = 0.0;
         return;
     }
-    const double max_wave = __longlong_as_double(static_cast<long long>(*word));
+    const double max_wave = sm[0];
     const double dt_adv = P.cfl * min_h / max_wave;
     const double dt_diff = 0.5 * (min_h * min_h) / P.mu;
     const double dt = smin(dt_adv, dt_diff);
@@ -575,7 +668,6 @@ __global__ __launch_bounds__(64) void k_step_begin(unsigned long long *word, Phy
     }
     if (P.gate) scal[kScalSteps] = scal[kScalSteps] + 1.0;
     if (hist) *hist = dt;
-    *word = static_cast<unsigned long long>(__double_as_longlong(1e-8));
 }
 
 // Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i): per-block
@@ -727,7 +819,7 @@ __global__ __launch_bounds__(kRedBlock) void k_wall_forces(Geom G, const int *__
             const int e = wall[k];
             use = !(skip_x && G.emx[e] > 1.3);
             const double nx = G.enx[e], ny = G.eny[e], A = G.elen[e];
-            const double rl = fL[e], rul = fL[ne + e], rvl = fL[2 * ne + e], El = fL[3 * ne + e];
+            const double rl = fL[4 * e], rul = fL[4 * e + 1], rvl = fL[4 * e + 2], El = fL[4 * e + 3];
             const double p = (gamma - 1.0) * (El - 0.5 * (rul * rul + rvl * rvl) / rl);
             const double txx = 0.0, txy = 0.0, tyy = 0.0;
             l = (-p * ny + txy * nx + tyy * ny) * A;
@@ -764,7 +856,7 @@ __global__ __launch_bounds__(64) void k_adjoint_rhs(Topo T, Geom G, const int *_
         const int e = wall[k];
         const int iL = T.ecl[e];
         const double A_f = G.elen[e], nx = G.enx[e], ny = G.eny[e];
-        const double rL = fL[e], ruL = fL[ne + e], rvL = fL[2 * ne + e];
+        const double rL = fL[4 * e], ruL = fL[4 * e + 1], rvL = fL[4 * e + 2];
         double dp[4];
         dp[0] = 0.5 * (gamma - 1.0) * ((ruL * ruL + rvL * rvL) / (rL * rL));
         dp[1] = -(gamma - 1.0) * (ruL / rL);
@@ -793,11 +885,8 @@ __global__ __launch_bounds__(256) void k_jt_blocks(Topo T, Geom G, const double 
     const char bc = T.ebc[e];
     const double enx = G.enx[e], eny = G.eny[e];
     double bL[4], bR[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        bL[q] = fL[q * ne + e];
-        bR[q] = fR[q * ne + e];
-    }
+    ld4(fL, e, bL);
+    ld4(fR, e, bR);
     for (int c = 0; c < 4; ++c) {
         double Lp[4], Lm[4], Rp[4], Rm[4], Fp[4], Fm[4];
 #pragma unroll
@@ -897,17 +986,18 @@ static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1)
 
 hipError_t launch_vertex_state(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_vertex_gather<4>, dim3(nblk(T.nv)), dim3(256), 0, s, T, G, U, vU, P);
+    hipLaunchKernelGGL((k_vertex_gather<4, false>), dim3(nblk(T.nv)), dim3(256), 0, s, T, G, U, vU, P);
     return hipGetLastError();
 }
 hipError_t launch_vertex_visc(const Topo &T, const Geom &G, const double *tq, double *vtq, const Phys &P,
                               hipStream_t s) {
-    hipLaunchKernelGGL(k_vertex_gather<5>, dim3(nblk(T.nv)), dim3(256), 0, s, T, G, tq, vtq, P);
+    hipLaunchKernelGGL((k_vertex_gather<5, true>), dim3(nblk(T.nv)), dim3(256), 0, s, T, G, tq, vtq, P);
     return hipGetLastError();
 }
 hipError_t launch_reconstruct(const Topo &T, const Geom &G, const double *U, const double *vU, double *fL,
                               double *fR, double *tq, double *Gout, const Phys &P, hipStream_t s) {
-    hipLaunchKernelGGL(k_cell_reconstruct, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, U, vU, fL, fR, tq, Gout, P);
+    hipLaunchKernelGGL(k_cell_reconstruct, dim3(nblk((long)kLanesPerCell * T.nc)), dim3(256), 0, s, T, G, U, vU, fL,
+                       fR, tq, Gout, P);
     return hipGetLastError();
 }
 hipError_t launch_stress(const Topo &T, const double *U, const double *Gs, double *tq, const Phys &P,
@@ -915,24 +1005,29 @@ hipError_t launch_stress(const Topo &T, const double *U, const double *Gs, doubl
     hipLaunchKernelGGL(k_cell_stress, dim3(nblk(T.nc)), dim3(256), 0, s, T, U, Gs, tq, P);
     return hipGetLastError();
 }
-hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *fL,
-                              const double *fR, const double *vtq, double *R, const double *dtp, double c0,
-                              double c1, int flags, const Phys &P, hipStream_t s, unsigned long long *speed) {
-    hipLaunchKernelGGL(k_cell_flux_update, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, U, Uold, fL, fR, vtq, R,
-                       dtp, c0, c1, flags, P, speed);
+hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *Fe,
+                              const double *vtq, double *R, const double *dtp, double c0, double c1, int flags,
+                              const Phys &P, hipStream_t s, double *speed) {
+    hipLaunchKernelGGL(k_cell_flux_update, dim3(nblk((long)kLanesPerCell * T.nc)), dim3(256), 0, s, T, G, U, Uold,
+                       Fe, vtq, R, dtp, c0, c1, flags, P, speed);
     return hipGetLastError();
 }
-hipError_t launch_speed_reset(unsigned long long *word, hipStream_t s) {
-    hipLaunchKernelGGL(k_speed_reset, dim3(1), dim3(64), 0, s, word);
+hipError_t launch_visc_and_flux(const Topo &T, const Geom &G, const double *tq, double *vtq, const double *fL,
+                                const double *fR, double *Fe, const Phys &P, int do_visc, hipStream_t s) {
+    const int nbv = (int)nblk(T.nv);
+    hipLaunchKernelGGL(k_viscous_gather_edge_flux, dim3(nbv + nblk(T.ne)), dim3(256), 0, s, T, G, tq, vtq, fL, fR,
+                       Fe, P, nbv, do_visc);
     return hipGetLastError();
 }
-hipError_t launch_speed_max(const double *U, int nc, const Phys &P, unsigned long long *word, hipStream_t s) {
-    hipLaunchKernelGGL(k_speed_max, dim3(nblk(nc)), dim3(256), 0, s, U, nc, P, word);
+int speed_parts(int nc) { return (int)nblk((long)kLanesPerCell * nc); }
+hipError_t launch_speed_max(const double *U, int nc, const Phys &P, double *parts, hipStream_t s) {
+    hipLaunchKernelGGL(k_speed_max, dim3(speed_parts(nc)), dim3(256), 0, s, U, nc, P, parts);
     return hipGetLastError();
 }
-hipError_t launch_step_begin(unsigned long long *word, const Phys &P, double min_h, double *scal, double *hist,
+hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double min_h, double *scal, double *hist,
                              int advance_t, hipStream_t s) {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, word, P, min_h, scal, hist, advance_t);
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, parts, speed_parts(nc), P, min_h, scal, hist,
+                       advance_t);
     return hipGetLastError();
 }
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,

@@ -86,14 +86,18 @@ hipError_t launch_reconstruct(const Topo &T, const Geom &G, const double *U, con
                               double *fR, double *tq, double *Gout, const Phys &P, hipStream_t s);
 hipError_t launch_stress(const Topo &T, const double *U, const double *Gs, double *tq, const Phys &P,
                          hipStream_t s);
-hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *fL,
-                              const double *fR, const double *vtq, double *R, const double *dtp, double c0,
-                              double c1, int flags, const Phys &P, hipStream_t s, unsigned long long *speed = nullptr);
-// max_c(|u|+c) accumulation word (the bits of a positive double; 1e-8 when armed)
-hipError_t launch_speed_reset(unsigned long long *word, hipStream_t s);
-hipError_t launch_speed_max(const double *U, int nc, const Phys &P, unsigned long long *word, hipStream_t s);
-// dt from the speed word, time advance, gate, re-arm (one thread)
-hipError_t launch_step_begin(unsigned long long *word, const Phys &P, double min_h, double *scal, double *hist,
+// Cell sum of the edge fluxes, viscous divergence, stage update.
+hipError_t launch_flux_update(const Topo &T, const Geom &G, double *U, double *Uold, const double *Fe,
+                              const double *vtq, double *R, const double *dtp, double c0, double c1, int flags,
+                              const Phys &P, hipStream_t s, double *speed = nullptr);
+// Vertex gather of the viscous quantities (do_visc) and HLLC once per edge.
+hipError_t launch_visc_and_flux(const Topo &T, const Geom &G, const double *tq, double *vtq, const double *fL,
+                                const double *fR, double *Fe, const Phys &P, int do_visc, hipStream_t s);
+// Per-block max_c(|u|+c) partials (speed_parts(nc) of them).
+int speed_parts(int nc);
+hipError_t launch_speed_max(const double *U, int nc, const Phys &P, double *parts, hipStream_t s);
+// dt from the speed partials, time advance, gate (one block)
+hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double min_h, double *scal, double *hist,
                              int advance_t, hipStream_t s);
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
                        double *partials, double *scal, int op, hipStream_t s);
