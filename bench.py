@@ -16,6 +16,14 @@ device synchronisations.  Prints one JSON line:
   cpu_baseline = the CPU restatement of the reference (oracle/liboracle.so,
               OpenMP over the host cores) on a bounded sample of the same
               workload.
+roofline.traffic is the dominant kernel's HBM bytes per launch from the
+committed FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_transonic200k.json,
+copy-calibrated; null when the kernel is not in it).
+
+    python bench.py --copy-calibration
+
+runs only a plain device copy of 512 MiB (the calibration kernel for the
+FETCH_SIZE / WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -29,7 +37,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-KERNEL_NAMES = {"vertex_gather": "o2f::k_vertex_gather<4>", "cell_reconstruct": "o2f::k_cell_reconstruct",
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "traffic_transonic200k.json")
+COPY_BYTES = 512 * 1024 * 1024
+KERNEL_NAMES = {"vertex_gather": "o2f::k_vertex_gather<4, false>", "cell_reconstruct": "o2f::k_cell_reconstruct",
                 "viscous_gather_edge_flux": "o2f::k_viscous_gather_edge_flux",
                 "flux_update": "o2f::k_cell_flux_update"}
 METRIC = "vertex-steps/s"
@@ -47,8 +57,8 @@ def kernel_bytes(desc, nv, ne, nc):
     the kernel reads or writes counted once (int32 ids, FP64 values)."""
     nnz = int(np.sum(desc["cell_nv"]))
     stencil = nv * (1 + 4 + 1) + nnz * (4 + 8)
-    geom_edge = ne * (8 + 1 + 5 * 8)          # 2 vertex ids, tag, len n_x n_y m_x m_y
-    cell_topo = nc * (1 + 16 + 16 + 24)       # info, 4 edge ids, 4 neighbour ids, V c_x c_y
+    geom_edge = ne * (1 + 5 * 8)              # tag, len n_x n_y m_x m_y
+    cell_topo = nc * (1 + 16 + 16 + 32 + 24)  # info, 4 edge ids, 4 neighbour ids, 8 edge-end ids, V c_x c_y
     return {
         "vertex_gather": stencil + 32 * nc + 32 * nv,
         "cell_reconstruct": cell_topo + geom_edge + 32 * nc + 32 * nv + 64 * ne + 40 * nc,
@@ -70,7 +80,7 @@ def step_algorithmic_bytes(nv, ne, nc, k=3):
     return 256 * nc + 3 * G
 
 
-def cpu_baseline(desc, flow, nv, budget_s=12.0):
+def cpu_baseline(desc, flow, nv, budget_s=20.0):
     from oracle import oracle as O
     p = O.params(**flow)
     m = O.Mesh(desc)
@@ -88,6 +98,27 @@ def cpu_baseline(desc, flow, nv, budget_s=12.0):
                       % (n, el)}
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary."""
+    try:
+        with open(TRAFFIC_JSON) as f:
+            k = json.load(f)["kernels"].get(kernel)
+    except (OSError, ValueError, KeyError):
+        return None
+    return None if k is None else k["traffic_bytes"]
+
+
+def copy_calibration():
+    import torch
+    n = COPY_BYTES // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,7 +128,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-steps", type=int, default=200,
                     help="steps of the per-kernel event-timed pass")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the 512 MiB device copy used to calibrate the HBM counters")
     a = ap.parse_args()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the per-mesh path runs on a single "
               "MI355X; multi-GPU sharding is described, not implemented); use --gpus 1",
@@ -152,7 +188,8 @@ def main():
                    "limiter": "venkatakrishnan K=5" if a.limiter == "venkat" else "barth_jespersen",
                    "interp_mode": "mvc", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": KERNEL_NAMES[dom],
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(KERNEL_NAMES[dom]),
+                     "kernel": KERNEL_NAMES[dom],
                      "bytes_per_launch": kb[dom], "avg_launch_us": avg_us[dom]},
         "kernels_avg_us": avg_us,
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes,

@@ -467,9 +467,9 @@ __global__ __launch_bounds__(256) void k_cell_stress(Topo T, const double *__res
 // Flux + update: HLLC on each of the cell's edges and the convective sum
 // (src/advection.cpp:433-464), the viscous divergence
 // (src/diffusion.cpp:145-201), and the Shu-Osher stage update
-// (src/simulation.cpp:159-167), fused.  Each interior edge's HLLC is
-// evaluated by both of its cells (bit-identical), trading 2x flux flops for
-// no flux array and no scatter.
+// (src/simulation.cpp:159-167), fused.  The edge fluxes come from the
+// per-edge HLLC pass; each cell gathers its edges' fluxes (no scatter, no
+// atomics) and sums them in ascending edge order.
 // ------------------------------------------------------------------------
 // |u| + c of one state (src/simulation.cpp:84-95)
 __device__ __forceinline__ double wave_speed(double rho, double ru, double rv, double E, double gamma) {
