@@ -38,6 +38,15 @@ extern "C" {
 
 typedef struct o2f_ctx o2f_ctx;
 
+/* Heat flux of the viscous term.  REF: q = -(mu c_p / Pr) (gamma-1) grad e
+ * as computeDiffusionTerm evaluates it (src/diffusion.cpp:233-240), i.e. R
+ * times Fourier's law; with the reference's dt limit 0.5 min_h^2 / mu this
+ * is unstable once viscosity matters (Re ~ 5000 at chord 1 m).  FOURIER:
+ * q = -(mu c_p / Pr) grad T with T = (gamma-1) e / R, R = c_p - c_p / gamma
+ * (src/params.cpp:5-8); an extension, not reference behaviour. */
+#define O2F_HEAT_FLUX_REF 0
+#define O2F_HEAT_FLUX_FOURIER 1
+
 /* Physical and numerical parameters: the members of SimulationParams that
  * the hot path reads (include/params.hpp:15-73), plus the limiter and
  * interpolation choices the reference hard-codes. */
@@ -55,6 +64,7 @@ typedef struct o2f_params {
     int interp_mode;        /* O2F_INTERP_*                       (REF)      */
     int bc_inflow_outflow_x;/* 1 iff boundary_conditions == "inflow_outflow_x"; 0 makes
                                every operator fail like apply_boundary_conditions */
+    int heat_flux;          /* O2F_HEAT_FLUX_*                    (REF)      */
 } o2f_params;
 
 /* Fills *p with the defaults of SimulationParams (include/params.hpp). */
@@ -208,6 +218,50 @@ int o2f_bicgstab(o2f_ctx *ctx, double beta, double eps, double tol, int chunk, i
 int o2f_shape_gradient(o2f_ctx *ctx, const char *const *names_all, const double *values_all,
                        int n_all, const char *const *opt_names, int n_opt, const double *lambda,
                        double beta, double eps, double *grad);
+
+/* ---- run to convergence --------------------------------------------- */
+
+/* The time loop of src/simulation.cpp:75-124 driven to a steady state
+ * instead of a final time (the reference stops only at final_time): every
+ * check_every steps (steps 0, k, 2k, ...) the density residual
+ * r = sqrt(mean_c R_rho(U^n)^2) / (rho_inf U_inf / c) of the step's first
+ * stage is reduced on the device; a step with r <= tol is completed and the
+ * loop stops after it.  *steps = steps taken, *res = last r evaluated.
+ * tol <= 0 runs max_steps steps.  use_graph: replay check_every steps as one
+ * captured hipGraph.  Device resident, one host synchronisation per group of
+ * steps. */
+int o2f_run_to_residual(o2f_ctx *ctx, double tol, int max_steps, int check_every, int use_graph, int *steps,
+                        double *res);
+
+/* ---- batches of shape candidates -------------------------------------- */
+
+/* A batch advances several contexts (meshes of any sizes, flow conditions
+ * of their own, all on the current device) together: each stage kernel is
+ * ONE launch over all members' workgroup tiles, optionally persistent and
+ * graph-captured.  Every member's results are bit-identical to its own
+ * single-mesh run (same per-entity arithmetic and sum orders); launch shapes
+ * change only speed.  Members must outlive the batch and must not be used
+ * on their own while a batch call runs.  No counterpart in the reference,
+ * which solves one shape per process (run_opti.sh:211-254). */
+typedef struct o2f_batch o2f_batch;
+int o2f_batch_create(o2f_ctx *const *members, int n_members, o2f_batch **out);
+void o2f_batch_destroy(o2f_batch *batch);
+const char *o2f_batch_last_error(const o2f_batch *batch);
+/* Launch shapes of the four batched stage kernels (vertex gather, cell
+ * reconstruction, viscous gather + edge flux, flux + update): shapes[2k] =
+ * threads per workgroup (64, 128 or 256), shapes[2k+1] = persistent grid
+ * size (0: one workgroup per tile). */
+int o2f_batch_set_shapes(o2f_batch *batch, const int *shapes);
+int o2f_batch_get_shapes(const o2f_batch *batch, int *shapes);
+/* Times every shape for each kernel on residual-only passes (the state is
+ * not changed) and keeps the fastest; shapes (8, may be NULL) out. */
+int o2f_batch_autotune(o2f_batch *batch, int reps, int *shapes);
+/* o2f_run_to_residual for every member at once: steps[i], res[i] per member
+ * (may be NULL). */
+int o2f_batch_run(o2f_batch *batch, double tol, int max_steps, int check_every, int use_graph, int *steps,
+                  double *res);
+/* Elapsed milliseconds of nsteps batched steps (tol <= 0, no early stop). */
+int o2f_batch_time_steps(o2f_batch *batch, int nsteps, int use_graph, double *total_ms);
 
 /* ---- measurement ----------------------------------------------------- */
 

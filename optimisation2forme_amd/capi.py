@@ -21,6 +21,7 @@ ERRORS = {-1: "O2F_ERR_ARG", -2: "O2F_ERR_BC", -3: "O2F_ERR_HIP", -4: "O2F_ERR_I
 
 LIMITERS = {"venkat": 0, "venkatakrishnan": 0, "barth_jespersen": 1, "bj": 1}
 INTERP_MODES = {"ref": 0, "mvc": 1}
+HEAT_FLUX = {"reference": 0, "ref": 0, "fourier": 1}
 
 # Every symbol include/o2f.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -31,7 +32,9 @@ EXPORTS = [
     "o2f_set_face_states", "o2f_hllc_flux_batch", "o2f_compute_dt", "o2f_rk3_step",
     "o2f_run_steps", "o2f_time_loop", "o2f_residual_norm", "o2f_lift_drag", "o2f_objective",
     "o2f_adjoint_rhs", "o2f_apply_jt", "o2f_bicgstab", "o2f_shape_gradient", "o2f_time_steps",
-    "o2f_mesh_topology", "o2f_run_until",
+    "o2f_mesh_topology", "o2f_run_until", "o2f_run_to_residual", "o2f_batch_create",
+    "o2f_batch_destroy", "o2f_batch_last_error", "o2f_batch_set_shapes", "o2f_batch_get_shapes",
+    "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps",
 ]
 
 
@@ -41,7 +44,7 @@ class Params(C.Structure):
         ("gamma", C.c_double), ("viscosity", C.c_double), ("Pr", C.c_double), ("cp", C.c_double),
         ("cfl_number", C.c_double), ("rho_ref", C.c_double), ("p_ref", C.c_double),
         ("inflow_velocity", C.c_double), ("venkat_k", C.c_double), ("limiter", C.c_int),
-        ("interp_mode", C.c_int), ("bc_inflow_outflow_x", C.c_int),
+        ("interp_mode", C.c_int), ("bc_inflow_outflow_x", C.c_int), ("heat_flux", C.c_int),
     ]
 
 
@@ -108,6 +111,15 @@ def lib():
             "o2f_time_steps": (C.c_int, [vp, C.c_int, D, D]),
             "o2f_mesh_topology": (C.c_int, [C.POINTER(MeshDesc), I, I, I, C.c_char_p, D, D, I]),
             "o2f_run_until": (C.c_int, [vp, C.c_double, C.c_int, I, D, D, D]),
+            "o2f_run_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
+            "o2f_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
+            "o2f_batch_destroy": (None, [vp]),
+            "o2f_batch_last_error": (C.c_char_p, [vp]),
+            "o2f_batch_set_shapes": (C.c_int, [vp, I]),
+            "o2f_batch_get_shapes": (C.c_int, [vp, I]),
+            "o2f_batch_autotune": (C.c_int, [vp, C.c_int, I]),
+            "o2f_batch_run": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
+            "o2f_batch_time_steps": (C.c_int, [vp, C.c_int, C.c_int, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -127,14 +139,16 @@ def _i(a):
 
 def make_params(gamma=1.4, viscosity=1e-6, Pr=0.71, cp=1005.0, cfl_number=0.1, rho_ref=1.225,
                 p_ref=101325.0, inflow_velocity=1.0, venkat_k=1e-2, limiter="venkat",
-                interp_mode="ref", bc_inflow_outflow_x=True):
+                interp_mode="ref", bc_inflow_outflow_x=True, heat_flux="reference"):
     """o2f_params; defaults are SimulationParams' (include/params.hpp:15-73)."""
     if isinstance(limiter, str):
         limiter = LIMITERS[limiter]
     if isinstance(interp_mode, str):
         interp_mode = INTERP_MODES[interp_mode]
+    if isinstance(heat_flux, str):
+        heat_flux = HEAT_FLUX[heat_flux]
     return Params(gamma, viscosity, Pr, cp, cfl_number, rho_ref, p_ref, inflow_velocity, venkat_k,
-                  int(limiter), int(interp_mode), int(bool(bc_inflow_outflow_x)))
+                  int(limiter), int(interp_mode), int(bool(bc_inflow_outflow_x)), int(heat_flux))
 
 
 def _mesh_desc(desc):
@@ -303,6 +317,14 @@ class Solver:
                                         C.byref(ts), C.byref(dl)))
         return n.value, tt.value, ts.value, dl.value
 
+    def run_to_residual(self, tol, max_steps, check_every=10, use_graph=True):
+        """Steps until the density residual of a checked step is <= tol;
+        returns (steps, last residual)."""
+        n, r = C.c_int(), C.c_double()
+        self._check(lib().o2f_run_to_residual(self.h, tol, max_steps, check_every, int(use_graph),
+                                              C.byref(n), C.byref(r)))
+        return n.value, r.value
+
     def time_loop(self, final_time, max_steps=-1):
         n, t = C.c_int(), C.c_double()
         self._check(lib().o2f_time_loop(self.h, final_time, max_steps, C.byref(n), C.byref(t)))
@@ -370,3 +392,62 @@ def hllc_flux_batch(qL, qR, normal, gamma=1.4):
     if st != O2F_OK:
         raise O2FError(st, lib().o2f_global_error().decode())
     return F
+
+
+class Batch:
+    """o2f_batch: several Solvers advanced together, one launch per stage
+    kernel over all members.  The Solvers must stay alive while the batch
+    exists."""
+
+    KINDS = ("vertex_gather", "cell_reconstruct", "viscous_gather_edge_flux", "flux_update")
+
+    def __init__(self, solvers):
+        L = lib()
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.h.value for s in self.solvers])
+        h = C.c_void_p()
+        st = L.o2f_batch_create(arr, len(self.solvers), C.byref(h))
+        if st != O2F_OK:
+            raise O2FError(st, "o2f_batch_create failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().o2f_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != O2F_OK:
+            raise O2FError(st, lib().o2f_batch_last_error(self.h).decode())
+
+    def shapes(self):
+        a = np.zeros(8, np.int32)
+        self._check(lib().o2f_batch_get_shapes(self.h, _i(a)))
+        return [(int(a[2 * k]), int(a[2 * k + 1])) for k in range(4)]
+
+    def set_shapes(self, shapes):
+        a = np.ascontiguousarray(np.asarray(shapes, np.int32).reshape(8))
+        self._check(lib().o2f_batch_set_shapes(self.h, _i(a)))
+
+    def autotune(self, reps=3):
+        a = np.zeros(8, np.int32)
+        self._check(lib().o2f_batch_autotune(self.h, reps, _i(a)))
+        return [(int(a[2 * k]), int(a[2 * k + 1])) for k in range(4)]
+
+    def run(self, tol, max_steps, check_every=10, use_graph=True):
+        n = len(self.solvers)
+        steps, res = np.zeros(n, np.int32), np.zeros(n)
+        self._check(lib().o2f_batch_run(self.h, tol, max_steps, check_every, int(use_graph), _i(steps),
+                                        _d(res)))
+        return steps, res
+
+    def time_steps(self, nsteps, use_graph=True):
+        ms = C.c_double()
+        self._check(lib().o2f_batch_time_steps(self.h, nsteps, int(use_graph), C.byref(ms)))
+        return ms.value

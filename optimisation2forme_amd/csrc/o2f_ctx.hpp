@@ -1,0 +1,110 @@
+// Internal: the context of one mesh (device arrays, stream, host mesh),
+// shared by the C ABI (o2f_capi.cpp) and the batch (o2f_batch.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "host_mesh.hpp"
+#include "o2f.h"
+#include "o2f_kernels.hpp"
+
+namespace o2f {
+
+template <class T>
+hipError_t dalloc(T *&p, size_t n) {
+    p = nullptr;
+    return hipMalloc(reinterpret_cast<void **>(&p), (n ? n : 1) * sizeof(T));
+}
+
+template <class T>
+hipError_t upload(T *&p, const std::vector<T> &v) {
+    hipError_t e = dalloc(p, v.size());
+    if (e != hipSuccess) return e;
+    if (v.empty()) return hipSuccess;
+    return hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
+struct GeomSet {
+    double *cvol = nullptr, *ccx = nullptr, *ccy = nullptr;
+    double *elen = nullptr, *enx = nullptr, *eny = nullptr, *emx = nullptr, *emy = nullptr;
+    double *sw = nullptr;
+    Geom view() const { return Geom{cvol, ccx, ccy, elen, enx, eny, emx, emy, sw}; }
+    hipError_t alloc(const HostMesh &hm) {
+        hipError_t e;
+        if ((e = dalloc(cvol, hm.nc)) || (e = dalloc(ccx, hm.nc)) || (e = dalloc(ccy, hm.nc)) ||
+            (e = dalloc(elen, hm.ne)) || (e = dalloc(enx, hm.ne)) || (e = dalloc(eny, hm.ne)) ||
+            (e = dalloc(emx, hm.ne)) || (e = dalloc(emy, hm.ne)) || (e = dalloc(sw, hm.sw.size())))
+            return e;
+        return hipSuccess;
+    }
+    hipError_t fill(const HostMesh &hm, hipStream_t s) {
+        auto cp = [&](double *d, const std::vector<double> &v) {
+            return v.empty() ? hipSuccess
+                             : hipMemcpyAsync(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s);
+        };
+        hipError_t e;
+        if ((e = cp(cvol, hm.cvol)) || (e = cp(ccx, hm.ccx)) || (e = cp(ccy, hm.ccy)) || (e = cp(elen, hm.elen)) ||
+            (e = cp(enx, hm.enx)) || (e = cp(eny, hm.eny)) || (e = cp(emx, hm.emx)) || (e = cp(emy, hm.emy)) ||
+            (e = cp(sw, hm.sw)))
+            return e;
+        return hipStreamSynchronize(s);
+    }
+    void release() {
+        for (double *p : {cvol, ccx, ccy, elen, enx, eny, emx, emy, sw})
+            if (p) (void)hipFree(p);
+        *this = GeomSet();
+    }
+};
+
+
+}  // namespace o2f
+
+struct o2f_ctx {
+    o2f::HostMesh hm;
+    o2f_params prm{};
+    o2f::Phys phys{};
+    double q_inf = 0.0;  // 0.5 rho_inf U_inf^2 (src/compute.cpp:96-100)
+    hipStream_t stream = nullptr;
+    std::string err;
+    // topology
+    int *cedge = nullptr, *cnbr = nullptr, *cevx = nullptr, *eva = nullptr, *evb = nullptr, *ecl = nullptr,
+        *ecr = nullptr;
+    int *sptr = nullptr, *scell = nullptr, *wall = nullptr;
+    uint8_t *cinfo = nullptr, *skind = nullptr;
+    char *ebc = nullptr, *vbc = nullptr;
+    o2f::GeomSet geo[3];  // 0: the mesh, 1/2: +/- shape perturbations
+    // state and per-stage intermediates
+    double *U = nullptr, *Uold = nullptr, *R = nullptr, *Gr = nullptr, *tq = nullptr;
+    double *vU = nullptr, *vtq = nullptr, *fL = nullptr, *fR = nullptr, *Fe = nullptr;
+    double *scal = nullptr, *partials = nullptr, *wall_out = nullptr;
+    double *speed = nullptr;              // per-block max_c(|u|+c) partials of the next step's dt
+    bool speed_valid = false;             // it holds the speed of the current U
+    bool have_grad = false;
+    // adjoint
+    double *blk = nullptr;
+    long faces_version = 0, blk_version = -1;
+    double blk_eps = 0.0;
+    double *kv[10] = {};  // BiCGSTAB / gradient work vectors (4 nc each)
+    double *Rp = nullptr, *Rm = nullptr;
+
+    int fail(int code, const std::string &msg) {
+        err = msg;
+        return code;
+    }
+    o2f::Topo topo() const {
+        return o2f::Topo{hm.nv, hm.ne, hm.nc, cedge, cnbr, cevx, cinfo, eva, evb, ecl, ecr, ebc, sptr, scell, skind, vbc};
+    }
+};
+
+#define HIPCHK(ctx, expr)                                                                       \
+    do {                                                                                        \
+        hipError_t e_ = (expr);                                                                 \
+        if (e_ != hipSuccess) return (ctx)->fail(O2F_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+
+// apply_boundary_conditions' check (src/boundary.cpp:84-92) for context c.
+int o2f_ctx_check_bc(o2f_ctx *c);

@@ -240,20 +240,29 @@ __device__ __forceinline__ void edge_flux(const Topo &T, const Geom &G, const do
 // One launch, two independent jobs that both follow the reconstruction:
 // blocks [0, nbv) gather the viscous quantities at the vertices
 // (src/diffusion.cpp:28-140), the other blocks evaluate HLLC once per edge.
+// `b` is the block's index in the mesh's own grid (blockIdx.x for a
+// single-mesh launch, the tile's block for a batched one).
+__device__ __forceinline__ void visc_edge_body(int b, const Topo &T, const Geom &G, const double *__restrict__ tq,
+                                               double *__restrict__ vtq, const double *__restrict__ fL,
+                                               const double *__restrict__ fR, double *__restrict__ F,
+                                               const Phys &P, int nbv, int do_visc) {
+    if (P.gate && *P.gate == 0.0) return;
+    if (b < nbv) {
+        const int v = b * blockDim.x + threadIdx.x;
+        if (do_visc && v < T.nv) gather_vertex<5, true>(T, G, tq, vtq, P, v);
+    } else {
+        const int e = (b - nbv) * blockDim.x + threadIdx.x;
+        if (e < T.ne) edge_flux(T, G, fL, fR, F, P, e);
+    }
+}
+
 __global__ __launch_bounds__(256) void k_viscous_gather_edge_flux(Topo T, Geom G, const double *__restrict__ tq,
                                                                   double *__restrict__ vtq,
                                                                   const double *__restrict__ fL,
                                                                   const double *__restrict__ fR,
                                                                   double *__restrict__ F, Phys P, int nbv,
                                                                   int do_visc) {
-    if (P.gate && *P.gate == 0.0) return;
-    if ((int)blockIdx.x < nbv) {
-        const int v = blockIdx.x * blockDim.x + threadIdx.x;
-        if (do_visc && v < T.nv) gather_vertex<5, true>(T, G, tq, vtq, P, v);
-    } else {
-        const int e = (blockIdx.x - nbv) * blockDim.x + threadIdx.x;
-        if (e < T.ne) edge_flux(T, G, fL, fR, F, P, e);
-    }
+    visc_edge_body(blockIdx.x, T, G, tq, vtq, fL, fR, F, P, nbv, do_visc);
 }
 
 // ------------------------------------------------------------------------
@@ -308,12 +317,12 @@ __device__ __forceinline__ void for_lanes(double v, F &&f) {
     f(3, from_lane<3>(v));
 }
 
-__global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const double *__restrict__ U,
-                                                          const double *__restrict__ vU,
-                                                          double *__restrict__ fL, double *__restrict__ fR,
-                                                          double *__restrict__ tq, double *__restrict__ Gout,
-                                                          Phys P) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const Geom &G,
+                                                 const double *__restrict__ U, const double *__restrict__ vU,
+                                                 double *__restrict__ fL, double *__restrict__ fR,
+                                                 double *__restrict__ tq, double *__restrict__ Gout,
+                                                 const Phys &P) {
+    const int gid = blk * blockDim.x + threadIdx.x;
     const int c = gid / kLanesPerCell, k = gid % kLanesPerCell;
     if (c >= T.nc || (P.gate && *P.gate == 0.0)) return;  // whole lane groups only
     const int nc = T.nc, nv = T.nv, ne = T.ne;
@@ -438,6 +447,14 @@ __global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const 
     tq[4 * (long)nc + c] = -kappa * T_y;
 }
 
+__global__ __launch_bounds__(256) void k_cell_reconstruct(Topo T, Geom G, const double *__restrict__ U,
+                                                          const double *__restrict__ vU,
+                                                          double *__restrict__ fL, double *__restrict__ fR,
+                                                          double *__restrict__ tq, double *__restrict__ Gout,
+                                                          Phys P) {
+    reconstruct_body(blockIdx.x, T, G, U, vU, fL, fR, tq, Gout, P);
+}
+
 // Stress / heat flux from stored gradients (computeDiffusionTerm called on
 // its own, src/diffusion.cpp:223-241).
 __global__ __launch_bounds__(256) void k_cell_stress(Topo T, const double *__restrict__ U,
@@ -482,7 +499,7 @@ __device__ __forceinline__ double wave_speed(double rho, double ru, double rv, d
 
 // Block max of m (NaN ignored like std::max) stored as this block's partial;
 // the step-begin block takes the max of the partials (exact, order-free).
-__device__ __forceinline__ void speed_fold(double m, double *parts) {
+__device__ __forceinline__ void speed_fold(double m, double *parts, int blk) {
     __shared__ double sm[256];
     sm[threadIdx.x] = m;
     __syncthreads();
@@ -490,18 +507,16 @@ __device__ __forceinline__ void speed_fold(double m, double *parts) {
         if (threadIdx.x < s) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + s]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) parts[blockIdx.x] = sm[0];
+    if (threadIdx.x == 0) parts[blk] = sm[0];
+    __syncthreads();  // sm is reused when one workgroup runs several tiles
 }
 
-__global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double *__restrict__ U,
-                                                          double *__restrict__ Uold,
-                                                          const double *__restrict__ Fe,
-                                                          const double *__restrict__ vtq,
-                                                          double *__restrict__ R,
-                                                          const double *__restrict__ dtp, double c0,
-                                                          double c1, int flags, Phys P,
-                                                          double *__restrict__ speed) {
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void flux_update_body(int blk, const Topo &T, const Geom &G, double *__restrict__ U,
+                                                 double *__restrict__ Uold, const double *__restrict__ Fe,
+                                                 const double *__restrict__ vtq, double *__restrict__ R,
+                                                 const double *__restrict__ dtp, double c0, double c1, int flags,
+                                                 const Phys &P, double *__restrict__ speed) {
+    const int gid = blk * blockDim.x + threadIdx.x;
     const int c = gid / kLanesPerCell, k = gid % kLanesPerCell;
     const bool gated = P.gate && *P.gate == 0.0;
     double m = 1e-8;
@@ -619,31 +634,48 @@ __global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double
             }
         }
     }
-    if ((flags & kFlagSpeed) && !gated) speed_fold(m, speed);
+    if ((flags & kFlagSpeed) && !gated) speed_fold(m, speed, blk);
+}
+
+__global__ __launch_bounds__(256) void k_cell_flux_update(Topo T, Geom G, double *__restrict__ U,
+                                                          double *__restrict__ Uold,
+                                                          const double *__restrict__ Fe,
+                                                          const double *__restrict__ vtq,
+                                                          double *__restrict__ R,
+                                                          const double *__restrict__ dtp, double c0,
+                                                          double c1, int flags, Phys P,
+                                                          double *__restrict__ speed) {
+    flux_update_body(blockIdx.x, T, G, U, Uold, Fe, vtq, R, dtp, c0, c1, flags, P, speed);
 }
 
 // Speed partials: per-block max_c(|u|+c) of the current state
 // (src/simulation.cpp:80-96), for the first step of a run or after the state
 // was set from the host (later steps get them from the last stage).
-__global__ __launch_bounds__(256) void k_speed_max(const double *__restrict__ U, int nc, Phys P,
-                                                   double *__restrict__ parts) {
+__device__ __forceinline__ void speed_max_body(int blk, const double *__restrict__ U, int nc, const Phys &P,
+                                               double *__restrict__ parts) {
     // same grid as the 4-lane cell kernels, so both leave the same partials
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gid = blk * blockDim.x + threadIdx.x;
     const int c = gid / kLanesPerCell;
     double m = 1e-8;
     if (c < nc && gid % kLanesPerCell == 0)
         m = smax(m, wave_speed(U[c], U[nc + c], U[2 * nc + c], U[3 * nc + c], P.gamma));
-    speed_fold(m, parts);
+    speed_fold(m, parts, blk);
+}
+
+__global__ __launch_bounds__(256) void k_speed_max(const double *__restrict__ U, int nc, Phys P,
+                                                   double *__restrict__ parts) {
+    speed_max_body(blockIdx.x, U, nc, P, parts);
 }
 
 // Start of a step (one block): max_wave = max of the speed partials, then
 // dt = min(CFL min_h / max_wave, 0.5 min_h^2 / mu) (src/simulation.cpp:108-110;
 // min_h is static, precomputed on the host), the time advance of :121 and,
 // when gated, the while (t < final_time) test of :75.
-__global__ __launch_bounds__(256) void k_step_begin(const double *__restrict__ parts, int nparts, Phys P,
-                                                    double min_h, double *__restrict__ scal,
-                                                    double *__restrict__ hist, int advance_t) {
+__device__ __forceinline__ void step_begin_body(const double *__restrict__ parts, int nparts, const Phys &P,
+                                                double min_h, double *__restrict__ scal,
+                                                double *__restrict__ hist, int advance_t) {
     __shared__ double sm[256];
+    if (P.gate && *P.gate == 0.0) return;  // uniform: read before any thread writes it
     double m = 1e-8;
     for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = smax(m, parts[b]);
     sm[threadIdx.x] = m;
@@ -653,7 +685,7 @@ __global__ __launch_bounds__(256) void k_step_begin(const double *__restrict__ p
         __syncthreads();
     }
     if (threadIdx.x != 0) return;
-    if (P.gate && !(scal[kScalT] < P.final_t)) {
+    if (P.gate && (!(scal[kScalT] < P.final_t) || scal[kScalConv] != 0.0)) {
         scal[kScalActive] = 0.0;
         return;
     }
@@ -668,6 +700,12 @@ __global__ __launch_bounds__(256) void k_step_begin(const double *__restrict__ p
     }
     if (P.gate) scal[kScalSteps] = scal[kScalSteps] + 1.0;
     if (hist) *hist = dt;
+}
+
+__global__ __launch_bounds__(256) void k_step_begin(const double *__restrict__ parts, int nparts, Phys P,
+                                                    double min_h, double *__restrict__ scal,
+                                                    double *__restrict__ hist, int advance_t) {
+    step_begin_body(parts, nparts, P, min_h, scal, hist, advance_t);
 }
 
 // Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i): per-block
@@ -980,6 +1018,111 @@ __global__ __launch_bounds__(256) void k_hllc_batch(const double *__restrict__ q
 }
 
 // ------------------------------------------------------------------------
+// Batched stage kernels (one launch for every member of a batch)
+// ------------------------------------------------------------------------
+template <class F>
+__device__ __forceinline__ void each_tile(const int2 *__restrict__ tiles, int ntiles, F &&f) {
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int2 tl = tiles[t];
+        f(tl.x, tl.y);
+    }
+}
+
+__global__ __launch_bounds__(256) void kb_speed_max(const Member *__restrict__ M, const int2 *__restrict__ tiles,
+                                                    int ntiles) {
+    each_tile(tiles, ntiles, [&](int m, int b) {
+        const Member &A = M[m];
+        speed_max_body(b, A.U, A.T.nc, A.P, A.speed);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_step_begin(const Member *__restrict__ M) {
+    const Member &A = M[blockIdx.x];
+    step_begin_body(A.speed, A.nparts, A.P, A.min_h, A.scal, nullptr, 1);
+}
+
+__global__ __launch_bounds__(256) void kb_vertex_gather(const Member *__restrict__ M,
+                                                        const int2 *__restrict__ tiles, int ntiles) {
+    each_tile(tiles, ntiles, [&](int m, int b) {
+        const Member &A = M[m];
+        const int v = b * blockDim.x + threadIdx.x;
+        if (v < A.T.nv && *A.P.gate != 0.0) gather_vertex<4, false>(A.T, A.G, A.U, A.vU, A.P, v);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_reconstruct(const Member *__restrict__ M, const int2 *__restrict__ tiles,
+                                                      int ntiles) {
+    each_tile(tiles, ntiles, [&](int m, int b) {
+        const Member &A = M[m];
+        reconstruct_body(b, A.T, A.G, A.U, A.vU, A.fL, A.fR, A.tq, nullptr, A.P);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_visc_edge(const Member *__restrict__ M, const int2 *__restrict__ tiles,
+                                                    int ntiles) {
+    each_tile(tiles, ntiles, [&](int m, int b) {
+        const Member &A = M[m];
+        const int nbv = (A.T.nv + (int)blockDim.x - 1) / (int)blockDim.x;
+        visc_edge_body(b, A.T, A.G, A.tq, A.vtq, A.fL, A.fR, A.Fe, A.P, nbv, 1);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_flux_update(const Member *__restrict__ M, const int2 *__restrict__ tiles,
+                                                      int ntiles, double c0, double c1, int flags) {
+    each_tile(tiles, ntiles, [&](int m, int b) {
+        const Member &A = M[m];
+        flux_update_body(b, A.T, A.G, A.U, A.Uold, A.Fe, A.vtq, A.R, A.scal + kScalDt, c0, c1, flags, A.P,
+                         A.speed);
+    });
+}
+
+// Sum of R_rho^2 over cells [chunk kResChunk, +kResChunk): thread t adds
+// cells t, t + 256, ... in order, then a fixed tree.
+__global__ __launch_bounds__(256) void kb_residual_partial(const Member *__restrict__ M,
+                                                           const int2 *__restrict__ tiles, int ntiles) {
+    __shared__ double sm[256];
+    each_tile(tiles, ntiles, [&](int m, int chunk) {
+        const Member &A = M[m];
+        if (*A.P.gate == 0.0) return;  // uniform per tile
+        const int nc = A.T.nc, c0 = chunk * kResChunk, c1 = min(nc, c0 + kResChunk);
+        double acc = 0.0;
+        for (int c = c0 + threadIdx.x; c < c1; c += 256) acc += A.R[c] * A.R[c];
+        sm[threadIdx.x] = acc;
+        __syncthreads();
+        for (int st = 128; st > 0; st >>= 1) {
+            if (threadIdx.x < st) sm[threadIdx.x] += sm[threadIdx.x + st];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) A.rparts[chunk] = sm[0];
+        __syncthreads();
+    });
+}
+
+// r = sqrt(mean_c R_rho^2) / (rho_inf U_inf / c), one block per member.
+__global__ __launch_bounds__(256) void kb_residual_final(const Member *__restrict__ M) {
+    __shared__ double sm[256];
+    const Member &A = M[blockIdx.x];
+    if (*A.P.gate == 0.0) return;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < A.nrparts; k += 256) acc += A.rparts[k];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sm[threadIdx.x] += sm[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double r = sqrt(sm[0] / A.T.nc) / A.res_scale;
+        A.scal[kScalRes] = r;
+        if (r <= A.res_tol) A.scal[kScalConv] = 1.0;
+    }
+}
+
+__global__ __launch_bounds__(64) void kb_collect(const Member *__restrict__ M, int n, double *__restrict__ flags) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) flags[i] = M[i].scal[kScalActive];
+}
+
+// ------------------------------------------------------------------------
 // Launch helpers
 // ------------------------------------------------------------------------
 static inline unsigned nblk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -1090,4 +1233,47 @@ hipError_t launch_hllc_batch(const double *qL, const double *qR, const double *n
     return hipGetLastError();
 }
 
+static inline dim3 tile_grid(int ntiles, TileShape sh) {
+    return dim3((unsigned)std::max(1, sh.grid > 0 ? std::min(sh.grid, ntiles) : ntiles));
+}
+hipError_t launch_b_speed_max(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s) {
+    hipLaunchKernelGGL(kb_speed_max, tile_grid(ntiles, sh), dim3(sh.block), 0, s, M, tiles, ntiles);
+    return hipGetLastError();
+}
+hipError_t launch_b_step_begin(const Member *M, int nmembers, hipStream_t s) {
+    hipLaunchKernelGGL(kb_step_begin, dim3(nmembers), dim3(256), 0, s, M);
+    return hipGetLastError();
+}
+hipError_t launch_b_vertex_gather(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s) {
+    hipLaunchKernelGGL(kb_vertex_gather, tile_grid(ntiles, sh), dim3(sh.block), 0, s, M, tiles, ntiles);
+    return hipGetLastError();
+}
+hipError_t launch_b_reconstruct(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s) {
+    hipLaunchKernelGGL(kb_reconstruct, tile_grid(ntiles, sh), dim3(sh.block), 0, s, M, tiles, ntiles);
+    return hipGetLastError();
+}
+hipError_t launch_b_visc_edge(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s) {
+    hipLaunchKernelGGL(kb_visc_edge, tile_grid(ntiles, sh), dim3(sh.block), 0, s, M, tiles, ntiles);
+    return hipGetLastError();
+}
+hipError_t launch_b_flux_update(const Member *M, const int2 *tiles, int ntiles, TileShape sh, double c0, double c1,
+                                int flags, hipStream_t s) {
+    hipLaunchKernelGGL(kb_flux_update, tile_grid(ntiles, sh), dim3(sh.block), 0, s, M, tiles, ntiles, c0, c1,
+                       flags);
+    return hipGetLastError();
+}
+hipError_t launch_b_residual(const Member *M, const int2 *chunk_tiles, int nchunks, int nmembers, hipStream_t s) {
+    hipLaunchKernelGGL(kb_residual_partial, dim3((unsigned)std::max(1, nchunks)), dim3(256), 0, s, M, chunk_tiles,
+                       nchunks);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(kb_residual_final, dim3(nmembers), dim3(256), 0, s, M);
+    return hipGetLastError();
+}
+hipError_t launch_b_collect(const Member *M, int nmembers, double *flags, hipStream_t s) {
+    hipLaunchKernelGGL(kb_collect, dim3(1), dim3(64), 0, s, M, nmembers, flags);
+    return hipGetLastError();
+}
+
 }  // namespace o2f
+

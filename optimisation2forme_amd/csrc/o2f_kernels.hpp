@@ -39,6 +39,8 @@ enum : int {
     kScalActive = 10,  // 1 while t < final_time (gated time loop), else 0
     kScalSteps = 11,   // steps taken by the gated time loop
     kScalTPrev = 12,   // t at the start of the last step
+    kScalConv = 13,    // set once the density residual met the tolerance: the next step stops
+    kScalRes = 14,     // last density residual norm evaluated on the device
     kScalCount = 16,
 };
 
@@ -77,6 +79,54 @@ struct Geom {
     const double *elen, *enx, *eny, *emx, *emy;
     const double *sw;
 };
+
+// ------------------------------------------------------------------------
+// Batched launches: one launch of a stage kernel covers every member of a
+// batch (meshes of any sizes).  Each member's kernels read its own entry of
+// a device-resident Member table; a launch walks a tile list of (member,
+// block of that member's own grid) pairs, so every member sees exactly the
+// per-entity arithmetic of its single-mesh launch.
+// ------------------------------------------------------------------------
+struct Member {
+    Topo T;
+    Geom G;
+    Phys P;  // P.gate = scal + kScalActive; P.final_t = +inf
+    double *U, *Uold, *R, *tq, *vU, *vtq, *fL, *fR, *Fe;
+    double *scal;    // the member's scalar block
+    double *speed;   // speed partials (one per flux-update block)
+    double *rparts;  // density-residual partials (one per kResChunk cells)
+    double min_h;
+    double res_scale;  // rho_inf U_inf / chord
+    double res_tol;    // stop once r <= res_tol (r from the last checked step)
+    int nparts;        // speed partials in use (flux-update grid of the member)
+    int nrparts;       // residual partials
+};
+
+constexpr int kResChunk = 2048;  // cells per residual partial (fixed: results do not depend on launch shapes)
+
+// Launch shape of one batched kernel: threads per workgroup (64, 128 or
+// 256) and grid (0: one workgroup per tile; G > 0: G persistent workgroups
+// striding over the tiles).
+struct TileShape {
+    int block;
+    int grid;
+};
+
+// Tile lists are int2 (member, block).  Stage flags as for the single-mesh
+// kernels; the dt of each member is its own scal[kScalDt].
+hipError_t launch_b_speed_max(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s);
+hipError_t launch_b_step_begin(const Member *M, int nmembers, hipStream_t s);
+hipError_t launch_b_vertex_gather(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s);
+hipError_t launch_b_reconstruct(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s);
+hipError_t launch_b_visc_edge(const Member *M, const int2 *tiles, int ntiles, TileShape sh, hipStream_t s);
+hipError_t launch_b_flux_update(const Member *M, const int2 *tiles, int ntiles, TileShape sh, double c0, double c1,
+                                int flags, hipStream_t s);
+// Density residual of the R stored by the last StoreR stage: partials over
+// fixed chunks of kResChunk cells, then one block per member sums them in
+// order, stores r in scal[kScalRes] and sets scal[kScalConv] when r <= tol.
+hipError_t launch_b_residual(const Member *M, const int2 *chunk_tiles, int nchunks, int nmembers, hipStream_t s);
+// flags[i] = member i still active
+hipError_t launch_b_collect(const Member *M, int nmembers, double *flags, hipStream_t s);
 
 hipError_t launch_vertex_state(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
                                hipStream_t s);

@@ -57,6 +57,7 @@ struct SimulationParams {
     std::string limiter = "venkat";   // "venkat" | "barth_jespersen"
     double venkat_K = 1e-2;           // k_eps of computeVenkatAlpha
     std::string interp_mode = "ref";  // "ref" | "mvc" (valence >= 5 vertices)
+    std::string heat_flux = "reference";  // "reference" (R x Fourier) | "fourier" (extension)
     double adjoint_tol = 1e-6;        // BiCGSTAB tolerance
     double adjoint_eps = 1e-6;        // FD step of the flux blocks and the shape partials
     int adjoint_chunk = 500;          // BiCGSTAB restart period

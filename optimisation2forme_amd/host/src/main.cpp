@@ -36,7 +36,8 @@ int main(int argc, char *argv[]) {
                   << "\nstate_load_path=" << params.state_load_path << "\nstate_save_path=" << params.state_save_path
                   << "\nshape_opt_enable=" << params.shape_opt_enable << "\nshape_param_file=" << params.shape_param_file
                   << "\nlimiter=" << params.limiter << "\nvenkat_K=" << params.venkat_K
-                  << "\ninterp_mode=" << params.interp_mode << "\ncv=" << params.cv << "\nR=" << params.R << "\n";
+                  << "\ninterp_mode=" << params.interp_mode << "\nheat_flux=" << params.heat_flux << "\ncv=" << params.cv
+                  << "\nR=" << params.R << "\n";
         return 0;
     }
     if (params.verbosity > 0) print_params(params);

@@ -69,6 +69,7 @@ bool load_config(const std::string &filename, SimulationParams &p) {
         else if (key == "limiter") p.limiter = val;
         else if (key == "venkat_K") ss >> p.venkat_K;
         else if (key == "interp_mode") p.interp_mode = val;
+        else if (key == "heat_flux") p.heat_flux = val;
         else if (key == "adjoint_tol") ss >> p.adjoint_tol;
         else if (key == "adjoint_eps") ss >> p.adjoint_eps;
         else if (key == "adjoint_chunk") ss >> p.adjoint_chunk;
@@ -92,6 +93,7 @@ void print_params(const SimulationParams &p) {
               << " Init:          " << p.initial_condition << "\n"
               << " Limiter:       " << p.limiter << " (K=" << p.venkat_K << ")\n"
               << " Interpolation: " << p.interp_mode << "\n"
+              << " Heat flux:     " << p.heat_flux << "\n"
               << " Save every:    " << p.save_interval << "\n"
               << " Final time:    " << p.final_time << "\n"
               << "=============================\n"
@@ -112,6 +114,7 @@ o2f_params to_o2f_params(const SimulationParams &p) {
     o.venkat_k = p.venkat_K;
     o.limiter = (p.limiter == "barth_jespersen" || p.limiter == "bj") ? O2F_LIMITER_BJ : O2F_LIMITER_VENKAT;
     o.interp_mode = p.interp_mode == "mvc" ? O2F_INTERP_MVC : O2F_INTERP_REF;
+    o.heat_flux = p.heat_flux == "fourier" ? O2F_HEAT_FLUX_FOURIER : O2F_HEAT_FLUX_REF;
     o.bc_inflow_outflow_x = p.boundary_conditions == "inflow_outflow_x" ? 1 : 0;
     return o;
 }

@@ -53,6 +53,7 @@ typedef struct {
     int limiter;     /* 0 Venkatakrishnan, 1 Barth-Jespersen */
     int interp_mode; /* 0 reference (valence >= 5 never written), 1 mean-value */
     int bc_known;    /* boundary_conditions == "inflow_outflow_x" */
+    int heat_flux;   /* 0: reference (gamma-1) grad e; 1: grad T, T = (gamma-1) e / R (extension) */
 } orc_params;
 
 typedef struct {
@@ -875,7 +876,8 @@ int orc_advection(orc_mesh *m, const orc_params *p) {
 int orc_diffusion(orc_mesh *m, const orc_params *p) {
     const double mu = p->viscosity;
     const double gamma = p->gamma;
-    const double kappa = p->viscosity * p->cp / p->Pr;
+    double kappa = p->viscosity * p->cp / p->Pr;
+    if (p->heat_flux == 1) kappa = kappa / (p->cp - p->cp / p->gamma);
     double **tq = m->ctq; /* tau_xx, tau_yy, tau_xy, q_x, q_y */
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < m->nc; ++c) {
@@ -1011,6 +1013,31 @@ double orc_residual_norm(const orc_mesh *m, const orc_params *p) {
     double a_inf = sqrt(p->gamma * p->p_ref / p->rho_ref);
     double scale = p->rho_ref * (p->mach * a_inf) / 1.0;
     return sqrt(s / m->nc) / scale;
+}
+
+/* The time loop run to a steady state instead of a final time (no
+ * counterpart in the reference, whose loop stops at final_time,
+ * src/simulation.cpp:75): steps n = 0, k, 2k, ... evaluate the density
+ * residual norm of R(U^n) (the first stage's residual); a step whose norm
+ * is <= tol completes and the loop stops after it.  tol <= 0: max_steps. */
+int orc_run_to_residual(orc_mesh *m, const orc_params *p, double tol, int max_steps, int check_every,
+                        int *steps, double *res) {
+    double r = -1.0;
+    int n = 0;
+    while (n < max_steps) {
+        const int check = tol > 0.0 && n % check_every == 0;
+        const double dt = orc_dt(m, p);
+        for (int q = 0; q < 4; ++q) memcpy(m->Uold[q], m->U[q], sizeof(double) * m->nc);
+        if (rk3_substep(m, p, 0.0, 1.0, dt)) return -1;
+        if (check) r = orc_residual_norm(m, p);
+        if (rk3_substep(m, p, 3.0 / 4.0, 1.0 / 4.0, dt)) return -1;
+        if (rk3_substep(m, p, 1.0 / 3.0, 2.0 / 3.0, dt)) return -1;
+        ++n;
+        if (check && r <= tol) break;
+    }
+    *steps = n;
+    *res = r;
+    return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -21,15 +21,16 @@ class Params(C.Structure):
         ("gamma", C.c_double), ("viscosity", C.c_double), ("Pr", C.c_double), ("cp", C.c_double),
         ("cfl", C.c_double), ("rho_ref", C.c_double), ("p_ref", C.c_double), ("mach", C.c_double),
         ("venkat_k", C.c_double), ("limiter", C.c_int), ("interp_mode", C.c_int),
-        ("bc_known", C.c_int),
+        ("bc_known", C.c_int), ("heat_flux", C.c_int),
     ]
 
 
 def params(gamma=1.4, viscosity=1e-6, Pr=0.71, cp=1005.0, cfl=0.1, rho_ref=1.225,
-           p_ref=101325.0, mach=1.0, venkat_k=1e-2, limiter=0, interp_mode=0, bc_known=1):
+           p_ref=101325.0, mach=1.0, venkat_k=1e-2, limiter=0, interp_mode=0, bc_known=1,
+           heat_flux=0):
     """Defaults are SimulationParams' (include/params.hpp:15-73)."""
     return Params(gamma, viscosity, Pr, cp, cfl, rho_ref, p_ref, mach, venkat_k, limiter,
-                  interp_mode, bc_known)
+                  interp_mode, bc_known, heat_flux)
 
 
 def lib():
@@ -66,6 +67,8 @@ def lib():
         L.orc_run_steps.restype = C.c_int
         L.orc_residual_norm.argtypes = [vp, P]
         L.orc_residual_norm.restype = C.c_double
+        L.orc_run_to_residual.argtypes = [vp, P, C.c_double, C.c_int, C.c_int, I, D]
+        L.orc_run_to_residual.restype = C.c_int
         L.orc_lift_drag.argtypes = [vp, P, D, D]
         L.orc_objective.argtypes = [vp, P, C.c_double]
         L.orc_objective.restype = C.c_double
@@ -217,6 +220,13 @@ class Mesh:
 
     def residual_norm(self, p):
         return lib().orc_residual_norm(self.h, C.byref(p))
+
+    def run_to_residual(self, p, tol, max_steps, check_every=10):
+        n, r = C.c_int(), C.c_double()
+        if lib().orc_run_to_residual(self.h, C.byref(p), tol, max_steps, check_every, C.byref(n),
+                                     C.byref(r)):
+            raise ValueError("Unknown boundary condition")
+        return n.value, r.value
 
     def lift_drag(self, p):
         cl, cd = C.c_double(), C.c_double()

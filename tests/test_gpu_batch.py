@@ -1,0 +1,164 @@
+"""Batched shape candidates and the run-to-residual loop on the GPU.
+
+Bar: every member of a batch (meshes of different sizes, flows of their
+own) ends bit-identical to its own single-mesh run, whatever the launch
+shapes (threads per workgroup, persistent grid), with or without hipGraph
+replay; the autotuner may change only speed.  The run-to-residual loop
+stops at the same step as the oracle's and leaves the same state.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from optimisation2forme_amd import capi, mesher
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print("\n[gpu case] %s" % request.node.nodeid, flush=True)
+    sys.stdout.flush()
+    yield
+
+
+# Three candidates: different NACA shapes, sizes and flow conditions.
+CANDIDATES = [
+    dict(m=0.0, p=0.4, t=0.12, n_around=40, n_radial=14, aoa=2.0, mach=0.6),
+    dict(m=0.03, p=0.45, t=0.11, n_around=72, n_radial=26, aoa=1.0, mach=0.85),
+    dict(m=0.05, p=0.3, t=0.14, n_around=56, n_radial=40, aoa=3.5, mach=0.5),
+]
+
+
+def candidate(k, limiter=0):
+    c = CANDIDATES[k]
+    desc = mesher.naca_mesh(m=c["m"], p=c["p"], t=c["t"], n_around=c["n_around"], n_radial=c["n_radial"],
+                            wall_spacing=4e-3, radius=20.0, aoa_deg=c["aoa"])
+    # Fourier heat flux: the reference's (R x Fourier) diverges at Re 5000
+    flow = dict(viscosity=H.re_viscosity(c["mach"]), Pr=0.72, cfl=0.5, mach=c["mach"], venkat_k=5.0,
+                limiter=limiter, interp_mode=1, heat_flux=1)
+    return desc, flow
+
+
+def solvers(limiter=0):
+    out = []
+    for k in range(len(CANDIDATES)):
+        desc, flow = candidate(k, limiter)
+        s = capi.Solver(desc, H.capi_params(flow))
+        s.init_freestream()
+        out.append(s)
+    return out
+
+
+def single_states(nsteps, limiter=0):
+    out = []
+    for s in solvers(limiter):
+        s.run_steps(nsteps)
+        out.append(s.state())
+    return out
+
+
+SHAPES = {
+    "default": [(256, 0)] * 4,
+    "small_blocks_persistent": [(64, 1024), (64, 2048), (128, 1024), (64, 512)],
+    "mixed": [(128, 0), (256, 1024), (64, 0), (128, 2048)],
+}
+
+
+@pytest.mark.parametrize("shape", sorted(SHAPES))
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_batch_members_bit_identical_to_single_runs(shape, use_graph):
+    n = 23  # two graph replays of 10 steps and a tail of 3
+    ref = single_states(n)
+    ss = solvers()
+    b = capi.Batch(ss)
+    b.set_shapes(SHAPES[shape])
+    assert b.shapes() == SHAPES[shape]
+    steps, res = b.run(-1.0, n, check_every=10, use_graph=use_graph)
+    assert list(steps) == [n] * len(ss)
+    for k, s in enumerate(ss):
+        assert np.array_equal(s.state(), ref[k]), (shape, use_graph, k)
+    b.close()
+
+
+def test_batch_barth_jespersen_and_consecutive_runs():
+    ref = single_states(30, limiter=1)
+    ss = solvers(limiter=1)
+    b = capi.Batch(ss)
+    b.set_shapes(SHAPES["mixed"])
+    b.run(-1.0, 12, check_every=4, use_graph=True)
+    b.run(-1.0, 18, check_every=6, use_graph=True)   # a new graph, same trajectory
+    for k, s in enumerate(ss):
+        assert np.array_equal(s.state(), ref[k]), k
+
+
+def test_autotuner_changes_only_speed():
+    ref = single_states(15)
+    ss = solvers()
+    b = capi.Batch(ss)
+    shapes = b.autotune(reps=2)
+    assert len(shapes) == 4
+    for blk, grid in shapes:
+        assert blk in (64, 128, 256) and grid >= 0
+    U0 = [s.state() for s in ss]
+    for k, s in enumerate(ss):   # residual-only timing passes leave the state alone
+        s2 = capi.Solver(*[candidate(k)[0], H.capi_params(candidate(k)[1])])
+        s2.init_freestream()
+        assert np.array_equal(U0[k], s2.state())
+    b.run(-1.0, 15, check_every=5, use_graph=True)
+    for k, s in enumerate(ss):
+        assert np.array_equal(s.state(), ref[k]), k
+
+
+def test_fourier_heat_flux_residual_and_steps_bit_identical():
+    desc, flow = candidate(1)
+    p = H.oracle_params(flow)
+    m = O.Mesh(desc)
+    m.init_freestream(p)
+    m.run_steps(p, 5)
+    s = capi.Solver(desc, H.capi_params(flow))
+    s.set_state(m.state())
+    assert np.array_equal(s.residual(), m.residual(p))
+    s.run_steps(4)
+    m.run_steps(p, 4)
+    assert H.rel_err(s.state(), m.state()) <= 1e-10
+
+
+def test_run_to_residual_matches_the_oracle():
+    desc, flow = candidate(0)
+    p = H.oracle_params(flow)
+    m = O.Mesh(desc)
+    m.init_freestream(p)
+    _, r0 = m.run_to_residual(p, 1e30, 5, 1)      # stops after step 0: r(U^0)
+    m.init_freestream(p)
+    tol = 0.6 * r0
+    n_cpu, r_cpu = m.run_to_residual(p, tol, 400, 5)
+    assert 1 < n_cpu < 400, (n_cpu, r_cpu, r0)
+    s = capi.Solver(desc, H.capi_params(flow))
+    s.init_freestream()
+    n_gpu, r_gpu = s.run_to_residual(tol, 400, check_every=5, use_graph=True)
+    assert n_gpu == n_cpu
+    assert abs(r_gpu - r_cpu) <= 1e-12 * r_cpu
+    assert H.rel_err(s.state(), m.state()) <= 1e-10
+
+
+def test_batch_members_stop_independently():
+    ss = solvers()
+    b = capi.Batch(ss)
+    # r at the cold start of every member, then a tolerance each reaches at its own step
+    steps1, r0 = b.run(1e30, 3, check_every=1, use_graph=False)
+    assert list(steps1) == [1, 1, 1]
+    for s in ss:
+        s.init_freestream()
+    tol = float(np.min(r0)) * 0.7
+    steps, res = b.run(tol, 300, check_every=3, use_graph=True)
+    for k, s in enumerate(ss):
+        ref = capi.Solver(candidate(k)[0], H.capi_params(candidate(k)[1]))
+        ref.init_freestream()
+        ref.run_steps(int(steps[k]))
+        assert np.array_equal(s.state(), ref.state()), k
+        if steps[k] < 300:
+            assert res[k] <= tol

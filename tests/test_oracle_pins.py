@@ -453,3 +453,52 @@ def test_shape_gradient_sign_matches_the_shipped_trajectory():
     gp = m.shape_gradient(p, lam, "p", m0=0.0, p0=0.4, t=0.12, beta=50.0)
     assert gm < 0
     assert gp == 0.0
+
+
+def test_run_to_residual_reduces_to_run_steps_and_the_residual_norm():
+    """tol <= 0 is exactly run_steps(max_steps); tol = inf stops after the
+    first step and reports r(U^0), the norm of the first stage's residual,
+    i.e. of a plain residual evaluation at the start state."""
+    desc = H.tiny_mesh()
+    flow = H.subsonic_small()
+    p = H.oracle_params(flow)
+    a, b = O.Mesh(desc), O.Mesh(desc)
+    a.init_freestream(p)
+    b.init_freestream(p)
+    n, r = a.run_to_residual(p, -1.0, 7, 3)
+    b.run_steps(p, 7)
+    assert n == 7 and r == -1.0
+    assert np.array_equal(a.state(), b.state())
+    c = O.Mesh(desc)
+    c.set_state(a.state())
+    c.residual(p)
+    r_expect = c.residual_norm(p)
+    n, r = a.run_to_residual(p, 1e300, 50, 1)
+    assert n == 1 and r == r_expect
+    b.run_steps(p, 1)
+    assert np.array_equal(a.state(), b.state())
+
+
+def test_fourier_heat_flux_is_the_reference_flux_over_R():
+    """At zero cell velocity the viscous energy residual is the heat-flux
+    divergence alone (the work terms use the cell-centre velocity): the
+    Fourier mode's is the reference mode's divided by the gas constant
+    R = c_p - c_p/gamma.  The stresses (boundary vertices carry the
+    freestream velocity) do not depend on the mode."""
+    desc = H.tiny_mesh()
+    m = O.Mesh(desc)
+    rng = np.random.default_rng(3)
+    nc = len(desc["cell_nv"])
+    rho = 1.225 * (1.0 + 0.05 * rng.standard_normal(nc))
+    E = 101325.0 / 0.4 * (1.0 + 0.05 * rng.standard_normal(nc))
+    U = np.array([rho, np.zeros(nc), np.zeros(nc), E])
+    out = {}
+    for hf in (0, 1):
+        p = O.params(viscosity=1e3, Pr=0.72, cfl=0.5, mach=0.5, interp_mode=1, heat_flux=hf)
+        m.set_state(U)
+        adv = m.advection(p).copy()
+        out[hf] = m.diffusion(p) - adv
+    Rgas = 1005.0 - 1005.0 / 1.4
+    assert np.array_equal(out[0][1:3], out[1][1:3])
+    assert np.max(np.abs(out[0][3])) > 0.0
+    np.testing.assert_allclose(out[1][3], out[0][3] / Rgas, rtol=1e-9, atol=1e-12 * np.max(np.abs(out[0][3])))
