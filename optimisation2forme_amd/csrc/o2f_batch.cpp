@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <utility>
 #include <string>
 #include <vector>
 
@@ -56,10 +58,8 @@ struct o2f_batch {
     int nrtiles = 0;
     double *dflags = nullptr, *hflags = nullptr;  // device / pinned host copies of the active flags
     std::vector<double> scal_init;                // host source of the per-run scalar reset
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int graph_k = 0;
-    bool graph_check = false;
+    // captured step chunks, keyed by (steps, check on the first step)
+    std::map<std::pair<int, bool>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
 
     int n() const { return static_cast<int>(ctx.size()); }
     int fail(int code, const std::string &m) {
@@ -67,11 +67,11 @@ struct o2f_batch {
         return code;
     }
     void drop_graph() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        exec = nullptr;
-        graph = nullptr;
-        graph_k = 0;
+        for (auto &kv : graphs) {
+            (void)hipGraphExecDestroy(kv.second.second);
+            (void)hipGraphDestroy(kv.second.first);
+        }
+        graphs.clear();
     }
 };
 
@@ -191,9 +191,17 @@ int enqueue_group(o2f_batch *b, int k, bool check) {
     return O2F_OK;
 }
 
-int ensure_graph(o2f_batch *b, int k, bool check) {
-    if (b->exec && b->graph_k == k && b->graph_check == check) return O2F_OK;
-    b->drop_graph();
+// At most this many steps are captured in one graph (a step is 14 kernel
+// nodes); longer groups replay several chunks.
+constexpr int kGraphSteps = 16;
+
+int get_graph(o2f_batch *b, int k, bool check, hipGraphExec_t *exec) {
+    const auto key = std::make_pair(k, check);
+    auto it = b->graphs.find(key);
+    if (it != b->graphs.end()) {
+        *exec = it->second.second;
+        return O2F_OK;
+    }
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     const int st = enqueue_group(b, k, check);
     hipGraph_t g = nullptr;
@@ -203,10 +211,42 @@ int ensure_graph(o2f_batch *b, int k, bool check) {
         return st;
     }
     BCHK(b, e);
-    b->graph = g;
-    BCHK(b, hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
-    b->graph_k = k;
-    b->graph_check = check;
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        BCHK(b, ei);
+    }
+    b->graphs[key] = std::make_pair(g, x);
+    *exec = x;
+    return O2F_OK;
+}
+
+// A group of k steps, the first one checked: chunks of at most kGraphSteps.
+std::vector<std::pair<int, bool>> chunks_of(int k, bool check) {
+    std::vector<std::pair<int, bool>> v;
+    int left = k;
+    bool first = true;
+    while (left > 0) {
+        const int n = std::min(left, kGraphSteps);
+        v.emplace_back(n, check && first);
+        first = false;
+        left -= n;
+    }
+    return v;
+}
+
+int enqueue_chunks(o2f_batch *b, const std::vector<std::pair<int, bool>> &plan, bool use_graph) {
+    int st;
+    for (const auto &c : plan) {
+        if (use_graph) {
+            hipGraphExec_t x = nullptr;
+            if ((st = get_graph(b, c.first, c.second, &x))) return st;
+            BCHK(b, hipGraphLaunch(x, b->stream));
+        } else if ((st = enqueue_group(b, c.first, c.second))) {
+            return st;
+        }
+    }
     return O2F_OK;
 }
 
@@ -244,8 +284,13 @@ int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph,
     const bool check = tol > 0.0;
     const int k = check_every;
     const int groups = max_steps / k, tail = max_steps % k;
-    // capture before any work is queued on the stream
-    if (groups > 0 && use_graph && (st = ensure_graph(b, k, check))) return st;
+    const auto plan = chunks_of(k, check), tail_plan = chunks_of(tail, check);
+    if (use_graph) {  // capture before any work is queued on the stream
+        hipGraphExec_t x;
+        for (const auto *p : {&plan, &tail_plan})
+            for (const auto &c : *p)
+                if ((st = get_graph(b, c.first, c.second, &x))) return st;
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (elapsed_ms) {
         BCHK(b, hipEventCreate(&e0));
@@ -257,16 +302,13 @@ int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph,
     const int per_sync = std::max(1, 64 / k);
     bool live = true;
     for (int g = 0; g < groups && live; ++g) {
-        if (use_graph)
-            BCHK(b, hipGraphLaunch(b->exec, b->stream));
-        else if ((st = enqueue_group(b, k, check)))
-            return st;
+        if ((st = enqueue_chunks(b, plan, use_graph))) return st;
         if (check && (g + 1) % per_sync == 0) {
             live = any_active(b, st);
             if (st) return st;
         }
     }
-    if (live && tail > 0 && (st = enqueue_group(b, tail, check))) return st;
+    if (live && tail > 0 && (st = enqueue_chunks(b, tail_plan, use_graph))) return st;
     if (elapsed_ms) {
         BCHK(b, hipEventRecord(e1, b->stream));
         BCHK(b, hipEventSynchronize(e1));
