@@ -24,6 +24,15 @@ copy-calibrated; null when the kernel is not in it).
 
 runs only a plain device copy of 512 MiB (the calibration kernel for the
 FETCH_SIZE / WRITE_SIZE counters).
+
+    python bench.py --batch [--steps K]
+
+measures the batched-candidate workload instead: 64 NACA 4-digit shapes
+(m ~ U[0, 0.06], p ~ U[0.2, 0.6], t ~ U[0.08, 0.15], seeded), each with its
+own O-mesh of U[20k, 120k] triangles and its own flow (Mach ~ U[0.5, 0.85],
+AoA ~ U[0, 4] deg, Re 5000, Fourier heat flux), advanced together by one
+launch per stage kernel (autotuned shapes, graph replay), against the same
+64 meshes solved one after another.
 """
 import argparse
 import json
@@ -119,6 +128,87 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "copies": 5}), flush=True)
 
 
+def batch_candidates(n=64, seed=20240611):
+    """The 64 shape candidates: shape, mesh size and flow drawn from a fixed seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m, p, t = rng.uniform(0.0, 0.06), rng.uniform(0.2, 0.6), rng.uniform(0.08, 0.15)
+        tri = rng.uniform(20e3, 120e3)
+        mach, aoa = rng.uniform(0.5, 0.85), rng.uniform(0.0, 4.0)
+        n_around = max(16, int(round(np.sqrt(tri / 2 * 2.6) / 4)) * 4)
+        n_radial = int(round(tri / (2 * n_around))) + 1
+        out.append(dict(m=m, p=p, t=t, n_around=n_around, n_radial=n_radial, mach=mach, aoa=aoa))
+    return out
+
+
+def bench_batch(a):
+    from optimisation2forme_amd import capi, mesher
+    t0 = time.perf_counter()
+    cands = batch_candidates()
+    solvers = []
+    for c in cands:
+        desc = mesher.naca_mesh(m=c["m"], p=c["p"], t=c["t"], n_around=c["n_around"], n_radial=c["n_radial"],
+                                wall_spacing=1e-3, radius=20.0, aoa_deg=c["aoa"])
+        mu = 1.225 * c["mach"] * np.sqrt(1.4 * 101325.0 / 1.225) / 5000.0
+        prm = capi.make_params(viscosity=mu, Pr=0.72, cfl_number=0.5, inflow_velocity=c["mach"], venkat_k=5.0,
+                               limiter="venkat", interp_mode="mvc", heat_flux="fourier")
+        solvers.append(capi.Solver(desc, prm))
+    setup_s = time.perf_counter() - t0
+    nv = np.array([s.nv for s in solvers])
+    nc = np.array([s.nc for s in solvers])
+    K = a.steps
+    # one after another: each mesh alone, direct launches and single-mesh graph replay
+    seq_direct = 0.0
+    for s in solvers:
+        s.init_freestream()
+        s.run_steps(2)
+        ms, _ = s.time_steps(K)
+        seq_direct += ms
+    seq_graph = 0.0
+    for s in solvers:
+        s.init_freestream()
+        one = capi.Batch([s])
+        one.time_steps(2)
+        seq_graph += one.time_steps(K, use_graph=True)
+        one.close()
+    b = capi.Batch(solvers)
+    shapes = b.autotune(reps=3)
+    for s in solvers:
+        s.init_freestream()
+    b.time_steps(2, use_graph=True)
+    for s in solvers:
+        s.init_freestream()
+    t1 = time.perf_counter()
+    batch_ms = b.time_steps(K, use_graph=True)
+    wall = time.perf_counter() - t1
+    finite = all(bool(np.isfinite(s.state()).all()) for s in solvers)
+    seq_ms = min(seq_direct, seq_graph)
+    total_vs = float(nv.sum()) * K
+    out = {
+        "metric": "vertex-steps/s (batched shape candidates)",
+        "value": total_vs / (wall),
+        "unit": "vertex-steps/s",
+        "n_gpus": 1, "steps": K, "warmup": 2,
+        "ms_per_step": 1e3 * wall / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (64 generated NACA 4-digit O-meshes, freestream cold starts)",
+        "config": {"workload": "batch64_naca4_20k_120k_triangles", "members": len(solvers),
+                   "cells": int(nc.sum()), "vertices": int(nv.sum()),
+                   "cells_min_max": [int(nc.min()), int(nc.max())], "mach": [0.5, 0.85], "aoa_deg": [0, 4],
+                   "re": 5000, "cfl": 0.5, "limiter": "venkatakrishnan K=5", "interp_mode": "mvc",
+                   "heat_flux": "fourier", "parallelism": "single GPU"},
+        "event_ms_per_step": batch_ms / K,
+        "sequential": {"direct_ms_per_step": seq_direct / K, "graph_ms_per_step": seq_graph / K,
+                       "value": total_vs / (seq_ms * 1e-3)},
+        "speedup_vs_sequential": seq_ms / batch_ms,
+        "autotuned_shapes": dict(zip(capi.Batch.KINDS, shapes)),
+        "setup_s": setup_s,
+        "state_finite": finite,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,9 +220,15 @@ def main():
                     help="steps of the per-kernel event-timed pass")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 512 MiB device copy used to calibrate the HBM counters")
+    ap.add_argument("--batch", action="store_true", help="the 64-candidate batched workload")
     a = ap.parse_args()
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.batch and a.gpus == 1:
+        if a.steps == ap.get_default("steps"):
+            a.steps = 200
+        bench_batch(a)
         return
     if a.gpus != 1:
         print("bench.py: this build measures one GPU only (the per-mesh path runs on a single "

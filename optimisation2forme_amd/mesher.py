@@ -76,17 +76,23 @@ def _unit(v):
     return v / np.linalg.norm(v, axis=1, keepdims=True)
 
 
-def march_layers(wall, d, centre, R):
+def march_layers(wall, d, centre, R, d_switch=0.25):
     """Layer-by-layer normal marching from the wall (distances d), with
-    normal smoothing and tangential point redistribution that grow with the
-    distance, then a radial blend of the outer layers onto the circle of
-    radius R around `centre`."""
+    light normal smoothing and tangential point redistribution, up to the
+    first layer at distance >= d_switch (a quarter chord: well inside the
+    radius of curvature of any concave part of a NACA 4-digit surface, so
+    the normals do not cross); beyond it every point moves out along its ray
+    from `centre` (a point inside the airfoil), its radius interpolated
+    between that layer's and R with the same layer distances.  The switch
+    layer is star-shaped around the centre, so the outer layers are nested
+    and cannot fold."""
     P = wall.copy()
     outward = np.sign(np.mean(np.sum(_unit(np.stack([(np.roll(P, -1, 0) - np.roll(P, 1, 0))[:, 1],
                                                         -(np.roll(P, -1, 0) - np.roll(P, 1, 0))[:, 0]], 1))
                                      * (P - centre), 1)))
     layers = [P.copy()]
     d_last = d[-1]
+    j_sw = len(d) - 1
     for j in range(1, len(d)):
         dd = d[j] - d[j - 1]
         T = np.roll(P, -1, 0) - np.roll(P, 1, 0)
@@ -102,20 +108,16 @@ def march_layers(wall, d, centre, R):
             Pn = Pn + nu * np.sum(Lp * Tn, 1, keepdims=True) * Tn
         P = Pn
         layers.append(P.copy())
-    xy = np.array(layers)
-    # radial blend onto the far-field circle over the outer layers
-    rel = xy - centre
-    r = np.linalg.norm(rel, axis=2)
-    rbar = r.mean(axis=1)
-    scale = R / rbar[-1]
-    for j in range(len(d)):
-        w = np.clip((d[j] - 0.25 * d_last) / (0.75 * d_last), 0.0, 1.0) ** 2
-        if w == 0.0:
-            continue
-        target = rbar[j] * scale
-        rn = (1.0 - w) * r[j] + w * target
-        xy[j] = centre + rel[j] * (rn / r[j])[:, None]
-    return xy
+        if d[j] >= d_switch:
+            j_sw = j
+            break
+    rel = layers[-1] - centre
+    r_sw = np.linalg.norm(rel, axis=1)
+    for j in range(j_sw + 1, len(d)):
+        s = (d[j] - d[j_sw]) / (d_last - d[j_sw])
+        rj = (1.0 - s) * r_sw + s * R
+        layers.append(centre + rel * (rj / r_sw)[:, None])
+    return np.array(layers)
 
 
 def naca_mesh(m=0.0, p=0.4, t=0.12, n_around=128, n_radial=40, wall_spacing=2e-3, radius=20.0,
@@ -131,8 +133,12 @@ def naca_mesh(m=0.0, p=0.4, t=0.12, n_around=128, n_radial=40, wall_spacing=2e-3
         wall = np.stack([pos_x + ca * dx + sa * dy, -sa * dx + ca * dy], 1)
     d, g = radial_distances(wall_spacing, n_radial, radius, growth)
     R = d[-1] if growth is not None else radius
-    centre = np.array([pos_x + 0.5 * chord, 0.0])
-    xy = march_layers(wall, d, centre, R)                 # (n_radial, n_around, 2)
+    # centre: the mean-line point at mid chord (inside the airfoil), rotated with it
+    yc_mid = float(naca4(m, p, t, np.array([0.5]))[0][0]) * chord
+    a = math.radians(aoa_deg)
+    centre = np.array([pos_x + math.cos(a) * 0.5 * chord + math.sin(a) * yc_mid,
+                       -math.sin(a) * 0.5 * chord + math.cos(a) * yc_mid])
+    xy = march_layers(wall, d, centre, R, d_switch=0.25 * chord)   # (n_radial, n_around, 2)
     ni, nj = n_around, n_radial
     idx = lambda j, i: j * ni + (i % ni)
     cells = []
