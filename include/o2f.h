@@ -202,6 +202,18 @@ int o2f_adjoint_rhs(o2f_ctx *ctx, double beta, double *b);
  * with central finite-difference (absolute step eps) HLLC blocks at the
  * current face states.  x, y are 4*n_cells. */
 int o2f_apply_jt(o2f_ctx *ctx, const double *x, double *y, double eps);
+/* Exact discrete adjoint: y = (dR/dU)^T x at the current state, R the full
+ * residual of o2f_residual (computeResidual, src/shape_optimization.cpp:
+ * 171-189).  Unlike applyJacobianTranspose (:237-484), which freezes the
+ * gradients and limiters and finite-differences HLLC, every dependence of R
+ * on U is differentiated exactly (interpolation, Green-Gauss, limiter
+ * through its active branch, HLLC, ghosts, stresses, heat flux); limiter
+ * state is recomputed per cell, not stored.  x, y are 4*n_cells SoA. */
+int o2f_apply_jt_exact(o2f_ctx *ctx, const double *x, double *y);
+/* solveAdjointSystem's BiCGSTAB (chunks, true residual, same error codes as
+ * o2f_bicgstab) on the exact operator of o2f_apply_jt_exact. */
+int o2f_bicgstab_exact(o2f_ctx *ctx, double beta, double tol, int chunk, int max_total, double *lambda,
+                       int *iters, double *relres);
 /* solveAdjointSystem (src/shape_optimization.cpp:658-740): BiCGSTAB with an
  * identity preconditioner on J^T lambda = b, restarted every `chunk`
  * iterations up to `max_total`, true residual checked after each chunk.

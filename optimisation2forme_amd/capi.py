@@ -34,7 +34,8 @@ EXPORTS = [
     "o2f_adjoint_rhs", "o2f_apply_jt", "o2f_bicgstab", "o2f_shape_gradient", "o2f_time_steps",
     "o2f_mesh_topology", "o2f_run_until", "o2f_run_to_residual", "o2f_batch_create",
     "o2f_batch_destroy", "o2f_batch_last_error", "o2f_batch_set_shapes", "o2f_batch_get_shapes",
-    "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps",
+    "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps", "o2f_apply_jt_exact",
+    "o2f_bicgstab_exact",
 ]
 
 
@@ -112,6 +113,8 @@ def lib():
             "o2f_mesh_topology": (C.c_int, [C.POINTER(MeshDesc), I, I, I, C.c_char_p, D, D, I]),
             "o2f_run_until": (C.c_int, [vp, C.c_double, C.c_int, I, D, D, D]),
             "o2f_run_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
+            "o2f_apply_jt_exact": (C.c_int, [vp, D, D]),
+            "o2f_bicgstab_exact": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D]),
             "o2f_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
             "o2f_batch_destroy": (None, [vp]),
             "o2f_batch_last_error": (C.c_char_p, [vp]),
@@ -355,6 +358,21 @@ class Solver:
         y = np.zeros((4, self.nc))
         self._check(lib().o2f_apply_jt(self.h, _d(x), _d(y), eps))
         return y
+
+    def apply_jt_exact(self, x):
+        """y = (dR/dU)^T x, the exact discrete adjoint of the full residual."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(4, self.nc)
+        y = np.zeros((4, self.nc))
+        self._check(lib().o2f_apply_jt_exact(self.h, _d(x), _d(y)))
+        return y
+
+    def bicgstab_exact(self, beta, tol=1e-6, chunk=500, max_total=100000, check=True):
+        lam = np.zeros((4, self.nc))
+        it, rr = C.c_int(), C.c_double()
+        st = lib().o2f_bicgstab_exact(self.h, beta, tol, chunk, max_total, _d(lam), C.byref(it), C.byref(rr))
+        if check:
+            self._check(st)
+        return lam, it.value, rr.value, st
 
     def bicgstab(self, beta, eps=1e-6, tol=1e-6, chunk=500, max_total=100000, check=True):
         lam = np.zeros((4, self.nc))

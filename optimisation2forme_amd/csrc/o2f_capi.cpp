@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <memory>
 #include <sstream>
 #include <string>
@@ -321,7 +322,8 @@ void o2f_ctx_destroy(o2f_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->cinfo,
                     c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
-                    c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx, c->Fe};
+                    c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx, c->Fe,
+                    c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c->kv)
@@ -746,9 +748,10 @@ int set_scal(o2f_ctx *c, int slot, double v) {
 
 // One call of Eigen's internal::bicgstab (identity preconditioner) starting
 // from x: returns the number of iterations it ran.
-int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol, int *iters) {
-    const Topo T = c->topo();
-    const Geom G = c->geo[0].view();
+// y = A x on the device, A = J^T (either form).
+using MatVec = std::function<hipError_t(const double *, double *)>;
+
+int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol, int *iters, const MatVec &mv) {
     hipStream_t s = c->stream;
     const long n = 4L * c->hm.nc;
     double *r = c->kv[1], *r0 = c->kv[2], *v = c->kv[3], *p = c->kv[4], *sv = c->kv[5], *t = c->kv[6];
@@ -756,7 +759,7 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     HostScal h;
     int st;
     // r = b - A x; r0 = r
-    HIPCHK(c, launch_jt_apply(T, G, c->blk, x, tmp, s));
+    HIPCHK(c, mv(x, tmp));
     HIPCHK(c, launch_sub(n, b, tmp, r, s));
     HIPCHK(c, launch_sub(n, r, nullptr, r0, s));
     HIPCHK(c, launch_dot2(r0, r0, b, b, n, c->partials, c->scal, kOpStore, s));
@@ -782,7 +785,7 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     while (h.v[kScalRR] > tol2 && i < maxit) {
         if (std::fabs(h.v[kScalRho]) < eps2 * r0_sqnorm) {
             // r became too orthogonal to r0: restart with r0 = r = b - A x
-            HIPCHK(c, launch_jt_apply(T, G, c->blk, x, tmp, s));
+            HIPCHK(c, mv(x, tmp));
             HIPCHK(c, launch_sub(n, b, tmp, r, s));
             HIPCHK(c, launch_sub(n, r, nullptr, r0, s));
             HIPCHK(c, launch_dot2(r, r, nullptr, nullptr, n, c->partials, c->scal, kOpRestart, s));
@@ -791,10 +794,10 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
             if (restarts++ == 0) i = 0;
         }
         HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
-        HIPCHK(c, launch_jt_apply(T, G, c->blk, p, v, s));
+        HIPCHK(c, mv(p, v));
         HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->scal, kOpAlpha, s));
         HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, s));
-        HIPCHK(c, launch_jt_apply(T, G, c->blk, sv, t, s));
+        HIPCHK(c, mv(sv, t));
         HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->scal, kOpOmega, s));
         HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s));
         ++i;
@@ -806,11 +809,14 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
 }
 }  // namespace
 
-int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int max_total, double *lambda,
-                 int *iters, double *relres) {
-    if (!c || !lambda || chunk <= 0) return O2F_ERR_ARG;
-    int st = ensure_blocks(c, eps);
-    if (st) return st;
+namespace {
+
+// solveAdjointSystem's loop (src/shape_optimization.cpp:690-736): chunks of
+// `chunk` iterations restarted from the current x, the true residual checked
+// after each; the RHS is assembled at the current face states.
+int bicgstab_solve(o2f_ctx *c, double beta, double tol, int chunk, int max_total, double *lambda, int *iters,
+                   double *relres, const MatVec &mv) {
+    int st;
     const long n = 4L * c->hm.nc;
     double *b = c->kv[0], *x = c->kv[8], *tmp = c->kv[9];
     if ((st = rhs_device(c, beta, b))) return st;
@@ -823,10 +829,10 @@ int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int
     int total = 0, status = O2F_OK;
     while (total < max_total) {
         int it = 0;
-        if ((st = bicgstab_chunk(c, b, x, chunk, tol, &it))) return st;
+        if ((st = bicgstab_chunk(c, b, x, chunk, tol, &it, mv))) return st;
         total += it;
-        // true residual b - J^T x (src/shape_optimization.cpp:715-717)
-        HIPCHK(c, launch_jt_apply(c->topo(), c->geo[0].view(), c->blk, x, tmp, c->stream));
+        // true residual b - A x (src/shape_optimization.cpp:715-717)
+        HIPCHK(c, mv(x, tmp));
         HIPCHK(c, launch_sub(n, b, tmp, tmp, c->stream));
         HIPCHK(c, launch_dot2(tmp, tmp, nullptr, nullptr, n, c->partials, c->scal, kOpStore, c->stream));
         if ((st = read_scal(c, h))) return st;
@@ -836,12 +842,147 @@ int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int
             status = c->fail(O2F_ERR_DIVERGED, "BiCGSTAB diverged");
             break;
         }
+        if (it == 0) break;  // no progress possible from this x
     }
     if (status == O2F_OK && rr > tol) status = c->fail(O2F_ERR_NOCONV, "BiCGSTAB did not converge");
     if (iters) *iters = total;
     if (relres) *relres = rr;
     st = copy_out(c, lambda, x, n);
     return st ? st : status;
+}
+
+}  // namespace
+
+int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int max_total, double *lambda,
+                 int *iters, double *relres) {
+    if (!c || !lambda || chunk <= 0) return O2F_ERR_ARG;
+    int st = ensure_blocks(c, eps);
+    if (st) return st;
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    const MatVec mv = [c, T, G](const double *x, double *y) {
+        return launch_jt_apply(T, G, c->blk, x, y, c->stream);
+    };
+    return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, mv);
+}
+
+// ---- exact adjoint ---------------------------------------------------------
+
+namespace {
+
+// Sub-array sizes (doubles) of the adjoint work block, each rounded up to a
+// multiple of 32 doubles so every array starts 256-byte aligned (the AoS
+// arrays are read as 16-byte pairs).
+size_t adj_round(size_t n) { return (n + 31) / 32 * 32; }
+
+AdjWork adj_work(o2f_ctx *c) {
+    const size_t nc = c->hm.nc, ne = c->hm.ne, nv = c->hm.nv;
+    double *p = c->a_work;
+    AdjWork W{};
+    auto take = [&](size_t n) {
+        double *q = p;
+        p += adj_round(n);
+        return q;
+    };
+    W.fbL = take(4 * ne);
+    W.fbR = take(4 * ne);
+    W.th = take(5 * ne);
+    W.vtb = take(5 * nv);
+    W.ub = take(4 * nc);
+    W.gb = take(8 * nc);
+    W.mval = take(4 * nc);
+    W.ha = take(4 * ne);
+    W.vi = take(4 * nv);
+    W.mtgt = c->a_mtgt;
+    return W;
+}
+
+size_t adj_work_size(const HostMesh &hm) {
+    const size_t nc = hm.nc, ne = hm.ne, nv = hm.nv;
+    return 3 * adj_round(4 * ne) + adj_round(5 * ne) + adj_round(5 * nv) + 2 * adj_round(4 * nc) +
+           adj_round(8 * nc) + adj_round(4 * nv);
+}
+
+// Vertex -> edge CSR, cell vertices, and the weight of each cell in the
+// interpolation stencil of each of its vertices (the transpose of the
+// cell -> vertex gather).
+int ensure_adjoint(o2f_ctx *c) {
+    int st = ensure_work(c);
+    if (st) return st;
+    const HostMesh &hm = c->hm;
+    if (!c->a_veptr) {
+        std::vector<int> ptr(hm.nv + 1, 0), ed(2 * (size_t)hm.ne);
+        for (int e = 0; e < hm.ne; ++e) {
+            ++ptr[hm.eva[e] + 1];
+            ++ptr[hm.evb[e] + 1];
+        }
+        for (int v = 0; v < hm.nv; ++v) ptr[v + 1] += ptr[v];
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (int e = 0; e < hm.ne; ++e) {
+            ed[fill[hm.eva[e]]++] = e;
+            ed[fill[hm.evb[e]]++] = e;
+        }
+        HIPCHK(c, upload(c->a_veptr, ptr));
+        HIPCHK(c, upload(c->a_vedge, ed));
+        HIPCHK(c, upload(c->a_cvtx, hm.cvtx));
+        HIPCHK(c, dalloc(c->a_work, adj_work_size(hm)));
+        HIPCHK(c, dalloc(c->a_mtgt, 4 * (size_t)hm.nc));
+        HIPCHK(c, dalloc(c->a_cvw, 4 * (size_t)hm.nc));
+    }
+    if (c->a_interp != c->prm.interp_mode) {
+        std::vector<double> w(4 * (size_t)hm.nc, 0.0);
+        for (int cc = 0; cc < hm.nc; ++cc)
+            for (int j = 0; j < 4; ++j) {
+                const int v = hm.cvtx[4 * cc + j];
+                if (v < 0) continue;
+                const int b = hm.vptr[v], n = hm.vptr[v + 1] - b;
+                const uint8_t kind = hm.skind[v];
+                for (int k = 0; k < n; ++k)
+                    if (hm.scell[b + k] == cc) {
+                        w[4 * cc + j] = kind == kStencilWsum ? hm.sw[b + k]
+                                        : kind == kStencilAvg2 ? 0.5
+                                        : kind == kStencilCopy ? 1.0
+                                                               : 0.0;
+                        break;
+                    }
+            }
+        HIPCHK(c, hipMemcpy(c->a_cvw, w.data(), w.size() * sizeof(double), hipMemcpyHostToDevice));
+        c->a_interp = c->prm.interp_mode;
+    }
+    return O2F_OK;
+}
+
+// Forward state of the linearisation point, then y = (dR/dU)^T x on the device.
+MatVec exact_matvec(o2f_ctx *c) {
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    const AdjTopo A{c->a_veptr, c->a_vedge, c->a_cvtx, c->a_cvw};
+    const AdjWork W = adj_work(c);
+    return [c, T, G, A, W](const double *x, double *y) {
+        return launch_adjoint_exact(T, G, c->phys, A, W, x, c->U, c->vU, c->fL, c->fR, c->vtq, y, c->stream);
+    };
+}
+
+}  // namespace
+
+int o2f_apply_jt_exact(o2f_ctx *c, const double *x, double *y) {
+    if (!c || !x || !y) return O2F_ERR_ARG;
+    int st = ensure_adjoint(c);
+    if (st) return st;
+    if ((st = residual_on(c, 0))) return st;
+    const size_t n = 4 * (size_t)c->hm.nc;
+    HIPCHK(c, hipMemcpyAsync(c->kv[0], x, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, exact_matvec(c)(c->kv[0], c->kv[1]));
+    return copy_out(c, y, c->kv[1], n);
+}
+
+int o2f_bicgstab_exact(o2f_ctx *c, double beta, double tol, int chunk, int max_total, double *lambda, int *iters,
+                       double *relres) {
+    if (!c || !lambda || chunk <= 0) return O2F_ERR_ARG;
+    int st = ensure_adjoint(c);
+    if (st) return st;
+    if ((st = residual_on(c, 0))) return st;
+    return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, exact_matvec(c));
 }
 
 int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *values_all, int n_all,

@@ -89,6 +89,11 @@ struct o2f_ctx {
     double blk_eps = 0.0;
     double *kv[10] = {};  // BiCGSTAB / gradient work vectors (4 nc each)
     double *Rp = nullptr, *Rm = nullptr;
+    // exact adjoint: vertex -> edge CSR, cell vertices and their stencil
+    // weights, work arrays (allocated on first use)
+    int *a_veptr = nullptr, *a_vedge = nullptr, *a_cvtx = nullptr, *a_mtgt = nullptr;
+    double *a_cvw = nullptr, *a_work = nullptr;
+    int a_interp = -1;  // interp mode a_cvw was built for
 
     int fail(int code, const std::string &msg) {
         err = msg;

@@ -1,0 +1,231 @@
+// Inline device helpers shared by the forward kernels (o2f_kernels.hip)
+// and the exact adjoint (o2f_adjoint.hip): min/max semantics, AoS loads,
+// boundary states, HLLC.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "o2f_kernels.hpp"
+
+namespace o2f {
+
+// Forward-mode dual number (value, one directional derivative): the HLLC
+// and stress Jacobians of the exact adjoint are evaluated with it, from the
+// same templated code as the forward values.
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) {
+    const double q = a.v / b.v;
+    return {q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual operator+(Dual a, double b) { return {a.v + b, a.d}; }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return {a + b.v, b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, double b) { return {a.v - b, a.d}; }
+__device__ __forceinline__ Dual operator-(double a, Dual b) { return {a - b.v, -b.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return {a.v * b, a.d * b}; }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return {a * b.v, a * b.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, double b) { return {a.v / b, a.d / b}; }
+__device__ __forceinline__ Dual operator/(double a, Dual b) {
+    const double q = a / b.v;
+    return {q, -q * b.d / b.v};
+}
+__device__ __forceinline__ bool operator<(Dual a, Dual b) { return a.v < b.v; }
+__device__ __forceinline__ bool operator<(Dual a, double b) { return a.v < b; }
+__device__ __forceinline__ bool operator<=(Dual a, double b) { return a.v <= b; }
+__device__ __forceinline__ bool operator>=(Dual a, double b) { return a.v >= b; }
+__device__ __forceinline__ bool operator>(Dual a, double b) { return a.v > b; }
+using ::sqrt;  // the double overloads stay visible next to sqrt(Dual)
+__device__ __forceinline__ Dual sqrt(Dual a) {
+    const double r = ::sqrt(a.v);
+    return {r, a.d / (2.0 * r)};
+}
+
+// std::min / std::max / std::clamp semantics (NaN handling included).
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+__device__ __forceinline__ double sclamp(double v, double lo, double hi) {
+    return (v < lo) ? lo : (hi < v) ? hi : v;
+}
+template <class T>
+__device__ __forceinline__ T smin_t(T a, T b) {
+    return (b < a) ? b : a;
+}
+template <class T>
+__device__ __forceinline__ T smax_t(T a, T b) {
+    return (a < b) ? b : a;
+}
+__device__ __forceinline__ bool tagged(char b) {
+    return b == 'i' || b == 'o' || b == 'f' || b == 'w';
+}
+
+// Per-stage intermediates (vertex values, face states, edge fluxes, cell and
+// vertex stresses) are stored AoS, four doubles per entity (a fifth
+// component, q_y, sits in a trailing SoA block): one gathered entity is one
+// 32-byte run of a cache line instead of four scattered 8-byte words.
+__device__ __forceinline__ void ld4(const double *__restrict__ p, long i, double (&v)[4]) {
+    const double2 *q = reinterpret_cast<const double2 *>(p) + 2 * i;
+    const double2 a = q[0], b = q[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+}
+__device__ __forceinline__ void st4(double *__restrict__ p, long i, const double (&v)[4]) {
+    double2 *q = reinterpret_cast<double2 *>(p) + 2 * i;
+    q[0] = make_double2(v[0], v[1]);
+    q[1] = make_double2(v[2], v[3]);
+}
+
+// apply_inflow_outflow_bc, src/boundary.cpp:7-45 (freestream state precomputed
+// on the host with the same expression order).
+__device__ __forceinline__ void bc_state(const Phys &P, char bc, double &r, double &ru, double &rv,
+                                         double &E) {
+    if (bc == 'i' || bc == 'f') {
+        r = P.fs_rho;
+        ru = P.fs_ru;
+        rv = P.fs_rv;
+        E = P.fs_E;
+    } else if (bc == 'w') {
+        const double kinetic_old = 0.5 * (ru * ru + rv * rv) / r;
+        const double e_internal = E - kinetic_old;
+        ru = 0.0;
+        rv = 0.0;
+        E = e_internal;
+    }
+}
+
+// apply_InterfaceStates_boundary_conditions, src/boundary.cpp:48-80.
+__device__ __forceinline__ void bc_ghost(const Phys &P, char bc, const double *L, double *R) {
+    R[0] = L[0];
+    R[1] = L[1];
+    R[2] = L[2];
+    R[3] = L[3];
+    if (bc == 'i' || bc == 'o' || bc == 'f') {
+        bc_state(P, bc, R[0], R[1], R[2], R[3]);
+    } else if (bc == 'w') {
+        R[1] = -L[1];
+        R[2] = -L[2];
+    }
+}
+
+// compute_hllc_flux, src/advection.cpp:325-412.  (enx, eny) is the stored
+// normal (into the left cell); the flux is taken along -n.
+template <class T>
+__device__ __forceinline__ void hllc_t(const T *qL, const T *qR, double enx, double eny, double gamma, T *F) {
+    const double nx = -enx, ny = -eny;
+    const T rho_L = qL[0];
+    const T u_L = (qL[1] * (nx) + qL[2] * ny) / rho_L;
+    const T v_L = (qL[1] * (-ny) + qL[2] * nx) / rho_L;
+    const T E_L = qL[3];
+    const T p_L = (gamma - 1.0) * (E_L - 0.5 * rho_L * (u_L * u_L + v_L * v_L));
+    const T c_L = sqrt(gamma * p_L / rho_L);
+    const T rho_R = qR[0];
+    const T u_R = (qR[1] * (nx) + qR[2] * ny) / rho_R;
+    const T v_R = (qR[1] * (-ny) + qR[2] * nx) / rho_R;
+    const T E_R = qR[3];
+    const T p_R = (gamma - 1.0) * (E_R - 0.5 * rho_R * (u_R * u_R + v_R * v_R));
+    const T c_R = sqrt(gamma * p_R / rho_R);
+
+    const T S_L = smin_t(u_L - c_L, u_R - c_R);
+    const T S_R = smax_t(u_L + c_L, u_R + c_R);
+    const T numer = p_R - p_L + rho_L * u_L * (S_L - u_L) - rho_R * u_R * (S_R - u_R);
+    const T denom = rho_L * (S_L - u_L) - rho_R * (S_R - u_R);
+    const T S_star = numer / denom;
+    const T p_star = p_L + rho_L * (S_L - u_L) * (S_star - u_L);
+    T f0, fu, fv, fE;
+    if (S_L >= 0.0) {
+        f0 = rho_L * u_L;
+        fu = rho_L * u_L * u_L + p_L;
+        fv = rho_L * u_L * v_L;
+        fE = (E_L + p_L) * u_L;
+    } else if (S_L <= 0.0 && S_star >= 0.0) {
+        const T rs = rho_L * (S_L - u_L) / (S_L - S_star);
+        const T rvs = rs * v_L;
+        const T Es = ((S_L - u_L) * E_L - p_L * u_L + p_star * S_star) / (S_L - S_star);
+        const T rus = rs * S_star;
+        f0 = rus;
+        fu = rus * rus / rs + p_star;
+        fv = rvs * rus / rs;
+        fE = (Es + p_star) * rus / rs;
+    } else if (S_star <= 0.0 && S_R >= 0.0) {
+        const T rs = rho_R * (S_R - u_R) / (S_R - S_star);
+        const T rvs = rs * v_R;
+        const T Es = ((S_R - u_R) * E_R - p_R * u_R + p_star * S_star) / (S_R - S_star);
+        const T rus = rs * S_star;
+        f0 = rus;
+        fu = rus * rus / rs + p_star;
+        fv = rvs * rus / rs;
+        fE = (Es + p_star) * rus / rs;
+    } else {
+        f0 = rho_R * u_R;
+        fu = rho_R * u_R * u_R + p_R;
+        fv = rho_R * u_R * v_R;
+        fE = (E_R + p_R) * u_R;
+    }
+    F[0] = f0;
+    F[1] = (fu * nx + fv * (-ny));
+    F[2] = (fu * ny + fv * (nx));
+    F[3] = fE;
+}
+
+
+__device__ __forceinline__ void hllc(const double *qL, const double *qR, double enx, double eny, double gamma,
+                                     double *F) {
+    hllc_t<double>(qL, qR, enx, eny, gamma, F);
+}
+
+// Limiters of computeInterfaceStates (src/advection.cpp:195-230).
+__device__ __forceinline__ double alpha_venkat(double phi_i, double phi_pred, double phi_min,
+                                               double phi_max, double cellArea, double k_eps) {
+    const double delta = phi_pred - phi_i;
+    if (fabs(delta) < DBL_EPSILON) return 1.0;
+    const double h = sqrt(cellArea / M_PI);
+    const double eps = k_eps * k_eps * k_eps * h * h * h;
+    const double a = (delta > 0.0) ? (phi_max - phi_i) : (phi_min - phi_i);
+    const double alpha = (a * a + eps + 2.0 * a * delta) / (a * a + eps + 2.0 * delta * delta);
+    return sclamp(alpha, 0.0, 1.0);
+}
+
+__device__ __forceinline__ double alpha_bj(double phi_i, double phi_pred, double phi_min,
+                                           double phi_max) {
+    double alpha = 1.0;
+    if (fabs(phi_pred - phi_i) < DBL_EPSILON) return alpha;
+    if (phi_pred > phi_i)
+        alpha = smin(1.0, (phi_max - phi_i) / (phi_pred - phi_i));
+    else if (phi_pred < phi_i)
+        alpha = smin(1.0, (phi_min - phi_i) / (phi_pred - phi_i));
+    return alpha;
+}
+
+// Cell kernels run four lanes per cell, one per edge slot (triangles leave
+// one idle): a cell's sums over its edges are formed by every lane of the
+// group from the lanes' terms in ascending slot order, i.e. exactly the
+// sequential order of the reference loops, so results stay bit-identical.
+constexpr int kLanesPerCell = 4;
+
+// Value of lane J of this lane's group of four: a DPP quad_perm broadcast
+// (register to register, no LDS round trip).
+template <int J>
+__device__ __forceinline__ double from_lane(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), J * 0x55, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), J * 0x55, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// f(j, value of lane j) for j = 0, 1, 2, 3 in order
+template <class F>
+__device__ __forceinline__ void for_lanes(double v, F &&f) {
+    f(0, from_lane<0>(v));
+    f(1, from_lane<1>(v));
+    f(2, from_lane<2>(v));
+    f(3, from_lane<3>(v));
+}
+
+}  // namespace o2f

@@ -128,6 +128,31 @@ hipError_t launch_b_residual(const Member *M, const int2 *chunk_tiles, int nchun
 // flags[i] = member i still active
 hipError_t launch_b_collect(const Member *M, int nmembers, double *flags, hipStream_t s);
 
+// ------------------------------------------------------------------------
+// Exact adjoint (o2f_adjoint.hip): y = (dR/dU)^T x at the state whose
+// forward vertex values vU, face states fL / fR and vertex stresses vtq the
+// last residual evaluation left.
+// ------------------------------------------------------------------------
+struct AdjTopo {
+    const int *veptr, *vedge;  // vertex -> incident edges (CSR, ascending edge id)
+    const int *cvtx;           // 4 per cell, -1 pad
+    const double *cvw;         // 4 per cell: the cell's weight in the stencil of its j-th vertex
+};
+struct AdjWork {
+    double *fbL, *fbR;  // adjoint face states (AoS 4 per edge)
+    double *th;         // half the adjoint edge stresses (AoS 4 + trailing block)
+    double *vtb;        // adjoint vertex stresses (AoS 4 + trailing block)
+    double *ub;         // adjoint cell state, direct part (AoS 4)
+    double *gb;         // adjoint cell gradients (AoS 8: x then y)
+    double *mval;       // limiter min / max adjoint per variable (AoS 4) ...
+    int *mtgt;          // ... and the neighbour cell it belongs to (-1: none)
+    double *ha;         // half the adjoint edge values (AoS 4)
+    double *vi;         // adjoint interpolated vertex states (AoS 4)
+};
+hipError_t launch_adjoint_exact(const Topo &T, const Geom &G, const Phys &P, const AdjTopo &A, const AdjWork &W,
+                                const double *x, const double *U, const double *vU, const double *fL,
+                                const double *fR, const double *vtq, double *y, hipStream_t s);
+
 hipError_t launch_vertex_state(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
                                hipStream_t s);
 hipError_t launch_vertex_visc(const Topo &T, const Geom &G, const double *tq, double *vtq, const Phys &P,
