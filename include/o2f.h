@@ -275,6 +275,43 @@ int o2f_batch_run(o2f_batch *batch, double tol, int max_steps, int check_every, 
 /* Elapsed milliseconds of nsteps batched steps (tol <= 0, no early stop). */
 int o2f_batch_time_steps(o2f_batch *batch, int nsteps, int use_graph, double *total_ms);
 
+/* ---- learned warm start ---------------------------------------------- */
+
+/* A per-cell MLP mapping (x, y, wall distance, m, p, t, Mach, AoA in
+ * degrees) plus 16 Fourier features of (x, y) to the conservative state
+ * (normalised: rho/rho_inf - 1, rho u/(rho_inf a_inf) - M,
+ * rho v/(rho_inf a_inf), (E - E_inf)/p_inf), 4 hidden layers of 256 with
+ * SiLU, trained with Adam on fields the solver produced.  BF16 operands on
+ * the matrix cores with FP32 accumulation and FP32 master weights.  Its
+ * prediction replaces the freestream cold start (init_one_x,
+ * src/init.cpp:59-81) as the initial state; the solver still iterates to
+ * the same residual tolerance, so only the step count changes.  No
+ * counterpart in the reference.  Weights: 5 layers W_l (K_l x N_l,
+ * row-major) then the 5 biases, fp32 (o2f_mlp_num_params gives the
+ * layout: K, N, weight offset, bias offset per layer). */
+typedef struct o2f_mlp o2f_mlp;
+int o2f_mlp_create(unsigned long long seed, o2f_mlp **out);
+void o2f_mlp_destroy(o2f_mlp *net);
+const char *o2f_mlp_last_error(const o2f_mlp *net);
+int o2f_mlp_num_params(const o2f_mlp *net, long *n_params, int *layout /* 20, may be NULL */);
+/* fp32 weights (n_params) and the Fourier frequencies (2 x 16: x row then y row). */
+int o2f_mlp_get(o2f_mlp *net, float *weights, float *fourier);
+int o2f_mlp_set(o2f_mlp *net, const float *weights);
+/* Training set: raw (n x 8 inputs above), targets (n x 4 normalised state). */
+int o2f_mlp_set_data(o2f_mlp *net, const float *raw, const float *targets, long n);
+/* steps Adam updates (lr, beta 0.9 / 0.999, eps 1e-8) on minibatches of
+ * `batch` samples (a multiple of 64) drawn from the training set by a
+ * seeded hash; losses[steps] (may be NULL) receives each step's MSE. */
+int o2f_mlp_train(o2f_mlp *net, int steps, int batch, float lr, unsigned long long seed, float *losses);
+/* out (n x 4) = network(raw (n x 8)). */
+int o2f_mlp_forward(o2f_mlp *net, const float *raw, long n, float *out);
+/* Mean squared error over the 4 outputs of n samples and its gradient with
+ * respect to every parameter (n_params, fp32). */
+int o2f_mlp_loss_grad(o2f_mlp *net, const float *raw, const float *targets, long n, float *loss, float *grad);
+/* Sets the context's state to the network's prediction for its cells
+ * (shape m, p, t and angle of attack as inputs; Mach from the context). */
+int o2f_mlp_warm_start(o2f_mlp *net, o2f_ctx *ctx, double m, double p, double t, double aoa_deg);
+
 /* ---- measurement ----------------------------------------------------- */
 
 /* Runs nsteps device-resident SSP-RK3 steps (as o2f_run_steps) between HIP

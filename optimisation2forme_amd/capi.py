@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libo2f.so")
 
 D = C.POINTER(C.c_double)
 I = C.POINTER(C.c_int)
+F = C.POINTER(C.c_float)
 
 O2F_OK = 0
 ERRORS = {-1: "O2F_ERR_ARG", -2: "O2F_ERR_BC", -3: "O2F_ERR_HIP", -4: "O2F_ERR_IO",
@@ -35,7 +36,9 @@ EXPORTS = [
     "o2f_mesh_topology", "o2f_run_until", "o2f_run_to_residual", "o2f_batch_create",
     "o2f_batch_destroy", "o2f_batch_last_error", "o2f_batch_set_shapes", "o2f_batch_get_shapes",
     "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps", "o2f_apply_jt_exact",
-    "o2f_bicgstab_exact",
+    "o2f_bicgstab_exact", "o2f_mlp_create", "o2f_mlp_destroy", "o2f_mlp_last_error", "o2f_mlp_num_params",
+    "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
+    "o2f_mlp_loss_grad", "o2f_mlp_warm_start",
 ]
 
 
@@ -114,6 +117,17 @@ def lib():
             "o2f_run_until": (C.c_int, [vp, C.c_double, C.c_int, I, D, D, D]),
             "o2f_run_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
             "o2f_apply_jt_exact": (C.c_int, [vp, D, D]),
+            "o2f_mlp_create": (C.c_int, [C.c_ulonglong, C.POINTER(vp)]),
+            "o2f_mlp_destroy": (None, [vp]),
+            "o2f_mlp_last_error": (C.c_char_p, [vp]),
+            "o2f_mlp_num_params": (C.c_int, [vp, C.POINTER(C.c_long), I]),
+            "o2f_mlp_get": (C.c_int, [vp, F, F]),
+            "o2f_mlp_set": (C.c_int, [vp, F]),
+            "o2f_mlp_set_data": (C.c_int, [vp, F, F, C.c_long]),
+            "o2f_mlp_train": (C.c_int, [vp, C.c_int, C.c_int, C.c_float, C.c_ulonglong, F]),
+            "o2f_mlp_forward": (C.c_int, [vp, F, C.c_long, F]),
+            "o2f_mlp_loss_grad": (C.c_int, [vp, F, F, C.c_long, F, F]),
+            "o2f_mlp_warm_start": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]),
             "o2f_bicgstab_exact": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D]),
             "o2f_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
             "o2f_batch_destroy": (None, [vp]),
@@ -469,3 +483,75 @@ class Batch:
         ms = C.c_double()
         self._check(lib().o2f_batch_time_steps(self.h, nsteps, int(use_graph), C.byref(ms)))
         return ms.value
+
+
+def _f(a):
+    return a.ctypes.data_as(F)
+
+
+class WarmStartNet:
+    """o2f_mlp: the learned warm start (BF16 MFMA forward / backward, Adam)."""
+
+    def __init__(self, seed=0):
+        h = C.c_void_p()
+        st = lib().o2f_mlp_create(seed, C.byref(h))
+        if st != O2F_OK:
+            raise O2FError(st, "o2f_mlp_create failed")
+        self.h = h
+        n = C.c_long()
+        lay = np.zeros(20, np.int32)
+        self._check(lib().o2f_mlp_num_params(self.h, C.byref(n), _i(lay)))
+        self.n_params = n.value
+        self.layout = [tuple(int(v) for v in lay[4 * l:4 * l + 4]) for l in range(5)]  # K, N, w_off, b_off
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().o2f_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st != O2F_OK:
+            raise O2FError(st, lib().o2f_mlp_last_error(self.h).decode())
+
+    def weights(self):
+        w = np.zeros(self.n_params, np.float32)
+        fb = np.zeros(32, np.float32)
+        self._check(lib().o2f_mlp_get(self.h, _f(w), _f(fb)))
+        return w, fb.reshape(2, 16)
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._check(lib().o2f_mlp_set(self.h, _f(w)))
+
+    def set_data(self, raw, targets):
+        raw = np.ascontiguousarray(raw, dtype=np.float32)
+        targets = np.ascontiguousarray(targets, dtype=np.float32)
+        self._check(lib().o2f_mlp_set_data(self.h, _f(raw), _f(targets), raw.shape[0]))
+
+    def train(self, steps, batch=65536, lr=1e-3, seed=1):
+        losses = np.zeros(max(steps, 1), np.float32)
+        self._check(lib().o2f_mlp_train(self.h, steps, batch, lr, seed, _f(losses)))
+        return losses[:steps]
+
+    def forward(self, raw):
+        raw = np.ascontiguousarray(raw, dtype=np.float32)
+        out = np.zeros((raw.shape[0], 4), np.float32)
+        self._check(lib().o2f_mlp_forward(self.h, _f(raw), raw.shape[0], _f(out)))
+        return out
+
+    def loss_grad(self, raw, targets):
+        raw = np.ascontiguousarray(raw, dtype=np.float32)
+        targets = np.ascontiguousarray(targets, dtype=np.float32)
+        loss = np.zeros(1, np.float32)
+        g = np.zeros(self.n_params, np.float32)
+        self._check(lib().o2f_mlp_loss_grad(self.h, _f(raw), _f(targets), raw.shape[0], _f(loss), _f(g)))
+        return float(loss[0]), g
+
+    def warm_start(self, solver, m, p, t, aoa_deg):
+        self._check(lib().o2f_mlp_warm_start(self.h, solver.h, m, p, t, aoa_deg))
