@@ -33,6 +33,14 @@ own O-mesh of U[20k, 120k] triangles and its own flow (Mach ~ U[0.5, 0.85],
 AoA ~ U[0, 4] deg, Re 5000, Fourier heat flux), advanced together by one
 launch per stage kernel (autotuned shapes, graph replay), against the same
 64 meshes solved one after another.
+
+    python bench.py --warm-start
+
+measures the learned warm start: training fields from the solver (seeded
+candidates in the same ranges, Fourier heat flux, each run to a density
+residual of --ws-train-tol), BF16-MFMA training of the network on the GPU,
+then held-out candidates run to --ws-tol from the freestream cold start and
+from the network's prediction.  Prints the steps and time of both.
 """
 import argparse
 import json
@@ -209,6 +217,38 @@ def bench_batch(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_warm_start(a):
+    from optimisation2forme_amd import warmstart as W
+    t0 = time.perf_counter()
+    train_c = W.candidates(a.ws_train, seed=2024)
+    fields = W.solve_fields(train_c, a.ws_train_tol, a.ws_max_steps)
+    t_fields = time.perf_counter() - t0
+    net, losses, t_train = W.train(fields, a.ws_steps, batch=a.ws_batch, lr=1e-3, seed=0)
+    test_c = W.candidates(a.ws_test, seed=777)
+    rows = W.compare(net, test_c, a.ws_tol, a.ws_max_steps)
+    cold = np.array([r["cold_steps"] for r in rows], float)
+    warm = np.array([r["warm_steps"] for r in rows], float)
+    out = {
+        "metric": "steps to density residual tol, surrogate warm start vs freestream cold start",
+        "value": float(np.mean(cold) / np.mean(warm)),
+        "unit": "x fewer steps (mean cold / mean warm)",
+        "higher_is_better": True, "n_gpus": 1, "dtype": "f64 solver, bf16 network (fp32 accumulate)",
+        "data": "synthetic (solver-produced fields on generated NACA 4-digit meshes)",
+        "config": {"workload": "warm_start_naca4_small_meshes", "train_candidates": a.ws_train,
+                   "train_samples": int(sum(f[2].shape[1] for f in fields)),
+                   "train_field_tol": a.ws_train_tol, "train_steps": a.ws_steps, "batch": a.ws_batch,
+                   "test_candidates": a.ws_test, "tol": a.ws_tol, "heat_flux": "fourier"},
+        "train_field_steps_mean": float(np.mean([f[3] for f in fields])),
+        "fields_s": t_fields, "train_s": t_train,
+        "loss_first_last": [float(losses[0]), float(np.mean(losses[-50:]))],
+        "cold_steps": cold.tolist(), "warm_steps": warm.tolist(),
+        "time_ratio": float(sum(r["cold_s"] for r in rows) / sum(r["warm_s"] for r in rows)),
+        "surrogate_rmse_mean": np.mean([r["surrogate_rmse"] for r in rows], 0).tolist(),
+        "per_candidate": rows,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,7 +261,18 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 512 MiB device copy used to calibrate the HBM counters")
     ap.add_argument("--batch", action="store_true", help="the 64-candidate batched workload")
+    ap.add_argument("--warm-start", action="store_true", help="the learned warm-start comparison")
+    ap.add_argument("--ws-train", type=int, default=32)
+    ap.add_argument("--ws-test", type=int, default=8)
+    ap.add_argument("--ws-train-tol", type=float, default=0.2)
+    ap.add_argument("--ws-tol", type=float, default=0.4)
+    ap.add_argument("--ws-max-steps", type=int, default=60000)
+    ap.add_argument("--ws-steps", type=int, default=3000)
+    ap.add_argument("--ws-batch", type=int, default=16384)
     a = ap.parse_args()
+    if a.warm_start and a.gpus == 1:
+        bench_warm_start(a)
+        return
     if a.copy_calibration:
         copy_calibration()
         return

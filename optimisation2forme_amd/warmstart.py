@@ -1,0 +1,157 @@
+"""Learned warm start: training fields from the solver, training, and the
+warm-versus-cold comparison.
+
+    fields = solve_fields(cands, tol, max_steps)      # the solver's own fields (batched)
+    net = train(fields, steps, batch)                 # BF16 MFMA training on the GPU
+    report = compare(net, held_out, tol, max_steps)   # steps to tol: warm vs cold start
+
+Every candidate is a NACA 4-digit shape with its own O-mesh and flow
+(Mach, angle of attack); the solver runs with the Fourier heat flux (the
+reference's R x Fourier flux diverges once viscosity matters).  The
+network's prediction only replaces the freestream cold start: both starts
+iterate to the same density-residual tolerance, so results are those of
+the solver and only the step count changes.
+"""
+import time
+
+import numpy as np
+
+from . import capi, mesher
+
+RHO, P_INF, GAMMA = 1.225, 101325.0, 1.4
+A_INF = float(np.sqrt(GAMMA * P_INF / RHO))
+
+
+def candidates(n, seed, n_around=(56, 96), n_radial=(24, 40)):
+    """Seeded shapes and flows in the ranges m ~ U[0, 0.06], p ~ U[0.2, 0.6],
+    t ~ U[0.08, 0.15], Mach ~ U[0.5, 0.85], AoA ~ U[0, 4] deg."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        out.append(dict(m=float(rng.uniform(0, 0.06)), p=float(rng.uniform(0.2, 0.6)),
+                        t=float(rng.uniform(0.08, 0.15)), mach=float(rng.uniform(0.5, 0.85)),
+                        aoa=float(rng.uniform(0, 4)),
+                        n_around=int(rng.integers(n_around[0] // 4, n_around[1] // 4 + 1)) * 4,
+                        n_radial=int(rng.integers(n_radial[0], n_radial[1] + 1))))
+    return out
+
+
+def build(c, wall_spacing=4e-3, radius=20.0):
+    desc = mesher.naca_mesh(m=c["m"], p=c["p"], t=c["t"], n_around=c["n_around"], n_radial=c["n_radial"],
+                            wall_spacing=wall_spacing, radius=radius, aoa_deg=c["aoa"])
+    mu = RHO * c["mach"] * A_INF / 5000.0
+    prm = capi.make_params(viscosity=mu, Pr=0.72, cfl_number=0.5, inflow_velocity=c["mach"], venkat_k=5.0,
+                           limiter="venkat", interp_mode="mvc", heat_flux="fourier")
+    return desc, prm
+
+
+def cell_centres(desc):
+    cv = np.asarray(desc["cell_vtx"])[:, :3]
+    xy = np.asarray(desc["xy"])
+    ctr = xy[cv].mean(axis=1)
+    return ctr[:, 0], ctr[:, 1]
+
+
+def wall_distance(desc, cx, cy):
+    """Distance of each point to the nearest wall edge segment."""
+    xy = np.asarray(desc["xy"])
+    bv = np.asarray(desc["bnd_vtx"])
+    wall = bv[np.asarray(desc["bnd_tag"]) == ord("w")]
+    a, b = xy[wall[:, 0]], xy[wall[:, 1]]
+    d = np.full(cx.shape, np.inf)
+    P = np.stack([cx, cy], 1)
+    for k in range(0, len(a), 64):
+        A, B = a[k:k + 64], b[k:k + 64]
+        AB = B - A
+        l2 = np.maximum(np.sum(AB * AB, 1), 1e-300)
+        s = np.clip(((P[:, None, :] - A[None]) * AB[None]).sum(2) / l2[None], 0, 1)
+        Q = A[None] + s[..., None] * AB[None]
+        d = np.minimum(d, np.sqrt(((Q - P[:, None, :]) ** 2).sum(2)).min(1))
+    return d
+
+
+def raw_inputs(desc, c):
+    cx, cy = cell_centres(desc)
+    wd = wall_distance(desc, cx, cy)
+    n = cx.shape[0]
+    return np.stack([cx, cy, wd, np.full(n, c["m"]), np.full(n, c["p"]), np.full(n, c["t"]),
+                     np.full(n, c["mach"]), np.full(n, c["aoa"])], 1).astype(np.float32)
+
+
+def normalise(U, mach):
+    """Network targets of a state (4 x n_cells)."""
+    u = mach * A_INF
+    Einf = P_INF / (GAMMA - 1.0) + 0.5 * RHO * u * u
+    return np.stack([U[0] / RHO - 1.0, U[1] / (RHO * A_INF) - mach, U[2] / (RHO * A_INF),
+                     (U[3] - Einf) / P_INF], 1).astype(np.float32)
+
+
+def solve_fields(cands, tol, max_steps, check_every=50):
+    """Cold-start every candidate and run it (batched, one launch per stage
+    kernel) until its density residual is <= tol; returns a list of
+    (candidate, mesh, state, steps, residual)."""
+    solvers, descs = [], []
+    for c in cands:
+        desc, prm = build(c)
+        s = capi.Solver(desc, prm)
+        s.init_freestream()
+        solvers.append(s)
+        descs.append(desc)
+    b = capi.Batch(solvers)
+    steps, res = b.run(tol, max_steps, check_every=check_every, use_graph=True)
+    out = [(c, d, s.state(), int(n), float(r)) for c, d, s, n, r in zip(cands, descs, solvers, steps, res)]
+    b.close()
+    for s in solvers:
+        s.close()
+    return out
+
+
+def dataset(fields):
+    raw = np.concatenate([raw_inputs(d, c) for c, d, _, _, _ in fields])
+    tgt = np.concatenate([normalise(U, c["mach"]) for c, _, U, _, _ in fields])
+    return raw, tgt
+
+
+def train(fields, steps, batch=65536, lr=1e-3, seed=0):
+    raw, tgt = dataset(fields)
+    net = capi.WarmStartNet(seed=seed)
+    net.set_data(raw, tgt)
+    t0 = time.perf_counter()
+    losses = net.train(steps, batch=batch, lr=lr, seed=seed + 1)
+    return net, losses, time.perf_counter() - t0
+
+
+def compare(net, cands, tol, max_steps, check_every=50):
+    """Per candidate: steps and wall time to the density-residual tolerance
+    from the freestream cold start and from the network's prediction
+    (inference included), C_L / C_D of both end states, and the
+    surrogate's error against the cold run's end state."""
+    rows = []
+    for c in cands:
+        desc, prm = build(c)
+        cold = capi.Solver(desc, prm)
+        warm = capi.Solver(desc, prm)
+        cold.init_freestream()
+        t0 = time.perf_counter()
+        n_cold, r_cold = cold.run_to_residual(tol, max_steps, check_every=check_every, use_graph=True)
+        t_cold = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        net.warm_start(warm, c["m"], c["p"], c["t"], c["aoa"])
+        t_inf = time.perf_counter() - t0
+        U0 = warm.state()
+        t0 = time.perf_counter()
+        n_warm, r_warm = warm.run_to_residual(tol, max_steps, check_every=check_every, use_graph=True)
+        t_warm = time.perf_counter() - t0 + t_inf
+        Uc, Uw = cold.state(), warm.state()
+        cold.residual()
+        warm.residual()
+        target = normalise(Uc, c["mach"])
+        pred = normalise(U0, c["mach"])
+        rows.append(dict(candidate=c, cells=cold.nc, cold_steps=int(n_cold), warm_steps=int(n_warm),
+                         cold_res=float(r_cold), warm_res=float(r_warm), cold_s=t_cold, warm_s=t_warm,
+                         inference_s=t_inf, cold_cl_cd=list(cold.lift_drag()), warm_cl_cd=list(warm.lift_drag()),
+                         surrogate_rmse=[float(v) for v in np.sqrt(np.mean((pred - target) ** 2, 0))],
+                         state_rel_diff=float(np.max(np.abs(Uw - Uc)) / np.max(np.abs(Uc)))))
+        cold.close()
+        warm.close()
+    return rows

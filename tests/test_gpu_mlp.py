@@ -85,3 +85,19 @@ def test_adam_training_reduces_the_loss():
     l2 = net2.train(300, batch=4096, lr=1e-3, seed=7)
     assert np.array_equal(losses, l2)
     assert np.array_equal(net.weights()[0], net2.weights()[0])
+
+
+def test_warm_start_reduces_the_steps_to_tolerance():
+    """Ablation: the network trained on solver fields (8 candidates) against
+    the freestream cold start on 2 held-out candidates; both iterate to the
+    same density-residual tolerance."""
+    from optimisation2forme_amd import warmstart as W
+    fields = W.solve_fields(W.candidates(8, seed=11), tol=0.25, max_steps=40000)
+    assert all(f[3] > 1000 for f in fields)     # developed fields (to 0.25 or 40000 steps)
+    net, losses, _ = W.train(fields, steps=600, batch=8192, lr=1e-3, seed=0)
+    assert losses[-20:].mean() < 0.2 * losses[:5].mean()
+    rows = W.compare(net, W.candidates(2, seed=12), tol=0.4, max_steps=40000)
+    for r in rows:
+        print("cells %d cold %d warm %d steps" % (r["cells"], r["cold_steps"], r["warm_steps"]), flush=True)
+        assert r["cold_res"] <= 0.4 and r["warm_res"] <= 0.4
+    assert sum(r["warm_steps"] for r in rows) < 0.7 * sum(r["cold_steps"] for r in rows)
