@@ -80,22 +80,30 @@ __global__ __launch_bounds__(256) void k_gemm(const u16 *__restrict__ A, long sa
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
-    const int ar = t >> 2, ak = (t & 3) * 8;   // A tile: row, first k of this thread
-    const int bn = t & 63, bk = (t >> 6) * 8;  // B tile: column, first k of this thread
     for (int k0 = kb; k0 < ke; k0 += kTK) {
-        // global -> LDS (every index below is inside the M x K and K x N operands:
-        // M, N are multiples of 64 and K of 32, checked by the host)
-        const long am = m0 + ar;
-        if (sak == 1) {
-            const v8u va = *reinterpret_cast<const v8u *>(A + am * sam + (k0 + ak));
-            *reinterpret_cast<v8u *>(&As[ar][ak]) = va;
-        } else {
+        // global -> LDS with 16-byte loads along whichever index is contiguous
+        // (every index is inside the M x K and K x N operands: M, N are
+        // multiples of 64 and K of 32, checked by the host)
+        if (sak == 1) {  // A rows contiguous in k
+            const int r = t >> 2, k = (t & 3) * 8;
+            *reinterpret_cast<v8u *>(&As[r][k]) =
+                *reinterpret_cast<const v8u *>(A + (long)(m0 + r) * sam + (k0 + k));
+        } else {  // A contiguous in m: 8 rows of one k, transposed into LDS
+            const int k = t >> 3, r = (t & 7) * 8;
+            const v8u va = *reinterpret_cast<const v8u *>(A + (long)(k0 + k) * sak + (m0 + r));
 #pragma unroll
-            for (int i = 0; i < 8; ++i) As[ar][ak + i] = A[am * sam + (long)(k0 + ak + i) * sak];
+            for (int i = 0; i < 8; ++i) As[r + i][k] = va[i];
         }
-        const long nn = n0 + bn;
+        if (sbk == 1) {  // B columns contiguous in k
+            const int n = t >> 2, k = (t & 3) * 8;
+            *reinterpret_cast<v8u *>(&Bt[n][k]) =
+                *reinterpret_cast<const v8u *>(B + (long)(n0 + n) * sbn + (k0 + k));
+        } else {  // B contiguous in n
+            const int k = t >> 3, n = (t & 7) * 8;
+            const v8u vb = *reinterpret_cast<const v8u *>(B + (long)(k0 + k) * sbk + (n0 + n));
 #pragma unroll
-        for (int i = 0; i < 8; ++i) Bt[bn][bk + i] = B[(long)(k0 + bk + i) * sbk + nn * sbn];
+            for (int i = 0; i < 8; ++i) Bt[n + i][k] = vb[i];
+        }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
@@ -346,6 +354,10 @@ int ensure_cap(o2f_mlp *o, int rows) {
 
 hipError_t gemm(o2f_mlp *o, const u16 *A, long sam, long sak, const u16 *B, long sbk, long sbn, int M, int N, int K,
                 int kchunk, const Epi &epi) {
+    // the tile loaders need one unit stride per operand and whole tiles
+    if (!((sak == 1 || sam == 1) && (sbk == 1 || sbn == 1)) || M % kTile || N % kTile || K % kTK ||
+        kchunk % kTK)
+        return hipErrorInvalidValue;
     const int splits = (K + kchunk - 1) / kchunk;
     hipLaunchKernelGGL(k_gemm, dim3(N / kTile, M / kTile, splits), dim3(256), 0, o->stream, A, sam, sak, B, sbk, sbn,
                        M, N, K, kchunk, epi);
