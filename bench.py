@@ -34,6 +34,11 @@ AoA ~ U[0, 4] deg, Re 5000, Fourier heat flux), advanced together by one
 launch per stage kernel (autotuned shapes, graph replay), against the same
 64 meshes solved one after another.
 
+    python bench.py --adjoint
+
+times J^T x products (reference-form operator and exact adjoint) on the
+transonic workload.
+
     python bench.py --warm-start
 
 measures the learned warm start: training fields from the solver (seeded
@@ -177,18 +182,18 @@ def bench_batch(a):
     for s in solvers:
         s.init_freestream()
         one = capi.Batch([s])
-        one.time_steps(2)
-        seq_graph += one.time_steps(K, use_graph=True)
+        one.time_steps(2, use_graph=not a.no_graph)
+        seq_graph += one.time_steps(K, use_graph=not a.no_graph)
         one.close()
     b = capi.Batch(solvers)
     shapes = b.autotune(reps=3)
     for s in solvers:
         s.init_freestream()
-    b.time_steps(2, use_graph=True)
+    b.time_steps(2, use_graph=not a.no_graph)
     for s in solvers:
         s.init_freestream()
     t1 = time.perf_counter()
-    batch_ms = b.time_steps(K, use_graph=True)
+    batch_ms = b.time_steps(K, use_graph=not a.no_graph)
     wall = time.perf_counter() - t1
     finite = all(bool(np.isfinite(s.state()).all()) for s in solvers)
     seq_ms = min(seq_direct, seq_graph)
@@ -221,11 +226,11 @@ def bench_warm_start(a):
     from optimisation2forme_amd import warmstart as W
     t0 = time.perf_counter()
     train_c = W.candidates(a.ws_train, seed=2024)
-    fields = W.solve_fields(train_c, a.ws_train_tol, a.ws_max_steps)
+    fields = W.solve_fields(train_c, a.ws_train_tol, a.ws_max_steps, use_graph=not a.no_graph)
     t_fields = time.perf_counter() - t0
     net, losses, t_train = W.train(fields, a.ws_steps, batch=a.ws_batch, lr=1e-3, seed=0)
     test_c = W.candidates(a.ws_test, seed=777)
-    rows = W.compare(net, test_c, a.ws_tol, a.ws_max_steps)
+    rows = W.compare(net, test_c, a.ws_tol, a.ws_max_steps, use_graph=not a.no_graph)
     cold = np.array([r["cold_steps"] for r in rows], float)
     warm = np.array([r["warm_steps"] for r in rows], float)
     out = {
@@ -249,6 +254,29 @@ def bench_warm_start(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(a):
+    """J^T x products on the transonic 200k workload after 50 steps: the
+    reference-form operator (FD HLLC blocks) and the exact adjoint."""
+    from optimisation2forme_amd import capi, mesher
+    flow = workload_params(a.limiter)
+    kw = dict(flow)
+    kw["cfl_number"] = kw.pop("cfl")
+    kw["inflow_velocity"] = kw.pop("mach")
+    s = capi.Solver(mesher.transonic_200k(), capi.make_params(**kw))
+    s.init_freestream()
+    s.run_steps(50)
+    n = max(2, a.steps)
+    ref_ms = s.time_adjoint(n, False) / n
+    exact_ms = s.time_adjoint(n, True) / n
+    B = 72 * s.nc + 96 * s.ne          # per product: blocks / state / result, the survey's figure
+    out = {"metric": "adjoint J^T x products", "unit": "ms per product", "higher_is_better": False,
+           "n_gpus": 1, "dtype": "f64", "steps": n,
+           "config": {"workload": "naca2412_transonic_200k_triangles", "cells": s.nc, "edges": s.ne},
+           "reference_form_ms": ref_ms, "exact_ms": exact_ms,
+           "reference_form_GBs": B / (ref_ms * 1e-3) / 1e9, "algorithmic_bytes_per_product": B}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -262,6 +290,9 @@ def main():
                     help="only run the 512 MiB device copy used to calibrate the HBM counters")
     ap.add_argument("--batch", action="store_true", help="the 64-candidate batched workload")
     ap.add_argument("--warm-start", action="store_true", help="the learned warm-start comparison")
+    ap.add_argument("--adjoint", action="store_true", help="J^T x product timings (reference form, exact)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="batch / warm-start modes: direct launches instead of graph replay (for tracing)")
     ap.add_argument("--ws-train", type=int, default=32)
     ap.add_argument("--ws-test", type=int, default=8)
     ap.add_argument("--ws-train-tol", type=float, default=0.2)
@@ -272,6 +303,11 @@ def main():
     a = ap.parse_args()
     if a.warm_start and a.gpus == 1:
         bench_warm_start(a)
+        return
+    if a.adjoint and a.gpus == 1:
+        if a.steps == ap.get_default("steps"):
+            a.steps = 50
+        bench_adjoint(a)
         return
     if a.copy_calibration:
         copy_calibration()

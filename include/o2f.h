@@ -320,6 +320,10 @@ int o2f_mlp_warm_start(o2f_mlp *net, o2f_ctx *ctx, double m, double p, double t,
  * per-stage kernels (vertex gather, cell reconstruction, vertex viscous
  * gather, flux + update), each bracketed by its own events. */
 int o2f_time_steps(o2f_ctx *ctx, int nsteps, double *total_ms, double *kernel_ms);
+/* nprod device-resident products y = J^T x at the current state between HIP
+ * events: exact = 0 the reference-form operator (FD blocks, built once
+ * before timing), 1 the exact adjoint.  Returns the elapsed milliseconds. */
+int o2f_time_adjoint(o2f_ctx *ctx, int nprod, int exact, double *total_ms);
 
 #ifdef __cplusplus
 }

@@ -38,7 +38,7 @@ EXPORTS = [
     "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps", "o2f_apply_jt_exact",
     "o2f_bicgstab_exact", "o2f_mlp_create", "o2f_mlp_destroy", "o2f_mlp_last_error", "o2f_mlp_num_params",
     "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
-    "o2f_mlp_loss_grad", "o2f_mlp_warm_start",
+    "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint",
 ]
 
 
@@ -117,6 +117,7 @@ def lib():
             "o2f_run_until": (C.c_int, [vp, C.c_double, C.c_int, I, D, D, D]),
             "o2f_run_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
             "o2f_apply_jt_exact": (C.c_int, [vp, D, D]),
+            "o2f_time_adjoint": (C.c_int, [vp, C.c_int, C.c_int, D]),
             "o2f_mlp_create": (C.c_int, [C.c_ulonglong, C.POINTER(vp)]),
             "o2f_mlp_destroy": (None, [vp]),
             "o2f_mlp_last_error": (C.c_char_p, [vp]),
@@ -406,6 +407,11 @@ class Solver:
         self._check(lib().o2f_shape_gradient(self.h, na, _d(va), len(names_all), no, len(opt_names),
                                              _d(lam), beta, eps, _d(g)))
         return g[:len(opt_names)]
+
+    def time_adjoint(self, nprod, exact):
+        ms = C.c_double()
+        self._check(lib().o2f_time_adjoint(self.h, nprod, int(exact), C.byref(ms)))
+        return ms.value
 
     def time_steps(self, n, per_kernel=False):
         tot = C.c_double()

@@ -1040,6 +1040,36 @@ int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *v
     return O2F_OK;
 }
 
+int o2f_time_adjoint(o2f_ctx *c, int nprod, int exact, double *total_ms) {
+    if (!c || nprod <= 0 || !total_ms) return O2F_ERR_ARG;
+    int st = exact ? ensure_adjoint(c) : ensure_work(c);
+    if (st) return st;
+    if ((st = residual_on(c, 0))) return st;
+    if (!exact && (st = ensure_blocks(c, 1e-6))) return st;
+    const long n = 4L * c->hm.nc;
+    double *x = c->kv[0], *y = c->kv[1];
+    HIPCHK(c, launch_fill(n, x, 1.0, c->stream));
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    const MatVec mv = exact ? exact_matvec(c) : MatVec([c, T, G](const double *a, double *b) {
+        return launch_jt_apply(T, G, c->blk, a, b, c->stream);
+    });
+    HIPCHK(c, mv(x, y));  // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    for (int k = 0; k < nprod; ++k) HIPCHK(c, mv(k & 1 ? y : x, k & 1 ? x : y));
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *total_ms = ms;
+    return O2F_OK;
+}
+
 int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) {
     if (!c || nsteps <= 0 || !total_ms) return O2F_ERR_ARG;
     int st = check_bc(c);

@@ -3,8 +3,8 @@
 // state, trained on fields the solver produced.  BF16 operands on the
 // gfx950 matrix cores (v_mfma_f32_16x16x32_bf16) with FP32 accumulation,
 // FP32 master weights and Adam, all hand-written: feature build, GEMM with
-// fused bias / SiLU / SiLU' epilogues, split-K weight gradients with a
-// fixed-order reduction, loss, Adam.  Deterministic: no atomics.
+// fused bias / SiLU / SiLU' epilogues, split-K weight and bias gradients
+// with a fixed-order reduction, loss, Adam.  Deterministic: no atomics.
 //
 // Layout: activations row-major [samples x width] in bf16 (stored as
 // uint16), pre-activations in fp32; weights W_l row-major [K_l x N_l].
@@ -154,19 +154,6 @@ __global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ P,
     out[i] = s;
 }
 
-// Column sums of dZ (bf16 values, fp32 sums) over chunks of kSplitChunk
-// rows: P[chunk][n], rows of a chunk in order; k_sum_splits then adds the
-// chunks in order.
-__global__ __launch_bounds__(256) void k_colsum_split(const u16 *__restrict__ D, int rows, int cols,
-                                                      float *__restrict__ P) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= cols) return;
-    const int r0 = blockIdx.y * kSplitChunk, r1 = min(rows, r0 + kSplitChunk);
-    float s = 0.f;
-    for (int m = r0; m < r1; ++m) s += from_bf16(D[(long)m * cols + n]);
-    P[(long)blockIdx.y * cols + n] = s;
-}
-
 // Input features of one sample from its 8 raw inputs
 // (x, y, wall distance, m, p, t, Mach, AoA in degrees) and the Fourier
 // frequencies B (2 x kFourier): scaled raw inputs, sin / cos (2 pi B^T (x, y)).
@@ -308,6 +295,8 @@ struct o2f_mlp {
     int cap = 0;  // rows of the activation buffers
     u16 *X[kLayers] = {}, *dA = nullptr, *dB = nullptr, *dY = nullptr;
     float *Z[kLayers - 1] = {}, *Y = nullptr, *T = nullptr, *split = nullptr, *lparts = nullptr, *lout = nullptr;
+    u16 *ones = nullptr;     // [cap x 64] bf16, column 0 = 1: bias gradients as a GEMM
+    float *dbtmp = nullptr;  // [64 x 256] sums of that GEMM (row 0 is the bias gradient)
     float *raw = nullptr, *tgt = nullptr;
     long ndata = 0;
     long step = 0;
@@ -334,7 +323,7 @@ int ensure_cap(o2f_mlp *o, int rows) {
     for (auto *p : o->X) (void)hipFree(p);
     for (auto *p : o->Z) (void)hipFree(p);
     for (void *p : {(void *)o->dA, (void *)o->dB, (void *)o->dY, (void *)o->Y, (void *)o->T, (void *)o->split,
-                    (void *)o->lparts})
+                    (void *)o->lparts, (void *)o->ones, (void *)o->dbtmp})
         (void)hipFree(p);
     const long r = rows;
     MCHK(o, dalloc(o->X[0], r * kIn));
@@ -348,6 +337,13 @@ int ensure_cap(o2f_mlp *o, int rows) {
     const long splits = (r + kSplitChunk - 1) / kSplitChunk;
     MCHK(o, dalloc(o->split, splits * kHid * kHid));
     MCHK(o, dalloc(o->lparts, (long)nblk(r)));
+    MCHK(o, dalloc(o->ones, r * kIn));
+    MCHK(o, dalloc(o->dbtmp, (long)kIn * kHid));
+    {
+        std::vector<u16> h((size_t)r * kIn, 0);
+        for (long i = 0; i < r; ++i) h[(size_t)i * kIn] = to_bf16(1.0f);
+        MCHK(o, hipMemcpy(o->ones, h.data(), h.size() * sizeof(u16), hipMemcpyHostToDevice));
+    }
     o->cap = rows;
     return O2F_OK;
 }
@@ -405,11 +401,12 @@ int backward(o2f_mlp *o, int n, int rows) {
         hipLaunchKernelGGL(k_sum_splits, dim3(nblk((long)Kl * Nl)), dim3(256), 0, s, o->split, splits, (long)Kl * Nl,
                            o->g + o->woff[l]);
         MCHK(o, hipGetLastError());
-        hipLaunchKernelGGL(k_colsum_split, dim3(nblk(Nl), splits), dim3(256), 0, s, dZ, rows, Nl, o->split);
+        // db_l = 1^T dZ: the same split GEMM with a ones column as the left operand
+        MCHK(o, gemm(o, o->ones, 1, kIn, dZ, Nl, 1, kIn, Nl, rows, kSplitChunk, ep));
+        hipLaunchKernelGGL(k_sum_splits, dim3(nblk((long)kIn * Nl)), dim3(256), 0, s, o->split, splits,
+                           (long)kIn * Nl, o->dbtmp);
         MCHK(o, hipGetLastError());
-        hipLaunchKernelGGL(k_sum_splits, dim3(nblk(Nl)), dim3(256), 0, s, o->split, splits, (long)Nl,
-                           o->g + o->boff[l]);
-        MCHK(o, hipGetLastError());
+        MCHK(o, hipMemcpyAsync(o->g + o->boff[l], o->dbtmp, Nl * sizeof(float), hipMemcpyDeviceToDevice, s));
         if (l == 0) break;
         // dZ_{l-1} = (dZ W_l^T) * silu'(Z_{l-1})
         Epi eb{};

@@ -86,7 +86,7 @@ def normalise(U, mach):
                      (U[3] - Einf) / P_INF], 1).astype(np.float32)
 
 
-def solve_fields(cands, tol, max_steps, check_every=50):
+def solve_fields(cands, tol, max_steps, check_every=50, use_graph=True):
     """Cold-start every candidate and run it (batched, one launch per stage
     kernel) until its density residual is <= tol; returns a list of
     (candidate, mesh, state, steps, residual)."""
@@ -98,7 +98,7 @@ def solve_fields(cands, tol, max_steps, check_every=50):
         solvers.append(s)
         descs.append(desc)
     b = capi.Batch(solvers)
-    steps, res = b.run(tol, max_steps, check_every=check_every, use_graph=True)
+    steps, res = b.run(tol, max_steps, check_every=check_every, use_graph=use_graph)
     out = [(c, d, s.state(), int(n), float(r)) for c, d, s, n, r in zip(cands, descs, solvers, steps, res)]
     b.close()
     for s in solvers:
@@ -121,7 +121,7 @@ def train(fields, steps, batch=65536, lr=1e-3, seed=0):
     return net, losses, time.perf_counter() - t0
 
 
-def compare(net, cands, tol, max_steps, check_every=50):
+def compare(net, cands, tol, max_steps, check_every=50, use_graph=True):
     """Per candidate: steps and wall time to the density-residual tolerance
     from the freestream cold start and from the network's prediction
     (inference included), C_L / C_D of both end states, and the
@@ -133,14 +133,14 @@ def compare(net, cands, tol, max_steps, check_every=50):
         warm = capi.Solver(desc, prm)
         cold.init_freestream()
         t0 = time.perf_counter()
-        n_cold, r_cold = cold.run_to_residual(tol, max_steps, check_every=check_every, use_graph=True)
+        n_cold, r_cold = cold.run_to_residual(tol, max_steps, check_every=check_every, use_graph=use_graph)
         t_cold = time.perf_counter() - t0
         t0 = time.perf_counter()
         net.warm_start(warm, c["m"], c["p"], c["t"], c["aoa"])
         t_inf = time.perf_counter() - t0
         U0 = warm.state()
         t0 = time.perf_counter()
-        n_warm, r_warm = warm.run_to_residual(tol, max_steps, check_every=check_every, use_graph=True)
+        n_warm, r_warm = warm.run_to_residual(tol, max_steps, check_every=check_every, use_graph=use_graph)
         t_warm = time.perf_counter() - t0 + t_inf
         Uc, Uw = cold.state(), warm.state()
         cold.residual()
