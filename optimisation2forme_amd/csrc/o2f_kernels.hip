@@ -206,15 +206,14 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
             ty[q] = val[q] * el * ny / vol;
         }
     }
+    // Lanes without an edge hold +0.0 terms; the sums start at +0.0 and so are
+    // never -0.0, and adding +0.0 then changes nothing: every lane's terms
+    // are added unconditionally, in slot order.
     double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        for_lanes(tx[q], [&](int j, double a) {
-            if (j < nk) gx[q] += a;
-        });
-        for_lanes(ty[q], [&](int j, double b) {
-            if (j < nk) gy[q] += b;
-        });
+        for_lanes(tx[q], [&](int, double a) { gx[q] += a; });
+        for_lanes(ty[q], [&](int, double b) { gy[q] += b; });
     }
     // neighbour min / max in neighbour-list order (a lane without a
     // neighbour offers the cell's own value, which changes nothing)
@@ -240,9 +239,7 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
     double al[4] = {1.0, 1.0, 1.0, 1.0};
 #pragma unroll
     for (int q = 0; q < 4; ++q)
-        for_lanes(a_k[q], [&](int j, double a) {
-            if (j < nk) al[q] = smin(al[q], a);
-        });
+        for_lanes(a_k[q], [&](int, double a) { al[q] = smin(al[q], a); });  // 1.0 from lanes without an edge
     // this edge's face state (+ boundary ghost)
     if (has_edge) {
         double f[4];
@@ -424,10 +421,8 @@ __device__ __forceinline__ void flux_update_body(int blk, const Topo &T, const G
         if (flags & kFlagAdv) {
             r[0] = r[1] = r[2] = r[3] = 0.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                for_lanes(adv[q], [&](int j, double a) {
-                    if (j < nk) r[q] += a;
-                });
+            for (int q = 0; q < 4; ++q)  // +0.0 from lanes without an edge (see the gradient sums)
+                for_lanes(adv[q], [&](int, double a) { r[q] += a; });
             const double invV = 1.0 / vol;
 #pragma unroll
             for (int q = 0; q < 4; ++q) r[q] = r[q] * -invV;
