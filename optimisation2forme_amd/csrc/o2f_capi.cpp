@@ -79,11 +79,16 @@ const long kNone = -1;
 // One residual evaluation (optionally fused with a stage update) on
 // geometry set g.  Four kernels: vertex gather, cell reconstruction,
 // viscous vertex gather, flux + update.
-int run_stage(o2f_ctx *c, int g, int flags, const double *dtp, double c0, double c1, bool store_grad) {
+// begin_step: the step's dt reduction rides in the vertex gather launch.
+int run_stage(o2f_ctx *c, int g, int flags, const double *dtp, double c0, double c1, bool store_grad,
+              bool begin_step = false, double *hist = nullptr) {
     const Topo T = c->topo();
     const Geom G = c->geo[g].view();
     hipStream_t s = c->stream;
-    HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, s));
+    if (begin_step)
+        HIPCHK(c, launch_step_gather(T, G, c->U, c->vU, c->phys, c->speed, c->hm.min_h, c->scal, hist, 1, s));
+    else
+        HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, s));
     HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, store_grad ? c->Gr : nullptr, c->phys, s));
     HIPCHK(c, launch_visc_and_flux(T, G, c->tq, c->vtq, c->fL, c->fR, c->Fe, c->phys, 1, s));
     HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, dtp, c0, c1,
@@ -108,10 +113,10 @@ int copy_out(o2f_ctx *c, double *host, const double *dev, size_t n) {
 
 // One SSP-RK3 step (src/simulation.cpp:130-142).  The dt in scal[kScalDt]
 // is read by the stage kernels on the device.
-int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed) {
+int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed, bool begin_step = false, double *hist = nullptr) {
     const double *dtp = c->scal + kScalDt;
     int st;
-    if ((st = run_stage(c, 0, kFlagRK | kFlagSaveOld, dtp, 0.0, 1.0, false))) return st;
+    if ((st = run_stage(c, 0, kFlagRK | kFlagSaveOld, dtp, 0.0, 1.0, false, begin_step, hist))) return st;
     if ((st = run_stage(c, 0, kFlagRK, dtp, 3.0 / 4.0, 1.0 / 4.0, false))) return st;
     return run_stage(c, 0, kFlagRK | (last_outputs ? kFlagStoreR : 0) | (next_speed ? kFlagSpeed : 0), dtp,
                      1.0 / 3.0, 2.0 / 3.0, last_outputs);
@@ -125,8 +130,7 @@ int one_step(o2f_ctx *c, bool last_outputs, double *hist) {
     if (!c->speed_valid) {
         HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
     }
-    HIPCHK(c, launch_step_begin(c->speed, c->hm.nc, c->phys, c->hm.min_h, c->scal, hist, 1, s));
-    int st = rk3_stages(c, last_outputs, true);
+    int st = rk3_stages(c, last_outputs, true, true, hist);
     c->speed_valid = (st == O2F_OK);
     return st;
 }

@@ -539,6 +539,25 @@ __global__ __launch_bounds__(256) void k_step_begin(const double *__restrict__ p
     step_begin_body(parts, nparts, P, min_h, scal, hist, advance_t);
 }
 
+// Start of a step fused with the first stage's vertex gather: the last
+// workgroup runs the step-begin reduction (dt is first read by the stage's
+// update kernel, two launches later), the others gather vertex states.  A
+// gather block that still sees the gate open while the last block closes it
+// only writes vertex values nothing reads once the gate is shut.
+__global__ __launch_bounds__(256) void k_step_gather(Topo T, Geom G, const double *__restrict__ U,
+                                                     double *__restrict__ vU, Phys P,
+                                                     const double *__restrict__ parts, int nparts, double min_h,
+                                                     double *__restrict__ scal, double *__restrict__ hist,
+                                                     int advance_t) {
+    if (blockIdx.x == gridDim.x - 1) {
+        step_begin_body(parts, nparts, P, min_h, scal, hist, advance_t);
+        return;
+    }
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= T.nv || (P.gate && *P.gate == 0.0)) return;
+    gather_vertex<4, false>(T, G, U, vU, P, v);
+}
+
 // Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i): per-block
 // partials in a fixed tree, then one block combines them in block order and
 // applies `op` (two launches: the kernel boundary orders the partials, no
@@ -996,6 +1015,13 @@ hipError_t launch_visc_and_flux(const Topo &T, const Geom &G, const double *tq, 
 int speed_parts(int nc) { return (int)nblk((long)kLanesPerCell * nc); }
 hipError_t launch_speed_max(const double *U, int nc, const Phys &P, double *parts, hipStream_t s) {
     hipLaunchKernelGGL(k_speed_max, dim3(speed_parts(nc)), dim3(256), 0, s, U, nc, P, parts);
+    return hipGetLastError();
+}
+hipError_t launch_step_gather(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
+                              const double *parts, double min_h, double *scal, double *hist, int advance_t,
+                              hipStream_t s) {
+    hipLaunchKernelGGL(k_step_gather, dim3(nblk(T.nv) + 1), dim3(256), 0, s, T, G, U, vU, P, parts, speed_parts(T.nc),
+                       min_h, scal, hist, advance_t);
     return hipGetLastError();
 }
 hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double min_h, double *scal, double *hist,

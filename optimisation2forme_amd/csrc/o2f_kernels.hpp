@@ -174,6 +174,10 @@ hipError_t launch_speed_max(const double *U, int nc, const Phys &P, double *part
 // dt from the speed partials, time advance, gate (one block)
 hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double min_h, double *scal, double *hist,
                              int advance_t, hipStream_t s);
+// The step begin and the first stage's vertex state gather in one launch.
+hipError_t launch_step_gather(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
+                              const double *parts, double min_h, double *scal, double *hist, int advance_t,
+                              hipStream_t s);
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
                        double *partials, double *scal, int op, hipStream_t s);
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s);
