@@ -245,6 +245,8 @@ def bench_warm_start(a):
                    "test_candidates": a.ws_test, "tol": a.ws_tol, "heat_flux": "fourier"},
         "train_field_steps_mean": float(np.mean([f[3] for f in fields])),
         "fields_s": t_fields, "train_s": t_train,
+        # useful flops of a training step: forward, input gradients, weight gradients
+        "train_tflops": 6.0 * (64 * 256 + 3 * 256 * 256 + 256 * 64) * a.ws_batch * a.ws_steps / t_train / 1e12,
         "loss_first_last": [float(losses[0]), float(np.mean(losses[-50:]))],
         "cold_steps": cold.tolist(), "warm_steps": warm.tolist(),
         "time_ratio": float(sum(r["cold_s"] for r in rows) / sum(r["warm_s"] for r in rows)),

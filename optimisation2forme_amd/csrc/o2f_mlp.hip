@@ -15,7 +15,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <random>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -63,70 +65,82 @@ struct Epi {
 };
 
 // C[M x N] = A[M x K] B[K x N]; A(m, k) = A[m sam + k sak], B(k, n) = B[k sbk + n sbn].
-// 256 threads = 4 waves as 2 x 2, each wave 32 x 32 = 2 x 2 MFMA tiles of
-// 16 x 16 x 32.  gridDim.z splits K into chunks of kchunk (partial outputs
-// at C + z M ldc).
+// A TM x TN workgroup tile: 256 threads = 4 waves as 2 x 2, each wave
+// (TM/2) x (TN/2) = (TM/32) x (TN/32) MFMA tiles of 16 x 16 x 32.  gridDim.z
+// splits K into chunks of kchunk (partial outputs at C + z M ldc).
+template <int TM, int TN>
 __global__ __launch_bounds__(256) void k_gemm(const u16 *__restrict__ A, long sam, long sak,
                                               const u16 *__restrict__ B, long sbk, long sbn, int M, int N,
                                               int K, int kchunk, Epi epi) {
-    __shared__ u16 As[kTile][kLdsRow];
-    __shared__ u16 Bt[kTile][kLdsRow];  // transposed: Bt[n][k]
+    constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
+    __shared__ u16 As[TM][kLdsRow];
+    __shared__ u16 Bt[TN][kLdsRow];  // transposed: Bt[n][k]
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
-    const int m0 = blockIdx.y * kTile, n0 = blockIdx.x * kTile;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
-    v4f acc[2][2];
+    v4f acc[FM][FN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
     for (int k0 = kb; k0 < ke; k0 += kTK) {
         // global -> LDS with 16-byte loads along whichever index is contiguous
         // (every index is inside the M x K and K x N operands: M, N are
-        // multiples of 64 and K of 32, checked by the host)
-        if (sak == 1) {  // A rows contiguous in k
-            const int r = t >> 2, k = (t & 3) * 8;
-            *reinterpret_cast<v8u *>(&As[r][k]) =
-                *reinterpret_cast<const v8u *>(A + (long)(m0 + r) * sam + (k0 + k));
-        } else {  // A contiguous in m: 8 rows of one k, transposed into LDS
-            const int k = t >> 3, r = (t & 7) * 8;
-            const v8u va = *reinterpret_cast<const v8u *>(A + (long)(k0 + k) * sak + (m0 + r));
+        // multiples of the tile and K of 32, checked by the host)
 #pragma unroll
-            for (int i = 0; i < 8; ++i) As[r + i][k] = va[i];
+        for (int it = 0; it < TM / 64; ++it) {
+            const int idx = t + 256 * it;
+            if (sak == 1) {  // A rows contiguous in k
+                const int r = idx >> 2, k = (idx & 3) * 8;
+                *reinterpret_cast<v8u *>(&As[r][k]) =
+                    *reinterpret_cast<const v8u *>(A + (long)(m0 + r) * sam + (k0 + k));
+            } else {  // A contiguous in m: 8 rows of one k, transposed into LDS
+                const int k = idx / (TM / 8), r = (idx % (TM / 8)) * 8;
+                const v8u va = *reinterpret_cast<const v8u *>(A + (long)(k0 + k) * sak + (m0 + r));
+#pragma unroll
+                for (int i = 0; i < 8; ++i) As[r + i][k] = va[i];
+            }
         }
-        if (sbk == 1) {  // B columns contiguous in k
-            const int n = t >> 2, k = (t & 3) * 8;
-            *reinterpret_cast<v8u *>(&Bt[n][k]) =
-                *reinterpret_cast<const v8u *>(B + (long)(n0 + n) * sbn + (k0 + k));
-        } else {  // B contiguous in n
-            const int k = t >> 3, n = (t & 7) * 8;
-            const v8u vb = *reinterpret_cast<const v8u *>(B + (long)(k0 + k) * sbk + (n0 + n));
 #pragma unroll
-            for (int i = 0; i < 8; ++i) Bt[n + i][k] = vb[i];
+        for (int it = 0; it < TN / 64; ++it) {
+            const int idx = t + 256 * it;
+            if (sbk == 1) {  // B columns contiguous in k
+                const int n = idx >> 2, k = (idx & 3) * 8;
+                *reinterpret_cast<v8u *>(&Bt[n][k]) =
+                    *reinterpret_cast<const v8u *>(B + (long)(n0 + n) * sbn + (k0 + k));
+            } else {  // B contiguous in n
+                const int k = idx / (TN / 8), n = (idx % (TN / 8)) * 8;
+                const v8u vb = *reinterpret_cast<const v8u *>(B + (long)(k0 + k) * sbk + (n0 + n));
+#pragma unroll
+                for (int i = 0; i < 8; ++i) Bt[n + i][k] = vb[i];
+            }
         }
         __syncthreads();
+        v8u bv[FN];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const v8u a = *reinterpret_cast<const v8u *>(&As[wm * 32 + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+        for (int j = 0; j < FN; ++j)
+            bv[j] = *reinterpret_cast<const v8u *>(&Bt[wn * WN + j * 16 + (lane & 15)][8 * (lane >> 4)]);
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const v8u b = *reinterpret_cast<const v8u *>(&Bt[wn * 32 + j * 16 + (lane & 15)][8 * (lane >> 4)]);
+        for (int i = 0; i < FM; ++i) {
+            const v8u a = *reinterpret_cast<const v8u *>(&As[wm * WM + i * 16 + (lane & 15)][8 * (lane >> 4)]);
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
-                                                                     __builtin_bit_cast(v8bf, b), acc[i][j], 0, 0,
-                                                                     0);
-            }
+                                                                     __builtin_bit_cast(v8bf, bv[j]), acc[i][j], 0,
+                                                                     0, 0);
         }
         __syncthreads();
     }
     // D[row = 4 (lane / 16) + r][col = lane % 16] of each 16 x 16 tile
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const long m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
-                const long n = n0 + wn * 32 + j * 16 + (lane & 15);
+                const long m = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+                const long n = n0 + wn * WN + j * 16 + (lane & 15);
                 const float v = acc[i][j][r];
                 const long o = m * epi.ldc + n;
                 if (epi.mode == kEpiHidden) {
@@ -300,6 +314,7 @@ struct o2f_mlp {
     float *raw = nullptr, *tgt = nullptr;
     long ndata = 0;
     long step = 0;
+    std::map<std::tuple<int, int, int, int, bool, bool, int>, int> tile_choice;
     hipStream_t stream = nullptr;
     std::string err;
     int fail(int code, const std::string &s) {
@@ -348,16 +363,59 @@ int ensure_cap(o2f_mlp *o, int rows) {
     return O2F_OK;
 }
 
+hipError_t launch_gemm(int variant, hipStream_t st, const u16 *A, long sam, long sak, const u16 *B, long sbk,
+                       long sbn, int M, int N, int K, int kchunk, const Epi &epi) {
+    const int splits = (K + kchunk - 1) / kchunk;
+    if (variant == 2)
+        hipLaunchKernelGGL((k_gemm<128, 128>), dim3(N / 128, M / 128, splits), dim3(256), 0, st, A, sam, sak, B, sbk,
+                           sbn, M, N, K, kchunk, epi);
+    else if (variant == 1)
+        hipLaunchKernelGGL((k_gemm<128, 64>), dim3(N / 64, M / 128, splits), dim3(256), 0, st, A, sam, sak, B, sbk,
+                           sbn, M, N, K, kchunk, epi);
+    else
+        hipLaunchKernelGGL((k_gemm<64, 64>), dim3(N / 64, M / 64, splits), dim3(256), 0, st, A, sam, sak, B, sbk,
+                           sbn, M, N, K, kchunk, epi);
+    return hipGetLastError();
+}
+
+// Workgroup tile per GEMM shape, chosen by timing the admissible tiles once
+// (each output element sums its k in the same order under every tile, so
+// the choice changes speed only).
 hipError_t gemm(o2f_mlp *o, const u16 *A, long sam, long sak, const u16 *B, long sbk, long sbn, int M, int N, int K,
                 int kchunk, const Epi &epi) {
     // the tile loaders need one unit stride per operand and whole tiles
     if (!((sak == 1 || sam == 1) && (sbk == 1 || sbn == 1)) || M % kTile || N % kTile || K % kTK ||
         kchunk % kTK)
         return hipErrorInvalidValue;
-    const int splits = (K + kchunk - 1) / kchunk;
-    hipLaunchKernelGGL(k_gemm, dim3(N / kTile, M / kTile, splits), dim3(256), 0, o->stream, A, sam, sak, B, sbk, sbn,
-                       M, N, K, kchunk, epi);
-    return hipGetLastError();
+    const auto key = std::make_tuple(M, N, K, kchunk, sak == 1, sbk == 1, epi.mode);
+    auto it = o->tile_choice.find(key);
+    if (it == o->tile_choice.end()) {
+        int best = 0;
+        float best_ms = -1.f;
+        hipEvent_t e0, e1;
+        hipError_t e;
+        if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) return e;
+        for (int v = 0; v < 3; ++v) {
+            if ((v >= 1 && M % 128) || (v == 2 && N % 128)) continue;
+            if ((e = launch_gemm(v, o->stream, A, sam, sak, B, sbk, sbn, M, N, K, kchunk, epi)) != hipSuccess) return e;
+            (void)hipEventRecord(e0, o->stream);
+            for (int r = 0; r < 3; ++r)
+                if ((e = launch_gemm(v, o->stream, A, sam, sak, B, sbk, sbn, M, N, K, kchunk, epi)) != hipSuccess)
+                    return e;
+            (void)hipEventRecord(e1, o->stream);
+            if ((e = hipEventSynchronize(e1)) != hipSuccess) return e;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (best_ms < 0.f || ms < best_ms) {
+                best_ms = ms;
+                best = v;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        it = o->tile_choice.emplace(key, best).first;
+    }
+    return launch_gemm(it->second, o->stream, A, sam, sak, B, sbk, sbn, M, N, K, kchunk, epi);
 }
 
 // Forward pass over `rows` rows of X[0] (a multiple of 64).
