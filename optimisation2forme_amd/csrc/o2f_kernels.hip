@@ -256,24 +256,32 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
             st4(fR, e, f);
         }
     }
-    if (k != 0) return;
-    if (Gout) {
+    if (Gout && k == 0) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             Gout[q * nc + c] = gx[q];
             Gout[(4 + q) * nc + c] = gy[q];
         }
     }
-    // viscous stress and heat flux of the cell (src/diffusion.cpp:227-240)
+    // viscous stress and heat flux of the cell (src/diffusion.cpp:227-240),
+    // the divisions spread over the four lanes: lane k forms one velocity
+    // derivative (u_x, v_y, u_y, v_x), lanes 0 / 1 one temperature
+    // derivative (T_x, T_y); every value is the same expression as in one
+    // lane, so the results are unchanged
     const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
     const double rho = u[0], E = u[3];
     const double uu = u[1] / rho, vvel = u[2] / rho;
-    const double u_x = (gx[1] - uu * gx[0]) / rho;
-    const double v_y = (gy[2] - vvel * gy[0]) / rho;
-    const double u_y = (gy[1] - uu * gy[0]) / rho;
-    const double v_x = (gx[2] - vvel * gx[0]) / rho;
-    const double T_x = (gamma - 1.0) * ((gx[3] * rho - E * gx[0]) / (rho * rho) - (uu * u_x + vvel * v_x));
-    const double T_y = (gamma - 1.0) * ((gy[3] * rho - E * gy[0]) / (rho * rho) - (uu * u_y + vvel * v_y));
+    const double num = (k == 0) ? (gx[1] - uu * gx[0])
+                       : (k == 1) ? (gy[2] - vvel * gy[0])
+                       : (k == 2) ? (gy[1] - uu * gy[0])
+                                  : (gx[2] - vvel * gx[0]);
+    const double d = num / rho;
+    const double u_x = from_lane<0>(d), v_y = from_lane<1>(d), u_y = from_lane<2>(d), v_x = from_lane<3>(d);
+    const double tn = (k & 1) ? (gy[3] * rho - E * gy[0]) : (gx[3] * rho - E * gx[0]);
+    const double tk = (k & 1) ? (uu * u_y + vvel * v_y) : (uu * u_x + vvel * v_x);
+    const double Tk = (gamma - 1.0) * (tn / (rho * rho) - tk);
+    const double T_x = from_lane<0>(Tk), T_y = from_lane<1>(Tk);
+    if (k != 0) return;
     const double t4[4] = {2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y), 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y),
                           mu * (u_y + v_x), -kappa * T_x};
     st4(tq, c, t4);
