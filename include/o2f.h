@@ -36,6 +36,12 @@ extern "C" {
 #define O2F_INTERP_REF 0 /* src/advection.cpp:26-131 exactly: valence >= 5 never written (stays 0) */
 #define O2F_INTERP_MVC 1 /* same, with the valence-4 mean-value weights generalised to any valence */
 
+/* How the derived geometry follows a vertex displacement (geometric
+ * derivatives below). */
+#define O2F_GEOM_REFERENCE 0 /* updateDerivedGeometry (src/shape_optimization.cpp:748-805): only edges touching
+                                a wall vertex and their left cells are refreshed */
+#define O2F_GEOM_FULL 1      /* every cell and edge follows its vertices (a re-built mesh) */
+
 typedef struct o2f_ctx o2f_ctx;
 
 /* Heat flux of the viscous term.  REF: q = -(mu c_p / Pr) (gamma-1) grad e
@@ -230,6 +236,40 @@ int o2f_bicgstab(o2f_ctx *ctx, double beta, double eps, double tol, int chunk, i
 int o2f_shape_gradient(o2f_ctx *ctx, const char *const *names_all, const double *values_all,
                        int n_all, const char *const *opt_names, int n_opt, const double *lambda,
                        double beta, double eps, double *grad);
+
+/* ---- geometric derivatives (forward mode) ----------------------------- */
+
+/* Wall-vertex displacement d(x, y)/d(param) of applyShapePerturbationNACA12
+ * (src/shape_optimization.cpp:807-890) for param "m" / "cambrure", "p" /
+ * "position_cambrure" or "t" (thickness: an extension, the reference
+ * perturbs only m and p): y = yc(m, p, x_r) + side * yt(t, x_r) with x
+ * fixed, zero for x_r >= 0.9 and off the wall.  names_all / values_all as
+ * for o2f_shape_gradient (pos_x, c, m, p, t defaults :830-834).  dxy
+ * (2*n_vertices, x then y per vertex) out.  O2F_ERR_ARG for another name. */
+int o2f_naca_displacement(o2f_ctx *ctx, const char *name, const char *const *names_all,
+                          const double *values_all, int n_all, double *dxy);
+/* Directional derivatives at the current state U (held fixed) along the
+ * vertex displacement dxy (2*n_vertices): dR = (dR/dX) dxy (4*n_cells SoA)
+ * and dJ = (dJ/dX) dxy for J = -C_L + beta C_D (computeDirectObjectiveTerm,
+ * src/shape_optimization.cpp:893-939), by forward-mode (dual number)
+ * differentiation of the geometry (host) and of the whole residual
+ * (device: interpolation weights, Green-Gauss, limiter, face states, HLLC
+ * with its moving normal, stresses, viscous and convective sums).  This is
+ * the eps -> 0 limit of computeShapeGradient's central differences
+ * (:942-1016) without the mesh copies.  geometry: O2F_GEOM_*.  R (may be
+ * NULL) receives the residual's value, bit-identical to o2f_residual's.
+ * Any of R, dR, dJ may be NULL.  Leaves the face states at the current
+ * state. */
+int o2f_residual_tangent(o2f_ctx *ctx, const double *dxy, int geometry, double beta, double *R, double *dR,
+                         double *dJ);
+/* computeShapeGradient's g_k = dJ/dk - lambda . dR/dk with both
+ * derivatives from o2f_residual_tangent along o2f_naca_displacement (one
+ * tangent evaluation per parameter instead of two perturbed residuals and
+ * mesh copies).  Arguments as o2f_shape_gradient; "t" is differentiable;
+ * other names give 0. */
+int o2f_shape_gradient_tangent(o2f_ctx *ctx, const char *const *names_all, const double *values_all, int n_all,
+                               const char *const *opt_names, int n_opt, const double *lambda, double beta,
+                               int geometry, double *grad);
 
 /* ---- run to convergence --------------------------------------------- */
 

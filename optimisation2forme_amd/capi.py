@@ -23,6 +23,7 @@ ERRORS = {-1: "O2F_ERR_ARG", -2: "O2F_ERR_BC", -3: "O2F_ERR_HIP", -4: "O2F_ERR_I
 LIMITERS = {"venkat": 0, "venkatakrishnan": 0, "barth_jespersen": 1, "bj": 1}
 INTERP_MODES = {"ref": 0, "mvc": 1}
 HEAT_FLUX = {"reference": 0, "ref": 0, "fourier": 1}
+GEOMETRY = {"reference": 0, "full": 1}  # O2F_GEOM_*
 
 # Every symbol include/o2f.h declares (checked by the CPU test suite).
 EXPORTS = [
@@ -38,7 +39,8 @@ EXPORTS = [
     "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps", "o2f_apply_jt_exact",
     "o2f_bicgstab_exact", "o2f_mlp_create", "o2f_mlp_destroy", "o2f_mlp_last_error", "o2f_mlp_num_params",
     "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
-    "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint",
+    "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_naca_displacement",
+    "o2f_residual_tangent", "o2f_shape_gradient_tangent",
 ]
 
 
@@ -112,6 +114,11 @@ def lib():
             "o2f_shape_gradient": (C.c_int, [vp, C.POINTER(C.c_char_p), D, C.c_int,
                                              C.POINTER(C.c_char_p), C.c_int, D, C.c_double,
                                              C.c_double, D]),
+            "o2f_naca_displacement": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_char_p), D, C.c_int, D]),
+            "o2f_residual_tangent": (C.c_int, [vp, D, C.c_int, C.c_double, D, D, D]),
+            "o2f_shape_gradient_tangent": (C.c_int, [vp, C.POINTER(C.c_char_p), D, C.c_int,
+                                                     C.POINTER(C.c_char_p), C.c_int, D, C.c_double,
+                                                     C.c_int, D]),
             "o2f_time_steps": (C.c_int, [vp, C.c_int, D, D]),
             "o2f_mesh_topology": (C.c_int, [C.POINTER(MeshDesc), I, I, I, C.c_char_p, D, D, I]),
             "o2f_run_until": (C.c_int, [vp, C.c_double, C.c_int, I, D, D, D]),
@@ -406,6 +413,34 @@ class Solver:
         g = np.zeros(max(len(opt_names), 1))
         self._check(lib().o2f_shape_gradient(self.h, na, _d(va), len(names_all), no, len(opt_names),
                                              _d(lam), beta, eps, _d(g)))
+        return g[:len(opt_names)]
+
+    def naca_displacement(self, name, names_all, values_all):
+        """d(x, y)/d(name) of the wall vertices, shape (n_vertices, 2)."""
+        na = (C.c_char_p * max(len(names_all), 1))(*[n.encode() for n in names_all])
+        va = np.ascontiguousarray(values_all if len(values_all) else [0.0], dtype=np.float64)
+        d = np.zeros((self.nv, 2))
+        self._check(lib().o2f_naca_displacement(self.h, name.encode(), na, _d(va), len(names_all), _d(d)))
+        return d
+
+    def residual_tangent(self, dxy, geometry="full", beta=1.0):
+        """(R, dR, dJ): the residual, its derivative along the vertex
+        displacement dxy (n_vertices x 2) and that of J = -C_L + beta C_D,
+        at the current state."""
+        dxy = np.ascontiguousarray(dxy, dtype=np.float64).reshape(self.nv, 2)
+        R, dR, dJ = np.zeros((4, self.nc)), np.zeros((4, self.nc)), C.c_double()
+        self._check(lib().o2f_residual_tangent(self.h, _d(dxy), GEOMETRY[geometry], beta, _d(R), _d(dR),
+                                               C.byref(dJ)))
+        return R, dR, dJ.value
+
+    def shape_gradient_tangent(self, names_all, values_all, opt_names, lam, beta, geometry="reference"):
+        na = (C.c_char_p * len(names_all))(*[n.encode() for n in names_all])
+        va = np.ascontiguousarray(values_all, dtype=np.float64)
+        no = (C.c_char_p * max(len(opt_names), 1))(*[n.encode() for n in opt_names])
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        g = np.zeros(max(len(opt_names), 1))
+        self._check(lib().o2f_shape_gradient_tangent(self.h, na, _d(va), len(names_all), no, len(opt_names),
+                                                     _d(lam), beta, GEOMETRY[geometry], _d(g)))
         return g[:len(opt_names)]
 
     def time_adjoint(self, nprod, exact):

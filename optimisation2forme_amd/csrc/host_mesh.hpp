@@ -64,6 +64,24 @@ double compute_min_h(const HostMesh &hm);
 bool perturb_naca(HostMesh &hm, int which, double pos_x, double c, double m0, double p0,
                   double t, double eps);
 
+// Derivative of the derived geometry along a vertex displacement dxy
+// (2 per vertex), in the layout of the HostMesh geometry arrays
+// (host_tangent.cpp).
+struct GeomTangent {
+    std::vector<double> cvol, ccx, ccy, elen, enx, eny, emx, emy, sw;
+};
+// geometry: O2F_GEOM_REFERENCE, only the edges touching a wall vertex and
+// their left cells follow the vertices (updateDerivedGeometry,
+// src/shape_optimization.cpp:748-805); O2F_GEOM_FULL, every cell and edge
+// does.  The interpolation weights follow the cell centres and vertices.
+void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomTangent &t);
+// d(vertex position)/d(parameter) of applyShapePerturbationNACA12
+// (src/shape_optimization.cpp:807-890): which 0 camber m, 1 camber
+// position p, 2 thickness t (y = yc + side * yt, x fixed, x_r >= 0.9
+// frozen).  Returns false for any other `which`.
+bool naca_displacement(const HostMesh &hm, int which, double pos_x, double c, double m0, double p0,
+                       std::vector<double> &dxy);
+
 // MSH 2.2 ASCII reader (Gmsh API element order, see o2f_msh2_read).
 bool read_msh2(const std::string &path, std::vector<double> &xy, std::vector<int> &cell_nv,
                std::vector<int> &cell_vtx, std::vector<int> &bnd_vtx, std::vector<char> &bnd_tag,

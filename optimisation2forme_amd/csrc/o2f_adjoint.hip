@@ -41,26 +41,6 @@ __device__ __forceinline__ int from_lane_int(int v, int j) {
     return static_cast<int>(r);
 }
 
-// Viscous stress and heat flux of one cell (src/diffusion.cpp:227-240), the
-// arithmetic of the forward reconstruction kernel, on any scalar type.
-template <class T>
-__device__ __forceinline__ void cell_stress_t(const T (&u)[4], const T (&gx)[4], const T (&gy)[4], double mu,
-                                              double gamma, double kappa, T (&out)[5]) {
-    const T rho = u[0], E = u[3];
-    const T uu = u[1] / rho, vvel = u[2] / rho;
-    const T u_x = (gx[1] - uu * gx[0]) / rho;
-    const T v_y = (gy[2] - vvel * gy[0]) / rho;
-    const T u_y = (gy[1] - uu * gy[0]) / rho;
-    const T v_x = (gx[2] - vvel * gx[0]) / rho;
-    const T T_x = (gamma - 1.0) * ((gx[3] * rho - E * gx[0]) / (rho * rho) - (uu * u_x + vvel * v_x));
-    const T T_y = (gamma - 1.0) * ((gy[3] * rho - E * gy[0]) / (rho * rho) - (uu * u_y + vvel * v_y));
-    out[0] = 2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y);
-    out[1] = 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y);
-    out[2] = mu * (u_y + v_x);
-    out[3] = -kappa * T_x;
-    out[4] = -kappa * T_y;
-}
-
 // Transpose of the wall state map (r, ru, rv, E) -> (r, 0, 0, E - (ru^2 +
 // rv^2) / (2 r)) of apply_inflow_outflow_bc (src/boundary.cpp:30-40) at w.
 __device__ __forceinline__ void wall_state_t(const double (&w)[4], const double (&s)[4], double (&o)[4]) {

@@ -327,7 +327,7 @@ void o2f_ctx_destroy(o2f_ctx *c) {
     void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->cinfo,
                     c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
                     c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx, c->Fe,
-                    c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work};
+                    c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work, c->t_work, c->t_out};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c->kv)
@@ -1039,6 +1039,151 @@ int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *v
         grad[k] = dJ_direct - h.v[kScalDotA];
     }
     // leave the context as computeResidual(mesh) left the reference's mesh
+    if ((st = residual_on(c, 0))) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+// ---- geometric derivatives ------------------------------------------------
+
+namespace {
+
+// getParamOrDefault (src/shape_optimization.cpp:101-109, defaults :830-834)
+struct NacaParams {
+    double pos_x, chord, m, p, t;
+};
+NacaParams naca_params(const char *const *names_all, const double *values_all, int n_all) {
+    auto param = [&](const char *key, double def) {
+        for (int i = 0; i < n_all; ++i)
+            if (std::strcmp(names_all[i], key) == 0) return values_all[i];
+        return def;
+    };
+    return {param("pos_x", 0.355), param("c", 1.0), param("m", 0.02), param("p", 0.4), param("t", 0.12)};
+}
+
+// 0 camber, 1 camber position, 2 thickness, -1 anything else
+int naca_which(const char *name) {
+    const std::string n = name ? name : "";
+    if (n == "m" || n == "cambrure") return 0;
+    if (n == "p" || n == "position_cambrure") return 1;
+    if (n == "t") return 2;
+    return -1;
+}
+
+int ensure_tangent(o2f_ctx *c) {
+    if (c->t_work) return O2F_OK;
+    const size_t nv = c->hm.nv, ne = c->hm.ne, nc = c->hm.nc;
+    HIPCHK(c, dalloc(c->t_work, 9 * nv + 12 * ne + 9 * nc));
+    HIPCHK(c, dalloc(c->t_out, 4));
+    return O2F_OK;
+}
+
+TanWork tan_work(o2f_ctx *c) {
+    const size_t nv = c->hm.nv, ne = c->hm.ne, nc = c->hm.nc;
+    double *p = c->t_work;
+    TanWork W;
+    W.dvU = p;
+    p += 4 * nv;
+    W.dvtq = p;
+    p += 5 * nv;
+    W.dfL = p;
+    p += 4 * ne;
+    W.dfR = p;
+    p += 4 * ne;
+    W.dFe = p;
+    p += 4 * ne;
+    W.dtq = p;
+    p += 5 * nc;
+    W.dR = p;
+    return W;
+}
+
+// The residual and its derivative along dxy at the current state: R in
+// c->R, dR in the tangent work; *dJ the objective's derivative.
+int tangent_on(o2f_ctx *c, const double *dxy, int geometry, double beta, double *dJ) {
+    if (geometry != O2F_GEOM_REFERENCE && geometry != O2F_GEOM_FULL)
+        return c->fail(O2F_ERR_ARG, "geometry must be O2F_GEOM_REFERENCE or O2F_GEOM_FULL");
+    int st = check_bc(c);
+    if (st) return st;
+    if ((st = ensure_tangent(c))) return st;
+    GeomTangent t;
+    geometry_tangent(c->hm, dxy, geometry, t);
+    GeomSet &gs = c->geo[2];
+    if (!gs.cvol) HIPCHK(c, gs.alloc(c->hm));
+    HIPCHK(c, gs.fill(t, c->stream));
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view(), dG = gs.view();
+    const TanWork W = tan_work(c);
+    HIPCHK(c, launch_residual_tangent(T, G, dG, c->phys, c->U, c->vU, c->fL, c->fR, c->tq, c->vtq, c->Fe, c->R, W,
+                                      c->stream));
+    ++c->faces_version;
+    HIPCHK(c, launch_objective_tangent(G, dG, c->wall, (int)c->hm.wall_edges.size(), c->fL, W.dfL, c->prm.gamma, 1,
+                                       c->t_out, c->stream));
+    double out[4];
+    HIPCHK(c, hipMemcpyAsync(out, c->t_out, sizeof out, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // J = -C_L + beta C_D with C = force / (q_inf c), c = 1 (objective_on)
+    const double corde = 1;
+    *dJ = -(out[2] / (c->q_inf * corde)) + beta * (out[3] / (c->q_inf * corde));
+    return O2F_OK;
+}
+
+}  // namespace
+
+int o2f_naca_displacement(o2f_ctx *c, const char *name, const char *const *names_all, const double *values_all,
+                          int n_all, double *dxy) {
+    if (!c || !name || !dxy || (n_all > 0 && (!names_all || !values_all))) return O2F_ERR_ARG;
+    const int which = naca_which(name);
+    if (which < 0) return c->fail(O2F_ERR_ARG, std::string("not a NACA shape parameter: ") + name);
+    const NacaParams np = naca_params(names_all, values_all, n_all);
+    std::vector<double> d;
+    naca_displacement(c->hm, which, np.pos_x, np.chord, np.m, np.p, d);
+    std::copy(d.begin(), d.end(), dxy);
+    return O2F_OK;
+}
+
+int o2f_residual_tangent(o2f_ctx *c, const double *dxy, int geometry, double beta, double *R, double *dR,
+                         double *dJ) {
+    if (!c || !dxy) return O2F_ERR_ARG;
+    double dj = 0.0;
+    int st = tangent_on(c, dxy, geometry, beta, &dj);
+    if (st) return st;
+    const size_t n = 4 * (size_t)c->hm.nc;
+    if ((st = copy_out(c, R, c->R, n))) return st;
+    if ((st = copy_out(c, dR, tan_work(c).dR, n))) return st;
+    if (dJ) *dJ = dj;
+    // leave the context as computeResidual(mesh) leaves it
+    if ((st = residual_on(c, 0))) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+int o2f_shape_gradient_tangent(o2f_ctx *c, const char *const *names_all, const double *values_all, int n_all,
+                               const char *const *opt_names, int n_opt, const double *lambda, double beta,
+                               int geometry, double *grad) {
+    if (!c || (n_all > 0 && (!names_all || !values_all)) || (n_opt > 0 && (!opt_names || !grad)) || !lambda)
+        return O2F_ERR_ARG;
+    const NacaParams np = naca_params(names_all, values_all, n_all);
+    int st = ensure_work(c);
+    if (st) return st;
+    const long n = 4L * c->hm.nc;
+    double *lam = c->kv[0];
+    HIPCHK(c, hipMemcpyAsync(lam, lambda, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    std::vector<double> dxy;
+    for (int k = 0; k < n_opt; ++k) {
+        const int which = naca_which(opt_names[k]);
+        if (which < 0) {
+            grad[k] = 0.0;
+            continue;
+        }
+        naca_displacement(c->hm, which, np.pos_x, np.chord, np.m, np.p, dxy);
+        double dJ = 0.0;
+        if ((st = tangent_on(c, dxy.data(), geometry, beta, &dJ))) return st;
+        HIPCHK(c, launch_dot2(lam, tan_work(c).dR, nullptr, nullptr, n, c->partials, c->scal, kOpStore, c->stream));
+        HostScal h;
+        if ((st = read_scal(c, h))) return st;
+        grad[k] = dJ - h.v[kScalDotA];
+    }
     if ((st = residual_on(c, 0))) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return O2F_OK;

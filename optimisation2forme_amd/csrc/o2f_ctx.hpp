@@ -40,7 +40,9 @@ struct GeomSet {
             return e;
         return hipSuccess;
     }
-    hipError_t fill(const HostMesh &hm, hipStream_t s) {
+    // M: a HostMesh, or a GeomTangent (the same arrays holding derivatives)
+    template <class M>
+    hipError_t fill(const M &hm, hipStream_t s) {
         auto cp = [&](double *d, const std::vector<double> &v) {
             return v.empty() ? hipSuccess
                              : hipMemcpyAsync(d, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, s);
@@ -94,6 +96,9 @@ struct o2f_ctx {
     int *a_veptr = nullptr, *a_vedge = nullptr, *a_cvtx = nullptr, *a_mtgt = nullptr;
     double *a_cvw = nullptr, *a_work = nullptr;
     int a_interp = -1;  // interp mode a_cvw was built for
+    // geometric tangent: derivative arrays of the residual's intermediates
+    // (allocated on first use) and its 4-double objective output
+    double *t_work = nullptr, *t_out = nullptr;
 
     int fail(int code, const std::string &msg) {
         err = msg;

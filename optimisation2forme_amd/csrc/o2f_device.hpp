@@ -16,6 +16,9 @@ namespace o2f {
 // same templated code as the forward values.
 struct Dual {
     double v, d;
+    Dual() = default;
+    __host__ __device__ constexpr Dual(double v_, double d_) : v(v_), d(d_) {}
+    __host__ __device__ constexpr explicit Dual(double v_) : v(v_), d(0.0) {}
 };
 __device__ __forceinline__ Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
 __device__ __forceinline__ Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
@@ -117,9 +120,11 @@ __device__ __forceinline__ void bc_ghost(const Phys &P, char bc, const double *L
 
 // compute_hllc_flux, src/advection.cpp:325-412.  (enx, eny) is the stored
 // normal (into the left cell); the flux is taken along -n.
-template <class T>
-__device__ __forceinline__ void hllc_t(const T *qL, const T *qR, double enx, double eny, double gamma, T *F) {
-    const double nx = -enx, ny = -eny;
+// T: the state type; N: the normal's type (double, or Dual when the
+// geometry itself is differentiated).
+template <class T, class N = double>
+__device__ __forceinline__ void hllc_t(const T *qL, const T *qR, N enx, N eny, double gamma, T *F) {
+    const N nx = -enx, ny = -eny;
     const T rho_L = qL[0];
     const T u_L = (qL[1] * (nx) + qL[2] * ny) / rho_L;
     const T v_L = (qL[1] * (-ny) + qL[2] * nx) / rho_L;
@@ -202,6 +207,80 @@ __device__ __forceinline__ double alpha_bj(double phi_i, double phi_pred, double
     else if (phi_pred < phi_i)
         alpha = smin(1.0, (phi_min - phi_i) / (phi_pred - phi_i));
     return alpha;
+}
+
+// Viscous stress and heat flux of one cell (src/diffusion.cpp:227-240), the
+// arithmetic of the forward reconstruction kernel, on any scalar type.
+template <class T>
+__device__ __forceinline__ void cell_stress_t(const T (&u)[4], const T (&gx)[4], const T (&gy)[4], double mu,
+                                              double gamma, double kappa, T (&out)[5]) {
+    const T rho = u[0], E = u[3];
+    const T uu = u[1] / rho, vvel = u[2] / rho;
+    const T u_x = (gx[1] - uu * gx[0]) / rho;
+    const T v_y = (gy[2] - vvel * gy[0]) / rho;
+    const T u_y = (gy[1] - uu * gy[0]) / rho;
+    const T v_x = (gx[2] - vvel * gx[0]) / rho;
+    const T T_x = (gamma - 1.0) * ((gx[3] * rho - E * gx[0]) / (rho * rho) - (uu * u_x + vvel * v_x));
+    const T T_y = (gamma - 1.0) * ((gy[3] * rho - E * gy[0]) / (rho * rho) - (uu * u_y + vvel * v_y));
+    out[0] = 2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y);
+    out[1] = 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y);
+    out[2] = mu * (u_y + v_x);
+    out[3] = -kappa * T_x;
+    out[4] = -kappa * T_y;
+}
+
+// Value part of a double or a Dual (branch decisions follow the value).
+__device__ __forceinline__ double vpart(double a) { return a; }
+__device__ __forceinline__ double vpart(Dual a) { return a.v; }
+
+// The limiters, the state boundary conditions and the ghost state on any
+// scalar type (the double versions above are what the forward kernels use;
+// these are the same expressions for the geometry tangent).
+template <class T>
+__device__ __forceinline__ T alpha_venkat_g(double phi_i, T phi_pred, double phi_min, double phi_max, T cellArea,
+                                            double k_eps) {
+    const T delta = phi_pred - phi_i;
+    if (fabs(vpart(delta)) < DBL_EPSILON) return T(1.0);
+    const T h = sqrt(cellArea / M_PI);
+    const T eps = k_eps * k_eps * k_eps * h * h * h;
+    const double a = (vpart(delta) > 0.0) ? (phi_max - phi_i) : (phi_min - phi_i);
+    const T alpha = (a * a + eps + 2.0 * a * delta) / (a * a + eps + 2.0 * delta * delta);
+    return (vpart(alpha) < 0.0) ? T(0.0) : (1.0 < vpart(alpha)) ? T(1.0) : alpha;
+}
+template <class T>
+__device__ __forceinline__ T alpha_bj_g(double phi_i, T phi_pred, double phi_min, double phi_max) {
+    T alpha = T(1.0);
+    if (fabs(vpart(phi_pred - phi_i)) < DBL_EPSILON) return alpha;
+    if (vpart(phi_pred) > phi_i)
+        alpha = smin_t(T(1.0), (phi_max - phi_i) / (phi_pred - phi_i));
+    else if (vpart(phi_pred) < phi_i)
+        alpha = smin_t(T(1.0), (phi_min - phi_i) / (phi_pred - phi_i));
+    return alpha;
+}
+template <class T>
+__device__ __forceinline__ void bc_state_g(const Phys &P, char bc, T (&q)[4]) {
+    if (bc == 'i' || bc == 'f') {
+        q[0] = T(P.fs_rho);
+        q[1] = T(P.fs_ru);
+        q[2] = T(P.fs_rv);
+        q[3] = T(P.fs_E);
+    } else if (bc == 'w') {
+        const T kinetic_old = 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0];
+        const T e_internal = q[3] - kinetic_old;
+        q[1] = T(0.0);
+        q[2] = T(0.0);
+        q[3] = e_internal;
+    }
+}
+template <class T>
+__device__ __forceinline__ void bc_ghost_g(const Phys &P, char bc, const T (&L)[4], T (&R)[4]) {
+    for (int q = 0; q < 4; ++q) R[q] = L[q];
+    if (bc == 'i' || bc == 'o' || bc == 'f') {
+        bc_state_g(P, bc, R);
+    } else if (bc == 'w') {
+        R[1] = -L[1];
+        R[2] = -L[2];
+    }
 }
 
 // Cell kernels run four lanes per cell, one per edge slot (triangles leave

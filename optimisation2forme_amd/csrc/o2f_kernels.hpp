@@ -153,6 +153,22 @@ hipError_t launch_adjoint_exact(const Topo &T, const Geom &G, const Phys &P, con
                                 const double *x, const double *U, const double *vU, const double *fL,
                                 const double *fR, const double *vtq, double *y, hipStream_t s);
 
+// ------------------------------------------------------------------------
+// Geometric tangent (o2f_tangent.hip): the residual at a fixed state and
+// its directional derivative along a geometry tangent dG (same layout as
+// the geometry).  Values go to the forward arrays (bit-identical to the
+// forward kernels), derivatives to the TanWork arrays of the same layouts.
+// ------------------------------------------------------------------------
+struct TanWork {
+    double *dvU, *dvtq, *dfL, *dfR, *dFe, *dtq, *dR;
+};
+hipError_t launch_residual_tangent(const Topo &T, const Geom &G, const Geom &dG, const Phys &P, const double *U,
+                                   double *vU, double *fL, double *fR, double *tq, double *vtq, double *Fe,
+                                   double *R, const TanWork &W, hipStream_t s);
+// out = L, D, dL, dD over the wall edges (skip_x: computeDirectObjectiveTerm's x > 1.3 rule)
+hipError_t launch_objective_tangent(const Geom &G, const Geom &dG, const int *wall, int nw, const double *fL,
+                                    const double *dfL, double gamma, int skip_x, double *out, hipStream_t s);
+
 hipError_t launch_vertex_state(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
                                hipStream_t s);
 hipError_t launch_vertex_visc(const Topo &T, const Geom &G, const double *tq, double *vtq, const Phys &P,
