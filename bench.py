@@ -39,6 +39,12 @@ launch per stage kernel (autotuned shapes, graph replay), against the same
 times J^T x products (reference-form operator and exact adjoint) on the
 transonic workload.
 
+    python bench.py --shape-gradient
+
+times computeShapeGradient per parameter on the transonic 200k mesh at zero
+incidence: the reference's central differences against the forward-mode
+geometric tangent.
+
     python bench.py --warm-start
 
 measures the learned warm start: training fields from the solver (seeded
@@ -279,6 +285,50 @@ def bench_adjoint(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_shape_gradient(a):
+    """computeShapeGradient on the transonic 200k mesh at zero incidence
+    (the reference's NACA perturbation acts on an unrotated airfoil) after
+    50 steps, for a fixed seeded lambda: the reference's central differences
+    (two perturbed residuals and geometry uploads per parameter) against
+    the forward-mode tangent (one dual residual per parameter)."""
+    from optimisation2forme_amd import capi, mesher
+    flow = workload_params(a.limiter)
+    kw = dict(flow)
+    kw["cfl_number"] = kw.pop("cfl")
+    kw["inflow_velocity"] = kw.pop("mach")
+    desc = mesher.transonic_200k(aoa_deg=0.0)
+    s = capi.Solver(desc, capi.make_params(**kw))
+    s.init_freestream()
+    s.run_steps(50)
+    s.residual()
+    lam = np.random.default_rng(5).standard_normal((4, s.nc)) * 1e-6
+    names, vals = ["pos_x", "c", "m", "p", "t"], [0.355, 1.0, 0.02, 0.4, 0.12]
+    n = max(1, a.steps)
+    s.shape_gradient(names, vals, ["m", "p"], lam, 0.1)             # warm-up
+    s.shape_gradient_tangent(names, vals, ["m", "p", "t"], lam, 0.1)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        g_fd = s.shape_gradient(names, vals, ["m", "p"], lam, 0.1)
+    fd_ms = (time.perf_counter() - t0) * 1e3 / (2 * n)
+    t0 = time.perf_counter()
+    for _ in range(n):
+        g_tan = s.shape_gradient_tangent(names, vals, ["m", "p", "t"], lam, 0.1)
+    tan_ms = (time.perf_counter() - t0) * 1e3 / (3 * n)
+    # central differences err by c eps^2 (plus kinks of the limiter):
+    # Richardson's extrapolation from eps = 1e-5 and 1e-6
+    g_fd5 = s.shape_gradient(names, vals, ["m", "p"], lam, 0.1, eps=1e-5)
+    rich = [float(g_fd[k] - (g_fd5[k] - g_fd[k]) / 99.0) for k in range(2)]
+    out = {"metric": "shape gradient", "unit": "ms per parameter", "higher_is_better": False,
+           "n_gpus": 1, "dtype": "f64", "steps": n,
+           "config": {"workload": "naca2412_transonic_200k_triangles_aoa0", "cells": s.nc},
+           "finite_difference_ms": fd_ms, "tangent_ms": tan_ms,
+           "grad_fd": [float(v) for v in g_fd], "grad_fd_eps1e-5": [float(v) for v in g_fd5],
+           "grad_fd_richardson": rich, "grad_tangent": [float(v) for v in g_tan],
+           "rel_diff_m_p": [float(abs(g_tan[k] - g_fd[k]) / abs(g_fd[k])) for k in range(2)],
+           "rel_diff_m_p_richardson": [float(abs(g_tan[k] - rich[k]) / abs(rich[k])) for k in range(2)]}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,6 +343,8 @@ def main():
     ap.add_argument("--batch", action="store_true", help="the 64-candidate batched workload")
     ap.add_argument("--warm-start", action="store_true", help="the learned warm-start comparison")
     ap.add_argument("--adjoint", action="store_true", help="J^T x product timings (reference form, exact)")
+    ap.add_argument("--shape-gradient", action="store_true",
+                    help="shape gradient timings (finite differences, forward-mode tangent)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch / warm-start modes: direct launches instead of graph replay (for tracing)")
     ap.add_argument("--ws-train", type=int, default=32)
@@ -310,6 +362,11 @@ def main():
         if a.steps == ap.get_default("steps"):
             a.steps = 50
         bench_adjoint(a)
+        return
+    if a.shape_gradient and a.gpus == 1:
+        if a.steps == ap.get_default("steps"):
+            a.steps = 5
+        bench_shape_gradient(a)
         return
     if a.copy_calibration:
         copy_calibration()

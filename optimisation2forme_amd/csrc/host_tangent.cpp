@@ -110,9 +110,17 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
     t.emx.assign(hm.ne, 0.0);
     t.emy.assign(hm.ne, 0.0);
     t.sw.assign(hm.sw.size(), 0.0);
-    // which cells and edges follow their vertices
-    std::vector<char> cell_on(hm.nc, geometry == O2F_GEOM_FULL), edge_on(hm.ne, geometry == O2F_GEOM_FULL);
-    if (geometry != O2F_GEOM_FULL) {
+    // which vertices move; which cells and edges follow their vertices
+    // (entities none of whose vertices move have a zero derivative and are
+    // skipped)
+    std::vector<char> moved(hm.nv, 0);
+    for (int v = 0; v < hm.nv; ++v) moved[v] = dxy[2 * v] != 0.0 || dxy[2 * v + 1] != 0.0;
+    std::vector<char> cell_on(hm.nc, 0), edge_on(hm.ne, 0);
+    if (geometry == O2F_GEOM_FULL) {
+        for (int c = 0; c < hm.nc; ++c)
+            for (int k = 0; k < hm.cnv[c]; ++k) cell_on[c] |= moved[hm.cvtx[4 * c + k]];
+        for (int e = 0; e < hm.ne; ++e) edge_on[e] = moved[hm.eva[e]] || moved[hm.evb[e]];
+    } else {
         // updateDerivedGeometry (src/shape_optimization.cpp:748-805): edges
         // touching a wall vertex and their left cells
         for (int e = 0; e < hm.ne; ++e)
@@ -122,6 +130,7 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
             }
     }
     std::vector<HD> cxd(hm.nc), cyd(hm.nc);
+    std::vector<char> centre_moved(hm.nc, 0);
     for (int c = 0; c < hm.nc; ++c) {
         cxd[c] = {hm.ccx[c], 0.0};
         cyd[c] = {hm.ccy[c], 0.0};
@@ -133,6 +142,7 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
         t.ccy[c] = cy.d;
         cxd[c].d = cx.d;
         cyd[c].d = cy.d;
+        centre_moved[c] = cx.d != 0.0 || cy.d != 0.0;
     }
     for (int e = 0; e < hm.ne; ++e)
         if (edge_on[e]) edge_geom_d(hm, dxy, e, t);
@@ -142,6 +152,9 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
     for (int v = 0; v < hm.nv; ++v) {
         if (hm.skind[v] != kStencilWsum) continue;
         const int b = hm.vptr[v], n = hm.vptr[v + 1] - b;
+        bool any = moved[v];
+        for (int k = 0; k < n && !any; ++k) any = centre_moved[hm.scell[b + k]];
+        if (!any) continue;  // neither the vertex nor a stencil centre moves
         const HD px = vxd(hm, dxy, v), py = vyd(hm, dxy, v);
         if (n == 3) {
             const int i0 = hm.scell[b], i1 = hm.scell[b + 1], i2 = hm.scell[b + 2];

@@ -19,6 +19,7 @@
 //   kt_edge_flux     per edge:   HLLC with a moving normal
 //   kt_flux_update   per cell:   convective and viscous sums -> R, dR
 //   kt_objective     one block:  wall forces -> L, D and their derivatives
+//                                (terms in parallel, summed in wall order)
 #include "o2f_kernels.hpp"
 
 #include "o2f_device.hpp"
@@ -269,28 +270,53 @@ __global__ __launch_bounds__(256) void kt_flux_update(Topo T, Geom G, Geom dG, P
 }
 
 // Wall forces of computeLiftDragCoefficients / computeDirectObjectiveTerm
-// (src/compute.cpp:92-132, src/shape_optimization.cpp:893-939) in wall-list
-// order (the order of k_wall_forces): out = L, D, dL, dD.
-__global__ __launch_bounds__(64) void kt_objective(Geom G, Geom dG, const int *__restrict__ wall, int nw,
-                                                   const double *__restrict__ fL, const double *__restrict__ dfL,
-                                                   double gamma, int skip_x, double *__restrict__ out) {
-    if (threadIdx.x != 0) return;
+// (src/compute.cpp:92-132, src/shape_optimization.cpp:893-939): each
+// work-item forms one wall edge's terms, one work-item adds them in
+// wall-list order (the order of k_wall_forces): out = L, D, dL, dD.
+__global__ __launch_bounds__(kRedBlock) void kt_objective(Geom G, Geom dG, const int *__restrict__ wall, int nw,
+                                                          const double *__restrict__ fL,
+                                                          const double *__restrict__ dfL, double gamma,
+                                                          int skip_x, double *__restrict__ out) {
+    __shared__ double tl[kRedBlock], td[kRedBlock], tdl[kRedBlock], tdd[kRedBlock];
+    __shared__ int flag[kRedBlock];
     Dual L(0.0), D(0.0);
-    for (int k = 0; k < nw; ++k) {
-        const int e = wall[k];
-        if (skip_x && G.emx[e] > 1.3) continue;
-        const Dual nx = gd(G.enx, dG.enx, e), ny = gd(G.eny, dG.eny, e), A = gd(G.elen, dG.elen, e);
-        Dual q[4];
-        ld4d(fL, dfL, e, q);
-        const Dual p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
-        const double txx = 0.0, txy = 0.0, tyy = 0.0;
-        L = L + (-p * ny + txy * nx + tyy * ny) * A;
-        D = D + (-p * nx + txx * nx + txy * ny) * A;
+    for (int base = 0; base < nw; base += kRedBlock) {
+        const int k = base + threadIdx.x;
+        Dual l(0.0), d(0.0);
+        bool use = false;
+        if (k < nw) {
+            const int e = wall[k];
+            use = !(skip_x && G.emx[e] > 1.3);
+            const Dual nx = gd(G.enx, dG.enx, e), ny = gd(G.eny, dG.eny, e), A = gd(G.elen, dG.elen, e);
+            Dual q[4];
+            ld4d(fL, dfL, e, q);
+            const Dual p = (gamma - 1.0) * (q[3] - 0.5 * (q[1] * q[1] + q[2] * q[2]) / q[0]);
+            const double txx = 0.0, txy = 0.0, tyy = 0.0;
+            l = (-p * ny + txy * nx + tyy * ny) * A;
+            d = (-p * nx + txx * nx + txy * ny) * A;
+        }
+        tl[threadIdx.x] = l.v;
+        td[threadIdx.x] = d.v;
+        tdl[threadIdx.x] = l.d;
+        tdd[threadIdx.x] = d.d;
+        flag[threadIdx.x] = use;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int m = min(kRedBlock, nw - base);
+            for (int j = 0; j < m; ++j)
+                if (flag[j]) {
+                    L = L + Dual(tl[j], tdl[j]);
+                    D = D + Dual(td[j], tdd[j]);
+                }
+        }
+        __syncthreads();
     }
-    out[0] = L.v;
-    out[1] = D.v;
-    out[2] = L.d;
-    out[3] = D.d;
+    if (threadIdx.x == 0) {
+        out[0] = L.v;
+        out[1] = D.v;
+        out[2] = L.d;
+        out[3] = D.d;
+    }
 }
 
 inline int nblk(long n) { return static_cast<int>((n + 255) / 256); }
@@ -312,7 +338,7 @@ hipError_t launch_residual_tangent(const Topo &T, const Geom &G, const Geom &dG,
 
 hipError_t launch_objective_tangent(const Geom &G, const Geom &dG, const int *wall, int nw, const double *fL,
                                     const double *dfL, double gamma, int skip_x, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(kt_objective, dim3(1), dim3(64), 0, s, G, dG, wall, nw, fL, dfL, gamma, skip_x, out);
+    hipLaunchKernelGGL(kt_objective, dim3(1), dim3(kRedBlock), 0, s, G, dG, wall, nw, fL, dfL, gamma, skip_x, out);
     return hipGetLastError();
 }
 

@@ -30,7 +30,7 @@ def _csv(path_or_dir, suffix):
 
 def short_name(name):
     """'void o2f::k_cell_reconstruct(o2f::Topo, ...)' -> 'o2f::k_cell_reconstruct'."""
-    n = name.split("(")[0]
+    n = name.replace("(anonymous namespace)::", "").split("(")[0]
     if n.startswith("void "):
         n = n[5:]
     return n.strip()
