@@ -216,10 +216,20 @@ int o2f_apply_jt(o2f_ctx *ctx, const double *x, double *y, double eps);
  * through its active branch, HLLC, ghosts, stresses, heat flux); limiter
  * state is recomputed per cell, not stored.  x, y are 4*n_cells SoA. */
 int o2f_apply_jt_exact(o2f_ctx *ctx, const double *x, double *y);
+/* The exact (dJ/dU)^T of J = -C_L + beta C_D as computeDirectObjectiveTerm
+ * evaluates it (src/shape_optimization.cpp:893-939: pressure of the wall
+ * face L states, wall edges with centre x > 1.3 skipped), differentiated
+ * through the face-state reconstruction (limited gradients, interpolation,
+ * boundary states) by the exact adjoint chain.  assembleAdjointRHS
+ * (:491-554) keeps only the face-state derivative, taken as the cell
+ * state's, on every wall edge.  b (4*n_cells SoA) out. */
+int o2f_adjoint_rhs_exact(o2f_ctx *ctx, double beta, double *b);
 /* solveAdjointSystem's BiCGSTAB (chunks, true residual, same error codes as
- * o2f_bicgstab) on the exact operator of o2f_apply_jt_exact. */
+ * o2f_bicgstab) on the exact operator of o2f_apply_jt_exact; right-hand
+ * side assembleAdjointRHS's (exact_rhs = 0) or o2f_adjoint_rhs_exact's
+ * (exact_rhs = 1). */
 int o2f_bicgstab_exact(o2f_ctx *ctx, double beta, double tol, int chunk, int max_total, double *lambda,
-                       int *iters, double *relres);
+                       int *iters, double *relres, int exact_rhs);
 /* solveAdjointSystem (src/shape_optimization.cpp:658-740): BiCGSTAB with an
  * identity preconditioner on J^T lambda = b, restarted every `chunk`
  * iterations up to `max_total`, true residual checked after each chunk.

@@ -40,7 +40,7 @@ EXPORTS = [
     "o2f_bicgstab_exact", "o2f_mlp_create", "o2f_mlp_destroy", "o2f_mlp_last_error", "o2f_mlp_num_params",
     "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
     "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_naca_displacement",
-    "o2f_residual_tangent", "o2f_shape_gradient_tangent",
+    "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
 ]
 
 
@@ -136,7 +136,8 @@ def lib():
             "o2f_mlp_forward": (C.c_int, [vp, F, C.c_long, F]),
             "o2f_mlp_loss_grad": (C.c_int, [vp, F, F, C.c_long, F, F]),
             "o2f_mlp_warm_start": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]),
-            "o2f_bicgstab_exact": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D]),
+            "o2f_bicgstab_exact": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D, C.c_int]),
+            "o2f_adjoint_rhs_exact": (C.c_int, [vp, C.c_double, D]),
             "o2f_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
             "o2f_batch_destroy": (None, [vp]),
             "o2f_batch_last_error": (C.c_char_p, [vp]),
@@ -388,10 +389,11 @@ class Solver:
         self._check(lib().o2f_apply_jt_exact(self.h, _d(x), _d(y)))
         return y
 
-    def bicgstab_exact(self, beta, tol=1e-6, chunk=500, max_total=100000, check=True):
+    def bicgstab_exact(self, beta, tol=1e-6, chunk=500, max_total=100000, check=True, exact_rhs=False):
         lam = np.zeros((4, self.nc))
         it, rr = C.c_int(), C.c_double()
-        st = lib().o2f_bicgstab_exact(self.h, beta, tol, chunk, max_total, _d(lam), C.byref(it), C.byref(rr))
+        st = lib().o2f_bicgstab_exact(self.h, beta, tol, chunk, max_total, _d(lam), C.byref(it), C.byref(rr),
+                                      int(exact_rhs))
         if check:
             self._check(st)
         return lam, it.value, rr.value, st
@@ -414,6 +416,12 @@ class Solver:
         self._check(lib().o2f_shape_gradient(self.h, na, _d(va), len(names_all), no, len(opt_names),
                                              _d(lam), beta, eps, _d(g)))
         return g[:len(opt_names)]
+
+    def adjoint_rhs_exact(self, beta):
+        """(dJ/dU)^T through the reconstruction (4 x n_cells)."""
+        b = np.zeros((4, self.nc))
+        self._check(lib().o2f_adjoint_rhs_exact(self.h, beta, _d(b)))
+        return b
 
     def naca_displacement(self, name, names_all, values_all):
         """d(x, y)/d(name) of the wall vertices, shape (n_vertices, 2)."""

@@ -142,6 +142,33 @@ __global__ __launch_bounds__(256) void ka_edge(Topo T, Geom G, Phys P, const dou
     W.th[4 * (long)ne + e] = zero ? 0.0 : 0.5 * tb[4];
 }
 
+// Objective seed: dJ/d(face L state) of J = -C_L + beta C_D over the wall
+// edges computeDirectObjectiveTerm counts (src/shape_optimization.cpp:
+// 893-939: p from the L state, edges with centre x > 1.3 skipped, wall
+// stresses zero), added to the adjoint face states.  With x = 0 the rest of
+// the chain then carries it to (dJ/dU)^T through the reconstruction.
+__global__ __launch_bounds__(256) void ka_seed_objective(Geom G, const int *__restrict__ wall, int nw,
+                                                         const double *__restrict__ fL, double gamma,
+                                                         double q_inf, double beta, AdjWork W) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nw) return;
+    const int e = wall[k];
+    if (G.emx[e] > 1.3) return;
+    const double nx = G.enx[e], ny = G.eny[e], A = G.elen[e];
+    double q[4], b[4];
+    ld4(fL, e, q);
+    ld4(W.fbL, e, b);
+    const double r = q[0], ru = q[1], rv = q[2];
+    // J = sum (p ny A - beta p nx A) / (q_inf c), c = 1
+    const double dJdp = (ny * A - beta * nx * A) / q_inf;
+    const double g1 = gamma - 1.0;
+    b[0] += dJdp * g1 * 0.5 * (ru * ru + rv * rv) / (r * r);
+    b[1] += dJdp * (-g1 * ru / r);
+    b[2] += dJdp * (-g1 * rv / r);
+    b[3] += dJdp * g1;
+    st4(W.fbL, e, b);
+}
+
 // ------------------------------------------------------------- vertices ---
 __global__ __launch_bounds__(256) void ka_vertex_visc(Topo T, AdjTopo A, AdjWork W) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -502,10 +529,16 @@ static inline unsigned nb256(long n) { return (unsigned)((n + 255) / 256); }
 
 hipError_t launch_adjoint_exact(const Topo &T, const Geom &G, const Phys &P, const AdjTopo &A, const AdjWork &W,
                                 const double *x, const double *U, const double *vU, const double *fL,
-                                const double *fR, const double *vtq, double *y, hipStream_t s) {
+                                const double *fR, const double *vtq, double *y, hipStream_t s,
+                                const ObjSeed *seed) {
     hipLaunchKernelGGL(ka_edge, dim3(nb256(T.ne)), dim3(256), 0, s, T, G, P, x, U, fL, fR, W);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (seed && seed->nw > 0) {
+        hipLaunchKernelGGL(ka_seed_objective, dim3(nb256(seed->nw)), dim3(256), 0, s, G, seed->wall, seed->nw, fL,
+                           P.gamma, seed->q_inf, seed->beta, W);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(ka_vertex_visc, dim3(nb256(T.nv)), dim3(256), 0, s, T, A, W);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     hipLaunchKernelGGL(ka_cell, dim3(nb256((long)kLanesPerCell * T.nc)), dim3(256), 0, s, T, G, P, A, x, U, vU, vtq,

@@ -818,12 +818,15 @@ namespace {
 // solveAdjointSystem's loop (src/shape_optimization.cpp:690-736): chunks of
 // `chunk` iterations restarted from the current x, the true residual checked
 // after each; the RHS is assembled at the current face states.
+// The right-hand side of a solve: b (4 nc, device) for beta.
+using RhsFn = std::function<int(double beta, double *b)>;
+
 int bicgstab_solve(o2f_ctx *c, double beta, double tol, int chunk, int max_total, double *lambda, int *iters,
-                   double *relres, const MatVec &mv) {
+                   double *relres, const MatVec &mv, const RhsFn &rhs = nullptr) {
     int st;
     const long n = 4L * c->hm.nc;
     double *b = c->kv[0], *x = c->kv[8], *tmp = c->kv[9];
-    if ((st = rhs_device(c, beta, b))) return st;
+    if ((st = rhs ? rhs(beta, b) : rhs_device(c, beta, b))) return st;
     HIPCHK(c, launch_fill(n, x, 0.0, c->stream));
     HostScal h;
     HIPCHK(c, launch_dot2(b, b, nullptr, nullptr, n, c->partials, c->scal, kOpStore, c->stream));
@@ -980,13 +983,43 @@ int o2f_apply_jt_exact(o2f_ctx *c, const double *x, double *y) {
     return copy_out(c, y, c->kv[1], n);
 }
 
+namespace {
+
+// (dJ/dU)^T through the reconstruction: the exact adjoint chain with x = 0
+// and the objective's face-state derivative as its seed.  Work vector
+// kv[8] is used as the zero x (the solve fills it with its zero start
+// afterwards anyway).
+int rhs_exact_device(o2f_ctx *c, double beta, double *b) {
+    const long n = 4L * c->hm.nc;
+    double *zero = c->kv[8];
+    HIPCHK(c, launch_fill(n, zero, 0.0, c->stream));
+    const AdjTopo A{c->a_veptr, c->a_vedge, c->a_cvtx, c->a_cvw};
+    const ObjSeed seed{c->wall, (int)c->hm.wall_edges.size(), c->q_inf, beta};
+    HIPCHK(c, launch_adjoint_exact(c->topo(), c->geo[0].view(), c->phys, A, adj_work(c), zero, c->U, c->vU, c->fL,
+                                   c->fR, c->vtq, b, c->stream, &seed));
+    return O2F_OK;
+}
+
+}  // namespace
+
+int o2f_adjoint_rhs_exact(o2f_ctx *c, double beta, double *b) {
+    if (!c || !b) return O2F_ERR_ARG;
+    int st = ensure_adjoint(c);
+    if (st) return st;
+    if ((st = residual_on(c, 0))) return st;
+    if ((st = rhs_exact_device(c, beta, c->kv[0]))) return st;
+    return copy_out(c, b, c->kv[0], 4 * (size_t)c->hm.nc);
+}
+
 int o2f_bicgstab_exact(o2f_ctx *c, double beta, double tol, int chunk, int max_total, double *lambda, int *iters,
-                       double *relres) {
+                       double *relres, int exact_rhs) {
     if (!c || !lambda || chunk <= 0) return O2F_ERR_ARG;
     int st = ensure_adjoint(c);
     if (st) return st;
     if ((st = residual_on(c, 0))) return st;
-    return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, exact_matvec(c));
+    RhsFn rhs = nullptr;
+    if (exact_rhs) rhs = [c](double beta_, double *b) { return rhs_exact_device(c, beta_, b); };
+    return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, exact_matvec(c), rhs);
 }
 
 int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *values_all, int n_all,

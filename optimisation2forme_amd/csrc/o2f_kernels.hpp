@@ -149,9 +149,17 @@ struct AdjWork {
     double *ha;         // half the adjoint edge values (AoS 4)
     double *vi;         // adjoint interpolated vertex states (AoS 4)
 };
+// seed: when given (with x = 0), the objective's face-state derivative is
+// added after the edge step, and y = (dJ/dU)^T exactly.
+struct ObjSeed {
+    const int *wall;
+    int nw;
+    double q_inf, beta;
+};
 hipError_t launch_adjoint_exact(const Topo &T, const Geom &G, const Phys &P, const AdjTopo &A, const AdjWork &W,
                                 const double *x, const double *U, const double *vU, const double *fL,
-                                const double *fR, const double *vtq, double *y, hipStream_t s);
+                                const double *fR, const double *vtq, double *y, hipStream_t s,
+                                const ObjSeed *seed = nullptr);
 
 // ------------------------------------------------------------------------
 // Geometric tangent (o2f_tangent.hip): the residual at a fixed state and

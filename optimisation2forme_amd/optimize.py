@@ -12,7 +12,10 @@ the GPU:
   instead of a state file (the reference chains state files the same way);
 - the flow runs a fixed number of steps or to a density-residual tolerance;
 - the adjoint is the reference's operator (BiCGSTAB on the FD HLLC blocks)
-  or the exact one;
+  or the exact one, with the reference's right-hand side or the exact
+  (dJ/dU)^T;
+- the gradient's geometric part is the reference's central difference or
+  the forward-mode tangent (its eps -> 0 limit, no mesh copies);
 - the parameter file and the shape log keep the reference's formats.
 """
 import os
@@ -104,11 +107,14 @@ def log_line(J, names, grads):
 
 
 def run(param_file, iterations, flow, out_dir, t=None, mesh=None, steps=2000, tol=None, adjoint="reference",
-        adjoint_tol=1e-6, adjoint_chunk=500, adjoint_max=100000, require_adjoint=True):
+        adjoint_tol=1e-6, adjoint_chunk=500, adjoint_max=100000, require_adjoint=True, exact_rhs=False,
+        gradient="reference"):
     """`iterations` outer iterations from the parameter file `param_file`
     (its `m`, `p`, `t`, `beta`; the marked keys are differentiated).
     flow: make_params keyword arguments; mesh: naca_mesh keyword arguments
-    (n_around, n_radial, wall_spacing, radius).  Writes
+    (n_around, n_radial, wall_spacing, radius).  adjoint: "reference" or
+    "exact" (exact_rhs: the exact right-hand side with it); gradient:
+    "reference" (central differences) or "tangent".  Writes
     shape_params_<i>.txt and shape_opt_<i>.log under out_dir and returns the
     history [(i, m, p, J, {name: grad}, adjoint iterations, relres)]."""
     os.makedirs(out_dir, exist_ok=True)
@@ -133,13 +139,17 @@ def run(param_file, iterations, flow, out_dir, t=None, mesh=None, steps=2000, to
         s.residual()                    # computeResidual before the adjoint
         J = s.objective(beta)
         if adjoint == "exact":
-            lam, its, rr, st = s.bicgstab_exact(beta, adjoint_tol, adjoint_chunk, adjoint_max, check=require_adjoint)
+            lam, its, rr, st = s.bicgstab_exact(beta, adjoint_tol, adjoint_chunk, adjoint_max, check=require_adjoint,
+                                                exact_rhs=exact_rhs)
         else:
             lam, its, rr, st = s.bicgstab(beta, 1e-6, adjoint_tol, adjoint_chunk, adjoint_max, check=require_adjoint)
         vals = dict(values)
         vals.update(m=m, p=p, t=t)
         names_all = list(vals)
-        g = s.shape_gradient(names_all, [vals[k] for k in names_all], marked, lam, beta)
+        if gradient == "tangent":
+            g = s.shape_gradient_tangent(names_all, [vals[k] for k in names_all], marked, lam, beta)
+        else:
+            g = s.shape_gradient(names_all, [vals[k] for k in names_all], marked, lam, beta)
         grads = dict(zip(marked, g))
         with open(os.path.join(out_dir, "shape_opt_%d.log" % i), "w") as f:
             f.write(log_line(J, marked, g))

@@ -214,3 +214,95 @@ def test_exact_bicgstab_solves_the_dense_system():
     lv = lam.reshape(-1)
     assert np.linalg.norm(M.T @ lv - b) <= 1e-8 * np.linalg.norm(b)
     assert np.linalg.norm(J.T @ lv - b) <= 5e-2 * np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("limiter", [0, 1])
+def test_exact_rhs_every_entry_against_finite_differences(limiter):
+    """(dJ/dU)^T of J = -C_L + beta C_D (pressure of the wall face states,
+    centre x > 1.3 skipped) against central differences of the oracle's
+    objective over every state entry of the 160-triangle mesh, whose
+    trailing edge crosses x = 1.3.  The reference's right-hand side keeps
+    only the face-state derivative of every wall edge, so it must differ."""
+    desc = H.tiny_mesh()
+    fl = flow(limiter)
+    m, p, U = state(desc, fl)
+    beta = 0.3
+    nc = U.shape[1]
+    scale = np.max(np.abs(U), axis=1)
+
+    def J_at(Us):
+        m.set_state(Us)
+        m.residual(p)
+        return m.objective(p, beta)
+
+    J0 = J_at(U)
+    g1, gf, gb, g2 = (np.zeros((4, nc)) for _ in range(4))
+    for q in range(4):
+        for c in range(nc):
+            vals = {}
+            for h in (1e-6, 1e-7):
+                Up = U.copy()
+                Up[q, c] += h * scale[q]
+                Jp = J_at(Up)
+                Up[q, c] -= 2 * h * scale[q]
+                Jm = J_at(Up)
+                vals[h] = (Jp, Jm, h * scale[q])
+            Jp, Jm, hh = vals[1e-6]
+            g1[q, c] = (Jp - Jm) / (2 * hh)
+            gf[q, c] = (Jp - J0) / hh
+            gb[q, c] = (J0 - Jm) / hh
+            Jp, Jm, hh = vals[1e-7]
+            g2[q, c] = (Jp - Jm) / (2 * hh)
+    m.set_state(U)
+    s = capi.Solver(desc, H.capi_params(fl))
+    s.set_state(U)
+    b = s.adjoint_rhs_exact(beta)
+    b_ref = s.adjoint_rhs(beta)
+    sc = np.max(np.abs(g1), axis=1, keepdims=True)
+    err = np.abs(b - g1) / sc
+    nz = (np.abs(g1) > 1e-12 * sc) | (np.abs(b) > 1e-12 * sc)
+    smooth = nz & (np.abs(gf - gb) / sc < 1e-5) & (np.abs(g1 - g2) / sc < 1e-6)
+    print("nonzeros %d smooth %d max err %.3g median %.3g; reference rhs differs by %.3g" %
+          (nz.sum(), smooth.sum(), err[smooth].max(), np.median(err[smooth]),
+           np.max(np.abs(b_ref - g1) / sc)), flush=True)
+    # the limiter ratio's second derivative splits the one-sided quotients of
+    # many entries by more than the filter allows; a kink whose slope jump
+    # passes the filter (< 1e-5) can shift a central quotient by that much
+    assert smooth.sum() > 0.6 * nz.sum()
+    assert err[smooth].max() < 1e-5
+    assert np.median(err[smooth]) < 1e-8
+    # where J's differences are exactly zero, the derivative is below what
+    # they resolve: a few ulps of J over the step
+    zero = (g1 == 0) & (g2 == 0)
+    resolution = 4 * np.spacing(abs(J0)) / (2e-7 * scale)[:, None]
+    print("off-pattern max |b| / resolution %.3g" %
+          np.max(np.abs(b[zero]) / np.broadcast_to(resolution, b.shape)[zero]), flush=True)
+    assert np.all(np.abs(b[zero]) <= np.broadcast_to(resolution, b.shape)[zero])
+    # the exact right-hand side reaches cells off the wall row (the
+    # reconstruction stencil); the reference's does not
+    assert np.count_nonzero(np.any(b != 0, axis=0)) > np.count_nonzero(np.any(b_ref != 0, axis=0))
+    assert np.max(np.abs(b_ref - g1) / sc) > 1e-3
+
+
+def test_exact_bicgstab_with_the_exact_rhs():
+    """The solve with exact_rhs: M^T lambda = (dJ/dU)^T with M the dense exact
+    operator (viscous state, as above)."""
+    desc = H.tiny_mesh()
+    fl = flow()
+    fl["viscosity"] = 1.0
+    m, p, U = state(desc, fl, noise=0.0)
+    s = capi.Solver(desc, H.capi_params(fl))
+    s.set_state(U)
+    beta = 0.1
+    lam, iters, rr, st = s.bicgstab_exact(beta, tol=1e-8, chunk=20000, max_total=80000, check=False,
+                                          exact_rhs=True)
+    print("iters %d relres %.3g status %d" % (iters, rr, st), flush=True)
+    assert st == 0 and rr <= 1e-8
+    b = s.adjoint_rhs_exact(beta).reshape(-1)
+    n = b.shape[0]
+    M = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        M[i] = s.apply_jt_exact(e.reshape(4, -1)).reshape(-1)
+    assert np.linalg.norm(M.T @ lam.reshape(-1) - b) <= 1e-7 * np.linalg.norm(b)
