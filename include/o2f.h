@@ -224,12 +224,23 @@ int o2f_apply_jt_exact(o2f_ctx *ctx, const double *x, double *y);
  * (:491-554) keeps only the face-state derivative, taken as the cell
  * state's, on every wall edge.  b (4*n_cells SoA) out. */
 int o2f_adjoint_rhs_exact(o2f_ctx *ctx, double beta, double *b);
+/* Options of o2f_bicgstab_exact (bitwise or). */
+#define O2F_ADJ_EXACT_RHS 1    /* right-hand side o2f_adjoint_rhs_exact (else assembleAdjointRHS's) */
+#define O2F_ADJ_BLOCK_JACOBI 2 /* right preconditioner: the inverse of each cell's 4x4 diagonal block of the
+                                  operator (else the identity, as solveAdjointSystem's
+                                  IdentityPreconditioner) */
 /* solveAdjointSystem's BiCGSTAB (chunks, true residual, same error codes as
- * o2f_bicgstab) on the exact operator of o2f_apply_jt_exact; right-hand
- * side assembleAdjointRHS's (exact_rhs = 0) or o2f_adjoint_rhs_exact's
- * (exact_rhs = 1). */
+ * o2f_bicgstab) on the exact operator of o2f_apply_jt_exact; options:
+ * O2F_ADJ_* flags (0: the reference's right-hand side, no
+ * preconditioner). */
 int o2f_bicgstab_exact(o2f_ctx *ctx, double beta, double tol, int chunk, int max_total, double *lambda,
-                       int *iters, double *relres, int exact_rhs);
+                       int *iters, double *relres, int options);
+/* The 4x4 diagonal blocks D_c = dR_c/dU_c (16 per cell, row q = residual
+ * variable, column r = state variable) of the exact operator (exact = 1)
+ * or of applyJacobianTranspose's (exact = 0), obtained by probing J^T with
+ * cells coloured two vertex rings apart (the block-Jacobi preconditioner's
+ * input).  No counterpart in the reference. */
+int o2f_adjoint_block_diagonal(o2f_ctx *ctx, int exact, double *blocks);
 /* solveAdjointSystem (src/shape_optimization.cpp:658-740): BiCGSTAB with an
  * identity preconditioner on J^T lambda = b, restarted every `chunk`
  * iterations up to `max_total`, true residual checked after each chunk.
@@ -238,6 +249,14 @@ int o2f_bicgstab_exact(o2f_ctx *ctx, double beta, double tol, int chunk, int max
  * O2F_ERR_NOCONV like the reference's throws (lambda still written). */
 int o2f_bicgstab(o2f_ctx *ctx, double beta, double eps, double tol, int chunk, int max_total,
                  double *lambda, int *iters, double *relres);
+/* o2f_bicgstab with options: O2F_ADJ_BLOCK_JACOBI right-preconditions the
+ * reference operator by the inverses of its 4x4 diagonal blocks (probed
+ * exactly, o2f_adjoint_block_diagonal).  solveAdjointSystem
+ * (src/shape_optimization.cpp:658-740) uses the identity; on this operator
+ * the blocks map the spectrum into the right half-plane and cut the
+ * iterations by an order of magnitude.  Other flags: O2F_ERR_ARG. */
+int o2f_bicgstab_options(o2f_ctx *ctx, double beta, double eps, double tol, int chunk, int max_total, int options,
+                         double *lambda, int *iters, double *relres);
 /* computeShapeGradient (src/shape_optimization.cpp:942-1016) with the
  * NACA 4-digit wall perturbation of applyShapePerturbationNACA12 (:828-890).
  * names_all/values_all: every key of the shape-parameter file (pos_x, c, m,

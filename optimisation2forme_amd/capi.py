@@ -41,6 +41,7 @@ EXPORTS = [
     "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
     "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_naca_displacement",
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
+    "o2f_adjoint_block_diagonal", "o2f_bicgstab_options",
 ]
 
 
@@ -138,6 +139,9 @@ def lib():
             "o2f_mlp_warm_start": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]),
             "o2f_bicgstab_exact": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D, C.c_int]),
             "o2f_adjoint_rhs_exact": (C.c_int, [vp, C.c_double, D]),
+            "o2f_adjoint_block_diagonal": (C.c_int, [vp, C.c_int, D]),
+            "o2f_bicgstab_options": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                               D, I, D]),
             "o2f_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(vp)]),
             "o2f_batch_destroy": (None, [vp]),
             "o2f_batch_last_error": (C.c_char_p, [vp]),
@@ -389,11 +393,22 @@ class Solver:
         self._check(lib().o2f_apply_jt_exact(self.h, _d(x), _d(y)))
         return y
 
-    def bicgstab_exact(self, beta, tol=1e-6, chunk=500, max_total=100000, check=True, exact_rhs=False):
+    def bicgstab_exact(self, beta, tol=1e-6, chunk=500, max_total=100000, check=True, exact_rhs=False,
+                       block_jacobi=False):
         lam = np.zeros((4, self.nc))
         it, rr = C.c_int(), C.c_double()
-        st = lib().o2f_bicgstab_exact(self.h, beta, tol, chunk, max_total, _d(lam), C.byref(it), C.byref(rr),
-                                      int(exact_rhs))
+        opts = (1 if exact_rhs else 0) | (2 if block_jacobi else 0)
+        st = lib().o2f_bicgstab_exact(self.h, beta, tol, chunk, max_total, _d(lam), C.byref(it), C.byref(rr), opts)
+        if check:
+            self._check(st)
+        return lam, it.value, rr.value, st
+
+    def bicgstab_block_jacobi(self, beta, eps=1e-6, tol=1e-6, chunk=500, max_total=100000, check=True):
+        """o2f_bicgstab right-preconditioned by the operator's diagonal blocks."""
+        lam = np.zeros((4, self.nc))
+        it, rr = C.c_int(), C.c_double()
+        st = lib().o2f_bicgstab_options(self.h, beta, eps, tol, chunk, max_total, 2, _d(lam), C.byref(it),
+                                        C.byref(rr))
         if check:
             self._check(st)
         return lam, it.value, rr.value, st
@@ -416,6 +431,12 @@ class Solver:
         self._check(lib().o2f_shape_gradient(self.h, na, _d(va), len(names_all), no, len(opt_names),
                                              _d(lam), beta, eps, _d(g)))
         return g[:len(opt_names)]
+
+    def adjoint_block_diagonal(self, exact=True):
+        """dR_c/dU_c per cell, shape (n_cells, 4, 4)."""
+        D = np.zeros((self.nc, 4, 4))
+        self._check(lib().o2f_adjoint_block_diagonal(self.h, int(exact), _d(D)))
+        return D
 
     def adjoint_rhs_exact(self, beta):
         """(dJ/dU)^T through the reconstruction (4 x n_cells)."""

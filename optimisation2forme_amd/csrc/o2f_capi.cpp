@@ -327,7 +327,8 @@ void o2f_ctx_destroy(o2f_ctx *c) {
     void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->cinfo,
                     c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
                     c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx, c->Fe,
-                    c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work, c->t_work, c->t_out};
+                    c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work, c->t_work, c->t_out,
+                    c->pc_color, c->pc_D, c->pc_P};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (double *p : c->kv)
@@ -755,11 +756,15 @@ int set_scal(o2f_ctx *c, int slot, double v) {
 // y = A x on the device, A = J^T (either form).
 using MatVec = std::function<hipError_t(const double *, double *)>;
 
-int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol, int *iters, const MatVec &mv) {
+// pc: a right preconditioner y = M^{-1} p (Eigen's preconditioned
+// recurrence: v = A M^{-1} p, t = A M^{-1} s, x += alpha M^{-1} p + w M^{-1} s);
+// empty: the identity, as solveAdjointSystem's IdentityPreconditioner.
+int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol, int *iters, const MatVec &mv,
+                   const MatVec &pc = nullptr) {
     hipStream_t s = c->stream;
     const long n = 4L * c->hm.nc;
     double *r = c->kv[1], *r0 = c->kv[2], *v = c->kv[3], *p = c->kv[4], *sv = c->kv[5], *t = c->kv[6];
-    double *tmp = c->kv[7];
+    double *tmp = c->kv[7], *py = c->kv[10], *sz = c->kv[11];
     HostScal h;
     int st;
     // r = b - A x; r0 = r
@@ -798,12 +803,14 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
             if (restarts++ == 0) i = 0;
         }
         HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
-        HIPCHK(c, mv(p, v));
+        if (pc) HIPCHK(c, pc(p, py));
+        HIPCHK(c, mv(pc ? py : p, v));
         HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->scal, kOpAlpha, s));
         HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, s));
-        HIPCHK(c, mv(sv, t));
+        if (pc) HIPCHK(c, pc(sv, sz));
+        HIPCHK(c, mv(pc ? sz : sv, t));
         HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->scal, kOpOmega, s));
-        HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s));
+        HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s, pc ? py : nullptr, pc ? sz : nullptr));
         ++i;
         HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s));
         if ((st = read_scal(c, h))) return st;
@@ -821,8 +828,10 @@ namespace {
 // The right-hand side of a solve: b (4 nc, device) for beta.
 using RhsFn = std::function<int(double beta, double *b)>;
 
+int build_block_jacobi(o2f_ctx *c, const MatVec &mv);
+
 int bicgstab_solve(o2f_ctx *c, double beta, double tol, int chunk, int max_total, double *lambda, int *iters,
-                   double *relres, const MatVec &mv, const RhsFn &rhs = nullptr) {
+                   double *relres, const MatVec &mv, const RhsFn &rhs = nullptr, const MatVec &pc = nullptr) {
     int st;
     const long n = 4L * c->hm.nc;
     double *b = c->kv[0], *x = c->kv[8], *tmp = c->kv[9];
@@ -836,7 +845,7 @@ int bicgstab_solve(o2f_ctx *c, double beta, double tol, int chunk, int max_total
     int total = 0, status = O2F_OK;
     while (total < max_total) {
         int it = 0;
-        if ((st = bicgstab_chunk(c, b, x, chunk, tol, &it, mv))) return st;
+        if ((st = bicgstab_chunk(c, b, x, chunk, tol, &it, mv, pc))) return st;
         total += it;
         // true residual b - A x (src/shape_optimization.cpp:715-717)
         HIPCHK(c, mv(x, tmp));
@@ -871,6 +880,26 @@ int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int
         return launch_jt_apply(T, G, c->blk, x, y, c->stream);
     };
     return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, mv);
+}
+
+int o2f_bicgstab_options(o2f_ctx *c, double beta, double eps, double tol, int chunk, int max_total, int options,
+                         double *lambda, int *iters, double *relres) {
+    if (!c || !lambda || chunk <= 0 || (options & ~O2F_ADJ_BLOCK_JACOBI)) return O2F_ERR_ARG;
+    int st = ensure_blocks(c, eps);
+    if (st) return st;
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    const MatVec mv = [c, T, G](const double *x, double *y) {
+        return launch_jt_apply(T, G, c->blk, x, y, c->stream);
+    };
+    MatVec pc = nullptr;
+    if (options & O2F_ADJ_BLOCK_JACOBI) {
+        if ((st = build_block_jacobi(c, mv))) return st;
+        pc = [c](const double *in, double *out) {
+            return launch_apply_blocks(c->hm.nc, c->pc_P, in, out, c->stream);
+        };
+    }
+    return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, mv, nullptr, pc);
 }
 
 // ---- exact adjoint ---------------------------------------------------------
@@ -985,6 +1014,63 @@ int o2f_apply_jt_exact(o2f_ctx *c, const double *x, double *y) {
 
 namespace {
 
+// Greedy colouring of the cells such that two cells of one colour are more
+// than two vertex rings apart: no cell of a colour lies in the residual
+// stencil of another (R_c depends on U only within two vertex rings of c:
+// the face neighbours' reconstructions and the vertex stresses of c's
+// vertices).
+int distance2_colors(const HostMesh &hm, std::vector<int> &color) {
+    color.assign(hm.nc, -1);
+    std::vector<int> seen;
+    int ncol = 0;
+    for (int c = 0; c < hm.nc; ++c) {
+        for (int k = 0; k < hm.cnv[c]; ++k) {
+            const int v = hm.cvtx[4 * c + k];
+            for (int i = hm.vptr[v]; i < hm.vptr[v + 1]; ++i) {
+                const int c1 = hm.vcell_asc[i];
+                for (int k1 = 0; k1 < hm.cnv[c1]; ++k1) {
+                    const int v1 = hm.cvtx[4 * c1 + k1];
+                    for (int j = hm.vptr[v1]; j < hm.vptr[v1 + 1]; ++j) {
+                        const int col = color[hm.vcell_asc[j]];
+                        if (col >= 0) {
+                            if (col >= (int)seen.size()) seen.resize(col + 1, -1);
+                            seen[col] = c;
+                        }
+                    }
+                }
+            }
+        }
+        int col = 0;
+        while (col < (int)seen.size() && seen[col] == c) ++col;
+        color[c] = col;
+        ncol = std::max(ncol, col + 1);
+    }
+    return ncol;
+}
+
+// The diagonal 4x4 blocks D_c = dR_c/dU_c of the operator behind mv (y =
+// J^T x) by probing, one colour and variable at a time, and P_c =
+// (D_c^T)^{-1}.  Uses work vectors kv[10] (x) and kv[11] (y).
+int build_block_jacobi(o2f_ctx *c, const MatVec &mv) {
+    const int nc = c->hm.nc;
+    if (!c->pc_color) {
+        std::vector<int> color;
+        c->pc_ncolors = distance2_colors(c->hm, color);
+        HIPCHK(c, upload(c->pc_color, color));
+    }
+    if (!c->pc_D) HIPCHK(c, dalloc(c->pc_D, 16 * (size_t)nc));
+    if (!c->pc_P) HIPCHK(c, dalloc(c->pc_P, 16 * (size_t)nc));
+    double *x = c->kv[10], *y = c->kv[11];
+    for (int k = 0; k < c->pc_ncolors; ++k)
+        for (int q = 0; q < 4; ++q) {
+            HIPCHK(c, launch_probe_seed(nc, c->pc_color, k, q, x, c->stream));
+            HIPCHK(c, mv(x, y));
+            HIPCHK(c, launch_probe_take(nc, c->pc_color, k, q, y, c->pc_D, c->stream));
+        }
+    HIPCHK(c, launch_invert_blocks(nc, c->pc_D, c->pc_P, c->stream));
+    return O2F_OK;
+}
+
 // (dJ/dU)^T through the reconstruction: the exact adjoint chain with x = 0
 // and the objective's face-state derivative as its seed.  Work vector
 // kv[8] is used as the zero x (the solve fills it with its zero start
@@ -1012,14 +1098,40 @@ int o2f_adjoint_rhs_exact(o2f_ctx *c, double beta, double *b) {
 }
 
 int o2f_bicgstab_exact(o2f_ctx *c, double beta, double tol, int chunk, int max_total, double *lambda, int *iters,
-                       double *relres, int exact_rhs) {
-    if (!c || !lambda || chunk <= 0) return O2F_ERR_ARG;
+                       double *relres, int options) {
+    if (!c || !lambda || chunk <= 0 || (options & ~(O2F_ADJ_EXACT_RHS | O2F_ADJ_BLOCK_JACOBI))) return O2F_ERR_ARG;
     int st = ensure_adjoint(c);
     if (st) return st;
     if ((st = residual_on(c, 0))) return st;
     RhsFn rhs = nullptr;
-    if (exact_rhs) rhs = [c](double beta_, double *b) { return rhs_exact_device(c, beta_, b); };
-    return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, exact_matvec(c), rhs);
+    if (options & O2F_ADJ_EXACT_RHS) rhs = [c](double beta_, double *b) { return rhs_exact_device(c, beta_, b); };
+    const MatVec mv = exact_matvec(c);
+    MatVec pc = nullptr;
+    if (options & O2F_ADJ_BLOCK_JACOBI) {
+        if ((st = build_block_jacobi(c, mv))) return st;
+        pc = [c](const double *in, double *out) {
+            return launch_apply_blocks(c->hm.nc, c->pc_P, in, out, c->stream);
+        };
+    }
+    return bicgstab_solve(c, beta, tol, chunk, max_total, lambda, iters, relres, mv, rhs, pc);
+}
+
+int o2f_adjoint_block_diagonal(o2f_ctx *c, int exact, double *blocks) {
+    if (!c || !blocks) return O2F_ERR_ARG;
+    int st = exact ? ensure_adjoint(c) : ensure_work(c);
+    if (st) return st;
+    if ((st = residual_on(c, 0))) return st;
+    MatVec mv;
+    if (exact) {
+        mv = exact_matvec(c);
+    } else {
+        if ((st = ensure_blocks(c, 1e-6))) return st;
+        const Topo T = c->topo();
+        const Geom G = c->geo[0].view();
+        mv = [c, T, G](const double *a, double *b) { return launch_jt_apply(T, G, c->blk, a, b, c->stream); };
+    }
+    if ((st = build_block_jacobi(c, mv))) return st;
+    return copy_out(c, blocks, c->pc_D, 16 * (size_t)c->hm.nc);
 }
 
 int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *values_all, int n_all,

@@ -89,7 +89,12 @@ struct o2f_ctx {
     double *blk = nullptr;
     long faces_version = 0, blk_version = -1;
     double blk_eps = 0.0;
-    double *kv[10] = {};  // BiCGSTAB / gradient work vectors (4 nc each)
+    double *kv[12] = {};  // BiCGSTAB / gradient work vectors (4 nc each)
+    // block-Jacobi preconditioner: cell colours (distance two in vertex
+    // rings), probed diagonal blocks and their inverses (16 per cell)
+    int *pc_color = nullptr;
+    int pc_ncolors = 0;
+    double *pc_D = nullptr, *pc_P = nullptr;
     double *Rp = nullptr, *Rm = nullptr;
     // exact adjoint: vertex -> edge CSR, cell vertices and their stencil
     // weights, work arrays (allocated on first use)

@@ -658,14 +658,83 @@ __global__ __launch_bounds__(256) void k_bicg_s(long n, const double *__restrict
     s[i] = r[i] - scal[kScalAlpha] * v[i];
 }
 
+// x += alpha px + w sx; r = s - w t (px = p, sx = s without a
+// preconditioner; the preconditioned directions with one)
 __global__ __launch_bounds__(256) void k_bicg_xr(long n, double *__restrict__ x, double *__restrict__ r,
-                                                 const double *__restrict__ p, const double *__restrict__ s,
-                                                 const double *__restrict__ t, const double *__restrict__ scal) {
+                                                 const double *__restrict__ px, const double *__restrict__ sx,
+                                                 const double *__restrict__ s, const double *__restrict__ t,
+                                                 const double *__restrict__ scal) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double alpha = scal[kScalAlpha], w = scal[kScalW];
-    x[i] += alpha * p[i] + w * s[i];
+    x[i] += alpha * px[i] + w * sx[i];
     r[i] = s[i] - w * t[i];
+}
+
+// Block-Jacobi preconditioner of the adjoint solves.  Probing: x = 1 on
+// variable q of every cell of colour k (cells of one colour are more than
+// two vertex rings apart, so none lies in another's residual stencil), y =
+// J^T x; then y_(c,r) = dR_(c,q)/dU_(c,r) for every cell c of that colour.
+__global__ __launch_bounds__(256) void k_probe_seed(int nc, const int *__restrict__ color, int k, int q,
+                                                    double *__restrict__ x) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    for (int r = 0; r < 4; ++r) x[(long)r * nc + c] = (r == q && color[c] == k) ? 1.0 : 0.0;
+}
+__global__ __launch_bounds__(256) void k_probe_take(int nc, const int *__restrict__ color, int k, int q,
+                                                    const double *__restrict__ y, double *__restrict__ D) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc || color[c] != k) return;
+    for (int r = 0; r < 4; ++r) D[16 * (long)c + 4 * q + r] = y[(long)r * nc + c];
+}
+// P_c = (D_c^T)^{-1} (the diagonal block of J^T is D_c^T), Gauss-Jordan
+// with partial pivoting; a singular block keeps the identity.
+__global__ __launch_bounds__(256) void k_invert_blocks(int nc, const double *__restrict__ D, double *__restrict__ P) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double a[4][8];
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) {
+            a[i][j] = D[16 * (long)c + 4 * j + i];  // transpose
+            a[i][4 + j] = (i == j) ? 1.0 : 0.0;
+        }
+    bool ok = true;
+    for (int col = 0; col < 4 && ok; ++col) {
+        int piv = col;
+        for (int i = col + 1; i < 4; ++i)
+            if (fabs(a[i][col]) > fabs(a[piv][col])) piv = i;
+        if (!(fabs(a[piv][col]) > 0.0)) {
+            ok = false;
+            break;
+        }
+        if (piv != col)
+            for (int j = 0; j < 8; ++j) {
+                const double t = a[col][j];
+                a[col][j] = a[piv][j];
+                a[piv][j] = t;
+            }
+        const double inv = 1.0 / a[col][col];
+        for (int j = 0; j < 8; ++j) a[col][j] *= inv;
+        for (int i = 0; i < 4; ++i)
+            if (i != col) {
+                const double f = a[i][col];
+                for (int j = 0; j < 8; ++j) a[i][j] -= f * a[col][j];
+            }
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) P[16 * (long)c + 4 * i + j] = ok ? a[i][4 + j] : (i == j ? 1.0 : 0.0);
+}
+// out_c = P_c in_c (SoA vectors of 4 nc)
+__global__ __launch_bounds__(256) void k_apply_blocks(int nc, const double *__restrict__ P,
+                                                      const double *__restrict__ in, double *__restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double v[4];
+    for (int r = 0; r < 4; ++r) v[r] = in[(long)r * nc + c];
+    for (int i = 0; i < 4; ++i) {
+        const double *row = P + 16 * (long)c + 4 * i;
+        out[(long)i * nc + c] = row[0] * v[0] + row[1] * v[1] + row[2] * v[2] + row[3] * v[3];
+    }
 }
 
 // out = a - b (r = b - A x); also used to copy (b == nullptr).
@@ -1056,8 +1125,24 @@ hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, c
     return hipGetLastError();
 }
 hipError_t launch_bicg_xr(long n, double *x, double *r, const double *p, const double *sv, const double *t,
-                          const double *scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_xr, dim3(nblk(n)), dim3(256), 0, s, n, x, r, p, sv, t, scal);
+                          const double *scal, hipStream_t s, const double *px, const double *sx) {
+    hipLaunchKernelGGL(k_bicg_xr, dim3(nblk(n)), dim3(256), 0, s, n, x, r, px ? px : p, sx ? sx : sv, sv, t, scal);
+    return hipGetLastError();
+}
+hipError_t launch_probe_seed(int nc, const int *color, int k, int q, double *x, hipStream_t s) {
+    hipLaunchKernelGGL(k_probe_seed, dim3(nblk(nc)), dim3(256), 0, s, nc, color, k, q, x);
+    return hipGetLastError();
+}
+hipError_t launch_probe_take(int nc, const int *color, int k, int q, const double *y, double *D, hipStream_t s) {
+    hipLaunchKernelGGL(k_probe_take, dim3(nblk(nc)), dim3(256), 0, s, nc, color, k, q, y, D);
+    return hipGetLastError();
+}
+hipError_t launch_invert_blocks(int nc, const double *D, double *P, hipStream_t s) {
+    hipLaunchKernelGGL(k_invert_blocks, dim3(nblk(nc)), dim3(256), 0, s, nc, D, P);
+    return hipGetLastError();
+}
+hipError_t launch_apply_blocks(int nc, const double *P, const double *in, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_apply_blocks, dim3(nblk(nc)), dim3(256), 0, s, nc, P, in, out);
     return hipGetLastError();
 }
 hipError_t launch_sub(long n, const double *a, const double *b, double *out, hipStream_t s) {

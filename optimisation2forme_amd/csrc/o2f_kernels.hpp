@@ -206,8 +206,15 @@ hipError_t launch_dot2(const double *a, const double *b, const double *c, const 
                        double *partials, double *scal, int op, hipStream_t s);
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s);
 hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s);
+// x += alpha (px ? px : p) + w (sx ? sx : sv); r = sv - w t
 hipError_t launch_bicg_xr(long n, double *x, double *r, const double *p, const double *sv, const double *t,
-                          const double *scal, hipStream_t s);
+                          const double *scal, hipStream_t s, const double *px = nullptr, const double *sx = nullptr);
+// Block-Jacobi preconditioner: probing seeds / extraction by colour,
+// P_c = (D_c^T)^{-1} (16 per cell, row-major), out_c = P_c in_c.
+hipError_t launch_probe_seed(int nc, const int *color, int k, int q, double *x, hipStream_t s);
+hipError_t launch_probe_take(int nc, const int *color, int k, int q, const double *y, double *D, hipStream_t s);
+hipError_t launch_invert_blocks(int nc, const double *D, double *P, hipStream_t s);
+hipError_t launch_apply_blocks(int nc, const double *P, const double *in, double *out, hipStream_t s);
 hipError_t launch_sub(long n, const double *a, const double *b, double *out, hipStream_t s);
 hipError_t launch_fill(long n, double *out, double v, hipStream_t s);
 hipError_t launch_fd_diff(long n, const double *Rp, const double *Rm, double eps, double *out, hipStream_t s);
