@@ -298,6 +298,43 @@ __device__ __forceinline__ double from_lane(double v) {
     return __hiloint2double(hi, lo);
 }
 
+template <int J>
+__device__ __forceinline__ int from_lane_i(int v) {
+    return __builtin_amdgcn_mov_dpp(v, J * 0x55, 0xF, 0xF, false);
+}
+
+// Value of register v in the lane the quad_perm control CTRL selects.
+template <int CTRL>
+__device__ __forceinline__ double quad_perm(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// 4 x 4 transpose within a group of four lanes: lane k holds row k on
+// entry (m[j] = M[k][j]) and column k on exit (m[j] = M[j][k]).  Two
+// butterfly stages: exchange with lane k ^ 1 the entries whose index
+// differs from k in bit 0 (quad_perm [1,0,3,2]), then with lane k ^ 2 those
+// differing in bit 1 (quad_perm [2,3,0,1]).
+__device__ __forceinline__ void quad_transpose(double (&m)[4], int k) {
+    constexpr int kXor1 = 0xB1, kXor2 = 0x4E;
+    const bool odd = k & 1, upper = k & 2;
+    double t0 = quad_perm<kXor1>(m[1]), t1 = quad_perm<kXor1>(m[0]);
+    double t2 = quad_perm<kXor1>(m[3]), t3 = quad_perm<kXor1>(m[2]);
+    m[0] = odd ? t0 : m[0];
+    m[1] = odd ? m[1] : t1;
+    m[2] = odd ? t2 : m[2];
+    m[3] = odd ? m[3] : t3;
+    t0 = quad_perm<kXor2>(m[2]);
+    t1 = quad_perm<kXor2>(m[3]);
+    t2 = quad_perm<kXor2>(m[0]);
+    t3 = quad_perm<kXor2>(m[1]);
+    m[0] = upper ? t0 : m[0];
+    m[1] = upper ? t1 : m[1];
+    m[2] = upper ? m[2] : t2;
+    m[3] = upper ? m[3] : t3;
+}
+
 // f(j, value of lane j) for j = 0, 1, 2, 3 in order
 template <class F>
 __device__ __forceinline__ void for_lanes(double v, F &&f) {

@@ -173,7 +173,7 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
     // this lane's edge and face neighbour
     const int e = has_edge ? T.cedge[4 * c + k] : 0;
     const int nb = T.cnbr[4 * c + k];
-    double el = 0.0, enx = 0.0, eny = 0.0, dx = 0.0, dy = 0.0, xa[4], xb[4], un[4];
+    double el = 0.0, enx = 0.0, eny = 0.0, dx = 0.0, dy = 0.0, xa[4], xb[4];
     char ebc = 0;
     if (has_edge) {
         const int va = T.cevx[8 * c + 2 * k], vb = T.cevx[8 * c + 2 * k + 1];
@@ -186,8 +186,6 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
         ld4(vU, va, xa);
         ld4(vU, vb, xb);
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) un[q] = (nb >= 0) ? U[q * nc + nb] : u[q];
     // edge value (+ edge BC) and this edge's Green-Gauss terms
     double tx[4] = {0.0, 0.0, 0.0, 0.0}, ty[4] = {0.0, 0.0, 0.0, 0.0};
     if (has_edge) {
@@ -206,68 +204,72 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
             ty[q] = val[q] * el * ny / vol;
         }
     }
-    // Lanes without an edge hold +0.0 terms; the sums start at +0.0 and so are
-    // never -0.0, and adding +0.0 then changes nothing: every lane's terms
-    // are added unconditionally, in slot order.
-    double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+    // From here lane k owns conservative variable k.  The 4 x 4 in-quad
+    // transpose gives lane k the terms of variable k of the edges in slot
+    // order; lanes without an edge contributed +0.0 terms, and the sums
+    // start at +0.0, so they are never -0.0 and adding +0.0 changes
+    // nothing.
+    quad_transpose(tx, k);
+    quad_transpose(ty, k);
+    double gxk = 0.0, gyk = 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        for_lanes(tx[q], [&](int, double a) { gx[q] += a; });
-        for_lanes(ty[q], [&](int, double b) { gy[q] += b; });
+    for (int j = 0; j < 4; ++j) {
+        gxk += tx[j];
+        gyk += ty[j];
     }
-    // neighbour min / max in neighbour-list order (a lane without a
-    // neighbour offers the cell's own value, which changes nothing)
-    double mn[4], mx[4];
+    const double uk = (k == 0) ? u[0] : (k == 1) ? u[1] : (k == 2) ? u[2] : u[3];
+    // neighbour min / max of variable k in neighbour-list order (no
+    // neighbour: the cell's own value, which changes nothing)
+    const int nbs[4] = {from_lane_i<0>(nb), from_lane_i<1>(nb), from_lane_i<2>(nb), from_lane_i<3>(nb)};
+    double mn = uk, mx = uk;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) mn[q] = mx[q] = u[q];
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-        for_lanes(un[q], [&](int, double v) {
-            if (v < mn[q]) mn[q] = v;
-            if (v > mx[q]) mx[q] = v;
-        });
-    // limiter: this edge's alpha, then the min over edges in slot order
-    double a_k[4] = {1.0, 1.0, 1.0, 1.0};
-    if (has_edge) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double pred = u[q] + gx[q] * dx + gy[q] * dy;
-            a_k[q] = (P.limiter == 1) ? alpha_bj(u[q], pred, mn[q], mx[q])
-                                      : alpha_venkat(u[q], pred, mn[q], mx[q], vol, P.venkat_k);
-        }
+    for (int j = 0; j < 4; ++j) {
+        const double v = (nbs[j] >= 0) ? U[(long)k * nc + nbs[j]] : uk;
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
     }
-    double al[4] = {1.0, 1.0, 1.0, 1.0};
+    // limiter of variable k: the min over the edges in slot order (edges
+    // beyond nk would offer 1.0, which changes nothing)
+    const double dxs[4] = {from_lane<0>(dx), from_lane<1>(dx), from_lane<2>(dx), from_lane<3>(dx)};
+    const double dys[4] = {from_lane<0>(dy), from_lane<1>(dy), from_lane<2>(dy), from_lane<3>(dy)};
+    double al = 1.0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        for_lanes(a_k[q], [&](int, double a) { al[q] = smin(al[q], a); });  // 1.0 from lanes without an edge
-    // this edge's face state (+ boundary ghost)
-    if (has_edge) {
-        double f[4];
+    for (int j = 0; j < 4; ++j) {
+        if (j >= nk) break;
+        const double pred = uk + gxk * dxs[j] + gyk * dys[j];
+        const double a = (P.limiter == 1) ? alpha_bj(uk, pred, mn, mx)
+                                          : alpha_venkat(uk, pred, mn, mx, vol, P.venkat_k);
+        al = smin(al, a);
+    }
+    // face states of variable k on every edge (+ boundary ghost,
+    // src/boundary.cpp:48-80)
+    const int es[4] = {from_lane_i<0>(e), from_lane_i<1>(e), from_lane_i<2>(e), from_lane_i<3>(e)};
+    const int bcs[4] = {from_lane_i<0>(ebc), from_lane_i<1>(ebc), from_lane_i<2>(ebc), from_lane_i<3>(ebc)};
+    const double fsk = (k == 0) ? P.fs_rho : (k == 1) ? P.fs_ru : (k == 2) ? P.fs_rv : P.fs_E;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) f[q] = u[q] + al[q] * (gx[q] * dx + gy[q] * dy);
-        if (is_left) {
-            st4(fL, e, f);
-            if (tagged(ebc)) {
-                double g[4];
-                bc_ghost(P, ebc, f, g);
-                st4(fR, e, g);
-            }
+    for (int j = 0; j < 4; ++j) {
+        if (j >= nk) break;
+        const double f = uk + al * (gxk * dxs[j] + gyk * dys[j]);
+        const long o = 4 * (long)es[j] + k;
+        if ((info >> j) & 1u) {
+            fL[o] = f;
+            const char bcj = (char)bcs[j];
+            if (tagged(bcj)) fR[o] = (bcj == 'i' || bcj == 'f') ? fsk : (bcj == 'w' && (k == 1 || k == 2)) ? -f : f;
         } else {
-            st4(fR, e, f);
+            fR[o] = f;
         }
     }
-    if (Gout && k == 0) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            Gout[q * nc + c] = gx[q];
-            Gout[(4 + q) * nc + c] = gy[q];
-        }
+    if (Gout) {
+        Gout[(long)k * nc + c] = gxk;
+        Gout[(long)(4 + k) * nc + c] = gyk;
     }
     // viscous stress and heat flux of the cell (src/diffusion.cpp:227-240),
     // the divisions spread over the four lanes: lane k forms one velocity
     // derivative (u_x, v_y, u_y, v_x), lanes 0 / 1 one temperature
     // derivative (T_x, T_y); every value is the same expression as in one
     // lane, so the results are unchanged
+    const double gx[4] = {from_lane<0>(gxk), from_lane<1>(gxk), from_lane<2>(gxk), from_lane<3>(gxk)};
+    const double gy[4] = {from_lane<0>(gyk), from_lane<1>(gyk), from_lane<2>(gyk), from_lane<3>(gyk)};
     const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
     const double rho = u[0], E = u[3];
     const double uu = u[1] / rho, vvel = u[2] / rho;
