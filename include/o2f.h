@@ -380,6 +380,14 @@ int o2f_mlp_loss_grad(o2f_mlp *net, const float *raw, const float *targets, long
 /* Sets the context's state to the network's prediction for its cells
  * (shape m, p, t and angle of attack as inputs; Mach from the context). */
 int o2f_mlp_warm_start(o2f_mlp *net, o2f_ctx *ctx, double m, double p, double t, double aoa_deg);
+/* The GEMM kernel choice: -1 (default) times the admissible kernels once
+ * per shape and keeps the fastest; 0-6 forces one wherever its tile divides
+ * the shape, the 64x64 kernel 0 elsewhere (0-2: single-buffered 64x64 /
+ * 128x64 / 128x128 workgroup tiles with K stages of 32; 3-6:
+ * double-buffered 128x128 / 128x64 / 64x128 / 64x64 with K stages of 64).
+ * Every kernel accumulates each output in the same k order, so the choice
+ * changes speed only. */
+int o2f_mlp_set_gemm_variant(o2f_mlp *net, int variant);
 
 /* ---- measurement ----------------------------------------------------- */
 

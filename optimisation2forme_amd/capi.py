@@ -41,7 +41,7 @@ EXPORTS = [
     "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
     "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_naca_displacement",
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
-    "o2f_adjoint_block_diagonal", "o2f_bicgstab_options",
+    "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant",
 ]
 
 
@@ -137,6 +137,7 @@ def lib():
             "o2f_mlp_forward": (C.c_int, [vp, F, C.c_long, F]),
             "o2f_mlp_loss_grad": (C.c_int, [vp, F, F, C.c_long, F, F]),
             "o2f_mlp_warm_start": (C.c_int, [vp, vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+            "o2f_mlp_set_gemm_variant": (C.c_int, [vp, C.c_int]),
             "o2f_bicgstab_exact": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D, C.c_int]),
             "o2f_adjoint_rhs_exact": (C.c_int, [vp, C.c_double, D]),
             "o2f_adjoint_block_diagonal": (C.c_int, [vp, C.c_int, D]),
@@ -588,6 +589,10 @@ class WarmStartNet:
     def _check(self, st):
         if st != O2F_OK:
             raise O2FError(st, lib().o2f_mlp_last_error(self.h).decode())
+
+    def set_gemm_variant(self, variant):
+        """-1: per-shape tuned GEMM kernel (default); 0-6: force one."""
+        self._check(lib().o2f_mlp_set_gemm_variant(self.h, int(variant)))
 
     def weights(self):
         w = np.zeros(self.n_params, np.float32)

@@ -158,6 +158,140 @@ __global__ __launch_bounds__(256) void k_gemm(const u16 *__restrict__ A, long sa
             }
 }
 
+// The same GEMM with K stages of 64 double-buffered in LDS: the next
+// stage's global loads are in flight while the matrix cores work on the
+// current one, and one barrier per stage separates the two buffers (each
+// wave issues FM x FN x 2 MFMAs per barrier instead of FM x FN).  The k
+// order of every accumulation (32-wide MFMA steps in increasing k) is that
+// of k_gemm, so both kernels give bit-identical results.
+constexpr int kTK2 = 64, kLdsRow2 = kTK2 + 8;
+
+template <int TM, int TN>
+__global__ __launch_bounds__(256) void k_gemm2(const u16 *__restrict__ A, long sam, long sak,
+                                               const u16 *__restrict__ B, long sbk, long sbn, int M, int N,
+                                               int K, int kchunk, Epi epi) {
+    constexpr int WM = TM / 2, WN = TN / 2, FM = WM / 16, FN = WN / 16;
+    constexpr int LA = TM * kTK2 / (256 * 8), LB = TN * kTK2 / (256 * 8);  // 16-byte loads per thread
+    __shared__ u16 As[2][TM][kLdsRow2];
+    __shared__ u16 Bt[2][TN][kLdsRow2];  // transposed: Bt[n][k]
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+    v8u ra[LA], rb[LB];
+    // global -> registers for the stage at k0 (every index inside the
+    // operands: M, N multiples of the tile, K and kchunk of 64, checked by
+    // the host)
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int idx = t + 256 * it;
+            if (sak == 1) {
+                const int r = idx >> 3, k = (idx & 7) * 8;
+                ra[it] = *reinterpret_cast<const v8u *>(A + (long)(m0 + r) * sam + (k0 + k));
+            } else {
+                const int k = idx / (TM / 8), r = (idx % (TM / 8)) * 8;
+                ra[it] = *reinterpret_cast<const v8u *>(A + (long)(k0 + k) * sak + (m0 + r));
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int idx = t + 256 * it;
+            if (sbk == 1) {
+                const int n = idx >> 3, k = (idx & 7) * 8;
+                rb[it] = *reinterpret_cast<const v8u *>(B + (long)(n0 + n) * sbn + (k0 + k));
+            } else {
+                const int k = idx / (TN / 8), n = (idx % (TN / 8)) * 8;
+                rb[it] = *reinterpret_cast<const v8u *>(B + (long)(k0 + k) * sbk + (n0 + n));
+            }
+        }
+    };
+    // registers -> LDS buffer `buf`
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < LA; ++it) {
+            const int idx = t + 256 * it;
+            if (sak == 1) {
+                const int r = idx >> 3, k = (idx & 7) * 8;
+                *reinterpret_cast<v8u *>(&As[buf][r][k]) = ra[it];
+            } else {
+                const int k = idx / (TM / 8), r = (idx % (TM / 8)) * 8;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) As[buf][r + i][k] = ra[it][i];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < LB; ++it) {
+            const int idx = t + 256 * it;
+            if (sbk == 1) {
+                const int n = idx >> 3, k = (idx & 7) * 8;
+                *reinterpret_cast<v8u *>(&Bt[buf][n][k]) = rb[it];
+            } else {
+                const int k = idx / (TN / 8), n = (idx % (TN / 8)) * 8;
+#pragma unroll
+                for (int i = 0; i < 8; ++i) Bt[buf][n + i][k] = rb[it][i];
+            }
+        }
+    };
+    v4f acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
+    load(kb);
+    store(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = kb; k0 < ke; k0 += kTK2) {
+        const bool more = k0 + kTK2 < ke;
+        if (more) load(k0 + kTK2);
+#pragma unroll
+        for (int kk = 0; kk < kTK2; kk += 32) {
+            v8u bv[FN];
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+                bv[j] = *reinterpret_cast<const v8u *>(&Bt[buf][wn * WN + j * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const v8u a =
+                    *reinterpret_cast<const v8u *>(&As[buf][wm * WM + i * 16 + (lane & 15)][kk + 8 * (lane >> 4)]);
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a),
+                                                                         __builtin_bit_cast(v8bf, bv[j]), acc[i][j],
+                                                                         0, 0, 0);
+            }
+        }
+        // the other buffer was last read before the previous barrier
+        if (more) store(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const long m = m0 + wm * WM + i * 16 + 4 * (lane >> 4) + r;
+                const long n = n0 + wn * WN + j * 16 + (lane & 15);
+                const float v = acc[i][j][r];
+                const long o = m * epi.ldc + n;
+                if (epi.mode == kEpiHidden) {
+                    const float z = v + epi.bias[n];
+                    epi.Z[o] = z;
+                    epi.act[o] = to_bf16(z * sigm(z));
+                } else if (epi.mode == kEpiOut) {
+                    epi.C[o] = v + epi.bias[n];
+                } else if (epi.mode == kEpiBack) {
+                    const float z = epi.Zprev[o], s = sigm(z);
+                    epi.act[o] = to_bf16(v * (s * (1.0f + z * (1.0f - s))));
+                } else {
+                    epi.C[(long)blockIdx.z * M * epi.ldc + o] = v;
+                }
+            }
+}
+
 // dW[i] = sum over split chunks s = 0, 1, ... in order of the partials.
 __global__ __launch_bounds__(256) void k_sum_splits(const float *__restrict__ P, int splits, long n,
                                                     float *__restrict__ out) {
@@ -315,6 +449,7 @@ struct o2f_mlp {
     long ndata = 0;
     long step = 0;
     std::map<std::tuple<int, int, int, int, bool, bool, int>, int> tile_choice;
+    int gemm_variant = -1;  // >= 0: that GEMM kernel for every shape (else the per-shape choice)
     hipStream_t stream = nullptr;
     std::string err;
     int fail(int code, const std::string &s) {
@@ -363,10 +498,33 @@ int ensure_cap(o2f_mlp *o, int rows) {
     return O2F_OK;
 }
 
+// GEMM kernels: 0-2 k_gemm <64,64>, <128,64>, <128,128>; 3-6 k_gemm2
+// <128,128>, <128,64>, <64,128>, <64,64>.
+constexpr int kGemmVariants = 7;
+constexpr int kVarTM[kGemmVariants] = {64, 128, 128, 128, 128, 64, 64};
+constexpr int kVarTN[kGemmVariants] = {64, 64, 128, 128, 64, 128, 64};
+constexpr int kVarTK[kGemmVariants] = {kTK, kTK, kTK, kTK2, kTK2, kTK2, kTK2};
+
+bool gemm_fits(int v, int M, int N, int K, int kchunk) {
+    return M % kVarTM[v] == 0 && N % kVarTN[v] == 0 && K % kVarTK[v] == 0 && kchunk % kVarTK[v] == 0;
+}
+
 hipError_t launch_gemm(int variant, hipStream_t st, const u16 *A, long sam, long sak, const u16 *B, long sbk,
                        long sbn, int M, int N, int K, int kchunk, const Epi &epi) {
     const int splits = (K + kchunk - 1) / kchunk;
-    if (variant == 2)
+    if (variant == 3)
+        hipLaunchKernelGGL((k_gemm2<128, 128>), dim3(N / 128, M / 128, splits), dim3(256), 0, st, A, sam, sak, B,
+                           sbk, sbn, M, N, K, kchunk, epi);
+    else if (variant == 4)
+        hipLaunchKernelGGL((k_gemm2<128, 64>), dim3(N / 64, M / 128, splits), dim3(256), 0, st, A, sam, sak, B, sbk,
+                           sbn, M, N, K, kchunk, epi);
+    else if (variant == 5)
+        hipLaunchKernelGGL((k_gemm2<64, 128>), dim3(N / 128, M / 64, splits), dim3(256), 0, st, A, sam, sak, B, sbk,
+                           sbn, M, N, K, kchunk, epi);
+    else if (variant == 6)
+        hipLaunchKernelGGL((k_gemm2<64, 64>), dim3(N / 64, M / 64, splits), dim3(256), 0, st, A, sam, sak, B, sbk,
+                           sbn, M, N, K, kchunk, epi);
+    else if (variant == 2)
         hipLaunchKernelGGL((k_gemm<128, 128>), dim3(N / 128, M / 128, splits), dim3(256), 0, st, A, sam, sak, B, sbk,
                            sbn, M, N, K, kchunk, epi);
     else if (variant == 1)
@@ -384,9 +542,12 @@ hipError_t launch_gemm(int variant, hipStream_t st, const u16 *A, long sam, long
 hipError_t gemm(o2f_mlp *o, const u16 *A, long sam, long sak, const u16 *B, long sbk, long sbn, int M, int N, int K,
                 int kchunk, const Epi &epi) {
     // the tile loaders need one unit stride per operand and whole tiles
-    if (!((sak == 1 || sam == 1) && (sbk == 1 || sbn == 1)) || M % kTile || N % kTile || K % kTK ||
-        kchunk % kTK)
+    if (!((sak == 1 || sam == 1) && (sbk == 1 || sbn == 1)) || !gemm_fits(0, M, N, K, kchunk))
         return hipErrorInvalidValue;
+    if (o->gemm_variant >= 0) {  // forced: that kernel where its tile fits, else the 64 x 64 one
+        const int v = gemm_fits(o->gemm_variant, M, N, K, kchunk) ? o->gemm_variant : 0;
+        return launch_gemm(v, o->stream, A, sam, sak, B, sbk, sbn, M, N, K, kchunk, epi);
+    }
     const auto key = std::make_tuple(M, N, K, kchunk, sak == 1, sbk == 1, epi.mode);
     auto it = o->tile_choice.find(key);
     if (it == o->tile_choice.end()) {
@@ -395,8 +556,8 @@ hipError_t gemm(o2f_mlp *o, const u16 *A, long sam, long sak, const u16 *B, long
         hipEvent_t e0, e1;
         hipError_t e;
         if ((e = hipEventCreate(&e0)) != hipSuccess || (e = hipEventCreate(&e1)) != hipSuccess) return e;
-        for (int v = 0; v < 3; ++v) {
-            if ((v >= 1 && M % 128) || (v == 2 && N % 128)) continue;
+        for (int v = 0; v < kGemmVariants; ++v) {
+            if (!gemm_fits(v, M, N, K, kchunk)) continue;
             if ((e = launch_gemm(v, o->stream, A, sam, sak, B, sbk, sbn, M, N, K, kchunk, epi)) != hipSuccess) return e;
             (void)hipEventRecord(e0, o->stream);
             for (int r = 0; r < 3; ++r)
@@ -497,6 +658,12 @@ int round64(long n) { return (int)((n + 63) / 64 * 64); }
 }  // namespace
 
 extern "C" {
+
+int o2f_mlp_set_gemm_variant(o2f_mlp *o, int variant) {
+    if (!o || variant < -1 || variant >= kGemmVariants) return O2F_ERR_ARG;
+    o->gemm_variant = variant;
+    return O2F_OK;
+}
 
 int o2f_mlp_create(unsigned long long seed, o2f_mlp **out) {
     if (!out) return O2F_ERR_ARG;

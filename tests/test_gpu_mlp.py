@@ -65,6 +65,29 @@ def test_gradient_matches_the_oracle():
             assert rel < 1e-2
 
 
+def test_every_gemm_kernel_gives_the_same_bits():
+    """The seven GEMM kernels (single / double-buffered, four tile shapes)
+    accumulate each output in the same k order: forward outputs, loss and
+    gradients are bit-identical whichever runs (a forced kernel runs where
+    its tile divides the shape, the 64 x 64 one elsewhere)."""
+    net = capi.WarmStartNet(seed=6)
+    rng = np.random.default_rng(4)
+    raw = sample(1024, rng)
+    tgt = smooth_target(raw)
+    ref = None
+    for v in range(7):
+        net.set_gemm_variant(v)
+        y = net.forward(raw)
+        loss, g = net.loss_grad(raw, tgt)
+        if ref is None:
+            ref = (y, loss, g)
+        else:
+            assert np.array_equal(y, ref[0]) and loss == ref[1] and np.array_equal(g, ref[2]), v
+    net.set_gemm_variant(-1)
+    assert np.array_equal(net.forward(raw), ref[0])
+    assert np.array_equal(net.loss_grad(raw, tgt)[1], ref[2])
+
+
 def test_adam_training_reduces_the_loss():
     net = capi.WarmStartNet(seed=5)
     rng = np.random.default_rng(3)
