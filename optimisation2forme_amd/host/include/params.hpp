@@ -63,6 +63,11 @@ struct SimulationParams {
     int adjoint_chunk = 500;          // BiCGSTAB restart period
     int adjoint_max_iters = 100000;   // BiCGSTAB iteration cap
     int max_steps = -1;               // cap on time steps (-1: none)
+    // adjoint and shape gradient of optimizeShape (defaults: the reference's)
+    std::string adjoint_operator = "reference";   // "reference" (FD HLLC blocks) | "exact"
+    std::string adjoint_rhs = "reference";        // "reference" | "exact" (with the exact operator)
+    std::string adjoint_preconditioner = "none";  // "none" | "block_jacobi"
+    std::string shape_gradient = "finite_difference";  // | "tangent" | "tangent_full" (forward mode)
 
     // c_v = c_p / gamma, R = c_p - c_v (src/params.cpp:5-8)
     void finalize();

@@ -36,7 +36,10 @@ int main(int argc, char *argv[]) {
                   << "\nstate_load_path=" << params.state_load_path << "\nstate_save_path=" << params.state_save_path
                   << "\nshape_opt_enable=" << params.shape_opt_enable << "\nshape_param_file=" << params.shape_param_file
                   << "\nlimiter=" << params.limiter << "\nvenkat_K=" << params.venkat_K
-                  << "\ninterp_mode=" << params.interp_mode << "\nheat_flux=" << params.heat_flux << "\ncv=" << params.cv
+                  << "\ninterp_mode=" << params.interp_mode << "\nheat_flux=" << params.heat_flux
+                  << "\nadjoint_operator=" << params.adjoint_operator << "\nadjoint_rhs=" << params.adjoint_rhs
+                  << "\nadjoint_preconditioner=" << params.adjoint_preconditioner
+                  << "\nshape_gradient=" << params.shape_gradient << "\ncv=" << params.cv
                   << "\nR=" << params.R << "\n";
         return 0;
     }

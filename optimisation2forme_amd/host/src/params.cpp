@@ -75,6 +75,10 @@ bool load_config(const std::string &filename, SimulationParams &p) {
         else if (key == "adjoint_chunk") ss >> p.adjoint_chunk;
         else if (key == "adjoint_max_iters") ss >> p.adjoint_max_iters;
         else if (key == "max_steps") ss >> p.max_steps;
+        else if (key == "adjoint_operator") p.adjoint_operator = val;
+        else if (key == "adjoint_rhs") p.adjoint_rhs = val;
+        else if (key == "adjoint_preconditioner") p.adjoint_preconditioner = val;
+        else if (key == "shape_gradient") p.shape_gradient = val;
     }
     p.finalize();
     return true;

@@ -102,16 +102,35 @@ void optimizeShape(Mesh &mesh, const SimulationParams &params, double eps) {
     std::vector<double> lambda(4 * nc);
     int iters = 0;
     double relres = 0.0;
-    const int st = o2f_bicgstab(mesh.ctx(), beta, eps, params.adjoint_tol, params.adjoint_chunk,
-                                params.adjoint_max_iters, lambda.data(), &iters, &relres);
+    // solveAdjointSystem (src/shape_optimization.cpp:658-740), or the
+    // extensions the config selects
+    const bool block_jacobi = params.adjoint_preconditioner == "block_jacobi";
+    int st;
+    if (params.adjoint_operator == "exact") {
+        const int opts = (params.adjoint_rhs == "exact" ? O2F_ADJ_EXACT_RHS : 0) |
+                         (block_jacobi ? O2F_ADJ_BLOCK_JACOBI : 0);
+        st = o2f_bicgstab_exact(mesh.ctx(), beta, params.adjoint_tol, params.adjoint_chunk,
+                                params.adjoint_max_iters, lambda.data(), &iters, &relres, opts);
+    } else if (block_jacobi) {
+        st = o2f_bicgstab_options(mesh.ctx(), beta, eps, params.adjoint_tol, params.adjoint_chunk,
+                                  params.adjoint_max_iters, O2F_ADJ_BLOCK_JACOBI, lambda.data(), &iters, &relres);
+    } else {
+        st = o2f_bicgstab(mesh.ctx(), beta, eps, params.adjoint_tol, params.adjoint_chunk,
+                          params.adjoint_max_iters, lambda.data(), &iters, &relres);
+    }
     std::cout << "[adjoint] iters=" << iters << "  relres=" << relres << std::endl;
     mesh.check(st);
     std::vector<const char *> na, no;
     for (auto &s : names_all) na.push_back(s.c_str());
     for (auto &s : names) no.push_back(s.c_str());
     std::vector<double> grad(names.size());
-    mesh.check(o2f_shape_gradient(mesh.ctx(), na.data(), values_all.data(), (int)na.size(), no.data(),
-                                  (int)no.size(), lambda.data(), beta, eps, grad.data()));
+    if (params.shape_gradient == "tangent" || params.shape_gradient == "tangent_full")
+        mesh.check(o2f_shape_gradient_tangent(
+            mesh.ctx(), na.data(), values_all.data(), (int)na.size(), no.data(), (int)no.size(), lambda.data(), beta,
+            params.shape_gradient == "tangent_full" ? O2F_GEOM_FULL : O2F_GEOM_REFERENCE, grad.data()));
+    else
+        mesh.check(o2f_shape_gradient(mesh.ctx(), na.data(), values_all.data(), (int)na.size(), no.data(),
+                                      (int)no.size(), lambda.data(), beta, eps, grad.data()));
     std::cout << std::fixed << std::setprecision(8);
     std::cout << "[preview] J courant = " << J_current << "\n";
     std::cout << "[preview] Gradient de forme (dJ/dθ) et sens pour diminuer J :\n";
