@@ -291,6 +291,13 @@ int o2f_naca_displacement(o2f_ctx *ctx, const char *name, const char *const *nam
  * state. */
 int o2f_residual_tangent(o2f_ctx *ctx, const double *dxy, int geometry, double beta, double *R, double *dR,
                          double *dJ);
+/* Jacobian-vector product y = (dR/dU) dU at the current state by the same
+ * forward-mode (dual number) residual, with the geometry fixed: the
+ * forward counterpart of o2f_apply_jt_exact, so <x, J y> = <J^T x, y> holds
+ * to rounding.  dU, dR: 4*n_cells SoA; R (may be NULL) receives the
+ * residual, bit-identical to o2f_residual's.  No counterpart in the
+ * reference (src/shape_optimization.cpp:192-227 finite-differences it). */
+int o2f_residual_jvp(o2f_ctx *ctx, const double *dU, double *R, double *dR);
 /* computeShapeGradient's g_k = dJ/dk - lambda . dR/dk with both
  * derivatives from o2f_residual_tangent along o2f_naca_displacement (one
  * tangent evaluation per parameter instead of two perturbed residuals and

@@ -41,7 +41,7 @@ EXPORTS = [
     "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
     "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_naca_displacement",
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
-    "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant",
+    "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
 ]
 
 
@@ -117,6 +117,7 @@ def lib():
                                              C.c_double, D]),
             "o2f_naca_displacement": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_char_p), D, C.c_int, D]),
             "o2f_residual_tangent": (C.c_int, [vp, D, C.c_int, C.c_double, D, D, D]),
+            "o2f_residual_jvp": (C.c_int, [vp, D, D, D]),
             "o2f_shape_gradient_tangent": (C.c_int, [vp, C.POINTER(C.c_char_p), D, C.c_int,
                                                      C.POINTER(C.c_char_p), C.c_int, D, C.c_double,
                                                      C.c_int, D]),
@@ -462,6 +463,13 @@ class Solver:
         self._check(lib().o2f_residual_tangent(self.h, _d(dxy), GEOMETRY[geometry], beta, _d(R), _d(dR),
                                                C.byref(dJ)))
         return R, dR, dJ.value
+
+    def residual_jvp(self, dU):
+        """(R, (dR/dU) dU) at the current state, forward mode."""
+        dU = np.ascontiguousarray(dU, dtype=np.float64).reshape(4, self.nc)
+        R, dR = np.zeros((4, self.nc)), np.zeros((4, self.nc))
+        self._check(lib().o2f_residual_jvp(self.h, _d(dU), _d(R), _d(dR)))
+        return R, dR
 
     def shape_gradient_tangent(self, names_all, values_all, opt_names, lam, beta, geometry="reference"):
         na = (C.c_char_p * len(names_all))(*[n.encode() for n in names_all])

@@ -1259,8 +1259,8 @@ int tangent_on(o2f_ctx *c, const double *dxy, int geometry, double beta, double 
     const Topo T = c->topo();
     const Geom G = c->geo[0].view(), dG = gs.view();
     const TanWork W = tan_work(c);
-    HIPCHK(c, launch_residual_tangent(T, G, dG, c->phys, c->U, c->vU, c->fL, c->fR, c->tq, c->vtq, c->Fe, c->R, W,
-                                      c->stream));
+    HIPCHK(c, launch_residual_tangent(T, G, dG, c->phys, c->U, nullptr, c->vU, c->fL, c->fR, c->tq, c->vtq, c->Fe,
+                                      c->R, W, c->stream));
     ++c->faces_version;
     HIPCHK(c, launch_objective_tangent(G, dG, c->wall, (int)c->hm.wall_edges.size(), c->fL, W.dfL, c->prm.gamma, 1,
                                        c->t_out, c->stream));
@@ -1298,6 +1298,25 @@ int o2f_residual_tangent(o2f_ctx *c, const double *dxy, int geometry, double bet
     if ((st = copy_out(c, dR, tan_work(c).dR, n))) return st;
     if (dJ) *dJ = dj;
     // leave the context as computeResidual(mesh) leaves it
+    if ((st = residual_on(c, 0))) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+int o2f_residual_jvp(o2f_ctx *c, const double *dU, double *R, double *dR) {
+    if (!c || !dU) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    if ((st = ensure_work(c)) || (st = ensure_tangent(c))) return st;
+    const size_t n = 4 * (size_t)c->hm.nc;
+    double *du = c->kv[10];
+    HIPCHK(c, hipMemcpyAsync(du, dU, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    const TanWork W = tan_work(c);
+    HIPCHK(c, launch_residual_tangent(c->topo(), c->geo[0].view(), Geom{}, c->phys, c->U, du, c->vU, c->fL, c->fR,
+                                      c->tq, c->vtq, c->Fe, c->R, W, c->stream));
+    ++c->faces_version;
+    if ((st = copy_out(c, R, c->R, n))) return st;
+    if ((st = copy_out(c, dR, W.dR, n))) return st;
     if ((st = residual_on(c, 0))) return st;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return O2F_OK;

@@ -44,6 +44,7 @@ __device__ __forceinline__ bool operator<(Dual a, double b) { return a.v < b; }
 __device__ __forceinline__ bool operator<=(Dual a, double b) { return a.v <= b; }
 __device__ __forceinline__ bool operator>=(Dual a, double b) { return a.v >= b; }
 __device__ __forceinline__ bool operator>(Dual a, double b) { return a.v > b; }
+__device__ __forceinline__ bool operator>(Dual a, Dual b) { return a.v > b.v; }
 using ::sqrt;  // the double overloads stay visible next to sqrt(Dual)
 __device__ __forceinline__ Dual sqrt(Dual a) {
     const double r = ::sqrt(a.v);
@@ -237,23 +238,22 @@ __device__ __forceinline__ double vpart(Dual a) { return a.v; }
 // scalar type (the double versions above are what the forward kernels use;
 // these are the same expressions for the geometry tangent).
 template <class T>
-__device__ __forceinline__ T alpha_venkat_g(double phi_i, T phi_pred, double phi_min, double phi_max, T cellArea,
-                                            double k_eps) {
+__device__ __forceinline__ T alpha_venkat_g(T phi_i, T phi_pred, T phi_min, T phi_max, T cellArea, double k_eps) {
     const T delta = phi_pred - phi_i;
     if (fabs(vpart(delta)) < DBL_EPSILON) return T(1.0);
     const T h = sqrt(cellArea / M_PI);
     const T eps = k_eps * k_eps * k_eps * h * h * h;
-    const double a = (vpart(delta) > 0.0) ? (phi_max - phi_i) : (phi_min - phi_i);
+    const T a = (vpart(delta) > 0.0) ? (phi_max - phi_i) : (phi_min - phi_i);
     const T alpha = (a * a + eps + 2.0 * a * delta) / (a * a + eps + 2.0 * delta * delta);
     return (vpart(alpha) < 0.0) ? T(0.0) : (1.0 < vpart(alpha)) ? T(1.0) : alpha;
 }
 template <class T>
-__device__ __forceinline__ T alpha_bj_g(double phi_i, T phi_pred, double phi_min, double phi_max) {
+__device__ __forceinline__ T alpha_bj_g(T phi_i, T phi_pred, T phi_min, T phi_max) {
     T alpha = T(1.0);
     if (fabs(vpart(phi_pred - phi_i)) < DBL_EPSILON) return alpha;
-    if (vpart(phi_pred) > phi_i)
+    if (vpart(phi_pred) > vpart(phi_i))
         alpha = smin_t(T(1.0), (phi_max - phi_i) / (phi_pred - phi_i));
-    else if (vpart(phi_pred) < phi_i)
+    else if (vpart(phi_pred) < vpart(phi_i))
         alpha = smin_t(T(1.0), (phi_min - phi_i) / (phi_pred - phi_i));
     return alpha;
 }

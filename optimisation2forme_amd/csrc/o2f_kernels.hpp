@@ -162,17 +162,18 @@ hipError_t launch_adjoint_exact(const Topo &T, const Geom &G, const Phys &P, con
                                 const ObjSeed *seed = nullptr);
 
 // ------------------------------------------------------------------------
-// Geometric tangent (o2f_tangent.hip): the residual at a fixed state and
-// its directional derivative along a geometry tangent dG (same layout as
-// the geometry).  Values go to the forward arrays (bit-identical to the
+// Tangent (o2f_tangent.hip): the residual and its directional derivative
+// along a geometry tangent dG (same layout as the geometry) and a state
+// tangent dU.  Values go to the forward arrays (bit-identical to the
 // forward kernels), derivatives to the TanWork arrays of the same layouts.
 // ------------------------------------------------------------------------
 struct TanWork {
     double *dvU, *dvtq, *dfL, *dfR, *dFe, *dtq, *dR;
 };
+// dU (4 nc, SoA) and the arrays of dG may be null (zero tangent).
 hipError_t launch_residual_tangent(const Topo &T, const Geom &G, const Geom &dG, const Phys &P, const double *U,
-                                   double *vU, double *fL, double *fR, double *tq, double *vtq, double *Fe,
-                                   double *R, const TanWork &W, hipStream_t s);
+                                   const double *dU, double *vU, double *fL, double *fR, double *tq, double *vtq,
+                                   double *Fe, double *R, const TanWork &W, hipStream_t s);
 // out = L, D, dL, dD over the wall edges (skip_x: computeDirectObjectiveTerm's x > 1.3 rule)
 hipError_t launch_objective_tangent(const Geom &G, const Geom &dG, const int *wall, int nw, const double *fL,
                                     const double *dfL, double gamma, int skip_x, double *out, hipStream_t s);

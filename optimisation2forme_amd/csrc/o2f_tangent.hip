@@ -1,7 +1,8 @@
-// Geometric tangent of the residual on gfx950: R(U, G) and its directional
-// derivative dR = (dR/dG) dG at a fixed state U, for a geometry tangent dG
-// (cell areas and centres, edge lengths, normals and midpoints, stencil
-// weights) that the host derives from a vertex displacement.
+// Tangent of the residual on gfx950: R(U, G) and its directional
+// derivative dR = (dR/dG) dG + (dR/dU) dU along a geometry tangent dG (cell
+// areas and centres, edge lengths, normals and midpoints, stencil weights,
+// derived on the host from a vertex displacement) and a state tangent dU
+// (either may be absent: null pointers are zero).
 //
 // Every quantity of the residual (computeResidual,
 // src/shape_optimization.cpp:171-189: advection src/advection.cpp:26-466,
@@ -28,7 +29,9 @@ namespace o2f {
 
 namespace {
 
-__device__ __forceinline__ Dual gd(const double *a, const double *da, long i) { return Dual(a[i], da[i]); }
+__device__ __forceinline__ Dual gd(const double *a, const double *da, long i) {
+    return da ? Dual(a[i], da[i]) : Dual(a[i]);
+}
 
 // Stencil gather of NV quantities at vertex v (gather_vertex of the forward
 // kernels): weights w0 a0 + w1 a1 + ... left to right, or the two-cell
@@ -55,11 +58,11 @@ __device__ __forceinline__ void gather_t(const Topo &T, const Geom &G, const Geo
     }
 }
 
-// the cell state (SoA [q][nc], no derivative: U is held fixed)
+// the cell state (SoA [q][nc]) with its tangent (null: zero)
 struct StateSrc {
-    const double *U;
+    const double *U, *dU;
     int nc;
-    __device__ Dual operator()(int c, int q) const { return Dual(U[(long)q * nc + c]); }
+    __device__ Dual operator()(int c, int q) const { return gd(U, dU, (long)q * nc + c); }
 };
 // the cell stresses (AoS 4 + trailing block) with their derivatives
 struct StressSrc {
@@ -82,10 +85,11 @@ __device__ __forceinline__ void ld4d(const double *v, const double *d, long i, D
 }
 
 __global__ __launch_bounds__(256) void kt_vertex_state(Topo T, Geom G, Geom dG, Phys P, const double *__restrict__ U,
-                                                       double *__restrict__ vU, double *__restrict__ dvU) {
+                                                       const double *__restrict__ dU, double *__restrict__ vU,
+                                                       double *__restrict__ dvU) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= T.nv) return;
-    const StateSrc src{U, T.nc};
+    const StateSrc src{U, dU, T.nc};
     Dual out[4];
     gather_t<4>(T, G, dG, v, src, out);
     const char bc = T.vbc[v];
@@ -114,7 +118,8 @@ __global__ __launch_bounds__(256) void kt_vertex_visc(Topo T, Geom G, Geom dG, P
 // limiter, face states and ghosts (:232-321, src/boundary.cpp:48-80), cell
 // stress / heat flux (src/diffusion.cpp:223-241).
 __global__ __launch_bounds__(256) void kt_reconstruct(Topo T, Geom G, Geom dG, Phys P, const double *__restrict__ U,
-                                                      const double *__restrict__ vU, const double *__restrict__ dvU,
+                                                      const double *__restrict__ dU, const double *__restrict__ vU,
+                                                      const double *__restrict__ dvU,
                                                       double *__restrict__ fL, double *__restrict__ dfL,
                                                       double *__restrict__ fR, double *__restrict__ dfR,
                                                       double *__restrict__ tq, double *__restrict__ dtq) {
@@ -123,8 +128,8 @@ __global__ __launch_bounds__(256) void kt_reconstruct(Topo T, Geom G, Geom dG, P
     const int nc = T.nc;
     const unsigned info = T.cinfo[c];
     const int nk = info >> 4;
-    double u[4];
-    for (int q = 0; q < 4; ++q) u[q] = U[(long)q * nc + c];
+    Dual u[4];
+    for (int q = 0; q < 4; ++q) u[q] = gd(U, dU, (long)q * nc + c);
     const Dual vol = gd(G.cvol, dG.cvol, c), cx = gd(G.ccx, dG.ccx, c), cy = gd(G.ccy, dG.ccy, c);
     Dual gx[4], gy[4];
     for (int q = 0; q < 4; ++q) gx[q] = gy[q] = Dual(0.0);
@@ -148,12 +153,12 @@ __global__ __launch_bounds__(256) void kt_reconstruct(Topo T, Geom G, Geom dG, P
             gy[q] = gy[q] + val[q] * el * ny / vol;
         }
     }
-    double mn[4], mx[4];
+    Dual mn[4], mx[4];
     for (int q = 0; q < 4; ++q) mn[q] = mx[q] = u[q];
     for (int j = 0; j < 4; ++j) {
         const int nb = T.cnbr[4 * c + j];
         for (int q = 0; q < 4; ++q) {
-            const double v = (nb >= 0) ? U[(long)q * nc + nb] : u[q];
+            const Dual v = (nb >= 0) ? gd(U, dU, (long)q * nc + nb) : u[q];
             if (v < mn[q]) mn[q] = v;
             if (v > mx[q]) mx[q] = v;
         }
@@ -187,9 +192,8 @@ __global__ __launch_bounds__(256) void kt_reconstruct(Topo T, Geom G, Geom dG, P
             st4d(fR, dfR, e, f);
         }
     }
-    const Dual du[4] = {Dual(u[0]), Dual(u[1]), Dual(u[2]), Dual(u[3])};
     Dual t5[5];
-    cell_stress_t<Dual>(du, gx, gy, P.mu, P.gamma, P.kappa, t5);
+    cell_stress_t<Dual>(u, gx, gy, P.mu, P.gamma, P.kappa, t5);
     const Dual t4[4] = {t5[0], t5[1], t5[2], t5[3]};
     st4d(tq, dtq, c, t4);
     tq[4 * (long)nc + c] = t5[4].v;
@@ -213,7 +217,8 @@ __global__ __launch_bounds__(256) void kt_edge_flux(Topo T, Geom G, Geom dG, Phy
 // then the viscous divergence (src/diffusion.cpp:145-201), edges in slot
 // (ascending id) order.
 __global__ __launch_bounds__(256) void kt_flux_update(Topo T, Geom G, Geom dG, Phys P, const double *__restrict__ U,
-                                                      const double *__restrict__ Fe, const double *__restrict__ dFe,
+                                                      const double *__restrict__ dU, const double *__restrict__ Fe,
+                                                      const double *__restrict__ dFe,
                                                       const double *__restrict__ vtq, const double *__restrict__ dvtq,
                                                       double *__restrict__ R, double *__restrict__ dR) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -221,8 +226,8 @@ __global__ __launch_bounds__(256) void kt_flux_update(Topo T, Geom G, Geom dG, P
     const int nc = T.nc, nv = T.nv;
     const unsigned info = T.cinfo[c];
     const int nk = info >> 4;
-    double u[4];
-    for (int q = 0; q < 4; ++q) u[q] = U[(long)q * nc + c];
+    Dual u[4];
+    for (int q = 0; q < 4; ++q) u[q] = gd(U, dU, (long)q * nc + c);
     const Dual vol = gd(G.cvol, dG.cvol, c);
     Dual r[4];
     for (int q = 0; q < 4; ++q) r[q] = Dual(0.0);
@@ -236,7 +241,7 @@ __global__ __launch_bounds__(256) void kt_flux_update(Topo T, Geom G, Geom dG, P
     }
     const Dual invV = 1.0 / vol;
     for (int q = 0; q < 4; ++q) r[q] = r[q] * -invV;
-    const double uu = u[1] / u[0], vv = u[2] / u[0];
+    const Dual uu = u[1] / u[0], vv = u[2] / u[0];
     for (int k = 0; k < nk; ++k) {
         const int e = T.cedge[4 * c + k];
         const int va = T.cevx[8 * c + 2 * k], vb = T.cevx[8 * c + 2 * k + 1];
@@ -324,15 +329,15 @@ inline int nblk(long n) { return static_cast<int>((n + 255) / 256); }
 }  // namespace
 
 hipError_t launch_residual_tangent(const Topo &T, const Geom &G, const Geom &dG, const Phys &P, const double *U,
-                                   double *vU, double *fL, double *fR, double *tq, double *vtq, double *Fe,
-                                   double *R, const TanWork &W, hipStream_t s) {
-    hipLaunchKernelGGL(kt_vertex_state, dim3(nblk(T.nv)), dim3(256), 0, s, T, G, dG, P, U, vU, W.dvU);
-    hipLaunchKernelGGL(kt_reconstruct, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, dG, P, U, vU, W.dvU, fL, W.dfL, fR,
-                       W.dfR, tq, W.dtq);
+                                   const double *dU, double *vU, double *fL, double *fR, double *tq, double *vtq,
+                                   double *Fe, double *R, const TanWork &W, hipStream_t s) {
+    hipLaunchKernelGGL(kt_vertex_state, dim3(nblk(T.nv)), dim3(256), 0, s, T, G, dG, P, U, dU, vU, W.dvU);
+    hipLaunchKernelGGL(kt_reconstruct, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, dG, P, U, dU, vU, W.dvU, fL, W.dfL,
+                       fR, W.dfR, tq, W.dtq);
     hipLaunchKernelGGL(kt_vertex_visc, dim3(nblk(T.nv)), dim3(256), 0, s, T, G, dG, P, tq, W.dtq, vtq, W.dvtq);
     hipLaunchKernelGGL(kt_edge_flux, dim3(nblk(T.ne)), dim3(256), 0, s, T, G, dG, P, fL, W.dfL, fR, W.dfR, Fe, W.dFe);
-    hipLaunchKernelGGL(kt_flux_update, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, dG, P, U, Fe, W.dFe, vtq, W.dvtq, R,
-                       W.dR);
+    hipLaunchKernelGGL(kt_flux_update, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, dG, P, U, dU, Fe, W.dFe, vtq, W.dvtq,
+                       R, W.dR);
     return hipGetLastError();
 }
 

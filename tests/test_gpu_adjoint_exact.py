@@ -359,3 +359,63 @@ def test_block_jacobi_reference_bicgstab(viscosity):
     else:
         assert st == 0 and it > 5 * it_bj
         assert np.linalg.norm(lam_bj - lam) <= 1e-6 * np.linalg.norm(lam)
+
+
+@pytest.mark.parametrize("mesh,limiter,heat_flux", [("tiny", 0, 1), ("tiny", 1, 1), ("tiny", 0, 0),
+                                                    ("small", 0, 1), ("small", 1, 1), ("shipped", 0, 0)])
+def test_transpose_identity_against_the_forward_jvp(mesh, limiter, heat_flux):
+    """<J^T x, y> (the hand-written reverse-mode kernels) against <x, J y>
+    (the dual-number forward-mode residual, o2f_residual_jvp) for seeded
+    random x, y: two implementations that share no derivative code agree to
+    rounding, entry pattern and branch choices included."""
+    if mesh == "shipped":
+        desc, fl = H.shipped_mesh(), dict(H.SHIPPED)
+        U0 = H.shipped_flow()
+        U = U0 * (1.0 + 1e-3 * np.random.default_rng(6).standard_normal(U0.shape))
+    else:
+        desc = H.tiny_mesh() if mesh == "tiny" else H.small_mesh()
+        fl = flow(limiter, heat_flux)
+        _, _, U = state(desc, fl, steps=15 if mesh == "tiny" else 10)
+    s = capi.Solver(desc, H.capi_params(fl))
+    s.set_state(U)
+    R_fwd = s.residual()
+    rng = np.random.default_rng(11)
+    for trial in range(3):
+        x = rng.standard_normal(U.shape)
+        y = rng.standard_normal(U.shape) * np.max(np.abs(U), axis=1, keepdims=True)
+        R, Jy = s.residual_jvp(y)
+        assert np.array_equal(R, R_fwd)
+        JTx = s.apply_jt_exact(x)
+        a, b = np.sum(JTx * y), np.sum(x * Jy)
+        scale = np.sum(np.abs(x * Jy))
+        print("%s trial %d: <J^T x, y> %.15e  <x, J y> %.15e  diff / scale %.3g" %
+              (mesh, trial, a, b, abs(a - b) / scale), flush=True)
+        assert abs(a - b) <= 1e-11 * scale
+
+
+def test_jvp_against_finite_differences():
+    """o2f_residual_jvp along a random direction against central differences
+    of the oracle's residual (160-triangle mesh; entries next to a limiter
+    or min / max switch excluded as above)."""
+    desc = H.tiny_mesh()
+    fl = flow()
+    m, p, U = state(desc, fl)
+    s = capi.Solver(desc, H.capi_params(fl))
+    s.set_state(U)
+    y = np.random.default_rng(12).standard_normal(U.shape) * np.max(np.abs(U), axis=1, keepdims=True)
+    _, Jy = s.residual_jvp(y)
+
+    def R_at(h):
+        m.set_state(U + h * y)
+        return m.residual(p).copy()
+    R0 = R_at(0.0)
+    c1 = (R_at(1e-7) - R_at(-1e-7)) / 2e-7
+    c2 = (R_at(1e-8) - R_at(-1e-8)) / 2e-8
+    f1, b1 = (R_at(1e-7) - R0) / 1e-7, (R0 - R_at(-1e-7)) / 1e-7
+    sc = np.max(np.abs(c1), axis=1, keepdims=True)
+    smooth = (np.abs(f1 - b1) / sc < 1e-4) & (np.abs(c1 - c2) / sc < 1e-5)
+    err = np.abs(Jy - c1) / sc
+    print("smooth %.3f  p99 %.3g  max %.3g" % (smooth.mean(), np.percentile(err[smooth], 99), err[smooth].max()),
+          flush=True)
+    assert smooth.mean() > 0.9
+    assert np.percentile(err[smooth], 99) < 1e-6
