@@ -273,11 +273,12 @@ __global__ __launch_bounds__(256) void ka_cell(Topo T, Geom G, Phys P, AdjTopo A
     }
     double a_k[4] = {1.0, 1.0, 1.0, 1.0};
     if (has_edge) {
+        const double veps = (P.limiter == 1) ? 0.0 : venkat_eps(vol, P.venkat_k);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const double pred = u[q] + gx[q] * dx + gy[q] * dy;
             a_k[q] = (P.limiter == 1) ? alpha_bj(u[q], pred, mn[q], mx[q])
-                                      : alpha_venkat(u[q], pred, mn[q], mx[q], vol, P.venkat_k);
+                                      : alpha_venkat_eps(u[q], pred, mn[q], mx[q], veps);
         }
     }
     double al[4] = {1.0, 1.0, 1.0, 1.0};

@@ -188,15 +188,26 @@ __device__ __forceinline__ void hllc(const double *qL, const double *qR, double 
 }
 
 // Limiters of computeInterfaceStates (src/advection.cpp:195-230).
-__device__ __forceinline__ double alpha_venkat(double phi_i, double phi_pred, double phi_min,
-                                               double phi_max, double cellArea, double k_eps) {
+// The limiter's eps = K^3 h^3, h = sqrt(V / pi): a function of the cell
+// alone, so a cell's edges can share one evaluation (same expression, same
+// rounding as computing it per edge).
+__device__ __forceinline__ double venkat_eps(double cellArea, double k_eps) {
+    const double h = sqrt(cellArea / M_PI);
+    return k_eps * k_eps * k_eps * h * h * h;
+}
+
+__device__ __forceinline__ double alpha_venkat_eps(double phi_i, double phi_pred, double phi_min,
+                                                   double phi_max, double eps) {
     const double delta = phi_pred - phi_i;
     if (fabs(delta) < DBL_EPSILON) return 1.0;
-    const double h = sqrt(cellArea / M_PI);
-    const double eps = k_eps * k_eps * k_eps * h * h * h;
     const double a = (delta > 0.0) ? (phi_max - phi_i) : (phi_min - phi_i);
     const double alpha = (a * a + eps + 2.0 * a * delta) / (a * a + eps + 2.0 * delta * delta);
     return sclamp(alpha, 0.0, 1.0);
+}
+
+__device__ __forceinline__ double alpha_venkat(double phi_i, double phi_pred, double phi_min,
+                                               double phi_max, double cellArea, double k_eps) {
+    return alpha_venkat_eps(phi_i, phi_pred, phi_min, phi_max, venkat_eps(cellArea, k_eps));
 }
 
 __device__ __forceinline__ double alpha_bj(double phi_i, double phi_pred, double phi_min,

@@ -200,22 +200,24 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            tx[q] = val[q] * el * nx / vol;
-            ty[q] = val[q] * el * ny / vol;
+            tx[q] = val[q] * el * nx;
+            ty[q] = val[q] * el * ny;
         }
     }
     // From here lane k owns conservative variable k.  The 4 x 4 in-quad
-    // transpose gives lane k the terms of variable k of the edges in slot
-    // order; lanes without an edge contributed +0.0 terms, and the sums
-    // start at +0.0, so they are never -0.0 and adding +0.0 changes
-    // nothing.
+    // transpose gives lane k the numerators of variable k's terms of the
+    // edges in slot order; it divides them by the volume (the same
+    // val * el * n / vol), so a triangle costs 3 division pairs per lane
+    // rather than 4.  The sums start at +0.0 and take the nk terms in slot
+    // order.
     quad_transpose(tx, k);
     quad_transpose(ty, k);
     double gxk = 0.0, gyk = 0.0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        gxk += tx[j];
-        gyk += ty[j];
+        if (j >= nk) break;
+        gxk += tx[j] / vol;
+        gyk += ty[j] / vol;
     }
     const double uk = (k == 0) ? u[0] : (k == 1) ? u[1] : (k == 2) ? u[2] : u[3];
     // neighbour min / max of variable k in neighbour-list order (no
@@ -232,13 +234,13 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
     // beyond nk would offer 1.0, which changes nothing)
     const double dxs[4] = {from_lane<0>(dx), from_lane<1>(dx), from_lane<2>(dx), from_lane<3>(dx)};
     const double dys[4] = {from_lane<0>(dy), from_lane<1>(dy), from_lane<2>(dy), from_lane<3>(dy)};
+    const double veps = (P.limiter == 1) ? 0.0 : venkat_eps(vol, P.venkat_k);
     double al = 1.0;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (j >= nk) break;
         const double pred = uk + gxk * dxs[j] + gyk * dys[j];
-        const double a = (P.limiter == 1) ? alpha_bj(uk, pred, mn, mx)
-                                          : alpha_venkat(uk, pred, mn, mx, vol, P.venkat_k);
+        const double a = (P.limiter == 1) ? alpha_bj(uk, pred, mn, mx) : alpha_venkat_eps(uk, pred, mn, mx, veps);
         al = smin(al, a);
     }
     // face states of variable k on every edge (+ boundary ghost,
@@ -272,7 +274,8 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
     const double gy[4] = {from_lane<0>(gyk), from_lane<1>(gyk), from_lane<2>(gyk), from_lane<3>(gyk)};
     const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
     const double rho = u[0], E = u[3];
-    const double uu = u[1] / rho, vvel = u[2] / rho;
+    const double vel = ((k & 1) ? u[2] : u[1]) / rho;  // lane 0: u, lane 1: v
+    const double uu = from_lane<0>(vel), vvel = from_lane<1>(vel);
     const double num = (k == 0) ? (gx[1] - uu * gx[0])
                        : (k == 1) ? (gy[2] - vvel * gy[0])
                        : (k == 2) ? (gy[1] - uu * gy[0])
