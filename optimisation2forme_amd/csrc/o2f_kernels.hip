@@ -266,25 +266,31 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
         Gout[(long)(4 + k) * nc + c] = gyk;
     }
     // viscous stress and heat flux of the cell (src/diffusion.cpp:227-240),
-    // the divisions spread over the four lanes: lane k forms one velocity
-    // derivative (u_x, v_y, u_y, v_x), lanes 0 / 1 one temperature
+    // the divisions spread over the four lanes: the velocities and the
+    // heat-flux quotients share one division, then lane k forms one velocity
+    // derivative (u_x, v_y, u_y, v_x) and lanes 0 / 1 one temperature
     // derivative (T_x, T_y); every value is the same expression as in one
     // lane, so the results are unchanged
     const double gx[4] = {from_lane<0>(gxk), from_lane<1>(gxk), from_lane<2>(gxk), from_lane<3>(gxk)};
     const double gy[4] = {from_lane<0>(gyk), from_lane<1>(gyk), from_lane<2>(gyk), from_lane<3>(gyk)};
     const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
     const double rho = u[0], E = u[3];
-    const double vel = ((k & 1) ? u[2] : u[1]) / rho;  // lane 0: u, lane 1: v
-    const double uu = from_lane<0>(vel), vvel = from_lane<1>(vel);
+    // one division for four quotients: lanes 0 / 1 the velocities u, v,
+    // lanes 2 / 3 the heat-flux terms (E_x rho - E rho_x) / rho^2 in x, y
+    const double qn = (k == 0)   ? u[1]
+                      : (k == 1) ? u[2]
+                      : (k == 2) ? (gx[3] * rho - E * gx[0])
+                                 : (gy[3] * rho - E * gy[0]);
+    const double qv = qn / ((k < 2) ? rho : rho * rho);
+    const double uu = from_lane<0>(qv), vvel = from_lane<1>(qv), ex = from_lane<2>(qv), ey = from_lane<3>(qv);
     const double num = (k == 0) ? (gx[1] - uu * gx[0])
                        : (k == 1) ? (gy[2] - vvel * gy[0])
                        : (k == 2) ? (gy[1] - uu * gy[0])
                                   : (gx[2] - vvel * gx[0]);
     const double d = num / rho;
     const double u_x = from_lane<0>(d), v_y = from_lane<1>(d), u_y = from_lane<2>(d), v_x = from_lane<3>(d);
-    const double tn = (k & 1) ? (gy[3] * rho - E * gy[0]) : (gx[3] * rho - E * gx[0]);
     const double tk = (k & 1) ? (uu * u_y + vvel * v_y) : (uu * u_x + vvel * v_x);
-    const double Tk = (gamma - 1.0) * (tn / (rho * rho) - tk);
+    const double Tk = (gamma - 1.0) * (((k & 1) ? ey : ex) - tk);
     const double T_x = from_lane<0>(Tk), T_y = from_lane<1>(Tk);
     if (k != 0) return;
     const double t4[4] = {2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y), 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y),
