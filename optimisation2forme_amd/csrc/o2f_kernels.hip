@@ -435,53 +435,49 @@ __device__ __forceinline__ void flux_update_body(int blk, const Topo &T, const G
             w2 = txy * el * nx / vol + tyy * el * ny / vol;
             w3 = (uu * txx + vv * txy - qx) * el * nx / vol + (uu * txy + vv * tyy - qy) * el * ny / vol;
         }
-        // ordered sums over the cell's edges
-        double r[4];
+        // From here lane k owns conservative variable k.  The in-quad
+        // transposes give lane k variable k's terms of the edges in slot
+        // order; it sums them, updates and stores variable k.
+        double rk;
         if (flags & kFlagAdv) {
-            r[0] = r[1] = r[2] = r[3] = 0.0;
+            // all four slots in order: +0.0 from lanes without an edge (the
+            // sum starts at +0.0, so it is never -0.0 and +0.0 changes nothing)
+            quad_transpose(adv, k);
+            rk = 0.0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q)  // +0.0 from lanes without an edge (see the gradient sums)
-                for_lanes(adv[q], [&](int, double a) { r[q] += a; });
+            for (int j = 0; j < 4; ++j) rk += adv[j];
             const double invV = 1.0 / vol;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) r[q] = r[q] * -invV;
+            rk = rk * -invV;
         } else {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) r[q] = R[q * nc + c];
+            rk = R[k * nc + c];
         }
         if (flags & kFlagVisc) {
-            // per edge, in slot order: R_rho_u, R_rho_v, R_E each += that edge's term
-            const double a0 = from_lane<0>(w1), b0 = from_lane<0>(w2), d0 = from_lane<0>(w3);
-            const double a1 = from_lane<1>(w1), b1 = from_lane<1>(w2), d1 = from_lane<1>(w3);
-            const double a2 = from_lane<2>(w1), b2 = from_lane<2>(w2), d2 = from_lane<2>(w3);
-            const double a3 = from_lane<3>(w1), b3 = from_lane<3>(w2), d3 = from_lane<3>(w3);
-            if (0 < nk) { r[1] += a0; r[2] += b0; r[3] += d0; }
-            if (1 < nk) { r[1] += a1; r[2] += b1; r[3] += d1; }
-            if (2 < nk) { r[1] += a2; r[2] += b2; r[3] += d2; }
-            if (3 < nk) { r[1] += a3; r[2] += b3; r[3] += d3; }
+            // per edge, in slot order: R_rho_u, R_rho_v, R_E each += that
+            // edge's term (slots j < nk only: rk may be -0.0 here)
+            double wv[4] = {0.0, w1, w2, w3};
+            quad_transpose(wv, k);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double t = rk + wv[j];
+                rk = (k > 0 && j < nk) ? t : rk;
+            }
         }
-        if (k == 0) {
-            if (flags & kFlagStoreR) {
-#pragma unroll
-                for (int q = 0; q < 4; ++q) R[q * nc + c] = r[q];
+        if (flags & kFlagStoreR) R[k * nc + c] = rk;
+        if (flags & kFlagRK) {
+            const double dt = *dtp;
+            const double uk = (k == 0) ? u[0] : (k == 1) ? u[1] : (k == 2) ? u[2] : u[3];
+            double old;
+            if (flags & kFlagSaveOld) {
+                old = uk;
+                Uold[k * nc + c] = uk;
+            } else {
+                old = Uold[k * nc + c];
             }
-            if (flags & kFlagRK) {
-                const double dt = *dtp;
-                double unew[4];
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    double old;
-                    if (flags & kFlagSaveOld) {
-                        old = u[q];
-                        Uold[q * nc + c] = u[q];
-                    } else {
-                        old = Uold[q * nc + c];
-                    }
-                    unew[q] = c0 * old + c1 * (u[q] + dt * (r[q]));
-                    U[q * nc + c] = unew[q];
-                }
-                if (flags & kFlagSpeed) m = smax(m, wave_speed(unew[0], unew[1], unew[2], unew[3], P.gamma));
-            }
+            const double unew = c0 * old + c1 * (uk + dt * rk);
+            U[k * nc + c] = unew;
+            if (flags & kFlagSpeed)
+                m = smax(m, wave_speed(from_lane<0>(unew), from_lane<1>(unew), from_lane<2>(unew),
+                                       from_lane<3>(unew), P.gamma));
         }
     }
     if ((flags & kFlagSpeed) && !gated) speed_fold(m, speed, blk);
