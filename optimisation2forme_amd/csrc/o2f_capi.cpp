@@ -288,7 +288,13 @@ int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx 
     if (!e) e = upload(c->ebc, hm.ebc);
     if (!e) e = upload(c->vbc, hm.vbc);
     if (!e) e = upload(c->sptr, sptr);
-    if (!e) e = upload(c->scell, hm.scell);
+    {
+        // kStencilSlots entries of padding: the vertex gathers read the
+        // first slots of a stencil in 16-byte loads
+        std::vector<int> padded(hm.scell);
+        padded.resize(hm.scell.size() + kStencilSlots, 0);
+        if (!e) e = upload(c->scell, padded);
+    }
     if (!e) e = upload(c->skind, hm.skind);
     if (!e) e = upload(c->wall, hm.wall_edges);
     if (!e) e = c->geo[0].alloc(hm);

@@ -36,7 +36,7 @@ struct GeomSet {
         hipError_t e;
         if ((e = dalloc(cvol, hm.nc)) || (e = dalloc(ccx, hm.nc)) || (e = dalloc(ccy, hm.nc)) ||
             (e = dalloc(elen, hm.ne)) || (e = dalloc(enx, hm.ne)) || (e = dalloc(eny, hm.ne)) ||
-            (e = dalloc(emx, hm.ne)) || (e = dalloc(emy, hm.ne)) || (e = dalloc(sw, hm.sw.size())))
+            (e = dalloc(emx, hm.ne)) || (e = dalloc(emy, hm.ne)) || (e = dalloc(sw, hm.sw.size() + kStencilSlots)))
             return e;
         return hipSuccess;
     }

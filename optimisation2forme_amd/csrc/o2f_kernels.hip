@@ -21,8 +21,6 @@ namespace o2f {
 // viscous quantities).  NV = 4 state variables with the state BC, or NV = 5
 // viscous quantities with apply_diffusion_bc (src/boundary.cpp:101-127).
 // ------------------------------------------------------------------------
-constexpr int kStencilSlots = 8;  // stencil entries loaded together (longer ones loop)
-
 // SRC_AOS: the source is AoS (4 + trailing block), else SoA [q][nc].
 // The destination is always AoS (4 + trailing block).
 template <int NV, bool SRC_AOS>
@@ -36,14 +34,35 @@ __device__ __forceinline__ void gather_vertex(const Topo &T, const Geom &G, cons
     const int nc = T.nc;
     const uint8_t kind = T.skind[v];
     const int b = T.sptr[v], n = T.sptr[v + 1] - b;
-    // All stencil ids and weights first, then all values: two dependent
-    // load levels instead of one per stencil entry.
+    // The first kStencilSlots ids and weights in 16-byte loads (both arrays
+    // carry kStencilSlots entries of padding; entries past the stencil are
+    // never used), then the values of the slots k < n: two dependent load
+    // levels, and few address computations.  (Per-entry loads under k < n
+    // let the compiler finish each id's address arithmetic inside its
+    // branch, which waits for the ids one at a time.)
     int cid[kStencilSlots];
     double w[kStencilSlots], val[kStencilSlots][NV];
+    {
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        typedef v4i v4i_a4 __attribute__((aligned(4)));
+        typedef v2d v2d_a8 __attribute__((aligned(8)));
+        const v4i_a4 *ip = reinterpret_cast<const v4i_a4 *>(T.scell + b);
+        const v2d_a8 *wp = reinterpret_cast<const v2d_a8 *>(G.sw + b);
 #pragma unroll
-    for (int k = 0; k < kStencilSlots; ++k) {
-        cid[k] = (k < n) ? T.scell[b + k] : 0;
-        w[k] = (k < n) ? G.sw[b + k] : 0.0;
+        for (int h = 0; h < kStencilSlots / 4; ++h) {
+            const v4i t = ip[h];
+            cid[4 * h] = t.x;
+            cid[4 * h + 1] = t.y;
+            cid[4 * h + 2] = t.z;
+            cid[4 * h + 3] = t.w;
+        }
+#pragma unroll
+        for (int h = 0; h < kStencilSlots / 2; ++h) {
+            const v2d t = wp[h];
+            w[2 * h] = t.x;
+            w[2 * h + 1] = t.y;
+        }
     }
 #pragma unroll
     for (int k = 0; k < kStencilSlots; ++k)

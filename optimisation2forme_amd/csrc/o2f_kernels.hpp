@@ -60,6 +60,11 @@ struct Phys {
     double final_t;
 };
 
+// Stencil entries a vertex gather loads together (longer stencils loop
+// over the rest); the stencil id and weight arrays carry this many entries
+// of padding past their end.
+constexpr int kStencilSlots = 8;
+
 // Read-only topology (device pointers).
 struct Topo {
     int nv, ne, nc;
