@@ -863,16 +863,18 @@ __global__ __launch_bounds__(64) void k_adjoint_rhs(Topo T, Geom G, const int *_
 // ------------------------------------------------------------------------
 // Adjoint J^T x (src/shape_optimization.cpp:237-484).
 // ------------------------------------------------------------------------
-// FD flux-Jacobian blocks per edge at the frozen face states:
-// blk[(r*4+c)*ne + e] = dF_r/dU_L,c, blk[(16 + r*4+c)*ne + e] = dF_r/dU_R,c.
-// A boundary edge rebuilds its ghost from each perturbed left state; its
-// right-state block is never read by the scatter and is stored as zero.
+// FD flux-Jacobian blocks per edge at the frozen face states, one 256-byte
+// record per edge: blk[32 e + r*4+c] = dF_r/dU_L,c and
+// blk[32 e + 16 + r*4+c] = dF_r/dU_R,c, so each side's block is one
+// 128-byte line.  A boundary edge rebuilds its ghost from each perturbed
+// left state; its right-state block is never read by the scatter and is
+// stored as zero.
 __global__ __launch_bounds__(256) void k_jt_blocks(Topo T, Geom G, const double *__restrict__ fL,
                                                    const double *__restrict__ fR, double eps, Phys P,
                                                    double *__restrict__ blk) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= T.ne) return;
-    const int ne = T.ne;
+    double *rec = blk + 32 * (size_t)e;
     const char bc = T.ebc[e];
     const double enx = G.enx[e], eny = G.eny[e];
     double bL[4], bR[4];
@@ -894,12 +896,12 @@ __global__ __launch_bounds__(256) void k_jt_blocks(Topo T, Geom G, const double 
         hllc(Lp, Rp, enx, eny, P.gamma, Fp);
         hllc(Lm, Rm, enx, eny, P.gamma, Fm);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) blk[(r * 4 + c) * ne + e] = (Fp[r] - Fm[r]) / (2 * eps);
+        for (int r = 0; r < 4; ++r) rec[r * 4 + c] = (Fp[r] - Fm[r]) / (2 * eps);
     }
     for (int c = 0; c < 4; ++c) {
         if (bc != '\0') {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) blk[(16 + r * 4 + c) * ne + e] = 0.0;
+            for (int r = 0; r < 4; ++r) rec[16 + r * 4 + c] = 0.0;
             continue;
         }
         double Rp[4], Rm[4], Fp[4], Fm[4];
@@ -910,47 +912,64 @@ __global__ __launch_bounds__(256) void k_jt_blocks(Topo T, Geom G, const double 
         hllc(bL, Rp, enx, eny, P.gamma, Fp);
         hllc(bL, Rm, enx, eny, P.gamma, Fm);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) blk[(16 + r * 4 + c) * ne + e] = (Fp[r] - Fm[r]) / (2 * eps);
+        for (int r = 0; r < 4; ++r) rec[16 + r * 4 + c] = (Fp[r] - Fm[r]) / (2 * eps);
     }
 }
 
 // y = J^T x as a per-cell gather in ascending edge id: the serialisation of
-// the reference's atomic scatter in edge order, without atomics.
+// the reference's atomic scatter in edge order, without atomics.  Four
+// lanes per cell: lane k forms edge slot k's contribution F_sideᵀ Delta
+// (its side's block read as one 128-byte record), the in-quad transpose
+// gives lane q variable q of every slot, and lane q sums them in slot order
+// (+0.0 from a slot without a contribution: the sum starts at +0.0, so it
+// is never -0.0 and adding +0.0 changes nothing).
 __global__ __launch_bounds__(256) void k_jt_apply(Topo T, Geom G, const double *__restrict__ blk,
                                                   const double *__restrict__ x, double *__restrict__ y) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= T.nc) return;
-    const int nc = T.nc, ne = T.ne;
-    const int nk = T.cinfo[c] >> 4;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < nk; ++k) {
-        const int e = T.cedge[4 * c + k];
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = gid / kLanesPerCell, k = gid % kLanesPerCell;
+    if (c >= T.nc) return;  // whole lane groups only
+    const int nc = T.nc;
+    const unsigned info = T.cinfo[c];
+    const int e = T.cedge[4 * c + k];  // -1 past the cell's edges
+    double g[4] = {0.0, 0.0, 0.0, 0.0};
+    if (e >= 0) {
+        const bool left = (info >> k) & 1u;  // c is the edge's left cell
         const double L = G.elen[e];
         const int iL = T.ecl[e];
+        const char bc = T.ebc[e];
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        const v2d *rp = reinterpret_cast<const v2d *>(blk + 32 * (size_t)e + (left ? 0 : 16));
+        double F[16];
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            const v2d t = rp[h];
+            F[2 * h] = t.x;
+            F[2 * h + 1] = t.y;
+        }
         const double invV_L = 1.0 / G.cvol[iL];
         double D[4];
-        int off;
-        if (T.ebc[e] != '\0') {
-            if (c != iL) continue;
+        bool use = true;
+        if (bc != '\0') {
+            use = left;
 #pragma unroll
             for (int q = 0; q < 4; ++q) D[q] = (-x[q * nc + iL] * invV_L) * L;
-            off = 0;
         } else {
             const int iR = T.ecr[e];
             const double invV_R = 1.0 / G.cvol[iR];
 #pragma unroll
             for (int q = 0; q < 4; ++q) D[q] = (-x[q * nc + iL] * invV_L + x[q * nc + iR] * invV_R) * L;
-            off = (c == iL) ? 0 : 16;
         }
+        if (use) {
 #pragma unroll
-        for (int kk = 0; kk < 4; ++kk) {
-            const double *F = blk + (size_t)(off + kk) * ne + e;  // column kk: rows r at stride 4*ne
-            acc[kk] += (F[0] * D[0] + F[(size_t)4 * ne] * D[1] + F[(size_t)8 * ne] * D[2] +
-                        F[(size_t)12 * ne] * D[3]);
+            for (int kk = 0; kk < 4; ++kk)  // column kk of the block: rows r at stride 4
+                g[kk] = (F[kk] * D[0] + F[4 + kk] * D[1] + F[8 + kk] * D[2] + F[12 + kk] * D[3]);
         }
     }
+    quad_transpose(g, k);
+    double acc = 0.0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) y[q * nc + c] = acc[q];
+    for (int j = 0; j < 4; ++j) acc += g[j];
+    y[k * nc + c] = acc;
 }
 
 // Standalone HLLC on a batch of faces (known-answer surface).
@@ -1200,7 +1219,7 @@ hipError_t launch_jt_blocks(const Topo &T, const Geom &G, const double *fL, cons
 }
 hipError_t launch_jt_apply(const Topo &T, const Geom &G, const double *blk, const double *x, double *y,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_jt_apply, dim3(nblk(T.nc)), dim3(256), 0, s, T, G, blk, x, y);
+    hipLaunchKernelGGL(k_jt_apply, dim3(nblk((long)kLanesPerCell * T.nc)), dim3(256), 0, s, T, G, blk, x, y);
     return hipGetLastError();
 }
 hipError_t launch_hllc_batch(const double *qL, const double *qR, const double *nrm, double gamma, double *F,
