@@ -785,42 +785,50 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
         *iters = 0;
         return O2F_OK;
     }
-    double r0_sqnorm = r0_sq;
-    // rho = alpha = w = 1; v = p = 0
+    // rho = alpha = w = 1; v = p = 0; r0_sqnorm = r0.r0; no iterations yet
     const double init[4] = {1.0, 1.0, 1.0, 1.0};
     HIPCHK(c, hipMemcpyAsync(c->scal + kScalRho, init, 4 * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(c, launch_fill(n, v, 0.0, s));
     HIPCHK(c, launch_fill(n, p, 0.0, s));
-    const double tol2 = tol * tol * rhs_sq;
-    const double eps2 = DBL_EPSILON * DBL_EPSILON;
-    int i = 0, restarts = 0;
+    if ((st = set_scal(c, kScalR0Sq, r0_sq)) || (st = set_scal(c, kScalIter, 0.0))) return st;
+    // The loop test (r.r > tol2 && i < maxit, then the restart test) runs on
+    // the device, in the last reduction of each iteration: it sets the flag
+    // scal[kScalGo] that every vector update and reduction of the body
+    // checks.  The host enqueues kGroup iterations and reads the flag once
+    // per group; iterations enqueued after a stop change nothing (only
+    // their matrix-vector products run).  A restart (flag 2) is done here.
+    const BicgTest test{tol * tol * rhs_sq, (double)maxit, DBL_EPSILON * DBL_EPSILON};
+    const double *go = c->scal + kScalGo;
+    constexpr int kGroup = 8;
+    int restarts = 0;
     // rho_old = rho; rho = r0.r, together with r.r for the loop test
-    HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s));
+    HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s, nullptr, test));
     if ((st = read_scal(c, h))) return st;
-    while (h.v[kScalRR] > tol2 && i < maxit) {
-        if (std::fabs(h.v[kScalRho]) < eps2 * r0_sqnorm) {
+    while (h.v[kScalGo] != 0.0) {
+        if (h.v[kScalGo] == 2.0) {
             // r became too orthogonal to r0: restart with r0 = r = b - A x
             HIPCHK(c, mv(x, tmp));
             HIPCHK(c, launch_sub(n, b, tmp, r, s));
             HIPCHK(c, launch_sub(n, r, nullptr, r0, s));
             HIPCHK(c, launch_dot2(r, r, nullptr, nullptr, n, c->partials, c->scal, kOpRestart, s));
-            if ((st = read_scal(c, h))) return st;
-            r0_sqnorm = h.v[kScalR0Sq];
-            if (restarts++ == 0) i = 0;
+            if (restarts++ == 0 && (st = set_scal(c, kScalIter, 0.0))) return st;
+            if ((st = set_scal(c, kScalGo, 1.0))) return st;
         }
-        HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
-        if (pc) HIPCHK(c, pc(p, py));
-        HIPCHK(c, mv(pc ? py : p, v));
-        HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->scal, kOpAlpha, s));
-        HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, s));
-        if (pc) HIPCHK(c, pc(sv, sz));
-        HIPCHK(c, mv(pc ? sz : sv, t));
-        HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->scal, kOpOmega, s));
-        HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s, pc ? py : nullptr, pc ? sz : nullptr));
-        ++i;
-        HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s));
+        for (int g = 0; g < kGroup; ++g) {
+            HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
+            if (pc) HIPCHK(c, pc(p, py));
+            HIPCHK(c, mv(pc ? py : p, v));
+            HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->scal, kOpAlpha, s, go));
+            HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, s));
+            if (pc) HIPCHK(c, pc(sv, sz));
+            HIPCHK(c, mv(pc ? sz : sv, t));
+            HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->scal, kOpOmega, s, go));
+            HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s, pc ? py : nullptr, pc ? sz : nullptr));
+            HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRhoLoop, s, go, test));
+        }
         if ((st = read_scal(c, h))) return st;
     }
+    const int i = (int)h.v[kScalIter];
     *iters = i;
     return O2F_OK;
 }

@@ -600,7 +600,9 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__rest
                                                            const double *__restrict__ b,
                                                            const double *__restrict__ c,
                                                            const double *__restrict__ d, long n,
-                                                           double *__restrict__ partials) {
+                                                           double *__restrict__ partials,
+                                                           const double *__restrict__ gate) {
+    if (gate && *gate != 1.0) return;  // a stopped BiCGSTAB loop
     __shared__ double s1[kRedBlock], s2[kRedBlock];
     double x = 0.0, y = 0.0;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -624,7 +626,9 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__rest
 }
 
 __global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restrict__ partials, int nparts,
-                                                         double *__restrict__ scal, int op) {
+                                                         double *__restrict__ scal, int op,
+                                                         const double *__restrict__ gate, BicgTest test) {
+    if (gate && *gate != 1.0) return;  // a stopped BiCGSTAB loop
     __shared__ double s1[kRedBlock], s2[kRedBlock];
     double x = 0.0, y = 0.0;
     for (int bb = threadIdx.x; bb < nparts; bb += blockDim.x) {
@@ -648,11 +652,18 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restri
             scal[kScalDotA] = ab;
             scal[kScalDotB] = cd;
             break;
-        case kOpRho:  // (r0.r, r.r): rho_old = rho; rho = r0.r
+        case kOpRho:  // (r0.r, r.r): rho_old = rho; rho = r0.r; then the loop test
+        case kOpRhoLoop: {
             scal[kScalRhoOld] = scal[kScalRho];
             scal[kScalRho] = ab;
             scal[kScalRR] = cd;
+            if (op == kOpRhoLoop) scal[kScalIter] += 1.0;
+            // while (r.r > tol2 && i < maxit), then the restart test of the
+            // loop body, in the host loop's order
+            const bool more = cd > test.tol2 && scal[kScalIter] < test.maxit;
+            scal[kScalGo] = !more ? 0.0 : (fabs(ab) < test.eps2 * scal[kScalR0Sq]) ? 2.0 : 1.0;
             break;
+        }
         case kOpRestart:  // (r.r): rho = r0_sqnorm = r.r
             scal[kScalRho] = ab;
             scal[kScalR0Sq] = ab;
@@ -667,11 +678,12 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restri
     }
 }
 
-// BiCGSTAB vector updates (Eigen's internal::bicgstab recurrence).
+// BiCGSTAB vector updates (Eigen's internal::bicgstab recurrence); each
+// runs only while the device loop flag scal[kScalGo] is 1.
 __global__ __launch_bounds__(256) void k_bicg_p(long n, const double *__restrict__ r, double *__restrict__ p,
                                                 const double *__restrict__ v, const double *__restrict__ scal) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || scal[kScalGo] != 1.0) return;
     const double beta = (scal[kScalRho] / scal[kScalRhoOld]) * (scal[kScalAlpha] / scal[kScalW]);
     const double w = scal[kScalW];
     p[i] = r[i] + beta * (p[i] - w * v[i]);
@@ -680,7 +692,7 @@ __global__ __launch_bounds__(256) void k_bicg_p(long n, const double *__restrict
 __global__ __launch_bounds__(256) void k_bicg_s(long n, const double *__restrict__ r, const double *__restrict__ v,
                                                 double *__restrict__ s, const double *__restrict__ scal) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || scal[kScalGo] != 1.0) return;
     s[i] = r[i] - scal[kScalAlpha] * v[i];
 }
 
@@ -691,7 +703,7 @@ __global__ __launch_bounds__(256) void k_bicg_xr(long n, double *__restrict__ x,
                                                  const double *__restrict__ s, const double *__restrict__ t,
                                                  const double *__restrict__ scal) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || scal[kScalGo] != 1.0) return;
     const double alpha = scal[kScalAlpha], w = scal[kScalW];
     x[i] += alpha * px[i] + w * sx[i];
     r[i] = s[i] - w * t[i];
@@ -1153,12 +1165,13 @@ hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double 
     return hipGetLastError();
 }
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
-                       double *partials, double *scal, int op, hipStream_t s) {
+                       double *partials, double *scal, int op, hipStream_t s, const double *gate,
+                       const BicgTest &test) {
     const unsigned g = (unsigned)std::min<long>(kRedGrid, std::max<long>(1, (long)nblk(n, kRedBlock)));
-    hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kRedBlock), 0, s, a, b, c, d, n, partials);
+    hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kRedBlock), 0, s, a, b, c, d, n, partials, gate);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedBlock), 0, s, partials, (int)g, scal, op);
+    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedBlock), 0, s, partials, (int)g, scal, op, gate, test);
     return hipGetLastError();
 }
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s) {

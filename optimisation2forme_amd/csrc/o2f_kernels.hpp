@@ -41,11 +41,21 @@ enum : int {
     kScalTPrev = 12,   // t at the start of the last step
     kScalConv = 13,    // set once the density residual met the tolerance: the next step stops
     kScalRes = 14,     // last density residual norm evaluated on the device
-    kScalCount = 16,
+    kScalGo = 15,      // BiCGSTAB loop: 1 run the next iteration, 0 stop, 2 restart first (host)
+    kScalIter = 16,    // BiCGSTAB iterations taken in the current chunk
+    kScalCount = 20,
 };
 
-// Epilogues of k_dot2.
-enum : int { kOpStore = 0, kOpRho = 1, kOpRestart = 2, kOpAlpha = 3, kOpOmega = 4 };
+// Epilogues of k_dot2.  kOpRho / kOpRhoLoop also evaluate the BiCGSTAB loop
+// test into kScalGo (kOpRhoLoop after counting the iteration that ended).
+enum : int { kOpStore = 0, kOpRho = 1, kOpRestart = 2, kOpAlpha = 3, kOpOmega = 4, kOpRhoLoop = 5 };
+
+// The loop test of a BiCGSTAB chunk, evaluated on the device.
+struct BicgTest {
+    double tol2;   // stop once r.r <= tol2
+    double maxit;  // or once this many iterations ran
+    double eps2;   // restart when |rho| < eps2 * r0_sqnorm
+};
 
 // Constants every kernel needs (host-computed with the reference's
 // expression order, see Context::make_phys).
@@ -209,7 +219,8 @@ hipError_t launch_step_gather(const Topo &T, const Geom &G, const double *U, dou
                               const double *parts, double min_h, double *scal, double *hist, int advance_t,
                               hipStream_t s);
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
-                       double *partials, double *scal, int op, hipStream_t s);
+                       double *partials, double *scal, int op, hipStream_t s, const double *gate = nullptr,
+                       const BicgTest &test = BicgTest{});
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s);
 hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s);
 // x += alpha (px ? px : p) + w (sx ? sx : sv); r = sv - w t
