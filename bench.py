@@ -276,12 +276,19 @@ def bench_adjoint(a):
     n = max(2, a.steps)
     ref_ms = s.time_adjoint(n, False) / n
     exact_ms = s.time_adjoint(n, True) / n
+    # solveAdjointSystem's BiCGSTAB on the reference-form operator, 300
+    # iterations under a tolerance it never meets (wall clock, host included)
+    s.bicgstab(50.0, tol=1e-300, chunk=20, max_total=20, check=False)
+    t0 = time.perf_counter()
+    _, bicg_it, _, _ = s.bicgstab(50.0, tol=1e-300, chunk=300, max_total=300, check=False)
+    bicg_us = 1e6 * (time.perf_counter() - t0) / max(1, bicg_it)
     B = 72 * s.nc + 96 * s.ne          # per product: blocks / state / result, the survey's figure
     out = {"metric": "adjoint J^T x products", "unit": "ms per product", "higher_is_better": False,
            "n_gpus": 1, "dtype": "f64", "steps": n,
            "config": {"workload": "naca2412_transonic_200k_triangles", "cells": s.nc, "edges": s.ne},
            "reference_form_ms": ref_ms, "exact_ms": exact_ms,
-           "reference_form_GBs": B / (ref_ms * 1e-3) / 1e9, "algorithmic_bytes_per_product": B}
+           "reference_form_GBs": B / (ref_ms * 1e-3) / 1e9, "algorithmic_bytes_per_product": B,
+           "bicgstab_us_per_iteration": bicg_us, "bicgstab_iterations_timed": bicg_it}
     print(json.dumps(out), flush=True)
 
 
