@@ -823,8 +823,8 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
             if (pc) HIPCHK(c, pc(sv, sz));
             HIPCHK(c, mv(pc ? sz : sv, t));
             HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->scal, kOpOmega, s, go));
-            HIPCHK(c, launch_bicg_xr(n, x, r, p, sv, t, c->scal, s, pc ? py : nullptr, pc ? sz : nullptr));
-            HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRhoLoop, s, go, test));
+            HIPCHK(c, launch_bicg_xr_dot(n, x, r, pc ? py : p, pc ? sz : sv, sv, t, r0, c->partials, c->scal, test,
+                                         s));
         }
         if ((st = read_scal(c, h))) return st;
     }

@@ -596,19 +596,12 @@ __global__ __launch_bounds__(256) void k_step_gather(Topo T, Geom G, const doubl
 // partials in a fixed tree, then one block combines them in block order and
 // applies `op` (two launches: the kernel boundary orders the partials, no
 // device-wide fence per block).
-__global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__restrict__ a,
-                                                           const double *__restrict__ b,
-                                                           const double *__restrict__ c,
-                                                           const double *__restrict__ d, long n,
-                                                           double *__restrict__ partials,
-                                                           const double *__restrict__ gate) {
-    if (gate && *gate != 1.0) return;  // a stopped BiCGSTAB loop
+// The block's two partial sums of a reduction (each thread's running sums x,
+// y, then a fixed tree) into partials[2 blockIdx.x + 0/1].  Every reduction
+// of the adjoint solve uses this one tree, so fused kernels reproduce the
+// plain reduction's partials bit for bit.
+__device__ __forceinline__ void block_partials2(double x, double y, double *__restrict__ partials) {
     __shared__ double s1[kRedBlock], s2[kRedBlock];
-    double x = 0.0, y = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        x += a[i] * b[i];
-        if (c) y += c[i] * d[i];
-    }
     s1[threadIdx.x] = x;
     s2[threadIdx.x] = y;
     __syncthreads();
@@ -623,6 +616,46 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__rest
         partials[2 * blockIdx.x] = s1[0];
         partials[2 * blockIdx.x + 1] = s2[0];
     }
+}
+
+__global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__restrict__ a,
+                                                           const double *__restrict__ b,
+                                                           const double *__restrict__ c,
+                                                           const double *__restrict__ d, long n,
+                                                           double *__restrict__ partials,
+                                                           const double *__restrict__ gate) {
+    if (gate && *gate != 1.0) return;  // a stopped BiCGSTAB loop
+    double x = 0.0, y = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        x += a[i] * b[i];
+        if (c) y += c[i] * d[i];
+    }
+    block_partials2(x, y, partials);
+}
+
+// The last update of a BiCGSTAB iteration fused with the reduction after
+// it: x += alpha px + w sx; r = s - w t; partials of (r0.r, r.r), with the
+// elements of k_dot2_partial's grid-stride loop in its order (the same
+// grid), so the partials are those of k_dot2_partial(r0, r, r, r).
+__global__ __launch_bounds__(kRedBlock) void k_bicg_xr_dot(long n, double *__restrict__ x, double *__restrict__ r,
+                                                          const double *__restrict__ px,
+                                                          const double *__restrict__ sx,
+                                                          const double *__restrict__ s,
+                                                          const double *__restrict__ t,
+                                                          const double *__restrict__ r0,
+                                                          const double *__restrict__ scal,
+                                                          double *__restrict__ partials) {
+    if (scal[kScalGo] != 1.0) return;  // a stopped BiCGSTAB loop
+    const double alpha = scal[kScalAlpha], w = scal[kScalW];
+    double a = 0.0, b = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        x[i] += alpha * px[i] + w * sx[i];
+        const double ri = s[i] - w * t[i];
+        r[i] = ri;
+        a += r0[i] * ri;
+        b += ri * ri;
+    }
+    block_partials2(a, b, partials);
 }
 
 __global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restrict__ partials, int nparts,
@@ -694,19 +727,6 @@ __global__ __launch_bounds__(256) void k_bicg_s(long n, const double *__restrict
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (i >= n || scal[kScalGo] != 1.0) return;
     s[i] = r[i] - scal[kScalAlpha] * v[i];
-}
-
-// x += alpha px + w sx; r = s - w t (px = p, sx = s without a
-// preconditioner; the preconditioned directions with one)
-__global__ __launch_bounds__(256) void k_bicg_xr(long n, double *__restrict__ x, double *__restrict__ r,
-                                                 const double *__restrict__ px, const double *__restrict__ sx,
-                                                 const double *__restrict__ s, const double *__restrict__ t,
-                                                 const double *__restrict__ scal) {
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n || scal[kScalGo] != 1.0) return;
-    const double alpha = scal[kScalAlpha], w = scal[kScalW];
-    x[i] += alpha * px[i] + w * sx[i];
-    r[i] = s[i] - w * t[i];
 }
 
 // Block-Jacobi preconditioner of the adjoint solves.  Probing: x = 1 on
@@ -1164,10 +1184,25 @@ hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double 
                        advance_t);
     return hipGetLastError();
 }
+// Blocks of a reduction over n elements (fixed, so the order is fixed).
+static unsigned red_grid(long n) {
+    return (unsigned)std::min<long>(kRedGrid, std::max<long>(1, (long)nblk(n, kRedBlock)));
+}
+hipError_t launch_bicg_xr_dot(long n, double *x, double *r, const double *px, const double *sx, const double *sv,
+                              const double *t, const double *r0, double *partials, double *scal,
+                              const BicgTest &test, hipStream_t s) {
+    const unsigned g = red_grid(n);
+    hipLaunchKernelGGL(k_bicg_xr_dot, dim3(g), dim3(kRedBlock), 0, s, n, x, r, px, sx, sv, t, r0, scal, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedBlock), 0, s, partials, (int)g, scal, kOpRhoLoop,
+                       scal + kScalGo, test);
+    return hipGetLastError();
+}
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
                        double *partials, double *scal, int op, hipStream_t s, const double *gate,
                        const BicgTest &test) {
-    const unsigned g = (unsigned)std::min<long>(kRedGrid, std::max<long>(1, (long)nblk(n, kRedBlock)));
+    const unsigned g = red_grid(n);
     hipLaunchKernelGGL(k_dot2_partial, dim3(g), dim3(kRedBlock), 0, s, a, b, c, d, n, partials, gate);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1180,11 +1215,6 @@ hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, co
 }
 hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s) {
     hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n)), dim3(256), 0, s, n, r, v, sv, scal);
-    return hipGetLastError();
-}
-hipError_t launch_bicg_xr(long n, double *x, double *r, const double *p, const double *sv, const double *t,
-                          const double *scal, hipStream_t s, const double *px, const double *sx) {
-    hipLaunchKernelGGL(k_bicg_xr, dim3(nblk(n)), dim3(256), 0, s, n, x, r, px ? px : p, sx ? sx : sv, sv, t, scal);
     return hipGetLastError();
 }
 hipError_t launch_probe_seed(int nc, const int *color, int k, int q, double *x, hipStream_t s) {
