@@ -573,29 +573,26 @@ __global__ __launch_bounds__(256) void k_step_begin(const double *__restrict__ p
     step_begin_body(parts, nparts, P, min_h, scal, hist, advance_t);
 }
 
-// Start of a step fused with the first stage's vertex gather: the last
-// workgroup runs the step-begin reduction (dt is first read by the stage's
-// update kernel, two launches later), the others gather vertex states.  A
-// gather block that still sees the gate open while the last block closes it
-// only writes vertex values nothing reads once the gate is shut.
+// Start of a step fused with the first stage's vertex gather: workgroup 0
+// runs the step-begin reduction (dt is first read by the stage's update
+// kernel, two launches later), concurrently with the gathers of the others
+// (dispatched first, it does not extend the launch's tail).  A gather block
+// that still sees the gate open while workgroup 0 closes it only writes
+// vertex values nothing reads once the gate is shut.
 __global__ __launch_bounds__(256) void k_step_gather(Topo T, Geom G, const double *__restrict__ U,
                                                      double *__restrict__ vU, Phys P,
                                                      const double *__restrict__ parts, int nparts, double min_h,
                                                      double *__restrict__ scal, double *__restrict__ hist,
                                                      int advance_t) {
-    if (blockIdx.x == gridDim.x - 1) {
+    if (blockIdx.x == 0) {
         step_begin_body(parts, nparts, P, min_h, scal, hist, advance_t);
         return;
     }
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = (blockIdx.x - 1) * blockDim.x + threadIdx.x;
     if (v >= T.nv || (P.gate && *P.gate == 0.0)) return;
     gather_vertex<4, false>(T, G, U, vU, P, v);
 }
 
-// Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i): per-block
-// partials in a fixed tree, then one block combines them in block order and
-// applies `op` (two launches: the kernel boundary orders the partials, no
-// device-wide fence per block).
 // The block's two partial sums of a reduction (each thread's running sums x,
 // y, then a fixed tree) into partials[2 blockIdx.x + 0/1].  Every reduction
 // of the adjoint solve uses this one tree, so fused kernels reproduce the
@@ -618,6 +615,10 @@ __device__ __forceinline__ void block_partials2(double x, double y, double *__re
     }
 }
 
+// Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i): per-block
+// partials in a fixed tree, then one block combines them in block order and
+// applies `op` (two launches: the kernel boundary orders the partials, no
+// device-wide fence per block).
 __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__restrict__ a,
                                                            const double *__restrict__ b,
                                                            const double *__restrict__ c,
