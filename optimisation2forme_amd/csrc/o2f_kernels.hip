@@ -425,6 +425,9 @@ __device__ __forceinline__ void flux_update_body(int blk, const Topo &T, const G
             }
         }
         const bool is_left = (info >> k) & 1u;
+        // one division for three quotients: lanes 0 / 1 / 2 form u, v, 1 / V
+        const double qv = ((k == 0) ? u[1] : (k == 1) ? u[2] : 1.0) / ((k < 2) ? u[0] : vol);
+        const double uu = from_lane<0>(qv), vv = from_lane<1>(qv), invV = from_lane<2>(qv);
         // convective term of this edge, src/advection.cpp:453-456
         double adv[4] = {0.0, 0.0, 0.0, 0.0};
         if (has_edge) {
@@ -435,8 +438,6 @@ __device__ __forceinline__ void flux_update_body(int blk, const Topo &T, const G
         // viscous terms of this edge, src/diffusion.cpp:183-200
         double w1 = 0.0, w2 = 0.0, w3 = 0.0;
         if ((flags & kFlagVisc) && has_edge) {
-            const double uu = u[1] / u[0];
-            const double vv = u[2] / u[0];
             double nx = enx, ny = eny;
             if (is_left) {
                 nx = -nx;
@@ -465,7 +466,6 @@ __device__ __forceinline__ void flux_update_body(int blk, const Topo &T, const G
             rk = 0.0;
 #pragma unroll
             for (int j = 0; j < 4; ++j) rk += adv[j];
-            const double invV = 1.0 / vol;
             rk = rk * -invV;
         } else {
             rk = R[k * nc + c];
