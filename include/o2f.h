@@ -408,6 +408,16 @@ int o2f_time_steps(o2f_ctx *ctx, int nsteps, double *total_ms, double *kernel_ms
  * events: exact = 0 the reference-form operator (FD blocks, built once
  * before timing), 1 the exact adjoint.  Returns the elapsed milliseconds. */
 int o2f_time_adjoint(o2f_ctx *ctx, int nprod, int exact, double *total_ms);
+/* `launches` back-to-back launches of one per-stage kernel at the current
+ * state between a single pair of HIP events (no event between launches, so
+ * the figure is the kernel's own duration plus the dispatch gap, as a
+ * kernel trace reports it): kernel 0 the vertex gather, 1 the cell
+ * reconstruction, 2 the vertex viscous gather + edge flux, 3 the flux sum
+ * and viscous divergence in residual form (no stage update).  Every launch
+ * recomputes the same outputs from the same inputs.  *avg_us receives the
+ * microseconds per launch.  A measurement entry point of this library (no
+ * counterpart in the reference). */
+int o2f_time_kernel(o2f_ctx *ctx, int kernel, int launches, double *avg_us);
 
 #ifdef __cplusplus
 }

@@ -39,7 +39,7 @@ EXPORTS = [
     "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps", "o2f_apply_jt_exact",
     "o2f_bicgstab_exact", "o2f_mlp_create", "o2f_mlp_destroy", "o2f_mlp_last_error", "o2f_mlp_num_params",
     "o2f_mlp_get", "o2f_mlp_set", "o2f_mlp_set_data", "o2f_mlp_train", "o2f_mlp_forward",
-    "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_naca_displacement",
+    "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_time_kernel", "o2f_naca_displacement",
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
 ]
@@ -127,6 +127,7 @@ def lib():
             "o2f_run_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
             "o2f_apply_jt_exact": (C.c_int, [vp, D, D]),
             "o2f_time_adjoint": (C.c_int, [vp, C.c_int, C.c_int, D]),
+            "o2f_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, D]),
             "o2f_mlp_create": (C.c_int, [C.c_ulonglong, C.POINTER(vp)]),
             "o2f_mlp_destroy": (None, [vp]),
             "o2f_mlp_last_error": (C.c_char_p, [vp]),
@@ -485,6 +486,14 @@ class Solver:
         ms = C.c_double()
         self._check(lib().o2f_time_adjoint(self.h, nprod, int(exact), C.byref(ms)))
         return ms.value
+
+    def time_kernel(self, kernel, launches):
+        """Microseconds per launch of one stage kernel (0 vertex gather,
+        1 reconstruction, 2 viscous gather + edge flux, 3 flux sum in
+        residual form), launched back to back at the current state."""
+        us = C.c_double()
+        self._check(lib().o2f_time_kernel(self.h, int(kernel), int(launches), C.byref(us)))
+        return us.value
 
     def time_steps(self, n, per_kernel=False):
         tot = C.c_double()

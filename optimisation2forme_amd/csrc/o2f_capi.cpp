@@ -1397,6 +1397,42 @@ int o2f_time_adjoint(o2f_ctx *c, int nprod, int exact, double *total_ms) {
     return O2F_OK;
 }
 
+int o2f_time_kernel(o2f_ctx *c, int kernel, int launches, double *avg_us) {
+    if (!c || kernel < 0 || kernel > 3 || launches <= 0 || !avg_us) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    // every intermediate the kernels read, at the current state
+    if ((st = residual_on(c, 0))) return st;
+    hipStream_t s = c->stream;
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    auto launch = [&]() -> hipError_t {
+        switch (kernel) {
+            case 0: return launch_vertex_state(T, G, c->U, c->vU, c->phys, s);
+            case 1: return launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, nullptr, c->phys, s);
+            case 2: return launch_visc_and_flux(T, G, c->tq, c->vtq, c->fL, c->fR, c->Fe, c->phys, 1, s);
+            default:
+                return launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, nullptr, 0.0, 0.0,
+                                          kFlagAdv | kFlagVisc | kFlagStoreR, c->phys, s);
+        }
+    };
+    HIPCHK(c, launch());  // warm-up
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    HIPCHK(c, hipEventRecord(e0, s));
+    for (int k = 0; k < launches; ++k) HIPCHK(c, launch());
+    HIPCHK(c, hipEventRecord(e1, s));
+    HIPCHK(c, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_us = 1e3 * ms / launches;
+    ++c->faces_version;
+    return O2F_OK;
+}
+
 int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) {
     if (!c || nsteps <= 0 || !total_ms) return O2F_ERR_ARG;
     int st = check_bc(c);
