@@ -290,22 +290,26 @@ __device__ __forceinline__ void reconstruct_body(int blk, const Topo &T, const G
     // derivative (u_x, v_y, u_y, v_x) and lanes 0 / 1 one temperature
     // derivative (T_x, T_y); every value is the same expression as in one
     // lane, so the results are unchanged
-    const double gx[4] = {from_lane<0>(gxk), from_lane<1>(gxk), from_lane<2>(gxk), from_lane<3>(gxk)};
-    const double gy[4] = {from_lane<0>(gyk), from_lane<1>(gyk), from_lane<2>(gyk), from_lane<3>(gyk)};
+    // Each lane reads only the gradient components it needs: the density
+    // gradient (lane 0), the energy gradient (lane 3) and, through one
+    // quad_perm [1, 2, 1, 2], (d rho u / dx, d rho v / dx) or
+    // (d rho u / dy, d rho v / dy).
+    const double gx0 = from_lane<0>(gxk), gy0 = from_lane<0>(gyk);
+    const double gx3 = from_lane<3>(gxk), gy3 = from_lane<3>(gyk);
+    const double px = quad_perm<0x99>(gxk), py = quad_perm<0x99>(gyk);
     const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
     const double rho = u[0], E = u[3];
     // one division for four quotients: lanes 0 / 1 the velocities u, v,
     // lanes 2 / 3 the heat-flux terms (E_x rho - E rho_x) / rho^2 in x, y
     const double qn = (k == 0)   ? u[1]
                       : (k == 1) ? u[2]
-                      : (k == 2) ? (gx[3] * rho - E * gx[0])
-                                 : (gy[3] * rho - E * gy[0]);
+                      : (k == 2) ? (gx3 * rho - E * gx0)
+                                 : (gy3 * rho - E * gy0);
     const double qv = qn / ((k < 2) ? rho : rho * rho);
     const double uu = from_lane<0>(qv), vvel = from_lane<1>(qv), ex = from_lane<2>(qv), ey = from_lane<3>(qv);
-    const double num = (k == 0) ? (gx[1] - uu * gx[0])
-                       : (k == 1) ? (gy[2] - vvel * gy[0])
-                       : (k == 2) ? (gy[1] - uu * gy[0])
-                                  : (gx[2] - vvel * gx[0]);
+    // lane 0: u_x, 1: v_y, 2: u_y, 3: v_x
+    const bool xk = (k == 0 || k == 3);
+    const double num = (xk ? px : py) - ((k & 1) ? vvel : uu) * (xk ? gx0 : gy0);
     const double d = num / rho;
     const double u_x = from_lane<0>(d), v_y = from_lane<1>(d), u_y = from_lane<2>(d), v_x = from_lane<3>(d);
     const double tk = (k & 1) ? (uu * u_y + vvel * v_y) : (uu * u_x + vvel * v_x);
