@@ -90,19 +90,27 @@ __global__ __launch_bounds__(256) void ka_edge(Topo T, Geom G, Phys P, const dou
     double qL[4], qR[4], bL[4], bR[4];
     ld4(fL, e, qL);
     ld4(fR, e, qR);
-    for (int j = 0; j < 8; ++j) {
-        Dual dl[4], dr[4], F[4];
+    // Fb^T dF/dq, two directions per HLLC evaluation: pass j takes
+    // components 2 (j % 2), 2 (j % 2) + 1 of the left (j < 2) or right state
+    double bq[8];
+#pragma unroll 1
+    for (int j = 0; j < 4; ++j) {
+        DualN<2> dl[4], dr[4], F[4];
+        const int side = j >> 1, q0 = 2 * (j & 1);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            dl[q] = Dual{qL[q], j == q ? 1.0 : 0.0};
-            dr[q] = Dual{qR[q], j == q + 4 ? 1.0 : 0.0};
+            dl[q] = DualN<2>::seed(qL[q], side == 0 ? q - q0 : -1);
+            dr[q] = DualN<2>::seed(qR[q], side == 1 ? q - q0 : -1);
         }
-        hllc_t<Dual>(dl, dr, enx, eny, P.gamma, F);
-        const double s = Fb[0] * F[0].d + Fb[1] * F[1].d + Fb[2] * F[2].d + Fb[3] * F[3].d;
-        if (j < 4)
-            bL[j] = s;
-        else
-            bR[j - 4] = s;
+        hllc_t<DualN<2>>(dl, dr, enx, eny, P.gamma, F);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+            bq[2 * j + i] = Fb[0] * F[0].d[i] + Fb[1] * F[1].d[i] + Fb[2] * F[2].d[i] + Fb[3] * F[3].d[i];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        bL[q] = bq[q];
+        bR[q] = bq[4 + q];
     }
     if (cr < 0) {
         // ghost R state = function of the L state (src/boundary.cpp:48-80)

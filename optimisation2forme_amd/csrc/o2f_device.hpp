@@ -51,6 +51,98 @@ __device__ __forceinline__ Dual sqrt(Dual a) {
     return {r, a.d / (2.0 * r)};
 }
 
+// Several tangent directions carried together: each component follows
+// exactly the arithmetic of a Dual (same operations in the same order), so
+// DualN<N> gives the N tangents of N separate Dual evaluations bit for
+// bit, with the value computed once.
+template <int N>
+struct DualN {
+    double v, d[N];
+    // value a, tangent 1 in direction `dir` (no direction: dir < 0)
+    __device__ static DualN seed(double a, int dir) {
+        DualN r;
+        r.v = a;
+#pragma unroll
+        for (int i = 0; i < N; ++i) r.d[i] = (i == dir) ? 1.0 : 0.0;
+        return r;
+    }
+};
+#define O2F_DUALN_OP(EXPR_V, EXPR_D)          \
+    DualN<N> r;                               \
+    r.v = EXPR_V;                             \
+    _Pragma("unroll") for (int i = 0; i < N; ++i) r.d[i] = EXPR_D; \
+    return r;
+template <int N>
+__device__ __forceinline__ DualN<N> operator+(const DualN<N> &a, const DualN<N> &b) {
+    O2F_DUALN_OP(a.v + b.v, a.d[i] + b.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator-(const DualN<N> &a, const DualN<N> &b) {
+    O2F_DUALN_OP(a.v - b.v, a.d[i] - b.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator-(const DualN<N> &a) {
+    O2F_DUALN_OP(-a.v, -a.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator*(const DualN<N> &a, const DualN<N> &b) {
+    O2F_DUALN_OP(a.v * b.v, a.d[i] * b.v + a.v * b.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator/(const DualN<N> &a, const DualN<N> &b) {
+    const double q = a.v / b.v;
+    O2F_DUALN_OP(q, (a.d[i] - q * b.d[i]) / b.v)
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator+(const DualN<N> &a, double b) {
+    O2F_DUALN_OP(a.v + b, a.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator+(double a, const DualN<N> &b) {
+    O2F_DUALN_OP(a + b.v, b.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator-(const DualN<N> &a, double b) {
+    O2F_DUALN_OP(a.v - b, a.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator-(double a, const DualN<N> &b) {
+    O2F_DUALN_OP(a - b.v, -b.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator*(const DualN<N> &a, double b) {
+    O2F_DUALN_OP(a.v * b, a.d[i] * b)
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator*(double a, const DualN<N> &b) {
+    O2F_DUALN_OP(a * b.v, a * b.d[i])
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator/(const DualN<N> &a, double b) {
+    O2F_DUALN_OP(a.v / b, a.d[i] / b)
+}
+template <int N>
+__device__ __forceinline__ DualN<N> operator/(double a, const DualN<N> &b) {
+    const double q = a / b.v;
+    O2F_DUALN_OP(q, -q * b.d[i] / b.v)
+}
+#undef O2F_DUALN_OP
+template <int N>
+__device__ __forceinline__ bool operator<(const DualN<N> &a, const DualN<N> &b) { return a.v < b.v; }
+template <int N>
+__device__ __forceinline__ bool operator<=(const DualN<N> &a, double b) { return a.v <= b; }
+template <int N>
+__device__ __forceinline__ bool operator>=(const DualN<N> &a, double b) { return a.v >= b; }
+template <int N>
+__device__ __forceinline__ DualN<N> sqrt(const DualN<N> &a) {
+    const double r = ::sqrt(a.v);
+    DualN<N> o;
+    o.v = r;
+#pragma unroll
+    for (int i = 0; i < N; ++i) o.d[i] = a.d[i] / (2.0 * r);
+    return o;
+}
+
 // std::min / std::max / std::clamp semantics (NaN handling included).
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
