@@ -7,6 +7,7 @@
 // back.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -297,6 +298,22 @@ int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx 
     }
     if (!e) e = upload(c->skind, hm.skind);
     if (!e) e = upload(c->wall, hm.wall_edges);
+    {
+        const int nw = (int)hm.wall_edges.size();
+        std::vector<int> order(nw);
+        for (int k = 0; k < nw; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return hm.ecl[hm.wall_edges[a]] < hm.ecl[hm.wall_edges[b]]; });
+        std::vector<int> by_cell(nw), grp;
+        for (int j = 0; j < nw; ++j) {
+            by_cell[j] = hm.wall_edges[order[j]];
+            if (j == 0 || hm.ecl[by_cell[j]] != hm.ecl[by_cell[j - 1]]) grp.push_back(j);
+        }
+        grp.push_back(nw);
+        c->wall_ngrp = (int)grp.size() - 1;
+        if (!e) e = upload(c->wall_by_cell, by_cell);
+        if (!e) e = upload(c->wall_grp, grp);
+    }
     if (!e) e = c->geo[0].alloc(hm);
     if (!e) e = c->geo[0].fill(hm, c->stream);
     if (!e) e = dalloc(c->U, 4 * nc);
@@ -330,7 +347,7 @@ int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx 
 void o2f_ctx_destroy(o2f_ctx *c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->cinfo,
+    void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->wall_by_cell, c->wall_grp, c->cinfo,
                     c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
                     c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx, c->Fe,
                     c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work, c->t_work, c->t_out,
@@ -707,7 +724,7 @@ static int ensure_work(o2f_ctx *c) {
 static int rhs_device(o2f_ctx *c, double beta, double *b) {
     const long n = 4L * c->hm.nc;
     HIPCHK(c, launch_fill(n, b, 0.0, c->stream));
-    HIPCHK(c, launch_adjoint_rhs(c->topo(), c->geo[0].view(), c->wall, (int)c->hm.wall_edges.size(), c->fL,
+    HIPCHK(c, launch_adjoint_rhs(c->topo(), c->geo[0].view(), c->wall_by_cell, c->wall_grp, c->wall_ngrp, c->fL,
                                  c->prm.gamma, c->q_inf, beta, b, c->stream));
     return O2F_OK;
 }

@@ -75,6 +75,11 @@ struct o2f_ctx {
     int *cedge = nullptr, *cnbr = nullptr, *cevx = nullptr, *eva = nullptr, *evb = nullptr, *ecl = nullptr,
         *ecr = nullptr;
     int *sptr = nullptr, *scell = nullptr, *wall = nullptr;
+    // wall edges grouped by their (left) cell, ascending edge id within a
+    // group, and the groups' offsets: the adjoint RHS adds each cell's wall
+    // edges in wall order with one thread per cell
+    int *wall_by_cell = nullptr, *wall_grp = nullptr;
+    int wall_ngrp = 0;
     uint8_t *cinfo = nullptr, *skind = nullptr;
     char *ebc = nullptr, *vbc = nullptr;
     o2f::GeomSet geo[3];  // 0: the mesh, 1/2: +/- shape perturbations

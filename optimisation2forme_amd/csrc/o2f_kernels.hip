@@ -874,14 +874,18 @@ __global__ __launch_bounds__(kRedBlock) void k_wall_forces(Geom G, const int *__
 }
 
 // assembleAdjointRHS (src/shape_optimization.cpp:491-554); b must be zero.
-__global__ __launch_bounds__(64) void k_adjoint_rhs(Topo T, Geom G, const int *__restrict__ wall, int nw,
-                                                   const double *__restrict__ fL, double gamma, double q_inf,
-                                                   double beta, double *__restrict__ b) {
-    if (threadIdx.x != 0) return;
-    const int nc = T.nc, ne = T.ne;
+__global__ __launch_bounds__(256) void k_adjoint_rhs(Topo T, Geom G, const int *__restrict__ wall_by_cell,
+                                                    const int *__restrict__ grp, int ngrp,
+                                                    const double *__restrict__ fL, double gamma, double q_inf,
+                                                    double beta, double *__restrict__ b) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngrp) return;
+    const int nc = T.nc;
     const double corde = 1;
-    for (int k = 0; k < nw; ++k) {
-        const int e = wall[k];
+    // one wall cell: its wall edges in ascending edge id, the order in
+    // which the serial loop adds them into this cell's entries
+    for (int j = grp[g]; j < grp[g + 1]; ++j) {
+        const int e = wall_by_cell[j];
         const int iL = T.ecl[e];
         const double A_f = G.elen[e], nx = G.enx[e], ny = G.eny[e];
         const double rL = fL[4 * e], ruL = fL[4 * e + 1], rvL = fL[4 * e + 2];
@@ -1255,9 +1259,12 @@ hipError_t launch_wall_forces(const Geom &G, const int *wall, int nw, const doub
     hipLaunchKernelGGL(k_wall_forces, dim3(1), dim3(kRedBlock), 0, s, G, wall, nw, fL, ne, gamma, skip_x, out);
     return hipGetLastError();
 }
-hipError_t launch_adjoint_rhs(const Topo &T, const Geom &G, const int *wall, int nw, const double *fL,
-                              double gamma, double q_inf, double beta, double *b, hipStream_t s) {
-    hipLaunchKernelGGL(k_adjoint_rhs, dim3(1), dim3(64), 0, s, T, G, wall, nw, fL, gamma, q_inf, beta, b);
+hipError_t launch_adjoint_rhs(const Topo &T, const Geom &G, const int *wall_by_cell, const int *grp, int ngrp,
+                              const double *fL, double gamma, double q_inf, double beta, double *b,
+                              hipStream_t s) {
+    if (ngrp == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adjoint_rhs, dim3(nblk(ngrp)), dim3(256), 0, s, T, G, wall_by_cell, grp, ngrp, fL, gamma,
+                       q_inf, beta, b);
     return hipGetLastError();
 }
 hipError_t launch_jt_blocks(const Topo &T, const Geom &G, const double *fL, const double *fR, double eps,

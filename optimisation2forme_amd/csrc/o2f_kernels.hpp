@@ -240,8 +240,8 @@ hipError_t launch_fill(long n, double *out, double v, hipStream_t s);
 hipError_t launch_fd_diff(long n, const double *Rp, const double *Rm, double eps, double *out, hipStream_t s);
 hipError_t launch_wall_forces(const Geom &G, const int *wall, int nw, const double *fL, int ne, double gamma,
                               int skip_x, double *out, hipStream_t s);
-hipError_t launch_adjoint_rhs(const Topo &T, const Geom &G, const int *wall, int nw, const double *fL,
-                              double gamma, double q_inf, double beta, double *b, hipStream_t s);
+hipError_t launch_adjoint_rhs(const Topo &T, const Geom &G, const int *wall_by_cell, const int *grp, int ngrp,
+                              const double *fL, double gamma, double q_inf, double beta, double *b, hipStream_t s);
 hipError_t launch_jt_blocks(const Topo &T, const Geom &G, const double *fL, const double *fR, double eps,
                             const Phys &P, double *blk, hipStream_t s);
 hipError_t launch_jt_apply(const Topo &T, const Geom &G, const double *blk, const double *x, double *y,
