@@ -631,9 +631,19 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__rest
                                                            const double *__restrict__ gate) {
     if (gate && *gate != 1.0) return;  // a stopped BiCGSTAB loop
     double x = 0.0, y = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        x += a[i] * b[i];
-        if (c) y += c[i] * d[i];
+    // unrolled (and the optional second product tested once, outside the
+    // loop): the loads of four elements are in flight together, the sums
+    // still add them one after another in element order
+    const long i0 = blockIdx.x * (long)blockDim.x + threadIdx.x, stride = (long)gridDim.x * blockDim.x;
+    if (c) {
+#pragma unroll 4
+        for (long i = i0; i < n; i += stride) {
+            x += a[i] * b[i];
+            y += c[i] * d[i];
+        }
+    } else {
+#pragma unroll 4
+        for (long i = i0; i < n; i += stride) x += a[i] * b[i];
     }
     block_partials2(x, y, partials);
 }
@@ -653,6 +663,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bicg_xr_dot(long n, double *__res
     if (scal[kScalGo] != 1.0) return;  // a stopped BiCGSTAB loop
     const double alpha = scal[kScalAlpha], w = scal[kScalW];
     double a = 0.0, b = 0.0;
+#pragma unroll 4
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
         x[i] += alpha * px[i] + w * sx[i];
         const double ri = s[i] - w * t[i];
