@@ -848,7 +848,6 @@ __global__ __launch_bounds__(kRedBlock) void k_wall_forces(Geom G, const int *__
                                                           double gamma, int skip_x,
                                                           double *__restrict__ out) {
     __shared__ double tl[kRedBlock], td[kRedBlock];
-    __shared__ int flag[kRedBlock];
     double L = 0.0, D = 0.0;
     for (int base = 0; base < nw; base += kRedBlock) {
         const int k = base + threadIdx.x;
@@ -866,15 +865,17 @@ __global__ __launch_bounds__(kRedBlock) void k_wall_forces(Geom G, const int *__
         }
         tl[threadIdx.x] = use ? l : 0.0;
         td[threadIdx.x] = use ? d : 0.0;
-        flag[threadIdx.x] = use;
         __syncthreads();
+        // the serial sum in wall order; a skipped edge adds +0.0, which
+        // changes nothing (the sums start at +0.0 and so are never -0.0),
+        // so the loop has no branch and its LDS reads run ahead
         if (threadIdx.x == 0) {
             const int m = min(kRedBlock, nw - base);
-            for (int j = 0; j < m; ++j)
-                if (flag[j]) {
-                    L += tl[j];
-                    D += td[j];
-                }
+#pragma unroll 8
+            for (int j = 0; j < m; ++j) {
+                L += tl[j];
+                D += td[j];
+            }
         }
         __syncthreads();
     }
