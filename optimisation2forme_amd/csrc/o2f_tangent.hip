@@ -283,7 +283,6 @@ __global__ __launch_bounds__(kRedBlock) void kt_objective(Geom G, Geom dG, const
                                                           const double *__restrict__ dfL, double gamma,
                                                           int skip_x, double *__restrict__ out) {
     __shared__ double tl[kRedBlock], td[kRedBlock], tdl[kRedBlock], tdd[kRedBlock];
-    __shared__ int flag[kRedBlock];
     Dual L(0.0), D(0.0);
     for (int base = 0; base < nw; base += kRedBlock) {
         const int k = base + threadIdx.x;
@@ -300,19 +299,21 @@ __global__ __launch_bounds__(kRedBlock) void kt_objective(Geom G, Geom dG, const
             l = (-p * ny + txy * nx + tyy * ny) * A;
             d = (-p * nx + txx * nx + txy * ny) * A;
         }
-        tl[threadIdx.x] = l.v;
-        td[threadIdx.x] = d.v;
-        tdl[threadIdx.x] = l.d;
-        tdd[threadIdx.x] = d.d;
-        flag[threadIdx.x] = use;
+        tl[threadIdx.x] = use ? l.v : 0.0;
+        td[threadIdx.x] = use ? d.v : 0.0;
+        tdl[threadIdx.x] = use ? l.d : 0.0;
+        tdd[threadIdx.x] = use ? d.d : 0.0;
         __syncthreads();
+        // the serial sum in wall order; a skipped edge adds +0.0, which
+        // changes nothing (the sums start at +0.0 and so are never -0.0),
+        // so the loop has no branch and its LDS reads run ahead
         if (threadIdx.x == 0) {
             const int m = min(kRedBlock, nw - base);
-            for (int j = 0; j < m; ++j)
-                if (flag[j]) {
-                    L = L + Dual(tl[j], tdl[j]);
-                    D = D + Dual(td[j], tdd[j]);
-                }
+#pragma unroll 8
+            for (int j = 0; j < m; ++j) {
+                L = L + Dual(tl[j], tdl[j]);
+                D = D + Dual(td[j], tdd[j]);
+            }
         }
         __syncthreads();
     }
