@@ -75,6 +75,15 @@ struct GeomTangent {
 // src/shape_optimization.cpp:748-805); O2F_GEOM_FULL, every cell and edge
 // does.  The interpolation weights follow the cell centres and vertices.
 void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomTangent &t);
+// The same derivatives as lists of the entries that can be non-zero (every
+// other entry of GeomTangent is exactly zero): cells with (dV, dc_x, dc_y)
+// in cval, edges with (dL, dn_x, dn_y, dmid_x, dmid_y) in eval, stencil
+// weights by their index into sw.
+struct GeomTangentSparse {
+    std::vector<int> cell, edge, swi;
+    std::vector<double> cval, eval, swv;
+};
+void geometry_tangent_sparse(const HostMesh &hm, const double *dxy, int geometry, GeomTangentSparse &t);
 // d(vertex position)/d(parameter) of applyShapePerturbationNACA12
 // (src/shape_optimization.cpp:807-890): which 0 camber m, 1 camber
 // position p, 2 thickness t (y = yc + side * yt, x fixed, x_r >= 0.9

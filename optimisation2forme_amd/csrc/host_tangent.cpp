@@ -81,8 +81,9 @@ void cell_geom_d(const HostMesh &hm, const double *dxy, int c, HD &vol, HD &cx, 
 
 // edge_geom (host_mesh.cpp) differentiated; the orientation test keeps the
 // sign of the mesh's own normal (a sign does not change under an
-// infinitesimal displacement).
-void edge_geom_d(const HostMesh &hm, const double *dxy, int e, GeomTangent &t) {
+// infinitesimal displacement).  d(length), d(normal), d(midpoint).
+void edge_geom_d(const HostMesh &hm, const double *dxy, int e, double &dlen, double &dnx, double &dny,
+                 double &dmx, double &dmy) {
     const int a = hm.eva[e], b = hm.evb[e];
     const HD tx = vxd(hm, dxy, a) - vxd(hm, dxy, b), ty = vyd(hm, dxy, a) - vyd(hm, dxy, b);
     const HD len = hd_sqrt(tx * tx + ty * ty);
@@ -91,25 +92,18 @@ void edge_geom_d(const HostMesh &hm, const double *dxy, int e, GeomTangent &t) {
         nx = -nx;
         ny = -ny;
     }
-    t.elen[e] = len.d;
-    t.enx[e] = nx.d;
-    t.eny[e] = ny.d;
-    t.emx[e] = 0.5 * (dxy[2 * a] + dxy[2 * b]);
-    t.emy[e] = 0.5 * (dxy[2 * a + 1] + dxy[2 * b + 1]);
+    dlen = len.d;
+    dnx = nx.d;
+    dny = ny.d;
+    dmx = 0.5 * (dxy[2 * a] + dxy[2 * b]);
+    dmy = 0.5 * (dxy[2 * a + 1] + dxy[2 * b + 1]);
 }
 
-}  // namespace
-
-void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomTangent &t) {
-    t.cvol.assign(hm.nc, 0.0);
-    t.ccx.assign(hm.nc, 0.0);
-    t.ccy.assign(hm.nc, 0.0);
-    t.elen.assign(hm.ne, 0.0);
-    t.enx.assign(hm.ne, 0.0);
-    t.eny.assign(hm.ne, 0.0);
-    t.emx.assign(hm.ne, 0.0);
-    t.emy.assign(hm.ne, 0.0);
-    t.sw.assign(hm.sw.size(), 0.0);
+// The derivative of every cell, edge and stencil weight that can follow
+// the displacement, handed to `out` (out.cell, out.edge, out.sw); every
+// other entry's derivative is exactly zero.
+template <class Out>
+void tangent_entities(const HostMesh &hm, const double *dxy, int geometry, Out &out) {
     // which vertices move; which cells and edges follow their vertices
     // (entities none of whose vertices move have a zero derivative and are
     // skipped)
@@ -137,15 +131,17 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
         if (!cell_on[c]) continue;
         HD vol, cx, cy;
         cell_geom_d(hm, dxy, c, vol, cx, cy);
-        t.cvol[c] = vol.d;
-        t.ccx[c] = cx.d;
-        t.ccy[c] = cy.d;
+        out.cell(c, vol.d, cx.d, cy.d);
         cxd[c].d = cx.d;
         cyd[c].d = cy.d;
         centre_moved[c] = cx.d != 0.0 || cy.d != 0.0;
     }
     for (int e = 0; e < hm.ne; ++e)
-        if (edge_on[e]) edge_geom_d(hm, dxy, e, t);
+        if (edge_on[e]) {
+            double dl, dnx, dny, dmx, dmy;
+            edge_geom_d(hm, dxy, e, dl, dnx, dny, dmx, dmy);
+            out.edge(e, dl, dnx, dny, dmx, dmy);
+        }
     // compute_stencils (host_mesh.cpp) differentiated: the weights follow the
     // cell centres and the vertex's own position
     std::vector<HD> w, dist, ux, uy;
@@ -164,9 +160,9 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
             const HD w0 = num0 / den;
             const HD num1 = (qx - px) * (ay - py) - (ax - px) * (qy - py);
             const HD w1 = num1 / den;
-            t.sw[b] = w0.d;
-            t.sw[b + 1] = w1.d;
-            t.sw[b + 2] = -w0.d - w1.d;
+            out.sw(b, w0.d);
+            out.sw(b + 1, w1.d);
+            out.sw(b + 2, -w0.d - w1.d);
         } else {
             // a weighted stencil of n != 3 cells is the mean-value form
             w.assign(n, HD{0.0, 0.0});
@@ -188,9 +184,64 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
                 w[k] = (hd_tan(0.5 * hd_acos(cp)) + hd_tan(0.5 * hd_acos(cn))) / dist[k];
                 wsum = wsum + w[k];
             }
-            for (int k = 0; k < n; ++k) t.sw[b + k] = (w[k] / wsum).d;
+            for (int k = 0; k < n; ++k) out.sw(b + k, (w[k] / wsum).d);
         }
     }
+}
+
+struct DenseOut {
+    GeomTangent &t;
+    void cell(int c, double v, double x, double y) {
+        t.cvol[c] = v;
+        t.ccx[c] = x;
+        t.ccy[c] = y;
+    }
+    void edge(int e, double l, double nx, double ny, double mx, double my) {
+        t.elen[e] = l;
+        t.enx[e] = nx;
+        t.eny[e] = ny;
+        t.emx[e] = mx;
+        t.emy[e] = my;
+    }
+    void sw(int i, double v) { t.sw[i] = v; }
+};
+
+struct SparseOut {
+    GeomTangentSparse &t;
+    void cell(int c, double v, double x, double y) {
+        t.cell.push_back(c);
+        t.cval.insert(t.cval.end(), {v, x, y});
+    }
+    void edge(int e, double l, double nx, double ny, double mx, double my) {
+        t.edge.push_back(e);
+        t.eval.insert(t.eval.end(), {l, nx, ny, mx, my});
+    }
+    void sw(int i, double v) {
+        t.swi.push_back(i);
+        t.swv.push_back(v);
+    }
+};
+
+}  // namespace
+
+void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomTangent &t) {
+    t.cvol.assign(hm.nc, 0.0);
+    t.ccx.assign(hm.nc, 0.0);
+    t.ccy.assign(hm.nc, 0.0);
+    t.elen.assign(hm.ne, 0.0);
+    t.enx.assign(hm.ne, 0.0);
+    t.eny.assign(hm.ne, 0.0);
+    t.emx.assign(hm.ne, 0.0);
+    t.emy.assign(hm.ne, 0.0);
+    t.sw.assign(hm.sw.size(), 0.0);
+    DenseOut out{t};
+    tangent_entities(hm, dxy, geometry, out);
+}
+
+void geometry_tangent_sparse(const HostMesh &hm, const double *dxy, int geometry, GeomTangentSparse &t) {
+    t = GeomTangentSparse();
+    SparseOut out{t};
+    tangent_entities(hm, dxy, geometry, out);
 }
 
 namespace {

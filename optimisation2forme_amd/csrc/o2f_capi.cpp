@@ -350,7 +350,7 @@ void o2f_ctx_destroy(o2f_ctx *c) {
     void *ptrs[] = {c->cedge, c->cnbr, c->eva, c->evb, c->ecl, c->ecr, c->sptr, c->scell, c->wall, c->wall_by_cell, c->wall_grp, c->cinfo,
                     c->skind, c->ebc, c->vbc, c->U, c->Uold, c->R, c->Gr, c->tq, c->vU, c->vtq, c->fL, c->fR,
                     c->scal, c->partials, c->wall_out, c->speed, c->blk, c->Rp, c->Rm, c->cevx, c->Fe,
-                    c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work, c->t_work, c->t_out,
+                    c->a_veptr, c->a_vedge, c->a_cvtx, c->a_mtgt, c->a_cvw, c->a_work, c->t_work, c->t_out, c->ts_i, c->ts_d,
                     c->pc_color, c->pc_D, c->pc_P};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1282,11 +1282,48 @@ int tangent_on(o2f_ctx *c, const double *dxy, int geometry, double beta, double 
     int st = check_bc(c);
     if (st) return st;
     if ((st = ensure_tangent(c))) return st;
-    GeomTangent t;
-    geometry_tangent(c->hm, dxy, geometry, t);
+    // the derivative set: zeros, then the entries that can be non-zero
+    GeomTangentSparse t;
+    geometry_tangent_sparse(c->hm, dxy, geometry, t);
     GeomSet &gs = c->geo[2];
     if (!gs.cvol) HIPCHK(c, gs.alloc(c->hm));
-    HIPCHK(c, gs.fill(t, c->stream));
+    {
+        const size_t nc = c->hm.nc, ne = c->hm.ne, nsw = c->hm.sw.size() + kStencilSlots;
+        const GeomW g = gs.writable();
+        for (double *p : {g.cvol, g.ccx, g.ccy}) HIPCHK(c, launch_fill((long)nc, p, 0.0, c->stream));
+        for (double *p : {g.elen, g.enx, g.eny, g.emx, g.emy}) HIPCHK(c, launch_fill((long)ne, p, 0.0, c->stream));
+        HIPCHK(c, launch_fill((long)nsw, g.sw, 0.0, c->stream));
+        std::vector<int> ip;
+        ip.reserve(t.cell.size() + t.edge.size() + t.swi.size());
+        ip.insert(ip.end(), t.cell.begin(), t.cell.end());
+        ip.insert(ip.end(), t.edge.begin(), t.edge.end());
+        ip.insert(ip.end(), t.swi.begin(), t.swi.end());
+        std::vector<double> dp;
+        dp.reserve(t.cval.size() + t.eval.size() + t.swv.size());
+        dp.insert(dp.end(), t.cval.begin(), t.cval.end());
+        dp.insert(dp.end(), t.eval.begin(), t.eval.end());
+        dp.insert(dp.end(), t.swv.begin(), t.swv.end());
+        if (ip.size() > c->ts_ni) {
+            if (c->ts_i) (void)hipFree(c->ts_i);
+            HIPCHK(c, dalloc(c->ts_i, ip.size()));
+            c->ts_ni = ip.size();
+        }
+        if (dp.size() > c->ts_nd) {
+            if (c->ts_d) (void)hipFree(c->ts_d);
+            HIPCHK(c, dalloc(c->ts_d, dp.size()));
+            c->ts_nd = dp.size();
+        }
+        if (!ip.empty()) {
+            HIPCHK(c, hipMemcpyAsync(c->ts_i, ip.data(), ip.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipMemcpyAsync(c->ts_d, dp.data(), dp.size() * sizeof(double), hipMemcpyHostToDevice,
+                                     c->stream));
+        }
+        const int ncell = (int)t.cell.size(), nedge = (int)t.edge.size(), nswe = (int)t.swi.size();
+        HIPCHK(c, launch_scatter_geometry(g, ncell, c->ts_i, c->ts_d, nedge, c->ts_i + ncell, c->ts_d + 3 * ncell, nswe,
+                                          c->ts_i + ncell + nedge, c->ts_d + 3 * ncell + 5 * nedge, c->stream));
+        // the host staging vectors are read by the copies above
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
     const Topo T = c->topo();
     const Geom G = c->geo[0].view(), dG = gs.view();
     const TanWork W = tan_work(c);

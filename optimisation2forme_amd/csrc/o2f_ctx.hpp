@@ -32,6 +32,7 @@ struct GeomSet {
     double *elen = nullptr, *enx = nullptr, *eny = nullptr, *emx = nullptr, *emy = nullptr;
     double *sw = nullptr;
     Geom view() const { return Geom{cvol, ccx, ccy, elen, enx, eny, emx, emy, sw}; }
+    GeomW writable() const { return GeomW{cvol, ccx, ccy, elen, enx, eny, emx, emy, sw}; }
     hipError_t alloc(const HostMesh &hm) {
         hipError_t e;
         if ((e = dalloc(cvol, hm.nc)) || (e = dalloc(ccx, hm.nc)) || (e = dalloc(ccy, hm.nc)) ||
@@ -109,6 +110,10 @@ struct o2f_ctx {
     // geometric tangent: derivative arrays of the residual's intermediates
     // (allocated on first use) and its 4-double objective output
     double *t_work = nullptr, *t_out = nullptr;
+    // staging of the sparse geometry tangent (indices, values; capacities)
+    int *ts_i = nullptr;
+    double *ts_d = nullptr;
+    size_t ts_ni = 0, ts_nd = 0;
 
     int fail(int code, const std::string &msg) {
         err = msg;

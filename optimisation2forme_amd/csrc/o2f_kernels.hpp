@@ -189,6 +189,16 @@ struct TanWork {
 hipError_t launch_residual_tangent(const Topo &T, const Geom &G, const Geom &dG, const Phys &P, const double *U,
                                    const double *dU, double *vU, double *fL, double *fR, double *tq, double *vtq,
                                    double *Fe, double *R, const TanWork &W, hipStream_t s);
+// Writable pointers of a geometry set, in Geom's layout.
+struct GeomW {
+    double *cvol, *ccx, *ccy, *elen, *enx, *eny, *emx, *emy, *sw;
+};
+// A geometry tangent given as its non-zero entries (GeomTangentSparse:
+// cells with 3 values each, edges with 5, stencil weights with 1) written
+// into a geometry set whose arrays hold zeros.
+hipError_t launch_scatter_geometry(const GeomW &g, int ncell, const int *cell, const double *cval, int nedge,
+                                   const int *edge, const double *eval, int nsw, const int *swi, const double *swv,
+                                   hipStream_t s);
 // out = L, D, dL, dD over the wall edges (skip_x: computeDirectObjectiveTerm's x > 1.3 rule)
 hipError_t launch_objective_tangent(const Geom &G, const Geom &dG, const int *wall, int nw, const double *fL,
                                     const double *dfL, double gamma, int skip_x, double *out, hipStream_t s);

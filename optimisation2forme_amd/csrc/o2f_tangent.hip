@@ -325,9 +325,49 @@ __global__ __launch_bounds__(kRedBlock) void kt_objective(Geom G, Geom dG, const
     }
 }
 
+// One thread per listed entry: cells [0, ncell), edges next, stencil
+// weights last.
+__global__ __launch_bounds__(256) void kt_scatter_geometry(GeomW g, int ncell, const int *__restrict__ cell,
+                                                           const double *__restrict__ cval, int nedge,
+                                                           const int *__restrict__ edge,
+                                                           const double *__restrict__ eval, int nsw,
+                                                           const int *__restrict__ swi,
+                                                           const double *__restrict__ swv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < ncell) {
+        const int c = cell[i];
+        g.cvol[c] = cval[3 * i];
+        g.ccx[c] = cval[3 * i + 1];
+        g.ccy[c] = cval[3 * i + 2];
+        return;
+    }
+    i -= ncell;
+    if (i < nedge) {
+        const int e = edge[i];
+        g.elen[e] = eval[5 * i];
+        g.enx[e] = eval[5 * i + 1];
+        g.eny[e] = eval[5 * i + 2];
+        g.emx[e] = eval[5 * i + 3];
+        g.emy[e] = eval[5 * i + 4];
+        return;
+    }
+    i -= nedge;
+    if (i < nsw) g.sw[swi[i]] = swv[i];
+}
+
 inline int nblk(long n) { return static_cast<int>((n + 255) / 256); }
 
 }  // namespace
+
+hipError_t launch_scatter_geometry(const GeomW &g, int ncell, const int *cell, const double *cval, int nedge,
+                                   const int *edge, const double *eval, int nsw, const int *swi, const double *swv,
+                                   hipStream_t s) {
+    const long n = (long)ncell + nedge + nsw;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(kt_scatter_geometry, dim3(nblk(n)), dim3(256), 0, s, g, ncell, cell, cval, nedge, edge, eval,
+                       nsw, swi, swv);
+    return hipGetLastError();
+}
 
 hipError_t launch_residual_tangent(const Topo &T, const Geom &G, const Geom &dG, const Phys &P, const double *U,
                                    const double *dU, double *vU, double *fL, double *fR, double *tq, double *vtq,

@@ -16,10 +16,15 @@
 //
 // over the entries whose central differences at h and h / 10 agree (no
 // orientation or clamp switch inside the step), and the number of entries
-// left out.
+// left out; and per case one line
+//
+//   <case> sparse <entries differing from the dense form> <entries listed>
+//
+// for geometry_tangent_sparse expanded into dense arrays.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <fstream>
 #include <random>
 #include <string>
@@ -75,6 +80,39 @@ void report(const char *label, const HostMesh &p1, const HostMesh &m1, const Hos
     }
 }
 
+// geometry_tangent_sparse expanded into dense arrays against
+// geometry_tangent: prints "<case> sparse <entries that differ bitwise>".
+void compare_sparse(const char *label, const HostMesh &hm, const double *dxy, int geometry, const GeomTangent &t) {
+    GeomTangentSparse sp;
+    geometry_tangent_sparse(hm, dxy, geometry, sp);
+    GeomTangent d;
+    d.cvol.assign(hm.nc, 0.0);
+    d.ccx.assign(hm.nc, 0.0);
+    d.ccy.assign(hm.nc, 0.0);
+    for (auto *v : {&d.elen, &d.enx, &d.eny, &d.emx, &d.emy}) v->assign(hm.ne, 0.0);
+    d.sw.assign(hm.sw.size(), 0.0);
+    for (size_t i = 0; i < sp.cell.size(); ++i) {
+        d.cvol[sp.cell[i]] = sp.cval[3 * i];
+        d.ccx[sp.cell[i]] = sp.cval[3 * i + 1];
+        d.ccy[sp.cell[i]] = sp.cval[3 * i + 2];
+    }
+    for (size_t i = 0; i < sp.edge.size(); ++i) {
+        const int e = sp.edge[i];
+        d.elen[e] = sp.eval[5 * i];
+        d.enx[e] = sp.eval[5 * i + 1];
+        d.eny[e] = sp.eval[5 * i + 2];
+        d.emx[e] = sp.eval[5 * i + 3];
+        d.emy[e] = sp.eval[5 * i + 4];
+    }
+    for (size_t i = 0; i < sp.swi.size(); ++i) d.sw[sp.swi[i]] = sp.swv[i];
+    long bad = 0;
+    for (const Field &f : kFields) {
+        const std::vector<double> &a = t.*f.tangent, &b = d.*f.tangent;
+        for (size_t i = 0; i < a.size(); ++i) bad += std::memcmp(&a[i], &b[i], sizeof(double)) != 0;
+    }
+    std::printf("%s sparse %ld %zu\n", label, bad, sp.cell.size() + sp.edge.size() + sp.swi.size());
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -115,6 +153,7 @@ int main(int argc, char **argv) {
         return build(X);
     };
     report("full", moved(h), moved(-h), moved(h / 10), moved(-h / 10), h, t);
+    compare_sparse("full", hm, dxy.data(), O2F_GEOM_FULL, t);
     // the NACA camber / camber position through the reference's in-place update
     const double pos_x = 0.355, chord = 1.0, m0 = 0.03, p0 = 0.45, t0 = 0.11;
     for (int which = 0; which < 2; ++which) {
@@ -130,6 +169,7 @@ int main(int argc, char **argv) {
         };
         report(which ? "naca_p" : "naca_m", perturbed(h), perturbed(-h), perturbed(h / 10), perturbed(-h / 10), h,
                tr);
+        compare_sparse(which ? "naca_p" : "naca_m", hm, d.data(), O2F_GEOM_REFERENCE, tr);
     }
     return 0;
 }

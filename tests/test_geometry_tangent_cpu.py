@@ -41,8 +41,11 @@ def run_case(exe, desc, d):
     assert r.returncode == 0, r.stderr
     rows = {}
     for line in r.stdout.strip().splitlines():
-        case, name, err, skipped, scale = line.split()
-        rows[(case, name)] = (float(err), int(skipped), float(scale))
+        case, name, a, b, *rest = line.split()
+        if name == "sparse":  # the sparse form expanded: entries differing, entries listed
+            rows[(case, name)] = (int(a), int(b))
+            continue
+        rows[(case, name)] = (float(a), int(b), float(rest[0]))
     return rows
 
 
@@ -55,7 +58,13 @@ def test_geometry_tangent_against_central_differences(checker, tmp_path):
     displacements through the reference's partial update."""
     desc = mesher.naca_mesh(m=0.03, p=0.45, t=0.11, n_around=68, n_radial=40, wall_spacing=2e-3, growth=1.15)
     rows = run_case(checker, desc, tmp_path)
-    assert len(rows) == 27
+    sparse = {k: rows.pop(k) for k in list(rows) if k[1] == "sparse"}
+    assert len(rows) == 27 and len(sparse) == 3
+    # the sparse form the device path uploads is the dense form, bit for bit
+    for case in ("full", "naca_m", "naca_p"):
+        bad, listed = sparse[(case, "sparse")]
+        print("%-7s sparse: %d entries listed, %d differ" % (case, listed, bad))
+        assert bad == 0 and listed > 0
     for (case, name), (err, skipped, scale) in sorted(rows.items()):
         print("%-7s %-5s err %.3g  left out %d  scale %.3g" % (case, name, err, skipped, scale))
         assert err < 3e-6, (case, name)
@@ -73,6 +82,7 @@ def test_quad_mesh_tangent(checker, tmp_path):
     valence-4 vertices), random displacement of every vertex."""
     from tests import helpers as H
     rows = run_case(checker, H.shipped_mesh(), tmp_path)
+    assert all(rows[(case, "sparse")][0] == 0 for case in ("full", "naca_m", "naca_p"))
     for name in ("cvol", "ccx", "ccy", "elen", "enx", "eny", "sw"):
         err, skipped, scale = rows[("full", name)]
         print("full %-5s err %.3g  left out %d" % (name, err, skipped))
