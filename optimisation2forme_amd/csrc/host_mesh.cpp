@@ -19,19 +19,21 @@ namespace {
 inline bool tagged(char b) { return b == 'i' || b == 'o' || b == 'f' || b == 'w'; }
 
 // Area and centre of one cell (triangle: src/mesh.cpp:123-130; quad with the
-// two-triangle area of :158-163, exact only for parallelograms).
-void cell_geom(const HostMesh &hm, int c, double &vol, double &cx, double &cy) {
+// two-triangle area of :158-163, exact only for parallelograms), from the
+// vertex positions vx / vy.
+void cell_geom_xy(const HostMesh &hm, const double *vx, const double *vy, int c, double &vol, double &cx,
+                  double &cy) {
     const int *v = &hm.cvtx[4 * c];
-    const double x0 = hm.vx[v[0]], y0 = hm.vy[v[0]];
-    const double x1 = hm.vx[v[1]], y1 = hm.vy[v[1]];
-    const double x2 = hm.vx[v[2]], y2 = hm.vy[v[2]];
+    const double x0 = vx[v[0]], y0 = vy[v[0]];
+    const double x1 = vx[v[1]], y1 = vy[v[1]];
+    const double x2 = vx[v[2]], y2 = vy[v[2]];
     if (hm.cnv[c] == 3) {
         const double e1x = x1 - x0, e1y = y1 - y0, e2x = x2 - x0, e2y = y2 - y0;
         vol = std::abs(0.5 * (e1x * e2y - e2x * e1y));
         cx = 1.0 / 3.0 * (x0 + x1 + x2);
         cy = 1.0 / 3.0 * (y0 + y1 + y2);
     } else {
-        const double x3 = hm.vx[v[3]], y3 = hm.vy[v[3]];
+        const double x3 = vx[v[3]], y3 = vy[v[3]];
         const double e1x = x1 - x0, e1y = y1 - y0, e2x = x2 - x0, e2y = y2 - y0;
         const double f1x = x1 - x2, f1y = y1 - y2, f2x = x3 - x2, f2y = y3 - y2;
         vol = 0.5 * (std::abs(e1x * e2y - e2x * e1y) + std::abs(f1x * f2y - f2x * f1y));
@@ -39,26 +41,88 @@ void cell_geom(const HostMesh &hm, int c, double &vol, double &cx, double &cy) {
         cy = 0.25 * (y0 + y1 + y2 + y3);
     }
 }
+void cell_geom(const HostMesh &hm, int c, double &vol, double &cx, double &cy) {
+    cell_geom_xy(hm, hm.vx.data(), hm.vy.data(), c, vol, cx, cy);
+}
 
 // Length, unit normal turned towards the left cell centre, midpoint
-// (src/mesh.cpp:199-211).
-void edge_geom(HostMesh &hm, int e) {
+// (src/mesh.cpp:199-211), from the vertex positions and cell centres given:
+// out = (length, n_x, n_y, mid_x, mid_y).
+void edge_geom_xy(const HostMesh &hm, const double *vx, const double *vy, const double *ccx, const double *ccy,
+                  int e, double *out) {
     const int a = hm.eva[e], b = hm.evb[e];
-    const double tx = hm.vx[a] - hm.vx[b], ty = hm.vy[a] - hm.vy[b], tz = 0.0;
+    const double tx = vx[a] - vx[b], ty = vy[a] - vy[b], tz = 0.0;
     const double len = std::sqrt(tx * tx + ty * ty + tz * tz);
     double nx = ty / len, ny = -tx / len;
-    const double mx = 0.5 * (hm.vx[a] + hm.vx[b]), my = 0.5 * (hm.vy[a] + hm.vy[b]);
+    const double mx = 0.5 * (vx[a] + vx[b]), my = 0.5 * (vy[a] + vy[b]);
     const int cl = hm.ecl[e];
-    const double to_cell_x = hm.ccx[cl] - mx, to_cell_y = hm.ccy[cl] - my;
+    const double to_cell_x = ccx[cl] - mx, to_cell_y = ccy[cl] - my;
     if (to_cell_x * nx + to_cell_y * ny < 0) {
         nx = -nx;
         ny = -ny;
     }
-    hm.elen[e] = len;
-    hm.enx[e] = nx;
-    hm.eny[e] = ny;
-    hm.emx[e] = mx;
-    hm.emy[e] = my;
+    out[0] = len;
+    out[1] = nx;
+    out[2] = ny;
+    out[3] = mx;
+    out[4] = my;
+}
+void edge_geom(HostMesh &hm, int e) {
+    double o[5];
+    edge_geom_xy(hm, hm.vx.data(), hm.vy.data(), hm.ccx.data(), hm.ccy.data(), e, o);
+    hm.elen[e] = o[0];
+    hm.enx[e] = o[1];
+    hm.eny[e] = o[2];
+    hm.emx[e] = o[3];
+    hm.emy[e] = o[4];
+}
+
+// The weights of vertex v's weighted stencil (n = 3: barycentric; n = 4,
+// or n >= 5 in mvc mode: mean value) from the vertex position and the cell
+// centres given, into w[0, n) in the stencil's slot order; false when the
+// vertex has no weighted stencil.  tmp: scratch of 4 n doubles.
+bool stencil_weights(const HostMesh &hm, int interp_mode, int v, double px, double py, const double *ccx,
+                     const double *ccy, double *w, std::vector<double> &tmp) {
+    const int b = hm.vptr[v], n = hm.vptr[v + 1] - b;
+    if (n == 3) {
+        // barycentric weights of the vertex in the triangle of the three
+        // cell centres (src/advection.cpp:29-50)
+        const int i0 = hm.vcell_asc[b], i1 = hm.vcell_asc[b + 1], i2 = hm.vcell_asc[b + 2];
+        const double ax = ccx[i0], ay = ccy[i0], bx = ccx[i1], by = ccy[i1];
+        const double qx = ccx[i2], qy = ccy[i2];
+        const double num0 = (bx - px) * (qy - py) - (qx - px) * (by - py);
+        const double den = (bx - ax) * (qy - ay) - (qx - ax) * (by - ay);
+        const double w0 = (num0 / den);
+        const double num1 = (qx - px) * (ay - py) - (ax - px) * (qy - py);
+        const double w1 = (num1 / den);
+        w[0] = w0;
+        w[1] = w1;
+        w[2] = 1 - w0 - w1;
+        return true;
+    }
+    if (n == 4 || (n >= 5 && interp_mode == O2F_INTERP_MVC)) {
+        // mean-value weights over the angularly ordered fan
+        // (src/advection.cpp:52-104); n >= 5 is the generalisation
+        tmp.assign(4 * static_cast<size_t>(n), 0.0);
+        double *ww = tmp.data(), *dist = ww + n, *ux = dist + n, *uy = ux + n;
+        for (int k = 0; k < n; ++k) {
+            const int c = hm.vcell_ang[b + k];
+            const double dx = ccx[c] - px, dy = ccy[c] - py;
+            dist[k] = std::hypot(dx, dy);
+            ux[k] = dx / dist[k];
+            uy[k] = dy / dist[k];
+        }
+        for (int k = 0; k < n; ++k) {
+            const int prev = (k + n - 1) % n, next = (k + 1) % n;
+            const double cp = std::clamp(ux[prev] * ux[k] + uy[prev] * uy[k], -1.0, 1.0);
+            const double cn = std::clamp(ux[k] * ux[next] + uy[k] * uy[next], -1.0, 1.0);
+            ww[k] = (std::tan(0.5 * std::acos(cp)) + std::tan(0.5 * std::acos(cn))) / dist[k];
+        }
+        const double wsum = std::accumulate(ww, ww + n, 0.0);
+        for (int k = 0; k < n; ++k) w[k] = ww[k] / wsum;
+        return true;
+    }
+    return false;
 }
 
 inline uint64_t pair_key(int a, int b) {
@@ -221,55 +285,13 @@ void compute_stencils(HostMesh &hm, int interp_mode) {
     hm.skind.assign(hm.nv, kStencilZero);
     hm.scell.assign(nnz, 0);
     hm.sw.assign(nnz, 0.0);
-    std::vector<double> w, dist, ux, uy;
+    std::vector<double> tmp;
     for (int v = 0; v < hm.nv; ++v) {
         const int b = hm.vptr[v], n = hm.vptr[v + 1] - b;
-        const double px = hm.vx[v], py = hm.vy[v];
-        if (n == 3) {
-            // barycentric weights of the vertex in the triangle of the three
-            // cell centres (src/advection.cpp:29-50)
-            const int i0 = hm.vcell_asc[b], i1 = hm.vcell_asc[b + 1], i2 = hm.vcell_asc[b + 2];
-            const double ax = hm.ccx[i0], ay = hm.ccy[i0], bx = hm.ccx[i1], by = hm.ccy[i1];
-            const double qx = hm.ccx[i2], qy = hm.ccy[i2];
-            const double num0 = (bx - px) * (qy - py) - (qx - px) * (by - py);
-            const double den = (bx - ax) * (qy - ay) - (qx - ax) * (by - ay);
-            const double w0 = (num0 / den);
-            const double num1 = (qx - px) * (ay - py) - (ax - px) * (qy - py);
-            const double w1 = (num1 / den);
-            const double w2 = 1 - w0 - w1;
+        if (stencil_weights(hm, interp_mode, v, hm.vx[v], hm.vy[v], hm.ccx.data(), hm.ccy.data(), &hm.sw[b], tmp)) {
             hm.skind[v] = kStencilWsum;
-            hm.scell[b] = i0;
-            hm.scell[b + 1] = i1;
-            hm.scell[b + 2] = i2;
-            hm.sw[b] = w0;
-            hm.sw[b + 1] = w1;
-            hm.sw[b + 2] = w2;
-        } else if (n == 4 || (n >= 5 && interp_mode == O2F_INTERP_MVC)) {
-            // mean-value weights over the angularly ordered fan
-            // (src/advection.cpp:52-104); n >= 5 is the generalisation
-            w.assign(n, 0.0);
-            dist.assign(n, 0.0);
-            ux.assign(n, 0.0);
-            uy.assign(n, 0.0);
-            for (int k = 0; k < n; ++k) {
-                const int c = hm.vcell_ang[b + k];
-                const double dx = hm.ccx[c] - px, dy = hm.ccy[c] - py;
-                dist[k] = std::hypot(dx, dy);
-                ux[k] = dx / dist[k];
-                uy[k] = dy / dist[k];
-            }
-            for (int k = 0; k < n; ++k) {
-                const int prev = (k + n - 1) % n, next = (k + 1) % n;
-                const double cp = std::clamp(ux[prev] * ux[k] + uy[prev] * uy[k], -1.0, 1.0);
-                const double cn = std::clamp(ux[k] * ux[next] + uy[k] * uy[next], -1.0, 1.0);
-                w[k] = (std::tan(0.5 * std::acos(cp)) + std::tan(0.5 * std::acos(cn))) / dist[k];
-            }
-            const double wsum = std::accumulate(w.begin(), w.end(), 0.0);
-            hm.skind[v] = kStencilWsum;
-            for (int k = 0; k < n; ++k) {
-                hm.scell[b + k] = hm.vcell_ang[b + k];
-                hm.sw[b + k] = w[k] / wsum;
-            }
+            const std::vector<int> &order = (n == 3) ? hm.vcell_asc : hm.vcell_ang;
+            for (int k = 0; k < n; ++k) hm.scell[b + k] = order[b + k];
         } else if (n == 2) {
             hm.skind[v] = kStencilAvg2;
             hm.scell[b] = hm.vcell_asc[b];
@@ -340,6 +362,76 @@ bool perturb_naca(HostMesh &hm, int which, double pos_x, double c, double m0, do
         const int cl = hm.ecl[e];
         cell_geom(hm, cl, hm.cvol[cl], hm.ccx[cl], hm.ccy[cl]);
         edge_geom(hm, e);
+    }
+    return true;
+}
+
+bool perturb_naca_sparse(const HostMesh &hm, int interp_mode, int which, double pos_x, double c, double m0,
+                         double p0, double t, double eps, GeomSparse &out) {
+    double m1 = m0, p1 = p0;
+    if (which == 0)
+        m1 = m0 + eps;
+    else if (which == 1)
+        p1 = p0 + eps;
+    else
+        return false;
+    out = GeomSparse();
+    // the wall vertices, moved as perturb_naca moves them
+    std::vector<double> vx(hm.vx), vy(hm.vy);
+    std::vector<char> vmoved(hm.nv, 0);
+    const double freeze_start = 0.9;
+    for (int v = 0; v < hm.nv; ++v) {
+        if (hm.vbc[v] != 'w') continue;
+        const double xr = (hm.vx[v] - pos_x) / c;
+        if (xr >= freeze_start) continue;
+        const double yc0 = naca_camber(m0, p0, xr);
+        const double yt0 = naca_thickness(t, c, xr);
+        const double side = (hm.vy[v] >= yc0) ? +1.0 : -1.0;
+        vx[v] = pos_x + xr * c;
+        vy[v] = naca_camber(m1, p1, xr) + side * yt0;
+        vmoved[v] = 1;
+    }
+    // updateDerivedGeometry: the edges touching a wall vertex and their left
+    // cells (each cell from its own vertices, each edge from the updated
+    // centre of its left cell, as the in-place update leaves them)
+    std::vector<double> ccx(hm.ccx), ccy(hm.ccy);
+    std::vector<char> cell_on(hm.nc, 0);
+    std::vector<int> edges;
+    for (int e = 0; e < hm.ne; ++e)
+        if (hm.ebc[e] == 'w' || hm.vbc[hm.eva[e]] == 'w' || hm.vbc[hm.evb[e]] == 'w') {
+            edges.push_back(e);
+            cell_on[hm.ecl[e]] = 1;
+        }
+    for (int cl = 0; cl < hm.nc; ++cl) {
+        if (!cell_on[cl]) continue;
+        double vol, cx, cy;
+        cell_geom_xy(hm, vx.data(), vy.data(), cl, vol, cx, cy);
+        ccx[cl] = cx;
+        ccy[cl] = cy;
+        out.cell.push_back(cl);
+        out.cval.insert(out.cval.end(), {vol, cx, cy});
+    }
+    for (int e : edges) {
+        double o[5];
+        edge_geom_xy(hm, vx.data(), vy.data(), ccx.data(), ccy.data(), e, o);
+        out.edge.push_back(e);
+        out.eval.insert(out.eval.end(), o, o + 5);
+    }
+    // compute_stencils: the weights of the vertices that moved or whose
+    // cells did (every other vertex's weights are hm's, recomputed from the
+    // same inputs)
+    std::vector<double> tmp, w;
+    for (int v = 0; v < hm.nv; ++v) {
+        const int b = hm.vptr[v], n = hm.vptr[v + 1] - b;
+        bool any = vmoved[v];
+        for (int k = 0; k < n && !any; ++k) any = cell_on[hm.vcell_asc[b + k]];
+        if (!any) continue;
+        w.resize(n);
+        if (!stencil_weights(hm, interp_mode, v, vx[v], vy[v], ccx.data(), ccy.data(), w.data(), tmp)) continue;
+        for (int k = 0; k < n; ++k) {
+            out.swi.push_back(b + k);
+            out.swv.push_back(w[k]);
+        }
     }
     return true;
 }

@@ -75,21 +75,30 @@ struct GeomTangent {
 // src/shape_optimization.cpp:748-805); O2F_GEOM_FULL, every cell and edge
 // does.  The interpolation weights follow the cell centres and vertices.
 void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomTangent &t);
-// The same derivatives as lists of the entries that can be non-zero (every
-// other entry of GeomTangent is exactly zero): cells with (dV, dc_x, dc_y)
-// in cval, edges with (dL, dn_x, dn_y, dmid_x, dmid_y) in eval, stencil
-// weights by their index into sw.
-struct GeomTangentSparse {
+// Entries of a geometry set given by index: cells with (V, c_x, c_y) in
+// cval, edges with (L, n_x, n_y, mid_x, mid_y) in eval, stencil weights by
+// their index into sw.  geometry_tangent_sparse lists the derivative's
+// entries that can be non-zero (every other one is exactly zero).
+struct GeomSparse {
     std::vector<int> cell, edge, swi;
     std::vector<double> cval, eval, swv;
 };
-void geometry_tangent_sparse(const HostMesh &hm, const double *dxy, int geometry, GeomTangentSparse &t);
+void geometry_tangent_sparse(const HostMesh &hm, const double *dxy, int geometry, GeomSparse &t);
 // d(vertex position)/d(parameter) of applyShapePerturbationNACA12
 // (src/shape_optimization.cpp:807-890): which 0 camber m, 1 camber
 // position p, 2 thickness t (y = yc + side * yt, x fixed, x_r >= 0.9
 // frozen).  Returns false for any other `which`.
 bool naca_displacement(const HostMesh &hm, int which, double pos_x, double c, double m0, double p0,
                        std::vector<double> &dxy);
+
+// perturb_naca followed by compute_stencils, without copying the mesh: the
+// entries of the perturbed geometry that can differ from hm's (the cells
+// and edges updateDerivedGeometry refreshes, the stencil weights of the
+// vertices that moved or whose cells did), each computed by the same code
+// as the full update, so hm's geometry with these entries replaced is the
+// full update's, bit for bit.
+bool perturb_naca_sparse(const HostMesh &hm, int interp_mode, int which, double pos_x, double c, double m0,
+                         double p0, double t, double eps, GeomSparse &out);
 
 // MSH 2.2 ASCII reader (Gmsh API element order, see o2f_msh2_read).
 bool read_msh2(const std::string &path, std::vector<double> &xy, std::vector<int> &cell_nv,

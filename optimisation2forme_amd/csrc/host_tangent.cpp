@@ -207,7 +207,7 @@ struct DenseOut {
 };
 
 struct SparseOut {
-    GeomTangentSparse &t;
+    GeomSparse &t;
     void cell(int c, double v, double x, double y) {
         t.cell.push_back(c);
         t.cval.insert(t.cval.end(), {v, x, y});
@@ -238,8 +238,8 @@ void geometry_tangent(const HostMesh &hm, const double *dxy, int geometry, GeomT
     tangent_entities(hm, dxy, geometry, out);
 }
 
-void geometry_tangent_sparse(const HostMesh &hm, const double *dxy, int geometry, GeomTangentSparse &t) {
-    t = GeomTangentSparse();
+void geometry_tangent_sparse(const HostMesh &hm, const double *dxy, int geometry, GeomSparse &t) {
+    t = GeomSparse();
     SparseOut out{t};
     tangent_entities(hm, dxy, geometry, out);
 }

@@ -1165,6 +1165,24 @@ int o2f_adjoint_block_diagonal(o2f_ctx *c, int exact, double *blocks) {
     return copy_out(c, blocks, c->pc_D, 16 * (size_t)c->hm.nc);
 }
 
+namespace {
+int scatter_entries(o2f_ctx *c, GeomSet &gs, const GeomSparse &t);
+// dst = src, every array of the geometry set (device to device)
+int copy_set(o2f_ctx *c, const GeomSet &src, GeomSet &dst) {
+    const size_t nc = c->hm.nc, ne = c->hm.ne, nsw = c->hm.sw.size() + kStencilSlots;
+    const GeomW a = src.writable(), b = dst.writable();
+    const std::pair<double *, double *> cells[] = {{a.cvol, b.cvol}, {a.ccx, b.ccx}, {a.ccy, b.ccy}};
+    const std::pair<double *, double *> edges[] = {
+        {a.elen, b.elen}, {a.enx, b.enx}, {a.eny, b.eny}, {a.emx, b.emx}, {a.emy, b.emy}};
+    for (const auto &pr : cells)
+        HIPCHK(c, hipMemcpyAsync(pr.second, pr.first, nc * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    for (const auto &pr : edges)
+        HIPCHK(c, hipMemcpyAsync(pr.second, pr.first, ne * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(b.sw, a.sw, nsw * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return O2F_OK;
+}
+}  // namespace
+
 int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *values_all, int n_all,
                        const char *const *opt_names, int n_opt, const double *lambda, double beta, double eps,
                        double *grad) {
@@ -1197,12 +1215,16 @@ int o2f_shape_gradient(o2f_ctx *c, const char *const *names_all, const double *v
         double J[2];
         double *Rpm[2] = {c->Rp, c->Rm};
         for (int side = 0; side < 2; ++side) {
-            HostMesh hmp = c->hm;
-            perturb_naca(hmp, which, pos_x, chord, m0, p0, t, side == 0 ? eps : -eps);
-            compute_stencils(hmp, c->prm.interp_mode);
+            // the perturbed mesh (applyShapePerturbationNACA12 on a copy, then
+            // the stencils): the base geometry with the entries the
+            // perturbation changes written over it, on the device
+            GeomSparse pert;
+            perturb_naca_sparse(c->hm, c->prm.interp_mode, which, pos_x, chord, m0, p0, t, side == 0 ? eps : -eps,
+                                pert);
             GeomSet &gs = c->geo[1 + side];
-            if (!gs.cvol) HIPCHK(c, gs.alloc(hmp));
-            HIPCHK(c, gs.fill(hmp, c->stream));
+            if (!gs.cvol) HIPCHK(c, gs.alloc(c->hm));
+            if ((st = copy_set(c, c->geo[0], gs))) return st;
+            if ((st = scatter_entries(c, gs, pert))) return st;
             if ((st = residual_on(c, 1 + side))) return st;
             HIPCHK(c, hipMemcpyAsync(Rpm[side], c->R, n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
             if ((st = objective_on(c, 1 + side, beta, &J[side]))) return st;
@@ -1246,6 +1268,46 @@ int naca_which(const char *name) {
     return -1;
 }
 
+// The listed entries of a geometry set written over its device arrays
+// (uploaded as index / value lists, then scattered).
+int scatter_entries(o2f_ctx *c, GeomSet &gs, const GeomSparse &t) {
+    std::vector<int> ip;
+    ip.reserve(t.cell.size() + t.edge.size() + t.swi.size());
+    ip.insert(ip.end(), t.cell.begin(), t.cell.end());
+    ip.insert(ip.end(), t.edge.begin(), t.edge.end());
+    ip.insert(ip.end(), t.swi.begin(), t.swi.end());
+    std::vector<double> dp;
+    dp.reserve(t.cval.size() + t.eval.size() + t.swv.size());
+    dp.insert(dp.end(), t.cval.begin(), t.cval.end());
+    dp.insert(dp.end(), t.eval.begin(), t.eval.end());
+    dp.insert(dp.end(), t.swv.begin(), t.swv.end());
+    if (ip.size() > c->ts_ni) {
+        if (c->ts_i) (void)hipFree(c->ts_i);
+        c->ts_i = nullptr;
+        c->ts_ni = 0;
+        HIPCHK(c, dalloc(c->ts_i, ip.size()));
+        c->ts_ni = ip.size();
+    }
+    if (dp.size() > c->ts_nd) {
+        if (c->ts_d) (void)hipFree(c->ts_d);
+        c->ts_d = nullptr;
+        c->ts_nd = 0;
+        HIPCHK(c, dalloc(c->ts_d, dp.size()));
+        c->ts_nd = dp.size();
+    }
+    if (!ip.empty()) {
+        HIPCHK(c, hipMemcpyAsync(c->ts_i, ip.data(), ip.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->ts_d, dp.data(), dp.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    const int ncell = (int)t.cell.size(), nedge = (int)t.edge.size(), nsw = (int)t.swi.size();
+    HIPCHK(c, launch_scatter_geometry(gs.writable(), ncell, c->ts_i, c->ts_d, nedge, c->ts_i + ncell,
+                                      c->ts_d + 3 * ncell, nsw, c->ts_i + ncell + nedge, c->ts_d + 3 * ncell + 5 * nedge,
+                                      c->stream));
+    // the host vectors are read by the copies above
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
 int ensure_tangent(o2f_ctx *c) {
     if (c->t_work) return O2F_OK;
     const size_t nv = c->hm.nv, ne = c->hm.ne, nc = c->hm.nc;
@@ -1283,7 +1345,7 @@ int tangent_on(o2f_ctx *c, const double *dxy, int geometry, double beta, double 
     if (st) return st;
     if ((st = ensure_tangent(c))) return st;
     // the derivative set: zeros, then the entries that can be non-zero
-    GeomTangentSparse t;
+    GeomSparse t;
     geometry_tangent_sparse(c->hm, dxy, geometry, t);
     GeomSet &gs = c->geo[2];
     if (!gs.cvol) HIPCHK(c, gs.alloc(c->hm));
@@ -1293,37 +1355,8 @@ int tangent_on(o2f_ctx *c, const double *dxy, int geometry, double beta, double 
         for (double *p : {g.cvol, g.ccx, g.ccy}) HIPCHK(c, launch_fill((long)nc, p, 0.0, c->stream));
         for (double *p : {g.elen, g.enx, g.eny, g.emx, g.emy}) HIPCHK(c, launch_fill((long)ne, p, 0.0, c->stream));
         HIPCHK(c, launch_fill((long)nsw, g.sw, 0.0, c->stream));
-        std::vector<int> ip;
-        ip.reserve(t.cell.size() + t.edge.size() + t.swi.size());
-        ip.insert(ip.end(), t.cell.begin(), t.cell.end());
-        ip.insert(ip.end(), t.edge.begin(), t.edge.end());
-        ip.insert(ip.end(), t.swi.begin(), t.swi.end());
-        std::vector<double> dp;
-        dp.reserve(t.cval.size() + t.eval.size() + t.swv.size());
-        dp.insert(dp.end(), t.cval.begin(), t.cval.end());
-        dp.insert(dp.end(), t.eval.begin(), t.eval.end());
-        dp.insert(dp.end(), t.swv.begin(), t.swv.end());
-        if (ip.size() > c->ts_ni) {
-            if (c->ts_i) (void)hipFree(c->ts_i);
-            HIPCHK(c, dalloc(c->ts_i, ip.size()));
-            c->ts_ni = ip.size();
-        }
-        if (dp.size() > c->ts_nd) {
-            if (c->ts_d) (void)hipFree(c->ts_d);
-            HIPCHK(c, dalloc(c->ts_d, dp.size()));
-            c->ts_nd = dp.size();
-        }
-        if (!ip.empty()) {
-            HIPCHK(c, hipMemcpyAsync(c->ts_i, ip.data(), ip.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-            HIPCHK(c, hipMemcpyAsync(c->ts_d, dp.data(), dp.size() * sizeof(double), hipMemcpyHostToDevice,
-                                     c->stream));
-        }
-        const int ncell = (int)t.cell.size(), nedge = (int)t.edge.size(), nswe = (int)t.swi.size();
-        HIPCHK(c, launch_scatter_geometry(g, ncell, c->ts_i, c->ts_d, nedge, c->ts_i + ncell, c->ts_d + 3 * ncell, nswe,
-                                          c->ts_i + ncell + nedge, c->ts_d + 3 * ncell + 5 * nedge, c->stream));
-        // the host staging vectors are read by the copies above
-        HIPCHK(c, hipStreamSynchronize(c->stream));
     }
+    if ((st = scatter_entries(c, gs, t))) return st;
     const Topo T = c->topo();
     const Geom G = c->geo[0].view(), dG = gs.view();
     const TanWork W = tan_work(c);

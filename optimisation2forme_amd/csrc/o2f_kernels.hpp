@@ -193,9 +193,8 @@ hipError_t launch_residual_tangent(const Topo &T, const Geom &G, const Geom &dG,
 struct GeomW {
     double *cvol, *ccx, *ccy, *elen, *enx, *eny, *emx, *emy, *sw;
 };
-// A geometry tangent given as its non-zero entries (GeomTangentSparse:
-// cells with 3 values each, edges with 5, stencil weights with 1) written
-// into a geometry set whose arrays hold zeros.
+// Listed entries of a geometry set (GeomSparse: cells with 3 values each,
+// edges with 5, stencil weights with 1) written over its arrays.
 hipError_t launch_scatter_geometry(const GeomW &g, int ncell, const int *cell, const double *cval, int nedge,
                                    const int *edge, const double *eval, int nsw, const int *swi, const double *swv,
                                    hipStream_t s);

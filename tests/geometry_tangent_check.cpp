@@ -20,7 +20,12 @@
 //
 //   <case> sparse <entries differing from the dense form> <entries listed>
 //
-// for geometry_tangent_sparse expanded into dense arrays.
+// for geometry_tangent_sparse expanded into dense arrays, and lines
+//
+//   <case> perturb <entries differing from the full update> <entries listed>
+//
+// for perturb_naca_sparse against a copy updated by perturb_naca and
+// compute_stencils (eps = 1e-6, -1e-6, 1e-2).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -83,7 +88,7 @@ void report(const char *label, const HostMesh &p1, const HostMesh &m1, const Hos
 // geometry_tangent_sparse expanded into dense arrays against
 // geometry_tangent: prints "<case> sparse <entries that differ bitwise>".
 void compare_sparse(const char *label, const HostMesh &hm, const double *dxy, int geometry, const GeomTangent &t) {
-    GeomTangentSparse sp;
+    GeomSparse sp;
     geometry_tangent_sparse(hm, dxy, geometry, sp);
     GeomTangent d;
     d.cvol.assign(hm.nc, 0.0);
@@ -111,6 +116,39 @@ void compare_sparse(const char *label, const HostMesh &hm, const double *dxy, in
         for (size_t i = 0; i < a.size(); ++i) bad += std::memcmp(&a[i], &b[i], sizeof(double)) != 0;
     }
     std::printf("%s sparse %ld %zu\n", label, bad, sp.cell.size() + sp.edge.size() + sp.swi.size());
+}
+
+// perturb_naca_sparse applied to hm's geometry against the full update
+// (a copy, perturb_naca, compute_stencils): prints
+// "<case> perturb <entries that differ bitwise> <entries listed>".
+void compare_perturbation(const char *label, const HostMesh &hm, int which, double pos_x, double chord, double m0,
+                          double p0, double t0, double eps) {
+    HostMesh q = hm;
+    perturb_naca(q, which, pos_x, chord, m0, p0, t0, eps);
+    compute_stencils(q, O2F_INTERP_MVC);
+    GeomSparse sp;
+    perturb_naca_sparse(hm, O2F_INTERP_MVC, which, pos_x, chord, m0, p0, t0, eps, sp);
+    HostMesh r = hm;
+    for (size_t i = 0; i < sp.cell.size(); ++i) {
+        r.cvol[sp.cell[i]] = sp.cval[3 * i];
+        r.ccx[sp.cell[i]] = sp.cval[3 * i + 1];
+        r.ccy[sp.cell[i]] = sp.cval[3 * i + 2];
+    }
+    for (size_t i = 0; i < sp.edge.size(); ++i) {
+        const int e = sp.edge[i];
+        r.elen[e] = sp.eval[5 * i];
+        r.enx[e] = sp.eval[5 * i + 1];
+        r.eny[e] = sp.eval[5 * i + 2];
+        r.emx[e] = sp.eval[5 * i + 3];
+        r.emy[e] = sp.eval[5 * i + 4];
+    }
+    for (size_t i = 0; i < sp.swi.size(); ++i) r.sw[sp.swi[i]] = sp.swv[i];
+    long bad = 0;
+    for (const Field &f : kFields) {
+        const std::vector<double> &a = q.*f.mesh, &b = r.*f.mesh;
+        for (size_t i = 0; i < a.size(); ++i) bad += std::memcmp(&a[i], &b[i], sizeof(double)) != 0;
+    }
+    std::printf("%s perturb %ld %zu\n", label, bad, sp.cell.size() + sp.edge.size() + sp.swi.size());
 }
 
 }  // namespace
@@ -170,6 +208,8 @@ int main(int argc, char **argv) {
         report(which ? "naca_p" : "naca_m", perturbed(h), perturbed(-h), perturbed(h / 10), perturbed(-h / 10), h,
                tr);
         compare_sparse(which ? "naca_p" : "naca_m", hm, d.data(), O2F_GEOM_REFERENCE, tr);
+        for (double e : {1e-6, -1e-6, 1e-2})
+            compare_perturbation(which ? "naca_p" : "naca_m", hm, which, pos_x, chord, m0, p0, t0, e);
     }
     return 0;
 }

@@ -45,6 +45,9 @@ def run_case(exe, desc, d):
         if name == "sparse":  # the sparse form expanded: entries differing, entries listed
             rows[(case, name)] = (int(a), int(b))
             continue
+        if name == "perturb":  # the sparse perturbation against the full update
+            rows.setdefault((case, name), []).append((int(a), int(b)))
+            continue
         rows[(case, name)] = (float(a), int(b), float(rest[0]))
     return rows
 
@@ -59,7 +62,16 @@ def test_geometry_tangent_against_central_differences(checker, tmp_path):
     desc = mesher.naca_mesh(m=0.03, p=0.45, t=0.11, n_around=68, n_radial=40, wall_spacing=2e-3, growth=1.15)
     rows = run_case(checker, desc, tmp_path)
     sparse = {k: rows.pop(k) for k in list(rows) if k[1] == "sparse"}
+    perturb = {k: rows.pop(k) for k in list(rows) if k[1] == "perturb"}
     assert len(rows) == 27 and len(sparse) == 3
+    # the FD gradient's perturbed geometry (base entries overwritten by the
+    # listed ones) is the full copy-and-update's, bit for bit, for +-1e-6
+    # and a large 1e-2
+    for case in ("naca_m", "naca_p"):
+        for bad, listed in perturb[(case, "perturb")]:
+            print("%-7s perturb: %d entries listed, %d differ" % (case, listed, bad))
+            assert bad == 0 and listed > 0
+        assert len(perturb[(case, "perturb")]) == 3
     # the sparse form the device path uploads is the dense form, bit for bit
     for case in ("full", "naca_m", "naca_p"):
         bad, listed = sparse[(case, "sparse")]
@@ -83,6 +95,7 @@ def test_quad_mesh_tangent(checker, tmp_path):
     from tests import helpers as H
     rows = run_case(checker, H.shipped_mesh(), tmp_path)
     assert all(rows[(case, "sparse")][0] == 0 for case in ("full", "naca_m", "naca_p"))
+    assert all(bad == 0 for case in ("naca_m", "naca_p") for bad, _ in rows[(case, "perturb")])
     for name in ("cvol", "ccx", "ccy", "elen", "enx", "eny", "sw"):
         err, skipped, scale = rows[("full", name)]
         print("full %-5s err %.3g  left out %d" % (name, err, skipped))
