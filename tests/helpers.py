@@ -82,6 +82,39 @@ def tiny_mesh():
                             radius=6.0)
 
 
+def corner_mesh(nx=8, ny=4, lx=2.0, ly=1.0):
+    """A structured triangulated box whose bottom and left sides are walls
+    (top far field, right outlet): the corner triangle owns two wall edges,
+    which no O-mesh has."""
+    xs = np.linspace(0.0, lx, nx + 1)
+    ys = np.linspace(0.0, ly, ny + 1)
+    xy = np.array([(x, y) for y in ys for x in xs], np.float64)
+
+    def vid(i, j):
+        return j * (nx + 1) + i
+
+    cells = []
+    for j in range(ny):
+        for i in range(nx):
+            a, b, c, d = vid(i, j), vid(i + 1, j), vid(i, j + 1), vid(i + 1, j + 1)
+            cells += [[a, b, c, -1], [b, d, c, -1]]
+    bnd, tag = [], []
+    for i in range(nx):
+        bnd.append((vid(i, 0), vid(i + 1, 0)))
+        tag.append(ord("w"))
+    for j in range(ny):
+        bnd.append((vid(0, j), vid(0, j + 1)))
+        tag.append(ord("w"))
+    for i in range(nx):
+        bnd.append((vid(i, ny), vid(i + 1, ny)))
+        tag.append(ord("f"))
+    for j in range(ny):
+        bnd.append((vid(nx, j), vid(nx, j + 1)))
+        tag.append(ord("o"))
+    return dict(xy=xy, cell_nv=np.full(len(cells), 3, np.int32), cell_vtx=np.array(cells, np.int32),
+                bnd_vtx=np.array(bnd, np.int32), bnd_tag=np.array(tag, np.uint8))
+
+
 @functools.lru_cache(maxsize=None)
 def small_mesh():
     return mesher.small_10k()

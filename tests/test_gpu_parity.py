@@ -294,3 +294,21 @@ def test_time_kernel_leaves_the_state():
     for k, n in ((4, 5), (-1, 5), (0, 0)):
         with pytest.raises(RuntimeError):
             s.time_kernel(k, n)
+
+
+def test_adjoint_rhs_and_forces_with_a_cell_of_two_wall_edges():
+    """assembleAdjointRHS adds every wall edge into its left cell in wall
+    order; the device adds each cell's wall edges in that order with one
+    thread per cell.  On a box whose corner triangle owns two wall edges the
+    right-hand side and C_L / C_D equal the oracle's bit for bit."""
+    desc = H.corner_mesh()
+    flow = H.subsonic_small()
+    p = H.oracle_params(flow)
+    m = O.Mesh(desc)
+    m.init_freestream(p)
+    m.run_steps(p, 5)
+    s, m2, _ = pair(desc, flow, m.state())
+    s.residual()
+    m2.residual(p)
+    assert same(s.adjoint_rhs(50.0), m2.adjoint_rhs(p, 50.0))
+    assert tuple(s.lift_drag()) == tuple(m2.lift_drag(p))
