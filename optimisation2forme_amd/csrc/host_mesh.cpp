@@ -333,6 +333,23 @@ double naca_thickness(double t, double c, double xr) {
            (0.2969 * std::sqrt(xr) - 0.1260 * xr - 0.3516 * xr * xr + 0.2843 * xr * xr * xr -
             0.1015 * xr * xr * xr * xr);
 }
+// The wall vertices applyShapePerturbationNACA12 moves, from hm's positions
+// into vx / vy (which may be hm's own arrays); moved[v] set for each.
+void move_wall_vertices(const HostMesh &hm, double pos_x, double c, double m0, double p0, double m1, double p1,
+                        double t, double *vx, double *vy, char *moved) {
+    const double freeze_start = 0.9;  // the last tenth of the chord stays fixed
+    for (int v = 0; v < hm.nv; ++v) {
+        if (hm.vbc[v] != 'w') continue;
+        const double xr = (hm.vx[v] - pos_x) / c;
+        if (xr >= freeze_start) continue;
+        const double yc0 = naca_camber(m0, p0, xr);
+        const double yt0 = naca_thickness(t, c, xr);
+        const double side = (hm.vy[v] >= yc0) ? +1.0 : -1.0;
+        vx[v] = pos_x + xr * c;
+        vy[v] = naca_camber(m1, p1, xr) + side * yt0;
+        if (moved) moved[v] = 1;
+    }
+}
 }  // namespace
 
 bool perturb_naca(HostMesh &hm, int which, double pos_x, double c, double m0, double p0,
@@ -344,17 +361,7 @@ bool perturb_naca(HostMesh &hm, int which, double pos_x, double c, double m0, do
         p1 = p0 + eps;
     else
         return false;
-    const double freeze_start = 0.9;  // the last tenth of the chord stays fixed
-    for (int v = 0; v < hm.nv; ++v) {
-        if (hm.vbc[v] != 'w') continue;
-        const double xr = (hm.vx[v] - pos_x) / c;
-        if (xr >= freeze_start) continue;
-        const double yc0 = naca_camber(m0, p0, xr);
-        const double yt0 = naca_thickness(t, c, xr);
-        const double side = (hm.vy[v] >= yc0) ? +1.0 : -1.0;
-        hm.vx[v] = pos_x + xr * c;
-        hm.vy[v] = naca_camber(m1, p1, xr) + side * yt0;
-    }
+    move_wall_vertices(hm, pos_x, c, m0, p0, m1, p1, t, hm.vx.data(), hm.vy.data(), nullptr);
     // updateDerivedGeometry: only edges touching a wall vertex, and only their
     // left cells, are refreshed (src/shape_optimization.cpp:748-805).
     for (int e = 0; e < hm.ne; ++e) {
@@ -379,18 +386,7 @@ bool perturb_naca_sparse(const HostMesh &hm, int interp_mode, int which, double 
     // the wall vertices, moved as perturb_naca moves them
     std::vector<double> vx(hm.vx), vy(hm.vy);
     std::vector<char> vmoved(hm.nv, 0);
-    const double freeze_start = 0.9;
-    for (int v = 0; v < hm.nv; ++v) {
-        if (hm.vbc[v] != 'w') continue;
-        const double xr = (hm.vx[v] - pos_x) / c;
-        if (xr >= freeze_start) continue;
-        const double yc0 = naca_camber(m0, p0, xr);
-        const double yt0 = naca_thickness(t, c, xr);
-        const double side = (hm.vy[v] >= yc0) ? +1.0 : -1.0;
-        vx[v] = pos_x + xr * c;
-        vy[v] = naca_camber(m1, p1, xr) + side * yt0;
-        vmoved[v] = 1;
-    }
+    move_wall_vertices(hm, pos_x, c, m0, p0, m1, p1, t, vx.data(), vy.data(), vmoved.data());
     // updateDerivedGeometry: the edges touching a wall vertex and their left
     // cells (each cell from its own vertices, each edge from the updated
     // centre of its left cell, as the in-place update leaves them)
