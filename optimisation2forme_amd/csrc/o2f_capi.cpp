@@ -327,7 +327,7 @@ int o2f_ctx_create(const o2f_mesh_desc *mesh, const o2f_params *params, o2f_ctx 
     if (!e) e = dalloc(c->fR, 4 * ne);
     if (!e) e = dalloc(c->Fe, 4 * ne);
     if (!e) e = dalloc(c->scal, kScalCount);
-    if (!e) e = dalloc(c->partials, 2 * kRedGrid);
+    if (!e) e = dalloc(c->partials, 4 * kRedGrid);  // two sets of block partials
     if (!e) e = dalloc(c->wall_out, 2);
     if (!e) e = dalloc(c->speed, speed_parts(hm.nc));
     if (!e) e = hipMemset(c->scal, 0, kScalCount * sizeof(double));
@@ -788,6 +788,7 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     const long n = 4L * c->hm.nc;
     double *r = c->kv[1], *r0 = c->kv[2], *v = c->kv[3], *p = c->kv[4], *sv = c->kv[5], *t = c->kv[6];
     double *tmp = c->kv[7], *py = c->kv[10], *sz = c->kv[11];
+    double *wparts = c->partials + 2 * kRedGrid;  // the (t.t, t.s) partials
     HostScal h;
     int st;
     // r = b - A x; r0 = r
@@ -835,13 +836,15 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
             HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
             if (pc) HIPCHK(c, pc(p, py));
             HIPCHK(c, mv(pc ? py : p, v));
-            HIPCHK(c, launch_dot2(r0, v, nullptr, nullptr, n, c->partials, c->scal, kOpAlpha, s, go));
-            HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, s));
+            // alpha and w are formed by the kernels that use them, each block
+            // from the partials (no one-block launch in between)
+            HIPCHK(c, launch_dot2_partial(r0, v, nullptr, nullptr, n, c->partials, go, s));
+            HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, c->partials, s));
             if (pc) HIPCHK(c, pc(sv, sz));
             HIPCHK(c, mv(pc ? sz : sv, t));
-            HIPCHK(c, launch_dot2(t, t, t, sv, n, c->partials, c->scal, kOpOmega, s, go));
-            HIPCHK(c, launch_bicg_xr_dot(n, x, r, pc ? py : p, pc ? sz : sv, sv, t, r0, c->partials, c->scal, test,
-                                         s));
+            HIPCHK(c, launch_dot2_partial(t, t, t, sv, n, wparts, go, s));
+            HIPCHK(c, launch_bicg_xr_dot(n, x, r, pc ? py : p, pc ? sz : sv, sv, t, r0, wparts, c->partials, c->scal,
+                                         test, s));
         }
         if ((st = read_scal(c, h))) return st;
     }

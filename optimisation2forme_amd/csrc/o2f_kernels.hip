@@ -619,10 +619,36 @@ __device__ __forceinline__ void block_partials2(double x, double y, double *__re
     }
 }
 
+// The two totals of nparts block partials, in block order then a fixed tree;
+// every thread of the block gets them.  Any block computes the same bits,
+// so a consumer kernel may redo this in each of its blocks instead of
+// reading a result a separate one-block launch stored.
+__device__ __forceinline__ void final_sums2(const double *__restrict__ partials, int nparts, double &ab,
+                                            double &cd) {
+    __shared__ double s1[kRedBlock], s2[kRedBlock];
+    double x = 0.0, y = 0.0;
+    for (int bb = threadIdx.x; bb < nparts; bb += blockDim.x) {
+        x += partials[2 * bb];
+        y += partials[2 * bb + 1];
+    }
+    s1[threadIdx.x] = x;
+    s2[threadIdx.x] = y;
+    __syncthreads();
+    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+            s2[threadIdx.x] += s2[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    ab = s1[0];
+    cd = s2[0];
+}
+
 // Sums of products sum_i a_i b_i (and optionally sum_i c_i d_i): per-block
-// partials in a fixed tree, then one block combines them in block order and
-// applies `op` (two launches: the kernel boundary orders the partials, no
-// device-wide fence per block).
+// partials in a fixed tree, then a later launch combines them in block order
+// (k_dot2_final, or every block of the kernel that uses the result): the
+// kernel boundary orders the partials, no device-wide fence per block.
 __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__restrict__ a,
                                                            const double *__restrict__ b,
                                                            const double *__restrict__ c,
@@ -648,20 +674,29 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__rest
     block_partials2(x, y, partials);
 }
 
-// The last update of a BiCGSTAB iteration fused with the reduction after
-// it: x += alpha px + w sx; r = s - w t; partials of (r0.r, r.r), with the
-// elements of k_dot2_partial's grid-stride loop in its order (the same
-// grid), so the partials are those of k_dot2_partial(r0, r, r, r).
+// The last update of a BiCGSTAB iteration fused with the reductions on
+// either side of it.  Each block first combines the (t.t, t.s) partials
+// (wparts, nparts of them) into w = t.s / t.t or 0, as k_dot2_final's
+// kOpOmega does (block 0 also stores it); then x += alpha px + w sx;
+// r = s - w t; and partials of (r0.r, r.r), with the elements of
+// k_dot2_partial's grid-stride loop in its order (the same grid), so the
+// partials are those of k_dot2_partial(r0, r, r, r).  wparts and partials
+// are distinct: other blocks still read wparts while this one writes.
 __global__ __launch_bounds__(kRedBlock) void k_bicg_xr_dot(long n, double *__restrict__ x, double *__restrict__ r,
                                                           const double *__restrict__ px,
                                                           const double *__restrict__ sx,
                                                           const double *__restrict__ s,
                                                           const double *__restrict__ t,
                                                           const double *__restrict__ r0,
-                                                          const double *__restrict__ scal,
+                                                          double *__restrict__ scal,
+                                                          const double *__restrict__ wparts, int nparts,
                                                           double *__restrict__ partials) {
     if (scal[kScalGo] != 1.0) return;  // a stopped BiCGSTAB loop
-    const double alpha = scal[kScalAlpha], w = scal[kScalW];
+    double tt, ts;
+    final_sums2(wparts, nparts, tt, ts);
+    const double w = (tt > 0) ? ts / tt : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[kScalW] = w;
+    const double alpha = scal[kScalAlpha];
     double a = 0.0, b = 0.0;
 #pragma unroll 4
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -678,24 +713,9 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restri
                                                          double *__restrict__ scal, int op,
                                                          const double *__restrict__ gate, BicgTest test) {
     if (gate && *gate != 1.0) return;  // a stopped BiCGSTAB loop
-    __shared__ double s1[kRedBlock], s2[kRedBlock];
-    double x = 0.0, y = 0.0;
-    for (int bb = threadIdx.x; bb < nparts; bb += blockDim.x) {
-        x += partials[2 * bb];
-        y += partials[2 * bb + 1];
-    }
-    s1[threadIdx.x] = x;
-    s2[threadIdx.x] = y;
-    __syncthreads();
-    for (int s = kRedBlock / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            s1[threadIdx.x] += s1[threadIdx.x + s];
-            s2[threadIdx.x] += s2[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
+    double ab, cd;
+    final_sums2(partials, nparts, ab, cd);
     if (threadIdx.x != 0) return;
-    const double ab = s1[0], cd = s2[0];
     switch (op) {
         case kOpStore:  // raw sums
             scal[kScalDotA] = ab;
@@ -738,11 +758,21 @@ __global__ __launch_bounds__(256) void k_bicg_p(long n, const double *__restrict
     p[i] = r[i] + beta * (p[i] - w * v[i]);
 }
 
-__global__ __launch_bounds__(256) void k_bicg_s(long n, const double *__restrict__ r, const double *__restrict__ v,
-                                                double *__restrict__ s, const double *__restrict__ scal) {
-    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i >= n || scal[kScalGo] != 1.0) return;
-    s[i] = r[i] - scal[kScalAlpha] * v[i];
+// s = r - alpha v, each block first forming alpha = rho / (r0.v) from the
+// nparts partials of r0.v as k_dot2_final's kOpAlpha does (block 0 also
+// stores it; no block of this kernel reads the stored value).
+__global__ __launch_bounds__(kRedBlock) void k_bicg_s(long n, const double *__restrict__ r,
+                                                     const double *__restrict__ v, double *__restrict__ s,
+                                                     double *__restrict__ scal,
+                                                     const double *__restrict__ aparts, int nparts) {
+    if (scal[kScalGo] != 1.0) return;  // a stopped BiCGSTAB loop
+    double rv, unused;
+    final_sums2(aparts, nparts, rv, unused);
+    const double alpha = scal[kScalRho] / rv;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[kScalAlpha] = alpha;
+#pragma unroll 4
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        s[i] = r[i] - alpha * v[i];
 }
 
 // Block-Jacobi preconditioner of the adjoint solves.  Probing: x = 1 on
@@ -1210,14 +1240,20 @@ static unsigned red_grid(long n) {
     return (unsigned)std::min<long>(kRedGrid, std::max<long>(1, (long)nblk(n, kRedBlock)));
 }
 hipError_t launch_bicg_xr_dot(long n, double *x, double *r, const double *px, const double *sx, const double *sv,
-                              const double *t, const double *r0, double *partials, double *scal,
-                              const BicgTest &test, hipStream_t s) {
+                              const double *t, const double *r0, const double *wparts, double *partials,
+                              double *scal, const BicgTest &test, hipStream_t s) {
     const unsigned g = red_grid(n);
-    hipLaunchKernelGGL(k_bicg_xr_dot, dim3(g), dim3(kRedBlock), 0, s, n, x, r, px, sx, sv, t, r0, scal, partials);
+    hipLaunchKernelGGL(k_bicg_xr_dot, dim3(g), dim3(kRedBlock), 0, s, n, x, r, px, sx, sv, t, r0, scal, wparts,
+                       (int)g, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedBlock), 0, s, partials, (int)g, scal, kOpRhoLoop,
                        scal + kScalGo, test);
+    return hipGetLastError();
+}
+hipError_t launch_dot2_partial(const double *a, const double *b, const double *c, const double *d, long n,
+                               double *partials, const double *gate, hipStream_t s) {
+    hipLaunchKernelGGL(k_dot2_partial, dim3(red_grid(n)), dim3(kRedBlock), 0, s, a, b, c, d, n, partials, gate);
     return hipGetLastError();
 }
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
@@ -1234,8 +1270,10 @@ hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, co
     hipLaunchKernelGGL(k_bicg_p, dim3(nblk(n)), dim3(256), 0, s, n, r, p, v, scal);
     return hipGetLastError();
 }
-hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s) {
-    hipLaunchKernelGGL(k_bicg_s, dim3(nblk(n)), dim3(256), 0, s, n, r, v, sv, scal);
+hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, double *scal, const double *aparts,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_s, dim3(red_grid(n)), dim3(kRedBlock), 0, s, n, r, v, sv, scal, aparts,
+                       (int)red_grid(n));
     return hipGetLastError();
 }
 hipError_t launch_probe_seed(int nc, const int *color, int k, int q, double *x, hipStream_t s) {

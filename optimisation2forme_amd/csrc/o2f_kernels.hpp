@@ -230,14 +230,23 @@ hipError_t launch_step_gather(const Topo &T, const Geom &G, const double *U, dou
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
                        double *partials, double *scal, int op, hipStream_t s, const double *gate = nullptr,
                        const BicgTest &test = BicgTest{});
+// The block partials of a reduction only (2 per block, red_grid(n) blocks),
+// for a consumer kernel that combines them itself.
+hipError_t launch_dot2_partial(const double *a, const double *b, const double *c, const double *d, long n,
+                               double *partials, const double *gate, hipStream_t s);
 // The iteration's last update (x += alpha px + w sx; r = s - w t) fused with
-// the (r0.r, r.r) reduction and the loop test (kOpRhoLoop); gated on
+// w = t.s / t.t from the (t.t, t.s) partials wparts (kOpOmega), and with the
+// (r0.r, r.r) reduction and the loop test (kOpRhoLoop) after it; gated on
 // scal[kScalGo].  Partials identical to launch_dot2(r0, r, r, r, ...).
+// wparts and partials must not overlap.
 hipError_t launch_bicg_xr_dot(long n, double *x, double *r, const double *px, const double *sx, const double *sv,
-                              const double *t, const double *r0, double *partials, double *scal,
-                              const BicgTest &test, hipStream_t s);
+                              const double *t, const double *r0, const double *wparts, double *partials,
+                              double *scal, const BicgTest &test, hipStream_t s);
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s);
-hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, const double *scal, hipStream_t s);
+// s = r - alpha v with alpha = rho / (r0.v) formed from the r0.v partials
+// aparts (kOpAlpha; stored to scal too).
+hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, double *scal, const double *aparts,
+                         hipStream_t s);
 // Block-Jacobi preconditioner: probing seeds / extraction by colour,
 // P_c = (D_c^T)^{-1} (16 per cell, row-major), out_c = P_c in_c.
 hipError_t launch_probe_seed(int nc, const int *color, int k, int q, double *x, hipStream_t s);
