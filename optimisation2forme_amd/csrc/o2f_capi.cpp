@@ -818,6 +818,32 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     const BicgTest test{tol * tol * rhs_sq, (double)maxit, DBL_EPSILON * DBL_EPSILON};
     const double *go = c->scal + kScalGo;
     constexpr int kGroup = 8;
+    const auto enqueue_group = [&]() -> hipError_t {
+        hipError_t e = hipSuccess;
+        for (int g = 0; g < kGroup && e == hipSuccess; ++g) {
+            if ((e = launch_bicg_p(n, r, p, v, c->scal, s)) != hipSuccess) break;
+            if (pc && (e = pc(p, py)) != hipSuccess) break;
+            if ((e = mv(pc ? py : p, v)) != hipSuccess) break;
+            // alpha and w are formed by the kernels that use them, each block
+            // from the partials (no one-block launch in between)
+            if ((e = launch_dot2_partial(r0, v, nullptr, nullptr, n, c->partials, go, s)) != hipSuccess) break;
+            if ((e = launch_bicg_s(n, r, v, sv, c->scal, c->partials, s)) != hipSuccess) break;
+            if (pc && (e = pc(sv, sz)) != hipSuccess) break;
+            if ((e = mv(pc ? sz : sv, t)) != hipSuccess) break;
+            if ((e = launch_dot2_partial(t, t, t, sv, n, wparts, go, s)) != hipSuccess) break;
+            e = launch_bicg_xr_dot(n, x, r, pc ? py : p, pc ? sz : sv, sv, t, r0, wparts, c->partials, c->scal,
+                                   test, s);
+        }
+        return e;
+    };
+    // the replayed group; destroyed on every way out of this call
+    hipGraphExec_t group = nullptr;
+    struct GroupGuard {
+        hipGraphExec_t &x;
+        ~GroupGuard() {
+            if (x) (void)hipGraphExecDestroy(x);
+        }
+    } guard{group};
     int restarts = 0;
     // rho_old = rho; rho = r0.r, together with r.r for the loop test
     HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s, nullptr, test));
@@ -832,20 +858,23 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
             if (restarts++ == 0 && (st = set_scal(c, kScalIter, 0.0))) return st;
             if ((st = set_scal(c, kScalGo, 1.0))) return st;
         }
-        for (int g = 0; g < kGroup; ++g) {
-            HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
-            if (pc) HIPCHK(c, pc(p, py));
-            HIPCHK(c, mv(pc ? py : p, v));
-            // alpha and w are formed by the kernels that use them, each block
-            // from the partials (no one-block launch in between)
-            HIPCHK(c, launch_dot2_partial(r0, v, nullptr, nullptr, n, c->partials, go, s));
-            HIPCHK(c, launch_bicg_s(n, r, v, sv, c->scal, c->partials, s));
-            if (pc) HIPCHK(c, pc(sv, sz));
-            HIPCHK(c, mv(pc ? sz : sv, t));
-            HIPCHK(c, launch_dot2_partial(t, t, t, sv, n, wparts, go, s));
-            HIPCHK(c, launch_bicg_xr_dot(n, x, r, pc ? py : p, pc ? sz : sv, sv, t, r0, wparts, c->partials, c->scal,
-                                         test, s));
+        if (!group) {
+            // the group is captured once per call and replayed: its kernels
+            // take only the call's fixed buffers and read every scalar from
+            // the device, so each replay is one eager group
+            HIPCHK(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            const hipError_t eq = enqueue_group();
+            hipGraph_t g = nullptr;
+            const hipError_t ec = hipStreamEndCapture(s, &g);
+            if (eq != hipSuccess || ec != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                HIPCHK(c, eq != hipSuccess ? eq : ec);
+            }
+            const hipError_t ei = hipGraphInstantiate(&group, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            HIPCHK(c, ei);
         }
+        HIPCHK(c, hipGraphLaunch(group, s));
         if ((st = read_scal(c, h))) return st;
     }
     const int i = (int)h.v[kScalIter];
