@@ -818,21 +818,26 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     const BicgTest test{tol * tol * rhs_sq, (double)maxit, DBL_EPSILON * DBL_EPSILON};
     const double *go = c->scal + kScalGo;
     constexpr int kGroup = 8;
+    // The loop body rotated to start after the direction update: v = A p,
+    // s, t = A s, the last update, then the next iteration's top (rho, the
+    // loop test) with its p update.  alpha, w and rho are formed by the
+    // kernels that use them, each block from the partials (no one-block
+    // launch in between).  The p update after the first top and after a
+    // restart runs on its own (launch_bicg_p).
     const auto enqueue_group = [&]() -> hipError_t {
         hipError_t e = hipSuccess;
         for (int g = 0; g < kGroup && e == hipSuccess; ++g) {
-            if ((e = launch_bicg_p(n, r, p, v, c->scal, s)) != hipSuccess) break;
             if (pc && (e = pc(p, py)) != hipSuccess) break;
             if ((e = mv(pc ? py : p, v)) != hipSuccess) break;
-            // alpha and w are formed by the kernels that use them, each block
-            // from the partials (no one-block launch in between)
             if ((e = launch_dot2_partial(r0, v, nullptr, nullptr, n, c->partials, go, s)) != hipSuccess) break;
-            if ((e = launch_bicg_s(n, r, v, sv, c->scal, c->partials, s)) != hipSuccess) break;
+            if ((e = launch_bicg_s(n, r, v, sv, c->scal, c->partials, g & 1, s)) != hipSuccess) break;
             if (pc && (e = pc(sv, sz)) != hipSuccess) break;
             if ((e = mv(pc ? sz : sv, t)) != hipSuccess) break;
             if ((e = launch_dot2_partial(t, t, t, sv, n, wparts, go, s)) != hipSuccess) break;
-            e = launch_bicg_xr_dot(n, x, r, pc ? py : p, pc ? sz : sv, sv, t, r0, wparts, c->partials, c->scal,
-                                   test, s);
+            if ((e = launch_bicg_xr_dot(n, x, r, pc ? py : p, pc ? sz : sv, sv, t, r0, wparts, c->partials,
+                                        c->scal, s)) != hipSuccess)
+                break;
+            e = launch_bicg_p_top(n, r, p, v, c->scal, c->partials, g & 1, test, s);
         }
         return e;
     };
@@ -845,6 +850,7 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
         }
     } guard{group};
     int restarts = 0;
+    bool owe_p = true;  // the first top (below) and a restarted one still owe their p update
     // rho_old = rho; rho = r0.r, together with r.r for the loop test
     HIPCHK(c, launch_dot2(r0, r, r, r, n, c->partials, c->scal, kOpRho, s, nullptr, test));
     if ((st = read_scal(c, h))) return st;
@@ -857,6 +863,12 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
             HIPCHK(c, launch_dot2(r, r, nullptr, nullptr, n, c->partials, c->scal, kOpRestart, s));
             if (restarts++ == 0 && (st = set_scal(c, kScalIter, 0.0))) return st;
             if ((st = set_scal(c, kScalGo, 1.0))) return st;
+            owe_p = true;
+        }
+        if (owe_p) {
+            // the p update of the top just evaluated, on its own
+            HIPCHK(c, launch_bicg_p(n, r, p, v, c->scal, s));
+            owe_p = false;
         }
         if (!group) {
             // the group is captured once per call and replayed: its kernels

@@ -676,8 +676,8 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_partial(const double *__rest
 
 // The last update of a BiCGSTAB iteration fused with the reductions on
 // either side of it.  Each block first combines the (t.t, t.s) partials
-// (wparts, nparts of them) into w = t.s / t.t or 0, as k_dot2_final's
-// kOpOmega does (block 0 also stores it); then x += alpha px + w sx;
+// (wparts, nparts of them) in k_dot2_final's order into w = t.s / t.t, or 0
+// when t.t is not positive (block 0 also stores it); then x += alpha px + w sx;
 // r = s - w t; and partials of (r0.r, r.r), with the elements of
 // k_dot2_partial's grid-stride loop in its order (the same grid), so the
 // partials are those of k_dot2_partial(r0, r, r, r).  wparts and partials
@@ -695,7 +695,10 @@ __global__ __launch_bounds__(kRedBlock) void k_bicg_xr_dot(long n, double *__res
     double tt, ts;
     final_sums2(wparts, nparts, tt, ts);
     const double w = (tt > 0) ? ts / tt : 0.0;
-    if (blockIdx.x == 0 && threadIdx.x == 0) scal[kScalW] = w;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[kScalW] = w;
+        scal[kScalIter] += 1.0;  // the iteration ends here; no block reads the count
+    }
     const double alpha = scal[kScalAlpha];
     double a = 0.0, b = 0.0;
 #pragma unroll 4
@@ -721,12 +724,10 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restri
             scal[kScalDotA] = ab;
             scal[kScalDotB] = cd;
             break;
-        case kOpRho:  // (r0.r, r.r): rho_old = rho; rho = r0.r; then the loop test
-        case kOpRhoLoop: {
+        case kOpRho: {  // (r0.r, r.r): rho_old = rho; rho = r0.r; then the loop test
             scal[kScalRhoOld] = scal[kScalRho];
             scal[kScalRho] = ab;
             scal[kScalRR] = cd;
-            if (op == kOpRhoLoop) scal[kScalIter] += 1.0;
             // while (r.r > tol2 && i < maxit), then the restart test of the
             // loop body, in the host loop's order
             const bool more = cd > test.tol2 && scal[kScalIter] < test.maxit;
@@ -738,17 +739,48 @@ __global__ __launch_bounds__(kRedBlock) void k_dot2_final(const double *__restri
             scal[kScalR0Sq] = ab;
             scal[kScalRR] = ab;
             break;
-        case kOpAlpha:  // (r0.v): alpha = rho / r0.v
-            scal[kScalAlpha] = scal[kScalRho] / ab;
-            break;
-        case kOpOmega:  // (t.t, t.s): w = t.s / t.t or 0
-            scal[kScalW] = (ab > 0) ? cd / ab : 0.0;
-            break;
     }
 }
 
 // BiCGSTAB vector updates (Eigen's internal::bicgstab recurrence); each
 // runs only while the device loop flag scal[kScalGo] is 1.
+
+// The top of a BiCGSTAB iteration, in every block: rho_old = rho, rho = r0.r
+// and r.r from the (r0.r, r.r) partials, the loop test as k_dot2_final's
+// kOpRho applies it (the iteration was counted by k_bicg_xr_dot), then
+// p = r + beta (p - w v) while the loop goes on.  Block 0 writes kScalGo, which the other blocks read at their
+// start: a block that sees the new value sees the value it would compute
+// itself, so either way it updates p only if the loop goes on.  The flag is
+// read once per block (one thread, then shared) so that the whole block
+// leaves or stays together.
+__global__ __launch_bounds__(kRedBlock) void k_bicg_p_top(long n, const double *__restrict__ r,
+                                                         double *__restrict__ p, const double *__restrict__ v,
+                                                         double *__restrict__ scal,
+                                                         const double *__restrict__ parts, int nparts, int odd,
+                                                         BicgTest test) {
+    __shared__ double go_in;
+    if (threadIdx.x == 0) go_in = scal[kScalGo];
+    __syncthreads();
+    if (go_in != 1.0) return;  // a stopped BiCGSTAB loop
+    double rho, rr;
+    final_sums2(parts, nparts, rho, rr);
+    const double rho_old = scal[odd ? kScalRhoAlt : kScalRho];
+    // while (r.r > tol2 && i < maxit), then the restart test of the loop body
+    const bool more = rr > test.tol2 && scal[kScalIter] < test.maxit;
+    const double go = !more ? 0.0 : (fabs(rho) < test.eps2 * scal[kScalR0Sq]) ? 2.0 : 1.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[kScalRhoOld] = rho_old;
+        scal[odd ? kScalRho : kScalRhoAlt] = rho;
+        scal[kScalRR] = rr;
+        scal[kScalGo] = go;
+    }
+    if (go != 1.0) return;
+    const double beta = (rho / rho_old) * (scal[kScalAlpha] / scal[kScalW]);
+    const double w = scal[kScalW];
+#pragma unroll 4
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = r[i] + beta * (p[i] - w * v[i]);
+}
 __global__ __launch_bounds__(256) void k_bicg_p(long n, const double *__restrict__ r, double *__restrict__ p,
                                                 const double *__restrict__ v, const double *__restrict__ scal) {
     const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -759,16 +791,17 @@ __global__ __launch_bounds__(256) void k_bicg_p(long n, const double *__restrict
 }
 
 // s = r - alpha v, each block first forming alpha = rho / (r0.v) from the
-// nparts partials of r0.v as k_dot2_final's kOpAlpha does (block 0 also
-// stores it; no block of this kernel reads the stored value).
+// nparts partials of r0.v in k_dot2_final's order (block 0 also stores it;
+// no block of this kernel reads the stored value).  rho is in kScalRho for
+// even iterations of a group, kScalRhoAlt for odd ones (k_bicg_p_top).
 __global__ __launch_bounds__(kRedBlock) void k_bicg_s(long n, const double *__restrict__ r,
                                                      const double *__restrict__ v, double *__restrict__ s,
                                                      double *__restrict__ scal,
-                                                     const double *__restrict__ aparts, int nparts) {
+                                                     const double *__restrict__ aparts, int nparts, int odd) {
     if (scal[kScalGo] != 1.0) return;  // a stopped BiCGSTAB loop
     double rv, unused;
     final_sums2(aparts, nparts, rv, unused);
-    const double alpha = scal[kScalRho] / rv;
+    const double alpha = scal[odd ? kScalRhoAlt : kScalRho] / rv;
     if (blockIdx.x == 0 && threadIdx.x == 0) scal[kScalAlpha] = alpha;
 #pragma unroll 4
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -1241,14 +1274,16 @@ static unsigned red_grid(long n) {
 }
 hipError_t launch_bicg_xr_dot(long n, double *x, double *r, const double *px, const double *sx, const double *sv,
                               const double *t, const double *r0, const double *wparts, double *partials,
-                              double *scal, const BicgTest &test, hipStream_t s) {
+                              double *scal, hipStream_t s) {
     const unsigned g = red_grid(n);
     hipLaunchKernelGGL(k_bicg_xr_dot, dim3(g), dim3(kRedBlock), 0, s, n, x, r, px, sx, sv, t, r0, scal, wparts,
                        (int)g, partials);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_dot2_final, dim3(1), dim3(kRedBlock), 0, s, partials, (int)g, scal, kOpRhoLoop,
-                       scal + kScalGo, test);
+    return hipGetLastError();
+}
+hipError_t launch_bicg_p_top(long n, const double *r, double *p, const double *v, double *scal,
+                             const double *parts, int odd, const BicgTest &test, hipStream_t s) {
+    hipLaunchKernelGGL(k_bicg_p_top, dim3(red_grid(n)), dim3(kRedBlock), 0, s, n, r, p, v, scal, parts,
+                       (int)red_grid(n), odd, test);
     return hipGetLastError();
 }
 hipError_t launch_dot2_partial(const double *a, const double *b, const double *c, const double *d, long n,
@@ -1271,9 +1306,9 @@ hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, co
     return hipGetLastError();
 }
 hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, double *scal, const double *aparts,
-                         hipStream_t s) {
+                         int odd, hipStream_t s) {
     hipLaunchKernelGGL(k_bicg_s, dim3(red_grid(n)), dim3(kRedBlock), 0, s, n, r, v, sv, scal, aparts,
-                       (int)red_grid(n));
+                       (int)red_grid(n), odd);
     return hipGetLastError();
 }
 hipError_t launch_probe_seed(int nc, const int *color, int k, int q, double *x, hipStream_t s) {

@@ -43,12 +43,14 @@ enum : int {
     kScalRes = 14,     // last density residual norm evaluated on the device
     kScalGo = 15,      // BiCGSTAB loop: 1 run the next iteration, 0 stop, 2 restart first (host)
     kScalIter = 16,    // BiCGSTAB iterations taken in the current chunk
+    kScalRhoAlt = 17,  // BiCGSTAB: rho of odd iterations of a group (kScalRho: even)
     kScalCount = 20,
 };
 
-// Epilogues of k_dot2.  kOpRho / kOpRhoLoop also evaluate the BiCGSTAB loop
-// test into kScalGo (kOpRhoLoop after counting the iteration that ended).
-enum : int { kOpStore = 0, kOpRho = 1, kOpRestart = 2, kOpAlpha = 3, kOpOmega = 4, kOpRhoLoop = 5 };
+// Epilogues of k_dot2.  kOpRho also evaluates the BiCGSTAB loop test into
+// kScalGo.  (alpha, w and the later tops are formed by the vector-update
+// kernels that use them.)
+enum : int { kOpStore = 0, kOpRho = 1, kOpRestart = 2 };
 
 // The loop test of a BiCGSTAB chunk, evaluated on the device.
 struct BicgTest {
@@ -235,18 +237,28 @@ hipError_t launch_dot2(const double *a, const double *b, const double *c, const 
 hipError_t launch_dot2_partial(const double *a, const double *b, const double *c, const double *d, long n,
                                double *partials, const double *gate, hipStream_t s);
 // The iteration's last update (x += alpha px + w sx; r = s - w t) fused with
-// w = t.s / t.t from the (t.t, t.s) partials wparts (kOpOmega), and with the
-// (r0.r, r.r) reduction and the loop test (kOpRhoLoop) after it; gated on
-// scal[kScalGo].  Partials identical to launch_dot2(r0, r, r, r, ...).
+// w = t.s / t.t (or 0) from the (t.t, t.s) partials wparts before it and
+// the (r0.r, r.r) partials after it (identical to launch_dot2(r0, r, r, r,
+// ...)'s); counts the iteration in scal[kScalIter].  Gated on scal[kScalGo].
 // wparts and partials must not overlap.
 hipError_t launch_bicg_xr_dot(long n, double *x, double *r, const double *px, const double *sx, const double *sv,
                               const double *t, const double *r0, const double *wparts, double *partials,
-                              double *scal, const BicgTest &test, hipStream_t s);
+                              double *scal, hipStream_t s);
 hipError_t launch_bicg_p(long n, const double *r, double *p, const double *v, const double *scal, hipStream_t s);
+// The top of a BiCGSTAB iteration and its direction update in one launch:
+// rho = r0.r and r.r from the partials of launch_bicg_xr_dot, the
+// loop test into scal[kScalGo] (as kOpRho, with the iteration already
+// counted), then, if the loop goes on, p = r + beta (p - w v).  Iteration
+// `odd` of a group reads rho_old from kScalRho (odd: kScalRhoAlt) and
+// writes the new rho to the other slot, so no block reads a slot that
+// block 0 writes; kScalRhoOld gets rho_old for a host restart.
+hipError_t launch_bicg_p_top(long n, const double *r, double *p, const double *v, double *scal,
+                             const double *parts, int odd, const BicgTest &test, hipStream_t s);
 // s = r - alpha v with alpha = rho / (r0.v) formed from the r0.v partials
-// aparts (kOpAlpha; stored to scal too).
+// aparts (stored to scal too); rho from kScalRho, or kScalRhoAlt in the odd
+// iterations of a group (see launch_bicg_p_top).
 hipError_t launch_bicg_s(long n, const double *r, const double *v, double *sv, double *scal, const double *aparts,
-                         hipStream_t s);
+                         int odd, hipStream_t s);
 // Block-Jacobi preconditioner: probing seeds / extraction by colour,
 // P_c = (D_c^T)^{-1} (16 per cell, row-major), out_c = P_c in_c.
 hipError_t launch_probe_seed(int nc, const int *color, int k, int q, double *x, hipStream_t s);
