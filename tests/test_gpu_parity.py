@@ -210,6 +210,36 @@ def test_bicgstab_converges_and_agrees_small_mesh():
         assert np.linalg.norm(lg - lo) <= 1e-2 * np.linalg.norm(lo)
 
 
+def test_bicgstab_graph_replays_are_stable():
+    # Each call captures the eight-iteration group once and replays it
+    # (here 12 times per 100-iteration chunk, with the true-residual check
+    # and the next chunk's set-up run eagerly between chunks).  Three calls,
+    # with other device work in between, must agree bit for bit, in every
+    # operator form that the loop drives.
+    flow = H.subsonic_small(interp_mode=1)
+    U = H.developed_state(H.small_mesh(), flow, steps=30)
+    s = capi.Solver(H.small_mesh(), H.capi_params(flow))
+    s.set_state(U)
+    s.residual()
+    x = np.random.default_rng(5).standard_normal((4, s.nc))
+    solves = [
+        lambda: s.bicgstab(0.1, tol=1e-30, chunk=100, max_total=300, check=False),
+        lambda: s.bicgstab_block_jacobi(0.1, tol=1e-30, chunk=100, max_total=300, check=False),
+        lambda: s.bicgstab_exact(0.1, tol=1e-30, chunk=100, max_total=200, check=False),
+    ]
+    for solve in solves:
+        runs = []
+        for _ in range(3):
+            runs.append(solve())
+            s.apply_jt(x)  # eager work on the same stream between the calls
+        (l0, it0, rr0, st0) = runs[0]
+        assert it0 > 0
+        for lam, it, rr, st in runs[1:]:
+            assert it == it0 and st == st0
+            assert np.array_equal([rr], [rr0], equal_nan=True)
+            assert np.array_equal(lam, l0, equal_nan=True)
+
+
 def test_shape_gradient_matches():
     s, m, p = pair(H.shipped_mesh(), H.SHIPPED, H.shipped_flow())
     m.residual(p)
