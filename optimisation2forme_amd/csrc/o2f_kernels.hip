@@ -758,8 +758,10 @@ __global__ __launch_bounds__(kRedBlock) void k_bicg_p_top(long n, const double *
                                                          double *__restrict__ scal,
                                                          const double *__restrict__ parts, int nparts, int odd,
                                                          BicgTest test) {
+    // the flag is the one slot that this launch both reads and writes:
+    // relaxed atomic accesses (either value read leads to the same outcome)
     __shared__ double go_in;
-    if (threadIdx.x == 0) go_in = scal[kScalGo];
+    if (threadIdx.x == 0) go_in = __hip_atomic_load(scal + kScalGo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __syncthreads();
     if (go_in != 1.0) return;  // a stopped BiCGSTAB loop
     double rho, rr;
@@ -772,7 +774,7 @@ __global__ __launch_bounds__(kRedBlock) void k_bicg_p_top(long n, const double *
         scal[kScalRhoOld] = rho_old;
         scal[odd ? kScalRho : kScalRhoAlt] = rho;
         scal[kScalRR] = rr;
-        scal[kScalGo] = go;
+        __hip_atomic_store(scal + kScalGo, go, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (go != 1.0) return;
     const double beta = (rho / rho_old) * (scal[kScalAlpha] / scal[kScalW]);
