@@ -172,6 +172,14 @@ int o2f_hllc_flux_batch(const double *qL, const double *qR, const double *normal
 int o2f_compute_dt(o2f_ctx *ctx, double *dt);
 /* step_SSP_RK3 with a given dt (src/simulation.cpp:130-171). */
 int o2f_rk3_step(o2f_ctx *ctx, double dt);
+/* The decomposition of step_SSP_RK3: the U_old <- U copy that opens it
+ * (src/simulation.cpp:130-133), then Simulator::rk3_substep(c0, c1, dt)
+ * (:145-171): computeAdvectionTerm + computeDiffusionTerm on the current U
+ * (gradients, face states and R left as those operators leave them), then
+ * U = c0 U_old + c1 (U + dt R).  o2f_rk3_save_old followed by substeps
+ * (0, 1), (3/4, 1/4), (1/3, 2/3) is bitwise o2f_rk3_step. */
+int o2f_rk3_save_old(o2f_ctx *ctx);
+int o2f_rk3_substep(o2f_ctx *ctx, double c0, double c1, double dt);
 /* nsteps x { dt reduction; SSP-RK3 step; t += dt }, device resident, no host
  * synchronisation inside.  dts (nsteps) may be NULL; t_inout (may be NULL)
  * carries the simulated time. */

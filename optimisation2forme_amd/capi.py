@@ -42,6 +42,7 @@ EXPORTS = [
     "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_time_kernel", "o2f_naca_displacement",
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
+    "o2f_rk3_save_old", "o2f_rk3_substep",
 ]
 
 
@@ -104,6 +105,8 @@ def lib():
             "o2f_hllc_flux_batch": (C.c_int, [D, D, D, C.c_double, D, C.c_int]),
             "o2f_compute_dt": (C.c_int, [vp, D]),
             "o2f_rk3_step": (C.c_int, [vp, C.c_double]),
+            "o2f_rk3_save_old": (C.c_int, [vp]),
+            "o2f_rk3_substep": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
             "o2f_run_steps": (C.c_int, [vp, C.c_int, D, D]),
             "o2f_time_loop": (C.c_int, [vp, C.c_double, C.c_int, I, D]),
             "o2f_residual_norm": (C.c_int, [vp, D]),
@@ -337,6 +340,12 @@ class Solver:
 
     def rk3_step(self, dt):
         self._check(lib().o2f_rk3_step(self.h, dt))
+
+    def rk3_save_old(self):
+        self._check(lib().o2f_rk3_save_old(self.h))
+
+    def rk3_substep(self, c0, c1, dt):
+        self._check(lib().o2f_rk3_substep(self.h, c0, c1, dt))
 
     def run_steps(self, n, t=0.0):
         dts = np.zeros(max(n, 1))

@@ -591,6 +591,28 @@ int o2f_rk3_step(o2f_ctx *c, double dt) {
     return O2F_OK;
 }
 
+int o2f_rk3_save_old(o2f_ctx *c) {
+    if (!c) return O2F_ERR_ARG;
+    HIPCHK(c, hipMemcpyAsync(c->Uold, c->U, 4 * sizeof(double) * c->hm.nc, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
+// One stage on its own: the residual (gradients, face states and R stored
+// as computeAdvectionTerm / computeDiffusionTerm leave them) and the update
+// U = c0 U_old + c1 (U + dt R) in the same kernels as the fused step, so
+// three substeps after o2f_rk3_save_old are bitwise o2f_rk3_step.
+int o2f_rk3_substep(o2f_ctx *c, double c0, double c1, double dt) {
+    if (!c) return O2F_ERR_ARG;
+    int st = check_bc(c);
+    if (st) return st;
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalDt, &dt, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    c->speed_valid = false;
+    if ((st = run_stage(c, 0, kFlagRK | kFlagStoreR, c->scal + kScalDt, c0, c1, true))) return st;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return O2F_OK;
+}
+
 int o2f_run_steps(o2f_ctx *c, int nsteps, double *dts, double *t_inout) {
     if (!c || nsteps < 0) return O2F_ERR_ARG;
     int st = check_bc(c);

@@ -6,6 +6,7 @@
 #ifndef O2F_HOST_PARAMS_HPP
 #define O2F_HOST_PARAMS_HPP
 
+#include <ostream>
 #include <string>
 
 #include "o2f.h"
@@ -68,6 +69,10 @@ struct SimulationParams {
     std::string adjoint_rhs = "reference";        // "reference" | "exact" (with the exact operator)
     std::string adjoint_preconditioner = "none";  // "none" | "block_jacobi"
     std::string shape_gradient = "finite_difference";  // | "tangent" | "tangent_full" (forward mode)
+    // 1: steps between two host events run as one device-gated batch; 0: the
+    // reference's loop body on the host, one dt reduction and one
+    // step_SSP_RK3 (three rk3_substep calls) per step
+    int device_time_loop = 1;
 
     // c_v = c_p / gamma, R = c_p - c_v (src/params.cpp:5-8)
     void finalize();
@@ -76,6 +81,8 @@ struct SimulationParams {
 // false when the file cannot be opened (src/params.cpp:18-83)
 bool load_config(const std::string &filename, SimulationParams &p);
 void print_params(const SimulationParams &p);
+// every key with its value, `key=value` one per line (navier_stokes --print-config)
+void dump_config(const SimulationParams &p, std::ostream &out);
 // The members of SimulationParams the device path reads.
 o2f_params to_o2f_params(const SimulationParams &p);
 

@@ -19,9 +19,13 @@ class Simulator {
 
     void initialize_fields();
     void time_loop();
+    void host_time_loop();
     void output_step(int iter, double t, double dt);
-    // the three Shu-Osher stages of src/simulation.cpp:130-171, fused on the device
+    // src/simulation.cpp:130-171: U_old <- U, then the three Shu-Osher
+    // substeps, each a residual and the update U = c0 U_old + c1 (U + dt R)
+    // on the device-resident state
     void step_SSP_RK3(double dt);
+    void rk3_substep(double c0, double c1, double dt);
 };
 
 }  // namespace navier_stokes

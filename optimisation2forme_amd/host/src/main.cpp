@@ -5,6 +5,7 @@
 
 #include "mesh.hpp"
 #include "params.hpp"
+#include "shape_optimization.hpp"
 #include "simulation.hpp"
 
 int main(int argc, char *argv[]) {
@@ -14,12 +15,32 @@ int main(int argc, char *argv[]) {
     for (int i = 1; i < argc; ++i) {
         const std::string arg = argv[i];
         if (arg == "-h" || arg == "--help") {
-            std::cout << "Usage: " << argv[0] << " [--print-config] [config_file]\n"
+            std::cout << "Usage: " << argv[0] << " [--print-config] [--print-shape-params file] [config_file]\n"
                       << "If no config_file is given, uses \"config.txt\" in CWD.\n";
             return 0;
         }
-        if (arg == "--print-config") dump = true;
-        else cfg_file = arg;
+        if (arg == "--print-config") {
+            dump = true;
+        } else if (arg == "--print-shape-params" && i + 1 < argc) {
+            // the parsed shape-parameter file: `name=value` in file order,
+            // then `optimised=` and the "*=" names (used by the tests)
+            std::vector<std::string> names_all, names;
+            std::vector<double> values;
+            try {
+                loadParamsGeneric(argv[++i], names_all, values, names);
+            } catch (const std::exception &e) {
+                std::cerr << e.what() << "\n";
+                return 1;
+            }
+            std::cout.precision(17);
+            for (size_t k = 0; k < names_all.size(); ++k) std::cout << names_all[k] << '=' << values[k] << '\n';
+            std::cout << "optimised=";
+            for (size_t k = 0; k < names.size(); ++k) std::cout << (k ? "," : "") << names[k];
+            std::cout << '\n';
+            return 0;
+        } else {
+            cfg_file = arg;
+        }
     }
     SimulationParams params;
     if (!load_config(cfg_file, params)) {
@@ -28,19 +49,7 @@ int main(int argc, char *argv[]) {
     }
     params.finalize();
     if (dump) {  // parsed values, one per line (used by the tests)
-        std::cout.precision(17);
-        std::cout << "mesh_name=" << params.mesh_name << "\ncfl_number=" << params.cfl_number
-                  << "\nfinal_time=" << params.final_time << "\nviscosity=" << params.viscosity
-                  << "\nPr=" << params.Pr << "\ninflow_velocity=" << params.inflow_velocity
-                  << "\nsave_interval=" << params.save_interval << "\nreload_config=" << params.reload_config
-                  << "\nstate_load_path=" << params.state_load_path << "\nstate_save_path=" << params.state_save_path
-                  << "\nshape_opt_enable=" << params.shape_opt_enable << "\nshape_param_file=" << params.shape_param_file
-                  << "\nlimiter=" << params.limiter << "\nvenkat_K=" << params.venkat_K
-                  << "\ninterp_mode=" << params.interp_mode << "\nheat_flux=" << params.heat_flux
-                  << "\nadjoint_operator=" << params.adjoint_operator << "\nadjoint_rhs=" << params.adjoint_rhs
-                  << "\nadjoint_preconditioner=" << params.adjoint_preconditioner
-                  << "\nshape_gradient=" << params.shape_gradient << "\ncv=" << params.cv
-                  << "\nR=" << params.R << "\n";
+        dump_config(params, std::cout);
         return 0;
     }
     if (params.verbosity > 0) print_params(params);

@@ -1,8 +1,26 @@
+// config.txt reader.  The file format is the reference driver's
+// (src/params.cpp:18-83), because scripts write and rewrite it
+// (run_opti.sh:160-190): one `key = value` assignment per line; blanks
+// (space, tab, CR, LF) around the key and the value do not count; a line
+// whose first non-blank character is '#', or that has no '=', is skipped;
+// keys this build does not know are ignored; a key assigned twice keeps its
+// last value.  Numbers are read the way `std::istream >>` reads them (the
+// longest valid prefix; a value that does not start with a number reads
+// as 0).
+//
+// Design: every key is one row of a table that binds its name to a member of
+// SimulationParams.  The same table parses a file and prints the parsed
+// values (`navier_stokes --print-config`), so a key added here is known to
+// both.
 #include "params.hpp"
 
+#include <algorithm>
 #include <fstream>
+#include <iterator>
 #include <iostream>
 #include <sstream>
+#include <string_view>
+#include <variant>
 
 namespace navier_stokes {
 
@@ -12,76 +30,144 @@ void SimulationParams::finalize() {
 }
 
 namespace {
-std::string trim(const std::string &s) {
-    const auto first = s.find_first_not_of(" \t\r\n");
-    if (first == std::string::npos) return "";
-    const auto last = s.find_last_not_of(" \t\r\n");
-    return s.substr(first, last - first + 1);
+
+using P = SimulationParams;
+using Member = std::variant<std::string P::*, double P::*, int P::*, bool P::*>;
+
+struct Key {
+    const char *name;
+    Member member;
+};
+
+// Sorted by name (byte order), so a key is found by binary search.
+constexpr Key kKeys[] = {
+    {"Lx", &P::Lx},
+    {"Ly", &P::Ly},
+    {"Nx", &P::Nx},
+    {"Ny", &P::Ny},
+    {"Pr", &P::Pr},
+    {"R", &P::R},
+    {"adjoint_chunk", &P::adjoint_chunk},
+    {"adjoint_eps", &P::adjoint_eps},
+    {"adjoint_max_iters", &P::adjoint_max_iters},
+    {"adjoint_operator", &P::adjoint_operator},
+    {"adjoint_preconditioner", &P::adjoint_preconditioner},
+    {"adjoint_rhs", &P::adjoint_rhs},
+    {"adjoint_tol", &P::adjoint_tol},
+    {"advection_scheme", &P::advection_scheme},
+    {"boundary_conditions", &P::boundary_conditions},
+    {"cfl_number", &P::cfl_number},
+    {"cp", &P::cp},
+    {"cv", &P::cv},
+    {"device_time_loop", &P::device_time_loop},
+    {"final_time", &P::final_time},
+    {"gamma", &P::gamma},
+    {"heat_flux", &P::heat_flux},
+    {"inflow_velocity", &P::inflow_velocity},
+    {"initial_condition", &P::initial_condition},
+    {"interp_mode", &P::interp_mode},
+    {"limiter", &P::limiter},
+    {"max_steps", &P::max_steps},
+    {"mesh_name", &P::mesh_name},
+    {"noise_level", &P::noise_level},
+    {"obstacle_center_x", &P::obstacle_center_x},
+    {"obstacle_center_y", &P::obstacle_center_y},
+    {"obstacle_size", &P::obstacle_size},
+    {"obstacle_type", &P::obstacle_type},
+    {"output_directory", &P::output_directory},
+    {"output_quantity", &P::output_quantity},
+    {"p_ref", &P::p_ref},
+    {"reload_config", &P::reload_config},
+    {"rho_ref", &P::rho_ref},
+    {"save_interval", &P::save_interval},
+    {"shape_gradient", &P::shape_gradient},
+    {"shape_log_file", &P::shape_log_file},
+    {"shape_opt_enable", &P::shape_opt_enable},
+    {"shape_param_file", &P::shape_param_file},
+    {"state_load_path", &P::state_load_path},
+    {"state_save_path", &P::state_save_path},
+    {"venkat_K", &P::venkat_K},
+    {"verbosity", &P::verbosity},
+    {"viscosity", &P::viscosity},
+};
+
+constexpr bool sorted_by_name() {
+    for (size_t i = 1; i < std::size(kKeys); ++i)
+        if (!(std::string_view(kKeys[i - 1].name) < std::string_view(kKeys[i].name))) return false;
+    return true;
 }
+static_assert(sorted_by_name(), "kKeys must stay sorted by name");
+
+const Key *find_key(std::string_view name) {
+    const Key *end = std::end(kKeys);
+    const Key *k = std::lower_bound(std::begin(kKeys), end, name,
+                                    [](const Key &a, std::string_view n) { return std::string_view(a.name) < n; });
+    return (k != end && name == k->name) ? k : nullptr;
+}
+
+constexpr std::string_view kBlanks = " \t\r\n";
+
+std::string_view strip(std::string_view s) {
+    const size_t a = s.find_first_not_of(kBlanks);
+    if (a == std::string_view::npos) return {};
+    return s.substr(a, s.find_last_not_of(kBlanks) + 1 - a);
+}
+
+// operator>> semantics: the value read, 0 when nothing could be read.
+template <class T>
+T number(std::string_view text) {
+    T v{};
+    std::istringstream in{std::string(text)};
+    in >> v;
+    return v;
+}
+
+void assign(P &p, const Key &k, std::string_view value) {
+    std::visit(
+        [&](auto member) {
+            using T = std::remove_reference_t<decltype(p.*member)>;
+            if constexpr (std::is_same_v<T, std::string>)
+                p.*member = std::string(value);
+            else
+                p.*member = number<T>(value);
+        },
+        k.member);
+}
+
+// Splits one line into (key, value); false for blank, comment and
+// assignment-free lines.
+bool split_assignment(std::string_view line, std::string_view &key, std::string_view &value) {
+    line = strip(line);
+    if (line.empty() || line.front() == '#') return false;
+    const size_t eq = line.find('=');
+    if (eq == std::string_view::npos) return false;
+    key = strip(line.substr(0, eq));
+    value = strip(line.substr(eq + 1));
+    return true;
+}
+
 }  // namespace
 
 bool load_config(const std::string &filename, SimulationParams &p) {
     std::ifstream in(filename);
     if (!in) return false;
-    std::string line;
-    while (std::getline(in, line)) {
-        const std::string t = trim(line);
-        if (t.empty() || t[0] == '#') continue;
-        const auto eq = t.find('=');
-        if (eq == std::string::npos) continue;
-        const std::string key = trim(t.substr(0, eq));
-        const std::string val = trim(t.substr(eq + 1));
-        std::istringstream ss(val);
-        if (key == "mesh_name") p.mesh_name = val;
-        else if (key == "Lx") ss >> p.Lx;
-        else if (key == "Ly") ss >> p.Ly;
-        else if (key == "Nx") ss >> p.Nx;
-        else if (key == "Ny") ss >> p.Ny;
-        else if (key == "cfl_number") ss >> p.cfl_number;
-        else if (key == "final_time") ss >> p.final_time;
-        else if (key == "gamma") ss >> p.gamma;
-        else if (key == "viscosity") ss >> p.viscosity;
-        else if (key == "Pr") ss >> p.Pr;
-        else if (key == "cp") ss >> p.cp;
-        else if (key == "cv") ss >> p.cv;
-        else if (key == "R") ss >> p.R;
-        else if (key == "advection_scheme") p.advection_scheme = val;
-        else if (key == "boundary_conditions") p.boundary_conditions = val;
-        else if (key == "initial_condition") p.initial_condition = val;
-        else if (key == "noise_level") ss >> p.noise_level;
-        else if (key == "obstacle_type") p.obstacle_type = val;
-        else if (key == "obstacle_center_x") ss >> p.obstacle_center_x;
-        else if (key == "obstacle_center_y") ss >> p.obstacle_center_y;
-        else if (key == "obstacle_size") ss >> p.obstacle_size;
-        else if (key == "inflow_velocity") ss >> p.inflow_velocity;
-        else if (key == "rho_ref") ss >> p.rho_ref;
-        else if (key == "p_ref") ss >> p.p_ref;
-        else if (key == "output_directory") p.output_directory = val;
-        else if (key == "save_interval") ss >> p.save_interval;
-        else if (key == "state_save_path") p.state_save_path = val;
-        else if (key == "state_load_path") p.state_load_path = val;
-        else if (key == "output_quantity") p.output_quantity = val;
-        else if (key == "verbosity") ss >> p.verbosity;
-        else if (key == "reload_config") ss >> p.reload_config;
-        else if (key == "shape_opt_enable") ss >> p.shape_opt_enable;
-        else if (key == "shape_param_file") p.shape_param_file = val;
-        else if (key == "shape_log_file") p.shape_log_file = val;
-        else if (key == "limiter") p.limiter = val;
-        else if (key == "venkat_K") ss >> p.venkat_K;
-        else if (key == "interp_mode") p.interp_mode = val;
-        else if (key == "heat_flux") p.heat_flux = val;
-        else if (key == "adjoint_tol") ss >> p.adjoint_tol;
-        else if (key == "adjoint_eps") ss >> p.adjoint_eps;
-        else if (key == "adjoint_chunk") ss >> p.adjoint_chunk;
-        else if (key == "adjoint_max_iters") ss >> p.adjoint_max_iters;
-        else if (key == "max_steps") ss >> p.max_steps;
-        else if (key == "adjoint_operator") p.adjoint_operator = val;
-        else if (key == "adjoint_rhs") p.adjoint_rhs = val;
-        else if (key == "adjoint_preconditioner") p.adjoint_preconditioner = val;
-        else if (key == "shape_gradient") p.shape_gradient = val;
+    for (std::string line; std::getline(in, line);) {
+        std::string_view key, value;
+        if (!split_assignment(line, key, value)) continue;
+        if (const Key *k = find_key(key)) assign(p, *k, value);
     }
     p.finalize();
     return true;
+}
+
+void dump_config(const SimulationParams &p, std::ostream &out) {
+    const auto old = out.precision(17);
+    for (const Key &k : kKeys) {
+        out << k.name << '=';
+        std::visit([&](auto member) { out << p.*member; }, k.member);
+        out << '\n';
+    }
+    out.precision(old);
 }
 
 void print_params(const SimulationParams &p) {

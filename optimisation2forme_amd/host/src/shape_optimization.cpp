@@ -4,14 +4,12 @@
 
 #include <algorithm>
 #include <cctype>
+#include <cstdio>
 #include <filesystem>
 #include <fstream>
-#include <iomanip>
 #include <iostream>
 #include <sstream>
 #include <stdexcept>
-#include <unordered_map>
-#include <unordered_set>
 
 namespace navier_stokes {
 
@@ -29,17 +27,123 @@ void computeResidual(Mesh &mesh, const SimulationParams &params, std::vector<dou
     R_E.assign(R.begin() + 3 * nc, R.end());
 }
 
+// ---- the shape-parameter file -------------------------------------------
+//
+// Grammar (the reference's, src/shape_optimization.cpp:19-98, which
+// run_opti.sh:91-129 rewrites between runs): text after '#' is a comment;
+// each remaining non-blank line assigns one value, as `key *= value` (an
+// optimised parameter), `key = value` or `key value`; "*=" is looked for
+// before '='; the value is read by std::stod.  A key assigned again keeps
+// its first position, takes the new value, and is optimised iff its last
+// assignment used "*=".
 namespace {
-void trim(std::string &s) {
-    s.erase(s.begin(), std::find_if(s.begin(), s.end(), [](unsigned char c) { return !std::isspace(c); }));
-    s.erase(std::find_if(s.rbegin(), s.rend(), [](unsigned char c) { return !std::isspace(c); }).base(), s.end());
+
+struct ShapeParam {
+    std::string name;
+    double value;
+    bool optimised;
+};
+
+bool is_blank(char ch) { return std::isspace(static_cast<unsigned char>(ch)) != 0; }
+
+std::string strip_blanks(const std::string &s, size_t from = 0, size_t to = std::string::npos) {
+    to = std::min(to, s.size());
+    while (from < to && is_blank(s[from])) ++from;
+    while (to > from && is_blank(s[to - 1])) --to;
+    return s.substr(from, to - from);
 }
 
-double param_or(const std::vector<std::string> &names, const std::vector<double> &values, const std::string &key,
-                double def) {
-    auto it = std::find(names.begin(), names.end(), key);
-    return it == names.end() ? def : values[it - names.begin()];
+// key and value of one line; false when the line assigns nothing
+bool parse_param_line(std::string line, std::string &key, std::string &value, bool &optimised) {
+    line = line.substr(0, line.find('#'));
+    optimised = false;
+    size_t at = line.find("*="), width = 2;
+    if (at != std::string::npos) {
+        optimised = true;
+    } else {
+        at = line.find('=');
+        width = 1;
+    }
+    if (at != std::string::npos) {
+        key = strip_blanks(line, 0, at);
+        value = strip_blanks(line, at + width);
+    } else {  // `key value`: the first two blank-separated words
+        std::istringstream words(line);
+        key.clear();
+        value.clear();
+        words >> key >> value;
+    }
+    return !key.empty() && !value.empty();
 }
+
+std::vector<ShapeParam> read_shape_params(const std::string &path) {
+    std::vector<ShapeParam> params;
+    if (path.empty() || path == "none") return params;
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("Impossible d'ouvrir le fichier de paramètres: " + path);
+    std::string key, value;
+    bool optimised;
+    for (std::string line; std::getline(in, line);) {
+        if (!parse_param_line(line, key, value, optimised)) continue;
+        const double v = std::stod(value);
+        auto known = std::find_if(params.begin(), params.end(), [&](const ShapeParam &q) { return q.name == key; });
+        if (known == params.end())
+            params.push_back({key, v, optimised});
+        else
+            *known = {key, v, optimised};
+    }
+    return params;
+}
+
+double value_or(const std::vector<ShapeParam> &params, const std::string &name, double fallback) {
+    for (const ShapeParam &q : params)
+        if (q.name == name) return q.value;
+    return fallback;
+}
+
+std::string format(const char *fmt, double v) {
+    char buf[64];
+    std::snprintf(buf, sizeof buf, fmt, v);
+    return buf;
+}
+
+// The shape log that run_opti.sh parses (parse_log_J_and_grads): one line,
+// " J=" in fixed notation with 8 decimals, then " name=gradient" per
+// optimised parameter in scientific notation with 6 (the reference's
+// appendShapeOptLog, src/shape_optimization.cpp:112-145, writes the same
+// bytes).  The file is replaced, not appended to.
+void write_shape_log(const std::string &path, double J, const std::vector<std::string> &names,
+                     const std::vector<double> &grad) {
+    if (path.empty() || path == "none") return;
+    const std::filesystem::path dir = std::filesystem::path(path).parent_path();
+    if (!dir.empty()) {
+        std::error_code ec;
+        std::filesystem::create_directories(dir, ec);
+        if (ec) throw std::runtime_error("Impossible de créer le dossier: " + dir.string());
+    }
+    std::string text = " J=" + format("%.8f", J);
+    for (size_t k = 0; k < names.size(); ++k) text += " " + names[k] + "=" + format("%.6e", grad[k]);
+    text += "\n";
+    std::ofstream out(path);
+    if (!out) throw std::runtime_error("Impossible d'ouvrir " + path + " en écriture.");
+    out << text;
+}
+
+// The console summary of optimizeShape (:1046-1063): J, then per parameter
+// its gradient and the direction that lowers J.
+void print_preview(double J, const std::vector<std::string> &names, const std::vector<double> &grad) {
+    std::string text = "[preview] J courant = " + format("%.8f", J) + "\n" +
+                       "[preview] Gradient de forme (dJ/dθ) et sens pour diminuer J :\n";
+    for (size_t k = 0; k < names.size(); ++k) {
+        const double g = grad[k];
+        const char *advice = g > 0.0   ? "-> diminuer ce paramètre (pas négatif)"
+                             : g < 0.0 ? "-> augmenter ce paramètre (pas positif)"
+                                       : "-> gradient ~ 0 (aucune direction claire)";
+        text += "  - " + names[k] + " : dJ/d" + names[k] + " = " + format("%.8f", g) + "    " + advice + "\n";
+    }
+    std::cout << text << std::flush;
+}
+
 }  // namespace
 
 void loadParamsGeneric(const std::string &filepath, std::vector<std::string> &names_all,
@@ -47,55 +151,23 @@ void loadParamsGeneric(const std::string &filepath, std::vector<std::string> &na
     names_all.clear();
     values_all.clear();
     names.clear();
-    if (filepath.empty() || filepath == "none") return;
-    std::ifstream in(filepath);
-    if (!in) throw std::runtime_error("Impossible d'ouvrir le fichier de paramètres: " + filepath);
-    std::unordered_map<std::string, size_t> pos;
-    std::unordered_set<std::string> opt;
-    std::string line;
-    while (std::getline(in, line)) {
-        if (auto h = line.find('#'); h != std::string::npos) line.erase(h);
-        trim(line);
-        if (line.empty()) continue;
-        bool marked = false;
-        std::string key, sval;
-        if (auto k = line.find("*="); k != std::string::npos) {
-            marked = true;
-            key = line.substr(0, k);
-            sval = line.substr(k + 2);
-        } else if (auto e = line.find('='); e != std::string::npos) {
-            key = line.substr(0, e);
-            sval = line.substr(e + 1);
-        } else {
-            std::istringstream ss(line);
-            ss >> key >> sval;
-        }
-        trim(key);
-        trim(sval);
-        if (key.empty() || sval.empty()) continue;
-        const double v = std::stod(sval);
-        auto it = pos.find(key);
-        if (it == pos.end()) {
-            pos[key] = names_all.size();
-            names_all.push_back(key);
-            values_all.push_back(v);
-        } else {
-            values_all[it->second] = v;
-        }
-        if (marked)
-            opt.insert(key);
-        else
-            opt.erase(key);
+    for (const ShapeParam &q : read_shape_params(filepath)) {
+        names_all.push_back(q.name);
+        values_all.push_back(q.value);
+        if (q.optimised) names.push_back(q.name);
     }
-    for (const auto &k : names_all)
-        if (opt.count(k)) names.push_back(k);
 }
 
 void optimizeShape(Mesh &mesh, const SimulationParams &params, double eps) {
+    const std::vector<ShapeParam> file = read_shape_params(params.shape_param_file);
     std::vector<double> values_all;
     std::vector<std::string> names, names_all;
-    loadParamsGeneric(params.shape_param_file, names_all, values_all, names);
-    const double beta = param_or(names_all, values_all, "beta", 1);
+    for (const ShapeParam &q : file) {
+        names_all.push_back(q.name);
+        values_all.push_back(q.value);
+        if (q.optimised) names.push_back(q.name);
+    }
+    const double beta = value_or(file, "beta", 1.0);
     double J_current = 0.0;
     mesh.check(o2f_objective(mesh.ctx(), beta, &J_current));
     const size_t nc = mesh.cells.size();
@@ -131,32 +203,8 @@ void optimizeShape(Mesh &mesh, const SimulationParams &params, double eps) {
     else
         mesh.check(o2f_shape_gradient(mesh.ctx(), na.data(), values_all.data(), (int)na.size(), no.data(),
                                       (int)no.size(), lambda.data(), beta, eps, grad.data()));
-    std::cout << std::fixed << std::setprecision(8);
-    std::cout << "[preview] J courant = " << J_current << "\n";
-    std::cout << "[preview] Gradient de forme (dJ/dθ) et sens pour diminuer J :\n";
-    for (size_t k = 0; k < names.size(); ++k) {
-        const double g = grad[k];
-        const char *sens = (g > 0.0)   ? "-> diminuer ce paramètre (pas négatif)"
-                           : (g < 0.0) ? "-> augmenter ce paramètre (pas positif)"
-                                       : "-> gradient ~ 0 (aucune direction claire)";
-        std::cout << "  - " << names[k] << " : dJ/d" << names[k] << " = " << g << "    " << sens << "\n";
-    }
-    // appendShapeOptLog (:112-145): " J=%.8f name=%e ..." parsed by run_opti.sh
-    const std::string &path = params.shape_log_file;
-    if (!path.empty() && path != "none") {
-        std::filesystem::path p(path);
-        if (!p.parent_path().empty()) {
-            std::error_code ec;
-            if (!std::filesystem::create_directories(p.parent_path(), ec) && ec)
-                throw std::runtime_error("Impossible de créer le dossier: " + p.parent_path().string());
-        }
-        std::ofstream out(path);
-        if (!out) throw std::runtime_error("Impossible d'ouvrir " + path + " en écriture.");
-        out << std::fixed << std::setprecision(8) << " J=" << J_current;
-        out << std::scientific << std::setprecision(6);
-        for (size_t k = 0; k < names.size(); ++k) out << ' ' << names[k] << '=' << grad[k];
-        out << '\n';
-    }
+    print_preview(J_current, names, grad);
+    write_shape_log(params.shape_log_file, J_current, names, grad);
 }
 
 }  // namespace navier_stokes

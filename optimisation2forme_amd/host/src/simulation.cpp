@@ -31,6 +31,10 @@ void Simulator::initialize_fields() {
 // reload_config; t += dt; ++it; }  (src/simulation.cpp:68-126).  Steps between
 // two host events run as one device-gated batch.
 void Simulator::time_loop() {
+    if (!params_.device_time_loop) {
+        host_time_loop();
+        return;
+    }
     mesh_.push_state();
     double t = 0.0;
     int it = 0;
@@ -61,10 +65,38 @@ void Simulator::time_loop() {
     mesh_.pull_faces();
 }
 
+// The loop body of src/simulation.cpp:75-124 on the host, one step at a
+// time: the dt reduction, step_SSP_RK3, output and reload.
+void Simulator::host_time_loop() {
+    mesh_.push_state();
+    double t = 0.0;
+    int it = 0;
+    const int si = std::max(params_.save_interval, 1);
+    while (t < params_.final_time) {
+        if (params_.max_steps >= 0 && it >= params_.max_steps) break;
+        mesh_.set_params(params_);
+        double dt = 0.0;
+        mesh_.check(o2f_compute_dt(mesh_.ctx(), &dt));
+        step_SSP_RK3(dt);
+        if (it % si == 0) output_step(it, t, dt);
+        if (params_.reload_config != 0 && it % params_.reload_config == 0) load_config("config.txt", params_);
+        t += dt;
+        ++it;
+    }
+    mesh_.pull_faces();
+}
+
 void Simulator::step_SSP_RK3(double dt) {
     mesh_.push_state();
-    mesh_.check(o2f_rk3_step(mesh_.ctx(), dt));
+    mesh_.check(o2f_rk3_save_old(mesh_.ctx()));
+    rk3_substep(0.0, 1.0, dt);
+    rk3_substep(3.0 / 4.0, 1.0 / 4.0, dt);
+    rk3_substep(1.0 / 3.0, 2.0 / 3.0, dt);
     mesh_.pull_state();
+}
+
+void Simulator::rk3_substep(double c0, double c1, double dt) {
+    mesh_.check(o2f_rk3_substep(mesh_.ctx(), c0, c1, dt));
 }
 
 void Simulator::output_step(int it, double t, double dt) {

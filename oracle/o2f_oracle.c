@@ -987,6 +987,15 @@ static int rk3_substep(orc_mesh *m, const orc_params *p, double c0, double c1, d
     return 0;
 }
 
+/* The two halves of step_SSP_RK3 on their own (src/simulation.cpp:130-133
+ * and Simulator::rk3_substep, :145-171). */
+void orc_save_old(orc_mesh *m) {
+    for (int q = 0; q < 4; ++q) memcpy(m->Uold[q], m->U[q], sizeof(double) * m->nc);
+}
+int orc_rk3_substep(orc_mesh *m, const orc_params *p, double c0, double c1, double dt) {
+    return rk3_substep(m, p, c0, c1, dt);
+}
+
 /* step_SSP_RK3, src/simulation.cpp:130-142 */
 int orc_rk3_step(orc_mesh *m, const orc_params *p, double dt) {
     for (int q = 0; q < 4; ++q) memcpy(m->Uold[q], m->U[q], sizeof(double) * m->nc);
@@ -1003,6 +1012,26 @@ int orc_run_steps(orc_mesh *m, const orc_params *p, int nsteps, double *dts) {
         if (dts) dts[s] = dt;
         if (orc_rk3_step(m, p, dt)) return -1;
     }
+    return 0;
+}
+
+/* Simulator::time_loop, src/simulation.cpp:68-126, without its output and
+ * hot-reload branches (they change no state when the config file does not
+ * change): t = 0; while (t < final_time) { dt; step_SSP_RK3(dt); t += dt },
+ * the last step may overshoot.  max_steps bounds the loop.  *steps and *t
+ * receive the step count and the final t. */
+int orc_time_loop(orc_mesh *m, const orc_params *p, double final_time, int max_steps, int *steps,
+                  double *t_out) {
+    double t = 0.0;
+    int it = 0;
+    while (t < final_time && it < max_steps) {
+        double dt = orc_dt(m, p);
+        if (orc_rk3_step(m, p, dt)) return -1;
+        t += dt;
+        ++it;
+    }
+    *steps = it;
+    *t_out = t;
     return 0;
 }
 

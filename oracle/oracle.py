@@ -63,8 +63,13 @@ def lib():
         L.orc_dt.restype = C.c_double
         L.orc_rk3_step.argtypes = [vp, P, C.c_double]
         L.orc_rk3_step.restype = C.c_int
+        L.orc_save_old.argtypes = [vp]
+        L.orc_rk3_substep.argtypes = [vp, P, C.c_double, C.c_double, C.c_double]
+        L.orc_rk3_substep.restype = C.c_int
         L.orc_run_steps.argtypes = [vp, P, C.c_int, D]
         L.orc_run_steps.restype = C.c_int
+        L.orc_time_loop.argtypes = [vp, P, C.c_double, C.c_int, I, D]
+        L.orc_time_loop.restype = C.c_int
         L.orc_residual_norm.argtypes = [vp, P]
         L.orc_residual_norm.restype = C.c_double
         L.orc_run_to_residual.argtypes = [vp, P, C.c_double, C.c_int, C.c_int, I, D]
@@ -212,11 +217,25 @@ class Mesh:
         if lib().orc_rk3_step(self.h, C.byref(p), dt):
             raise ValueError("Unknown boundary condition")
 
+    def save_old(self):
+        lib().orc_save_old(self.h)
+
+    def rk3_substep(self, p, c0, c1, dt):
+        if lib().orc_rk3_substep(self.h, C.byref(p), c0, c1, dt):
+            raise ValueError("Unknown boundary condition")
+
     def run_steps(self, p, n):
         dts = np.zeros(n)
         if lib().orc_run_steps(self.h, C.byref(p), n, _d(dts)):
             raise ValueError("Unknown boundary condition")
         return dts
+
+    def time_loop(self, p, final_time, max_steps=10 ** 9):
+        """Simulator::time_loop: steps until t >= final_time; (steps, t)."""
+        n, t = C.c_int(), C.c_double()
+        if lib().orc_time_loop(self.h, C.byref(p), final_time, max_steps, C.byref(n), C.byref(t)):
+            raise ValueError("Unknown boundary condition")
+        return n.value, t.value
 
     def residual_norm(self, p):
         return lib().orc_residual_norm(self.h, C.byref(p))

@@ -4,6 +4,14 @@ committed, the GPU machines never read the reference).
 
 - naca_2D_shape_0.msh.gz      mesh_opti/naca_2D_shape_0.msh (25,274 quads)
 - naca_2D_flow_1.gz           mesh_opti/naca_2D_flow_1 (state produced on shape 0)
+- naca_2D_flow_0.gz           mesh_opti/naca_2D_flow_0, the state the reference's
+                              run started from on shape 0 (config.txt:62-63):
+                              flow_1 is flow_0 integrated for final_time =
+                              0.005 s on shape 0 with config.txt's parameters
+- naca_2D_shape_K.msh.gz, naca_2D_flow_K.gz, naca_2D_flow_{K+1}.gz for K in
+                              CHAINED: more (mesh, start, end) triples of the
+                              same chain (run_opti.sh:211-254 bumps the three
+                              paths together)
 - naca_2D_shape_params_0.txt  mesh_opti/naca_2D_shape_params_0.txt
 - trajectory.json             (m, p) of mesh_opti/naca_2D_shape_params_{0..30}.txt:
                               the sign of each step is minus the sign of the
@@ -18,6 +26,7 @@ import sys
 
 REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
 HERE = os.path.dirname(os.path.abspath(__file__))
+CHAINED = (15, 29)
 
 
 def gz(src, dst):
@@ -28,7 +37,12 @@ def gz(src, dst):
 def main():
     mo = os.path.join(REF, "mesh_opti")
     gz(os.path.join(mo, "naca_2D_shape_0.msh"), os.path.join(HERE, "naca_2D_shape_0.msh.gz"))
-    gz(os.path.join(mo, "naca_2D_flow_1"), os.path.join(HERE, "naca_2D_flow_1.gz"))
+    for k in (0, 1):
+        gz(os.path.join(mo, "naca_2D_flow_%d" % k), os.path.join(HERE, "naca_2D_flow_%d.gz" % k))
+    for k in CHAINED:
+        gz(os.path.join(mo, "naca_2D_shape_%d.msh" % k), os.path.join(HERE, "naca_2D_shape_%d.msh.gz" % k))
+        for j in (k, k + 1):
+            gz(os.path.join(mo, "naca_2D_flow_%d" % j), os.path.join(HERE, "naca_2D_flow_%d.gz" % j))
     shutil.copy(os.path.join(mo, "naca_2D_shape_params_0.txt"), os.path.join(HERE, "naca_2D_shape_params_0.txt"))
     traj = []
     for k in range(31):

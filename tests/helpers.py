@@ -75,6 +75,38 @@ def shipped_flow_path():
     return _gunzip("naca_2D_flow_1.gz")
 
 
+# (mesh, start, end) triples of the reference's shipped optimisation chain:
+# run k loads naca_2D_flow_k on naca_2D_shape_k.msh, integrates it for
+# final_time = 0.005 s with mesh_opti/config.txt's parameters and saves
+# naca_2D_flow_{k+1} (mesh_opti/config.txt:62-70, run_opti.sh:211-254).
+CHAIN = (0, 15, 29)
+CHAIN_FINAL_TIME = 0.005
+
+
+@functools.lru_cache(maxsize=None)
+def chain_mesh(k):
+    return msh2.read_msh2(_gunzip("naca_2D_shape_%d.msh.gz" % k))
+
+
+def chain_mesh_path(k):
+    return _gunzip("naca_2D_shape_%d.msh.gz" % k)
+
+
+@functools.lru_cache(maxsize=None)
+def chain_flow(k):
+    return msh2.read_state(_gunzip("naca_2D_flow_%d.gz" % k))
+
+
+def chain_flow_path(k):
+    return _gunzip("naca_2D_flow_%d.gz" % k)
+
+
+def var_rel_err(U, Uref):
+    """Per conservative variable: max |U - Uref| / max |Uref|."""
+    U, Uref = np.asarray(U), np.asarray(Uref)
+    return [float(np.max(np.abs(U[q] - Uref[q])) / np.max(np.abs(Uref[q]))) for q in range(4)]
+
+
 @functools.lru_cache(maxsize=None)
 def tiny_mesh():
     """160-triangle O-mesh (brute-force sizes)."""

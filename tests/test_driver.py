@@ -42,6 +42,9 @@ adjoint_preconditioner = block_jacobi
 shape_gradient = tangent
 state_load_path = a/b
 mesh_name = mesh_opti/naca_2D_shape_0.msh
+Lx = 2.5e0xyz
+Ny = not_a_number
+  obstacle_type =   two words  
 """)
     r = run(["--print-config", str(cfg)])
     assert r.returncode == 0, r.stderr
@@ -55,6 +58,9 @@ mesh_name = mesh_opti/naca_2D_shape_0.msh
     assert kv["adjoint_preconditioner"] == "block_jacobi" and kv["shape_gradient"] == "tangent"
     # finalize(): c_v = c_p / gamma, R = c_p - c_v (src/params.cpp:5-8)
     assert float(kv["cv"]) == 1005.0 / 1.4 and float(kv["R"]) == 1005.0 - 1005.0 / 1.4
+    # numbers as operator>> reads them: the longest valid prefix, 0 when none
+    assert float(kv["Lx"]) == 2.5 and int(kv["Ny"]) == 0
+    assert kv["obstacle_type"] == "two words" and "unknown_key" not in kv
 
 
 def test_missing_config_fails():
@@ -156,3 +162,72 @@ shape_gradient = %s
     # against their Richardson extrapolation is pinned to 1e-8 in
     # test_gpu_geometry_tangent.py
     assert abs(grads["tangent"] - grads["finite_difference"]) <= 1e-3 * abs(grads["finite_difference"])
+
+
+def test_shape_parameter_file_grammar(tmp_path):
+    """loadParamsGeneric's grammar (src/shape_optimization.cpp:19-98): '#'
+    comments, "*=" before '=', the `key value` form, a reassigned key keeps
+    its first position and takes the last value and the last marking."""
+    f = tmp_path / "params.txt"
+    f.write_text("""# header
+pos_x = 0.355
+c      1.0   trailing words are ignored
+m *= 0.02   # optimised
+p *= 0.4
+t *= 0.12
+t = 0.125
+beta = 50
+m *= 2.5e-2
+bad_line_without_value
+   
+""")
+    r = run(["--print-shape-params", str(f)])
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.strip().split("\n")
+    kv = [tuple(x.split("=", 1)) for x in lines[:-1]]
+    assert [k for k, _ in kv] == ["pos_x", "c", "m", "p", "t", "beta"]
+    assert [float(v) for _, v in kv] == [0.355, 1.0, 0.025, 0.4, 0.125, 50.0]
+    assert lines[-1] == "optimised=m,p"
+    # the shipped file
+    r = run(["--print-shape-params", os.path.join(H.GOLDEN, "naca_2D_shape_params_0.txt")])
+    assert r.returncode == 0 and r.stdout.strip().split("\n")[-1] == "optimised=m,p"
+    r = run(["--print-shape-params", str(tmp_path / "missing.txt")])
+    assert r.returncode == 1 and "Impossible d'ouvrir le fichier de paramètres" in r.stderr
+
+
+@pytest.mark.gpu
+def test_host_time_loop_equals_the_device_loop(tmp_path):
+    """device_time_loop = 0 runs the reference's loop body on the host (one
+    dt reduction and one step_SSP_RK3 of three rk3_substep calls per step):
+    the same printed lines and the same saved state, bit for bit, as the
+    device-gated loop."""
+    m = O.Mesh(H.shipped_mesh())
+    m.set_state(H.shipped_flow())
+    dt0 = m.dt(H.oracle_params(H.SHIPPED))
+    outs = {}
+    for mode in (0, 1):
+        d = tmp_path / ("mode%d" % mode)
+        d.mkdir()
+        cfg = d / "config.txt"
+        cfg.write_text("""mesh_name = %s
+viscosity = 1.8e-5
+Pr = 0.71
+cfl_number = 0.5
+inflow_velocity = 0.3
+boundary_conditions = inflow_outflow_x
+initial_condition = one_x
+final_time = %.17g
+save_interval = 2
+reload_config = 3
+verbosity = 1
+output_directory = %s
+state_load_path = %s
+state_save_path = %s
+device_time_loop = %d
+""" % (H.shipped_mesh_path(), 6.5 * dt0, d / "results", H.shipped_flow_path(), d / "state", mode))
+        r = run([str(cfg)], cwd=str(d))
+        assert r.returncode == 0, r.stdout + r.stderr
+        lines = [x for x in r.stdout.split("\n") if x.startswith("it=") or x.startswith("C_L=")]
+        outs[mode] = (lines, msh2.read_state(str(d / "state")))
+    assert outs[0][0] == outs[1][0] and len(outs[0][0]) >= 6
+    assert np.array_equal(outs[0][1], outs[1][1])

@@ -130,6 +130,26 @@ def test_rk3_steps_per_step_parity_shipped():
         assert same(s.state(), m.state()), "step %d" % k
 
 
+def test_rk3_substeps_are_the_step_decomposition():
+    """step_SSP_RK3 as the reference decomposes it (U_old <- U, then
+    rk3_substep (0, 1), (3/4, 1/4), (1/3, 2/3)): every substep's state and
+    residual bitwise the oracle's, the end state bitwise the fused step."""
+    s, m, p = pair(H.shipped_mesh(), H.SHIPPED, H.shipped_flow())
+    fused = capi.Solver(H.shipped_mesh(), H.capi_params(H.SHIPPED))
+    fused.set_state(H.shipped_flow())
+    for k in range(2):
+        dt = m.dt(p)
+        s.rk3_save_old()
+        m.save_old()
+        for c0, c1 in ((0.0, 1.0), (3.0 / 4.0, 1.0 / 4.0), (1.0 / 3.0, 2.0 / 3.0)):
+            s.rk3_substep(c0, c1, dt)
+            m.rk3_substep(p, c0, c1, dt)
+            assert same(s.state(), m.state()), (k, c0)
+            assert same(s.gradients(), m.gradients()), (k, c0)
+        fused.rk3_step(dt)
+        assert same(fused.state(), s.state()), k
+
+
 def test_run_steps_transonic_triangles_within_1e_10():
     flow = H.transonic(limiter=1)
     desc = H.small_mesh()
