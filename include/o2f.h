@@ -249,6 +249,13 @@ int o2f_bicgstab_exact(o2f_ctx *ctx, double beta, double tol, int chunk, int max
  * cells coloured two vertex rings apart (the block-Jacobi preconditioner's
  * input).  No counterpart in the reference. */
 int o2f_adjoint_block_diagonal(o2f_ctx *ctx, int exact, double *blocks);
+/* The epsilon of the orthogonality restart test |r0.r| < eps^2 |r0|^2 of
+ * every BiCGSTAB of this context (Eigen's internal::bicgstab, called by
+ * solveAdjointSystem, src/shape_optimization.cpp:699-717, uses
+ * NumTraits<double>::epsilon(), the default here).  A larger value makes
+ * the restart branch reachable on small test systems.  O2F_ERR_ARG for a
+ * negative or NaN value. */
+int o2f_bicgstab_set_restart_eps(o2f_ctx *ctx, double restart_eps);
 /* solveAdjointSystem (src/shape_optimization.cpp:658-740): BiCGSTAB with an
  * identity preconditioner on J^T lambda = b, restarted every `chunk`
  * iterations up to `max_total`, true residual checked after each chunk.

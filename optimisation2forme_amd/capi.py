@@ -42,7 +42,7 @@ EXPORTS = [
     "o2f_mlp_loss_grad", "o2f_mlp_warm_start", "o2f_time_adjoint", "o2f_time_kernel", "o2f_naca_displacement",
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
-    "o2f_rk3_save_old", "o2f_rk3_substep",
+    "o2f_rk3_save_old", "o2f_rk3_substep", "o2f_bicgstab_set_restart_eps",
 ]
 
 
@@ -106,6 +106,7 @@ def lib():
             "o2f_compute_dt": (C.c_int, [vp, D]),
             "o2f_rk3_step": (C.c_int, [vp, C.c_double]),
             "o2f_rk3_save_old": (C.c_int, [vp]),
+            "o2f_bicgstab_set_restart_eps": (C.c_int, [vp, C.c_double]),
             "o2f_rk3_substep": (C.c_int, [vp, C.c_double, C.c_double, C.c_double]),
             "o2f_run_steps": (C.c_int, [vp, C.c_int, D, D]),
             "o2f_time_loop": (C.c_int, [vp, C.c_double, C.c_int, I, D]),
@@ -424,6 +425,9 @@ class Solver:
         if check:
             self._check(st)
         return lam, it.value, rr.value, st
+
+    def set_restart_eps(self, restart_eps):
+        self._check(lib().o2f_bicgstab_set_restart_eps(self.h, restart_eps))
 
     def bicgstab(self, beta, eps=1e-6, tol=1e-6, chunk=500, max_total=100000, check=True):
         lam = np.zeros((4, self.nc))

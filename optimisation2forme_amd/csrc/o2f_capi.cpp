@@ -837,7 +837,7 @@ int bicgstab_chunk(o2f_ctx *c, const double *b, double *x, int maxit, double tol
     // checks.  The host enqueues kGroup iterations and reads the flag once
     // per group; iterations enqueued after a stop change nothing (only
     // their matrix-vector products run).  A restart (flag 2) is done here.
-    const BicgTest test{tol * tol * rhs_sq, (double)maxit, DBL_EPSILON * DBL_EPSILON};
+    const BicgTest test{tol * tol * rhs_sq, (double)maxit, c->bicg_restart_eps * c->bicg_restart_eps};
     const double *go = c->scal + kScalGo;
     constexpr int kGroup = 8;
     // The loop body rotated to start after the direction update: v = A p,
@@ -965,6 +965,12 @@ int bicgstab_solve(o2f_ctx *c, double beta, double tol, int chunk, int max_total
 }
 
 }  // namespace
+
+int o2f_bicgstab_set_restart_eps(o2f_ctx *c, double restart_eps) {
+    if (!c || !(restart_eps >= 0.0)) return O2F_ERR_ARG;
+    c->bicg_restart_eps = restart_eps;
+    return O2F_OK;
+}
 
 int o2f_bicgstab(o2f_ctx *c, double beta, double eps, double tol, int chunk, int max_total, double *lambda,
                  int *iters, double *relres) {

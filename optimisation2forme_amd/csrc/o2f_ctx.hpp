@@ -95,6 +95,9 @@ struct o2f_ctx {
     double *blk = nullptr;
     long faces_version = 0, blk_version = -1;
     double blk_eps = 0.0;
+    // epsilon of the BiCGSTAB restart test |r0.r| < eps^2 |r0|^2 (Eigen's
+    // NumTraits<double>::epsilon(), settable for tests)
+    double bicg_restart_eps = 2.220446049250313e-16;
     double *kv[12] = {};  // BiCGSTAB / gradient work vectors (4 nc each)
     // block-Jacobi preconditioner: cell colours (distance two in vertex
     // rings), probed diagonal blocks and their inverses (16 per cell)

@@ -1238,7 +1238,61 @@ static void jt_packed(const orc_mesh *m, const orc_params *p, double eps, const 
         for (int k = 0; k < 4; ++k) yp[4 * (size_t)i + k] = sy[(size_t)k * nc + i];
 }
 
+/* Dot products of the BiCGSTAB restatement.  Sequential by default (the
+ * reference's Eigen sums in its own packet order; the difference is
+ * rounding).  In device order the sum follows the gfx950 reduction exactly,
+ * so a device solve can be compared bit for bit: the vectors are taken in
+ * the device's SoA element order (element j = variable j / nc of cell
+ * j % nc), thread t of block b of a grid of G = min(512, ceil(n / 256))
+ * blocks of 256 sums elements b*256 + t, b*256 + t + 256 G, ... in order,
+ * each block adds its 256 sums by a halving tree (s[t] += s[t + h] for
+ * h = 128 .. 1), and the G block sums are added the same way (thread t
+ * sums blocks t, t + 256 in order, then the tree). */
+static int g_dot_device_order = 0;
+static double g_restart_eps = DBL_EPSILON; /* Eigen's NumTraits<double>::epsilon() */
+static int g_restarts = 0;                 /* restarts taken by the last orc_bicgstab */
+
+int orc_bicgstab_restarts(void) { return g_restarts; }
+
+void orc_bicgstab_config(int device_order, double restart_eps) {
+    g_dot_device_order = device_order;
+    g_restart_eps = restart_eps;
+}
+
+static void halving_tree(double *s) {
+    for (int h = 128; h > 0; h >>= 1)
+        for (int t = 0; t < h; ++t) s[t] += s[t + h];
+}
+
+static double dot_device_order(const double *a, const double *b, size_t n) {
+    const size_t nc = n / 4;
+    size_t blocks = (n + 255) / 256;
+    const int G = (int)(blocks < 1 ? 1 : blocks > 512 ? 512 : blocks);
+    const size_t stride = (size_t)G * 256;
+    double part[512], s[256];
+    for (int blk = 0; blk < G; ++blk) {
+        for (int t = 0; t < 256; ++t) {
+            double x = 0.0;
+            for (size_t j = (size_t)blk * 256 + t; j < n; j += stride) {
+                const size_t k = 4 * (j % nc) + j / nc; /* packed index of SoA element j */
+                x += a[k] * b[k];
+            }
+            s[t] = x;
+        }
+        halving_tree(s);
+        part[blk] = s[0];
+    }
+    for (int t = 0; t < 256; ++t) {
+        double x = 0.0;
+        for (int bb = t; bb < G; bb += 256) x += part[bb];
+        s[t] = x;
+    }
+    halving_tree(s);
+    return s[0];
+}
+
 static double dotp(const double *a, const double *b, size_t n) {
+    if (g_dot_device_order) return dot_device_order(a, b, n);
     double s = 0.0;
     for (size_t i = 0; i < n; ++i) s += a[i] * b[i];
     return s;
@@ -1269,7 +1323,7 @@ static int bicgstab_chunk(const orc_mesh *m, const orc_params *p, double eps, co
     memset(v, 0, n * sizeof(double));
     memset(pp, 0, n * sizeof(double));
     double tol2 = tol * tol * rhs_sqnorm;
-    double eps2 = DBL_EPSILON * DBL_EPSILON;
+    double eps2 = g_restart_eps * g_restart_eps;
     int i = 0, restarts = 0;
     while (dotp(r, r, n) > tol2 && i < maxit) {
         double rho_old = rho;
@@ -1279,6 +1333,7 @@ static int bicgstab_chunk(const orc_mesh *m, const orc_params *p, double eps, co
             for (size_t k = 0; k < n; ++k) r[k] = b[k] - r[k];
             memcpy(r0, r, n * sizeof(double));
             rho = r0_sqnorm = dotp(r, r, n);
+            ++g_restarts;
             if (restarts++ == 0) i = 0;
         }
         double beta = (rho / rho_old) * (alpha / w);
@@ -1307,11 +1362,13 @@ int orc_bicgstab(const orc_mesh *m, const orc_params *p, double beta, double eps
     orc_adjoint_rhs(m, p, beta, bs);
     for (int i = 0; i < nc; ++i)
         for (int k = 0; k < 4; ++k) b[4 * (size_t)i + k] = bs[(size_t)k * nc + i];
+    g_restarts = 0;
     double bnorm = sqrt(dotp(b, b, n));
     double relres = INFINITY;
     int it_total = 0, status = 0;
     while (it_total < max_total) {
-        it_total += bicgstab_chunk(m, p, eps, b, u, chunk, tol, work);
+        const int it = bicgstab_chunk(m, p, eps, b, u, chunk, tol, work);
+        it_total += it;
         jt_packed(m, p, eps, u, r, work + 6 * n, work + 7 * n);
         for (size_t k = 0; k < n; ++k) r[k] = b[k] - r[k];
         relres = sqrt(dotp(r, r, n)) / (bnorm > 0 ? bnorm : 1.0);
@@ -1320,6 +1377,9 @@ int orc_bicgstab(const orc_mesh *m, const orc_params *p, double beta, double eps
             status = -2;
             break;
         }
+        /* a chunk that ran no iteration leaves x as it was: stop instead of
+         * repeating it forever (the reference's loop would not end) */
+        if (it == 0) break;
     }
     if (status == 0 && relres > tol) status = -3;
     for (int i = 0; i < nc; ++i)

@@ -82,6 +82,8 @@ def lib():
         L.orc_edge_blocks.argtypes = [vp, P, C.c_int, C.c_double, D, D]
         L.orc_bicgstab.argtypes = [vp, P, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, D, I, D]
         L.orc_bicgstab.restype = C.c_int
+        L.orc_bicgstab_config.argtypes = [C.c_int, C.c_double]
+        L.orc_bicgstab_restarts.restype = C.c_int
         L.orc_perturb_shape.argtypes = [vp, C.c_int] + [C.c_double] * 6
         L.orc_perturb_shape.restype = C.c_int
         L.orc_shape_gradient.argtypes = [vp, P, D, C.c_int] + [C.c_double] * 7 + [D]
@@ -288,6 +290,18 @@ class Mesh:
         lib().orc_shape_gradient(self.h, C.byref(p), _d(lam), PARAM_CODES.get(name, -1), pos_x, c,
                                  m0, p0, t, beta, eps, C.byref(g))
         return g.value
+
+
+def bicgstab_config(device_order=False, restart_eps=2.220446049250313e-16):
+    """Dot-product order of the oracle's BiCGSTAB (sequential, or the
+    device's reduction order for bitwise comparison) and the epsilon of its
+    orthogonality restart test (default DBL_EPSILON, Eigen's)."""
+    lib().orc_bicgstab_config(int(device_order), restart_eps)
+
+
+def bicgstab_restarts():
+    """Orthogonality restarts taken by the last Mesh.bicgstab call."""
+    return lib().orc_bicgstab_restarts()
 
 
 def num_threads():

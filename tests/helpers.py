@@ -176,3 +176,20 @@ def rel_err(a, b):
     den = np.maximum(np.abs(b), 1e-300)
     scale = np.max(np.abs(b)) if b.size else 1.0
     return float(np.max(np.abs(a - b) / np.maximum(den, 1e-12 * scale))) if a.size else 0.0
+
+
+def quad_box_mesh(nx=16, ny=16, lx=1.0, ly=1.0, tag="o"):
+    """A uniform nx x ny grid of rectangles over [0, lx] x [0, ly], every
+    boundary line tagged `tag` (outlet: state and stresses pass through)."""
+    xs = np.linspace(0.0, lx, nx + 1)
+    ys = np.linspace(0.0, ly, ny + 1)
+    xy = np.array([(x, y) for y in ys for x in xs], np.float64)
+
+    def vid(i, j):
+        return j * (nx + 1) + i
+
+    cells = [[vid(i, j), vid(i + 1, j), vid(i + 1, j + 1), vid(i, j + 1)] for j in range(ny) for i in range(nx)]
+    bnd = [(vid(i, 0), vid(i + 1, 0)) for i in range(nx)] + [(vid(i, ny), vid(i + 1, ny)) for i in range(nx)]
+    bnd += [(vid(0, j), vid(0, j + 1)) for j in range(ny)] + [(vid(nx, j), vid(nx, j + 1)) for j in range(ny)]
+    return dict(xy=xy, cell_nv=np.full(len(cells), 4, np.int32), cell_vtx=np.array(cells, np.int32),
+                bnd_vtx=np.array(bnd, np.int32), bnd_tag=np.full(len(bnd), ord(tag), np.uint8))

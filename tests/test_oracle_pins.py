@@ -502,3 +502,172 @@ def test_fourier_heat_flux_is_the_reference_flux_over_R():
     assert np.array_equal(out[0][1:3], out[1][1:3])
     assert np.max(np.abs(out[0][3])) > 0.0
     np.testing.assert_allclose(out[1][3], out[0][3] / Rgas, rtol=1e-9, atol=1e-12 * np.max(np.abs(out[0][3])))
+
+
+# ------------------------------------------------- closed forms, limiters ----
+
+def _viscous_part(m, p, U):
+    m.set_state(U)
+    adv = m.advection(p).copy()
+    return m.diffusion(p) - adv
+
+
+@pytest.mark.parametrize("field", ["u=ay2", "u=ax2", "v=ax2", "e=bx2"])
+def test_viscous_divergence_of_quadratic_fields_is_exact_on_uniform_quads(field):
+    """On a uniform grid of rectangles every stage of computeDiffusionTerm
+    (src/diffusion.cpp:206-243) is exact for a quadratic field: the
+    valence-4 vertex values are the average of the four cell values (the
+    quadratic's value plus a constant), Green-Gauss of edge averages is the
+    central difference, exact for quadratics at the centre, so the stresses
+    are the exact linear fields, interpolated exactly, and their Green-Gauss
+    divergence is the exact constant.  With uniform density and pressure:
+      u = a y^2:  R_rho_u = d(tau_xy)/dy = 2 a mu
+      u = a x^2:  R_rho_u = d(tau_xx)/dx = (2 - 2/3) mu 2a,  R_rho_v = 0
+      v = a x^2:  R_rho_v = d(tau_xy)/dx = 2 a mu
+      u = v = 0, E / rho = e0 (1 + b x^2):  R_E = -div q
+                  = (mu c_p / Pr)(gamma - 1) 2 b e0   (the reference's heat
+                  flux, R times Fourier's law)
+    A dropped 2/3 term, a transposed stress or the Fourier heat flux each
+    miss these by O(1)."""
+    desc = H.quad_box_mesh(24, 24)
+    mu, Pr, cp, g = 0.02, 0.72, 1005.0, 1.4
+    p = O.params(viscosity=mu, Pr=Pr, cp=cp, cfl=0.5, mach=0.3, interp_mode=0)
+    m = O.Mesh(desc)
+    c = m.cells()
+    x, y = c["cx"], c["cy"]
+    rho = np.full(m.nc, 1.2)
+    u = np.zeros(m.nc)
+    v = np.zeros(m.nc)
+    a, b, e0 = 3.0, 0.5, 2.0e5
+    e_int = np.full(m.nc, e0)
+    if field == "u=ay2":
+        u = a * y * y
+    elif field == "u=ax2":
+        u = a * x * x
+    elif field == "v=ax2":
+        v = a * x * x
+    else:
+        e_int = e0 * (1.0 + b * x * x)
+    E = rho * e_int + 0.5 * rho * (u * u + v * v)
+    R = _viscous_part(m, p, np.array([rho, rho * u, rho * v, E]))
+    inner = (x > 3.5 / 24) & (x < 1 - 3.5 / 24) & (y > 3.5 / 24) & (y < 1 - 3.5 / 24)
+    assert inner.sum() >= 16 * 16
+    expect = {"u=ay2": (1, 2 * a * mu), "u=ax2": (1, (2.0 - 2.0 / 3.0) * mu * 2 * a),
+              "v=ax2": (2, 2 * a * mu), "e=bx2": (3, mu * cp / Pr * (g - 1.0) * 2 * b * e0)}
+    q, val = expect[field]
+    np.testing.assert_allclose(R[q][inner], val, rtol=1e-7)
+    if field == "u=ax2":
+        assert np.max(np.abs(R[2][inner])) <= 1e-7 * val
+
+
+def test_barth_jespersen_face_states_stay_within_the_neighbour_range():
+    """computeBarthJespersenAlpha (src/advection.cpp:195-207) with the
+    per-variable minimum over the cell's edges (:264-294): the limited face
+    states of every cell lie within [min, max] of the cell and its face
+    neighbours, and the limiter is as large as that allows -- either no face
+    needed limiting (alpha = 1: the face state is the unlimited linear
+    reconstruction) or one face state sits on the bound it was limited to."""
+    desc = H.shipped_mesh()
+    U = H.shipped_flow() * (1.0 + 1e-3 * np.random.default_rng(11).standard_normal((4, len(desc["cell_nv"]))))
+    p = H.oracle_params(dict(H.SHIPPED, limiter=1))
+    m = O.Mesh(desc)
+    m.set_state(U)
+    m.advection(p)
+    G = m.gradients()
+    fL, fR = m.faces()
+    e, c = m.edges(), m.cells()
+    ce = c["edges"]
+    lo, hi = U.copy(), U.copy()
+    for k in range(4):
+        ek = ce[:, k]
+        has = ek >= 0
+        ekc = np.maximum(ek, 0)
+        nb = np.where(e["cl"][ekc] == np.arange(m.nc), e["cr"][ekc], e["cl"][ekc])
+        ok = has & (nb >= 0)
+        lo[:, ok] = np.minimum(lo[:, ok], U[:, nb[ok]])
+        hi[:, ok] = np.maximum(hi[:, ok], U[:, nb[ok]])
+    # per cell and variable, floored where the variable crosses zero
+    tol = 1e-12 * np.maximum(np.maximum(np.abs(lo), np.abs(hi)), 1e-6 * np.abs(U).max(axis=1)[:, None])
+    limited = 0
+    for q in range(4):
+        attained = np.zeros(m.nc, bool)
+        unlimited = np.ones(m.nc, bool)
+        fits = np.ones(m.nc, bool)        # strictly inside the range at every face
+        exceeds = np.zeros(m.nc, bool)    # clearly outside at some face
+        for k in range(4):
+            ek = ce[:, k]
+            has = ek >= 0
+            ekc = np.maximum(ek, 0)
+            left = e["cl"][ekc] == np.arange(m.nc)
+            face = np.where(left, fL[q, ekc], fR[q, ekc])
+            # boundary edges: the right state is the ghost, not this cell's
+            own = has & (left | (e["cr"][ekc] >= 0))
+            assert np.all(face[own] >= (lo[q] - tol[q])[own]) and np.all(face[own] <= (hi[q] + tol[q])[own])
+            lin = U[q] + G[q] * (e["mx"][ekc] - c["cx"]) + G[4 + q] * (e["my"][ekc] - c["cy"])
+            unlimited &= ~own | (np.abs(face - lin) <= tol[q])
+            fits &= ~own | ((lin >= lo[q] + tol[q]) & (lin <= hi[q] - tol[q]))
+            exceeds |= own & ((lin < lo[q] - tol[q]) | (lin > hi[q] + tol[q]))
+            attained |= own & ((np.abs(face - lo[q]) <= tol[q]) | (np.abs(face - hi[q]) <= tol[q]))
+        # alpha = 1 where the unlimited reconstruction fits the range, else
+        # the largest alpha that keeps every face in it (cells within
+        # rounding of the bound are neither)
+        assert np.all(unlimited[fits]), q
+        assert np.all(attained[exceeds]), q
+        assert fits.sum() + exceeds.sum() > 0.8 * m.nc, (q, fits.sum(), exceeds.sum())
+        limited += int(exceeds.sum())
+    assert limited > 0.05 * m.nc      # the developed state does limit
+
+
+def test_objective_skips_wall_edges_behind_x_1_3():
+    """computeDirectObjectiveTerm (src/shape_optimization.cpp:893-939) drops
+    the wall edges whose centre has x > 1.3, computeLiftDragCoefficients
+    (src/compute.cpp:92-132) keeps them: a pressure change on those edges
+    alone moves C_L and C_D but leaves J unchanged; the same change on the
+    wall edges ahead of x = 1.3 moves J."""
+    p = H.oracle_params(H.SHIPPED)
+    m = O.Mesh(H.shipped_mesh())
+    m.set_state(H.shipped_flow())
+    m.residual(p)
+    fl, fr = m.faces()
+    e = m.edges()
+    wall = e["bc"] == ord("w")
+    aft = wall & (e["mx"] > 1.3)
+    assert 0 < aft.sum() < wall.sum()
+    J0, (cl0, cd0) = m.objective(p, 50.0), m.lift_drag(p)
+    # the two wall edges on either side of x = 1.3 (centres 1.2953, 1.2956
+    # and 1.3001, 1.3003 on this mesh) pin the threshold and its direction
+    below = wall & (e["mx"] <= 1.3) & (e["mx"] > 1.295)
+    above = wall & (e["mx"] > 1.3) & (e["mx"] < 1.3005)
+    assert below.sum() == 2 and above.sum() == 2
+    for region, moves_J in ((aft, False), (above, False), (below, True), (wall & ~aft, True)):
+        f2 = fl.copy()
+        f2[3, region] -= 2000.0 / 0.4          # lower the wall pressure there
+        m.set_faces(f2, fr)
+        J1, (cl1, cd1) = m.objective(p, 50.0), m.lift_drag(p)
+        assert (cl1 != cl0) and (cd1 != cd0)
+        assert (J1 != J0) == moves_J
+    m.set_faces(fl, fr)
+
+
+def test_wall_ghost_state_carries_no_mass_or_energy_flux():
+    """apply_InterfaceStates_boundary_conditions (src/boundary.cpp:48-80)
+    mirrors the momentum at a wall: HLLC between a state and its mirror has
+    S* = 0, so the mass and energy fluxes through the wall vanish and the
+    momentum flux is the wall pressure along the normal.  A copied or
+    freestream ghost fails this."""
+    p = H.oracle_params(H.SHIPPED)
+    m = O.Mesh(H.shipped_mesh())
+    m.set_state(H.shipped_flow())
+    m.residual(p)
+    fl, fr = m.faces()
+    e = m.edges()
+    w = e["bc"] == ord("w")
+    n = np.array([e["nx"][w], e["ny"][w]])
+    F = O.hllc_batch(fl[:, w], fr[:, w], n)
+    rho, ru, rv, E = fl[:, w]
+    c = np.sqrt(1.4 * 0.4 * (E - 0.5 * (ru * ru + rv * rv) / rho) / rho)
+    assert np.all(np.abs(F[0]) <= 1e-12 * rho * c)
+    assert np.all(np.abs(F[3]) <= 1e-12 * E * c)
+    # momentum flux along the normal only
+    tang = -F[1] * n[1] + F[2] * n[0]
+    assert np.all(np.abs(tang) <= 1e-9 * np.abs(F[1] * n[0] + F[2] * n[1]))
