@@ -166,6 +166,17 @@ int o2f_set_face_states(o2f_ctx *ctx, const double *face_L, const double *face_R
 int o2f_hllc_flux_batch(const double *qL, const double *qR, const double *normal, double gamma,
                         double *F, int n);
 
+/* o2f_hllc_flux_batch for callers that must not allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments and bit-identical results; the faces are
+ * staged through a fixed device array of the library (one per GPU, held
+ * until the process ends) instead of a hipMalloc'd buffer, so at most
+ * O2F_NOALLOC_MAX_FACES faces per call (O2F_ERR_ARG beyond).  Calls on one
+ * GPU share that array and are serialised. */
+#define O2F_NOALLOC_MAX_FACES 16384
+int o2f_hllc_flux_batch_noalloc(const double *qL, const double *qR, const double *normal, double gamma,
+                                double *F, int n);
+
 /* ---- time integration (Simulator::time_loop / step_SSP_RK3) --------- */
 
 /* dt = min(CFL min_h / max_wave, 0.5 min_h^2 / mu) (src/simulation.cpp:77-110). */

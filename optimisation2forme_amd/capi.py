@@ -43,6 +43,7 @@ EXPORTS = [
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
     "o2f_rk3_save_old", "o2f_rk3_substep", "o2f_bicgstab_set_restart_eps",
+    "o2f_hllc_flux_batch_noalloc",
 ]
 
 
@@ -103,6 +104,7 @@ def lib():
             "o2f_get_face_states": (C.c_int, [vp, D, D]),
             "o2f_set_face_states": (C.c_int, [vp, D, D]),
             "o2f_hllc_flux_batch": (C.c_int, [D, D, D, C.c_double, D, C.c_int]),
+            "o2f_hllc_flux_batch_noalloc": (C.c_int, [D, D, D, C.c_double, D, C.c_int]),
             "o2f_compute_dt": (C.c_int, [vp, D]),
             "o2f_rk3_step": (C.c_int, [vp, C.c_double]),
             "o2f_rk3_save_old": (C.c_int, [vp]),
@@ -660,3 +662,20 @@ class WarmStartNet:
 
     def warm_start(self, solver, m, p, t, aoa_deg):
         self._check(lib().o2f_mlp_warm_start(self.h, solver.h, m, p, t, aoa_deg))
+
+
+NOALLOC_MAX_FACES = 16384  # O2F_NOALLOC_MAX_FACES
+
+
+def hllc_flux_batch_noalloc(qL, qR, normal, gamma=1.4):
+    """o2f_hllc_flux_batch_noalloc: the same fluxes, staged through the
+    library's fixed per-GPU array (at most NOALLOC_MAX_FACES faces)."""
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    normal = np.ascontiguousarray(normal, dtype=np.float64)
+    n = qL.shape[1]
+    F = np.zeros((4, n))
+    st = lib().o2f_hllc_flux_batch_noalloc(_d(qL), _d(qR), _d(normal), gamma, _d(F), n)
+    if st != O2F_OK:
+        raise O2FError(st, lib().o2f_global_error().decode())
+    return F

@@ -26,11 +26,10 @@
 
 using namespace o2f;
 
-namespace {
-
+namespace o2f {
+// the message of the last failed call without a context (o2f_global_error)
 thread_local std::string g_error;
-
-}  // namespace
+}  // namespace o2f
 
 namespace {
 
