@@ -10,15 +10,17 @@ state starts uniform).  One step = dt reduction + 3 x (residual + stage
 update), all on the device.  W untimed steps, then K timed steps between
 device synchronisations.  Prints one JSON line:
   value     = N_vertices * K / seconds (vertex-steps/s, one GPU)
-  roofline  = the dominant kernel's compulsory bytes per launch / its mean
-              launch time (HIP events on the solver's stream), against
-              8 TB/s HBM3E
+  roofline  = the residual's algorithmic bytes B_residual = 64 N_c + G
+              (state in, residual out, geometry once; intermediates
+              excluded) over the summed mean durations of its four kernels
+              (each timed over back-to-back launches with HIP events on the
+              solver's stream), against 8 TB/s HBM3E
   cpu_baseline = the CPU restatement of the reference (oracle/liboracle.so,
               OpenMP over the host cores) on a bounded sample of the same
               workload.
-roofline.traffic is the dominant kernel's HBM bytes per launch from the
+roofline.traffic is the residual's HBM bytes (its four kernels) from the
 committed FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_transonic200k.json,
-copy-calibrated; null when the kernel is not in it).
+copy-calibrated; null when a kernel is not in it).
 
     python bench.py --copy-calibration
 
@@ -99,13 +101,39 @@ def kernel_bytes(desc, nv, ne, nc):
     }
 
 
-def step_algorithmic_bytes(nv, ne, nc, k=3):
-    """Algorithmic bytes of one SSP-RK3 step: B_step = 256 N_c + 3 G,
-    G = (24 + 4k) N_c + 56 N_e + (20 + 4 vbar) N_v (state once per stage,
-    geometry once per stage, intermediates excluded)."""
+def geometry_bytes(nv, ne, nc, k=3):
+    """G = (24 + 4k) N_c + 56 N_e + (20 + 4 vbar) N_v: cell volume, centre
+    and k edge ids; edge normal, length, midpoint, 2 cells, 2 vertices;
+    vertex position, CSR offset and vbar cell ids (FP64 values, int32 ids)."""
     vbar = k * nc / nv
-    G = (24 + 4 * k) * nc + 56 * ne + (20 + 4 * vbar) * nv
-    return 256 * nc + 3 * G
+    return (24 + 4 * k) * nc + 56 * ne + (20 + 4 * vbar) * nv
+
+
+def step_algorithmic_bytes(nv, ne, nc, k=3):
+    """Algorithmic bytes of one SSP-RK3 step: B_step = 256 N_c + 3 G (state
+    once per stage, geometry once per stage, intermediates excluded)."""
+    return 256 * nc + 3 * geometry_bytes(nv, ne, nc, k)
+
+
+def residual_algorithmic_bytes(nv, ne, nc, k=3):
+    """Algorithmic bytes of one residual evaluation: B_residual = 64 N_c + G
+    (U read, R written, geometry read once; intermediates excluded)."""
+    return 64 * nc + geometry_bytes(nv, ne, nc, k)
+
+
+def cpu_model():
+    """Model name, family and model number of the host CPU (/proc/cpuinfo)."""
+    info = {}
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" in line:
+                    k, v = line.split(":", 1)
+                    info.setdefault(k.strip(), v.strip())
+    except OSError:
+        return None
+    return "%s (family %s, model %s), %d logical CPUs" % (info.get("model name", "?"), info.get("cpu family", "?"),
+                                                          info.get("model", "?"), os.cpu_count() or 0)
 
 
 def cpu_baseline(desc, flow, nv, budget_s=20.0):
@@ -120,7 +148,7 @@ def cpu_baseline(desc, flow, nv, budget_s=20.0):
     t0 = time.perf_counter()
     m.run_steps(p, n)
     el = time.perf_counter() - t0
-    return {"value": nv * n / el, "unit": METRIC, "cores": O.num_threads(), "kind": "port",
+    return {"value": nv * n / el, "unit": METRIC, "cores": O.num_threads(), "kind": "port", "cpu": cpu_model(),
             "sample": "%d SSP-RK3 steps of the same 199,680-triangle workload after 2 warm-up steps, "
                       "oracle/liboracle.so (C restatement of the reference's OpenMP loop), %.1f s"
                       % (n, el)}
@@ -421,9 +449,17 @@ def main():
     # each kernel launched back to back between one event pair (the flux
     # kernel in residual form): its own duration, as the kernel trace has it
     rep_us = {n: s.time_kernel(i, launches) for i, n in enumerate(names)}
+    # the residual as a whole: its four launches in order, back to back
+    res_b2b_us = s.time_kernel(4, launches)
     kb = kernel_bytes(desc, nv, ne, nc)
+    # roofline of the residual path (the hot kernel sequence): its
+    # algorithmic bytes B_residual = 64 N_c + G over the summed durations of
+    # its four kernels
+    res_bytes = residual_algorithmic_bytes(nv, ne, nc)
+    res_us = sum(rep_us.values())
+    achieved = res_bytes / (res_us * 1e-6) / 1e9
+    traffic = [measured_traffic(KERNEL_NAMES[n]) for n in names]
     dom = max(names, key=lambda n: rep_us[n])
-    achieved = kb[dom] / (rep_us[dom] * 1e-6) / 1e9
     step_bytes = step_algorithmic_bytes(nv, ne, nc)
     ms_step = total_ms / a.steps
 
@@ -445,12 +481,16 @@ def main():
                    "limiter": "venkatakrishnan K=5" if a.limiter == "venkat" else "barth_jespersen",
                    "interp_mode": "mvc", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(KERNEL_NAMES[dom]),
-                     "kernel": KERNEL_NAMES[dom],
-                     "bytes_per_launch": kb[dom], "avg_launch_us": rep_us[dom],
-                     "timing": "HIP events around back-to-back launches on the solver's stream"},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": sum(traffic) if all(t is not None for t in traffic) else None,
+                     "kernel": "residual = " + " + ".join(KERNEL_NAMES[n] for n in names),
+                     "bytes_per_launch": res_bytes, "avg_launch_us": res_us,
+                     "dominant_kernel": KERNEL_NAMES[dom], "dominant_share_of_time": rep_us[dom] / res_us,
+                     "residual_back_to_back_us": res_b2b_us,
+                     "timing": "HIP events around back-to-back launches of each kernel on the solver's stream"},
         "kernels_avg_us": rep_us,
         "kernels_event_bracketed_us": avg_us,
+        "kernels_compulsory_bytes": kb,
         "step_roofline": {"algorithmic_bytes_per_step": step_bytes,
                           "event_ms_per_step": ms_step,
                           "achieved_GBs": step_bytes / (ms_step * 1e-3) / 1e9,

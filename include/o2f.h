@@ -439,7 +439,8 @@ int o2f_time_adjoint(o2f_ctx *ctx, int nprod, int exact, double *total_ms);
  * the figure is the kernel's own duration plus the dispatch gap, as a
  * kernel trace reports it): kernel 0 the vertex gather, 1 the cell
  * reconstruction, 2 the vertex viscous gather + edge flux, 3 the flux sum
- * and viscous divergence in residual form (no stage update).  Every launch
+ * and viscous divergence in residual form (no stage update), 4 the whole
+ * residual (those four launches in order, per "launch").  Every launch
  * recomputes the same outputs from the same inputs.  *avg_us receives the
  * microseconds per launch.  A measurement entry point of this library (no
  * counterpart in the reference). */
