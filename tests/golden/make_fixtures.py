@@ -12,7 +12,8 @@ committed, the GPU machines never read the reference).
                               CHAINED: more (mesh, start, end) triples of the
                               same chain (run_opti.sh:211-254 bumps the three
                               paths together)
-- naca_2D_shape_params_0.txt  mesh_opti/naca_2D_shape_params_0.txt
+- naca_2D_shape_params_K.txt  mesh_opti/naca_2D_shape_params_K.txt for K = 0 and the
+                              CHAINED runs (the shape-parameter file run K read)
 - trajectory.json             (m, p) of mesh_opti/naca_2D_shape_params_{0..30}.txt:
                               the sign of each step is minus the sign of the
                               reference's shape gradient (run_opti.sh:91-120)
@@ -43,7 +44,9 @@ def main():
         gz(os.path.join(mo, "naca_2D_shape_%d.msh" % k), os.path.join(HERE, "naca_2D_shape_%d.msh.gz" % k))
         for j in (k, k + 1):
             gz(os.path.join(mo, "naca_2D_flow_%d" % j), os.path.join(HERE, "naca_2D_flow_%d.gz" % j))
-    shutil.copy(os.path.join(mo, "naca_2D_shape_params_0.txt"), os.path.join(HERE, "naca_2D_shape_params_0.txt"))
+    for k in (0,) + CHAINED:
+        name = "naca_2D_shape_params_%d.txt" % k
+        shutil.copy(os.path.join(mo, name), os.path.join(HERE, name))
     traj = []
     for k in range(31):
         vals = {}
