@@ -290,6 +290,129 @@ def bench_warm_start(a):
     print(json.dumps(out), flush=True)
 
 
+def _golden(name):
+    """A data file of tests/golden (the reference's shipped mesh, states and
+    shape-parameter files), gunzipped to a temporary file if needed."""
+    import gzip
+    import tempfile
+    path = os.path.join(ROOT, "tests", "golden", name)
+    if not name.endswith(".gz"):
+        return path
+    fd, out = tempfile.mkstemp(suffix="_" + name[:-3])
+    with gzip.open(path, "rb") as f, os.fdopen(fd, "wb") as g:
+        g.write(f.read())
+    return out
+
+
+def _shape_params(k):
+    vals = {}
+    with open(_golden("naca_2D_shape_params_%d.txt" % k)) as f:
+        for line in f:
+            line = line.split("#")[0].replace("*=", "=")
+            if "=" in line:
+                key, v = line.split("=", 1)
+                vals[key.strip()] = float(v)
+    return vals
+
+
+def bench_time_to_converged(a):
+    """Time to a converged flow + adjoint + shape gradient on the
+    reference's own case: its shipped 25,274-quad NACA mesh (shape 0, m = 0,
+    p = 0.4, t = 0.12), Mach 0.3, mu 1.8e-5, Pr 0.71, CFL 0.5
+    (mesh_opti/config.txt) with the Venkatakrishnan constant K = 5, FP64.
+    Flow: SSP-RK3 until the density residual is <= --ttc-tol; adjoint:
+    solveAdjointSystem's BiCGSTAB on the reference-form operator to
+    --ttc-adjoint-tol; gradient: computeShapeGradient (central differences,
+    m and p).  Three starts: the freestream cold start (init_one_x), the
+    learned warm start (network trained on the solver's converged fields
+    of shapes 15 and 29 at Mach 0.3 and of shape 0 at Mach 0.25 and 0.35;
+    shape 0 at Mach 0.3 is held out) and the reference's own chaining (the
+    state file its optimisation run loaded, naca_2D_flow_0)."""
+    from optimisation2forme_amd import capi, warmstart as W
+    tol, adj_tol = a.ttc_tol, a.ttc_adjoint_tol
+    meshes = {k: capi.read_msh2(_golden("naca_2D_shape_%d.msh.gz" % k)) for k in (0, 15, 29)}
+
+    def params(mach):
+        return capi.make_params(viscosity=1.8e-5, Pr=0.71, cfl_number=0.5, inflow_velocity=mach, venkat_k=5.0,
+                                limiter="venkat")
+
+    def case(k, mach):
+        sp = _shape_params(k)
+        return dict(desc=meshes[k], params=params(mach), m=sp["m"], p=sp["p"], t=sp["t"], mach=mach, aoa=0.0)
+
+    # training fields (reported separately from the time to converge)
+    t0 = time.perf_counter()
+    train_cases = [case(15, 0.3), case(29, 0.3), case(0, 0.25), case(0, 0.35)]
+    fields = W.converge(train_cases, tol, a.ttc_max_steps)
+    t_fields = time.perf_counter() - t0
+    net, losses, t_train = W.train(fields, a.ws_steps, batch=a.ws_batch, lr=1e-3, seed=0)
+
+    sp0 = _shape_params(0)
+    names = ["pos_x", "c", "m", "p", "t", "beta"]
+    vals = [sp0["pos_x"], sp0["c"], sp0["m"], sp0["p"], sp0["t"], sp0["beta"]]
+    beta = sp0["beta"]
+
+    def solve(start):
+        s = capi.Solver(meshes[0], params(0.3))
+        s.init_freestream()
+        s.run_steps(1)  # warm-up of the kernels (untimed), then the start state
+        if start == "cold":
+            s.init_freestream()
+        t_begin = time.perf_counter()
+        t_inf = 0.0
+        if start == "warm":
+            net.warm_start(s, sp0["m"], sp0["p"], sp0["t"], 0.0)
+            t_inf = time.perf_counter() - t_begin
+        elif start == "chained":
+            s.load_state_file(_golden("naca_2D_flow_0.gz"))
+        U0 = s.state() if start == "warm" else None
+        t1 = time.perf_counter()
+        steps, r = s.run_to_residual(tol, a.ttc_max_steps, check_every=100)
+        t_flow = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        lam, its, rr, st = s.bicgstab(beta, eps=1e-6, tol=adj_tol, chunk=500, max_total=100000, check=False)
+        t_adj = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        g = s.shape_gradient(names, vals, ["m", "p"], lam, beta)
+        t_grad = time.perf_counter() - t1
+        cl, cd = s.lift_drag()
+        out = dict(start=start, flow_steps=int(steps), flow_residual=float(r), flow_s=t_flow, inference_s=t_inf,
+                   adjoint_iterations=int(its), adjoint_relres=float(rr), adjoint_status=int(st), adjoint_s=t_adj,
+                   gradient_s=t_grad, total_s=t_inf + t_flow + t_adj + t_grad, C_L=cl, C_D=cd,
+                   J=s.objective(beta), grad_m_p=[float(v) for v in g])
+        U = s.state()
+        s.close()
+        return out, U, U0
+
+    cold, Uc, _ = solve("cold")
+    warm, Uw, Upred = solve("warm")
+    chained, Uh, _ = solve("chained")
+    scale = np.abs(Uc).max(axis=1)
+    for row, U in ((warm, Uw), (chained, Uh)):
+        row["state_rel_diff_vs_cold"] = [float(np.max(np.abs(U[q] - Uc[q])) / scale[q]) for q in range(4)]
+        row["C_L_rel_diff"] = abs(row["C_L"] - cold["C_L"]) / abs(cold["C_L"]) if cold["C_L"] else None
+        row["C_D_rel_diff"] = abs(row["C_D"] - cold["C_D"]) / abs(cold["C_D"])
+        row["grad_rel_diff"] = [abs(x - y) / max(abs(y), 1e-300) for x, y in zip(row["grad_m_p"], cold["grad_m_p"])]
+    warm["surrogate_rel_err"] = [float(np.sqrt(np.mean((Upred[q] - Uc[q]) ** 2)) / scale[q]) for q in range(4)]
+    out = {
+        "metric": "time to converged flow+adjoint, surrogate vs cold start",
+        "value": cold["total_s"] / warm["total_s"],
+        "unit": "x (cold-start seconds / warm-start seconds)",
+        "higher_is_better": True, "n_gpus": 1, "dtype": "f64 solver, bf16 network (fp32 accumulate)",
+        "data": "the reference's shipped mesh naca_2D_shape_0.msh and its flow conditions (mesh_opti/config.txt)",
+        "config": {"workload": "shipped_naca0012_quads_25k_mach0.3", "cells": int(len(meshes[0]["cell_nv"])),
+                   "mach": 0.3, "viscosity": 1.8e-5, "Pr": 0.71, "cfl": 0.5, "limiter": "venkatakrishnan K=5",
+                   "heat_flux": "reference", "flow_tol": tol, "adjoint_tol": adj_tol, "beta": beta,
+                   "train_fields": ["shape 15 M0.3", "shape 29 M0.3", "shape 0 M0.25", "shape 0 M0.35"],
+                   "train_steps": a.ws_steps, "batch": a.ws_batch},
+        "cold": cold, "warm": warm, "chained": chained,
+        "training": {"fields_s": t_fields, "train_s": t_train, "field_steps": [f[3] for f in fields],
+                     "field_residuals": [f[4] for f in fields],
+                     "loss_first_last": [float(losses[0]), float(np.mean(losses[-50:]))]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_adjoint(a):
     """J^T x products on the transonic 200k workload after 50 steps: the
     reference-form operator (FD HLLC blocks) and the exact adjoint."""
@@ -382,6 +505,12 @@ def main():
                     help="shape gradient timings (finite differences, forward-mode tangent)")
     ap.add_argument("--no-graph", action="store_true",
                     help="batch / warm-start modes: direct launches instead of graph replay (for tracing)")
+    ap.add_argument("--time-to-converged", action="store_true",
+                    help="flow + adjoint + gradient to tolerance on the reference's shipped case: cold, "
+                         "learned warm start and state chaining")
+    ap.add_argument("--ttc-tol", type=float, default=1e-8)
+    ap.add_argument("--ttc-adjoint-tol", type=float, default=1e-8)
+    ap.add_argument("--ttc-max-steps", type=int, default=400000)
     ap.add_argument("--ws-train", type=int, default=32)
     ap.add_argument("--ws-test", type=int, default=8)
     ap.add_argument("--ws-train-tol", type=float, default=0.2)
@@ -392,6 +521,9 @@ def main():
     a = ap.parse_args()
     if a.warm_start and a.gpus == 1:
         bench_warm_start(a)
+        return
+    if a.time_to_converged and a.gpus == 1:
+        bench_time_to_converged(a)
         return
     if a.adjoint and a.gpus == 1:
         if a.steps == ap.get_default("steps"):

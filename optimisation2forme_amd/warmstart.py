@@ -46,9 +46,15 @@ def build(c, wall_spacing=4e-3, radius=20.0):
 
 
 def cell_centres(desc):
-    cv = np.asarray(desc["cell_vtx"])[:, :3]
+    """Vertex means of the cells (Cell::centre, src/mesh.cpp:123-130),
+    triangles and quads."""
+    cv = np.asarray(desc["cell_vtx"])
+    nv = np.asarray(desc["cell_nv"])
     xy = np.asarray(desc["xy"])
-    ctr = xy[cv].mean(axis=1)
+    ctr = xy[cv[:, :3]].sum(axis=1)
+    quad = nv == 4
+    ctr[quad] += xy[cv[quad, 3]]
+    ctr /= nv[:, None]
     return ctr[:, 0], ctr[:, 1]
 
 
@@ -100,6 +106,25 @@ def solve_fields(cands, tol, max_steps, check_every=50, use_graph=True):
     b = capi.Batch(solvers)
     steps, res = b.run(tol, max_steps, check_every=check_every, use_graph=use_graph)
     out = [(c, d, s.state(), int(n), float(r)) for c, d, s, n, r in zip(cands, descs, solvers, steps, res)]
+    b.close()
+    for s in solvers:
+        s.close()
+    return out
+
+
+def converge(cases, tol, max_steps, check_every=100, use_graph=True):
+    """Each case (desc, params, shape / flow labels m, p, t, mach, aoa) from
+    its freestream cold start, batched, until its density residual is <=
+    tol; returns (case, desc, state, steps, residual) per case, the
+    training fields of train()."""
+    solvers = []
+    for c in cases:
+        s = capi.Solver(c["desc"], c["params"])
+        s.init_freestream()
+        solvers.append(s)
+    b = capi.Batch(solvers)
+    steps, res = b.run(tol, max_steps, check_every=check_every, use_graph=use_graph)
+    out = [(c, c["desc"], s.state(), int(n), float(r)) for c, s, n, r in zip(cases, solvers, steps, res)]
     b.close()
     for s in solvers:
         s.close()

@@ -11,6 +11,7 @@ import pytest
 
 from optimisation2forme_amd import capi
 from oracle import mlp_oracle as MO
+from tests import helpers as H
 
 pytestmark = pytest.mark.gpu
 
@@ -110,17 +111,39 @@ def test_adam_training_reduces_the_loss():
     assert np.array_equal(net.weights()[0], net2.weights()[0])
 
 
-def test_warm_start_reduces_the_steps_to_tolerance():
-    """Ablation: the network trained on solver fields (8 candidates) against
-    the freestream cold start on 2 held-out candidates; both iterate to the
-    same density-residual tolerance."""
+def test_warm_start_converges_to_the_cold_start_state():
+    """Ablation on a case that converges: the reference's shipped quad mesh
+    (shape 0) at its Mach 0.3 flow with Venkatakrishnan K = 5.  The network
+    is trained on the solver's fields of shapes 15 and 29 (same flow); from
+    its prediction and from the freestream, the solver iterates to the same
+    density residual, 1e-8.  The warm start takes fewer steps and ends on
+    the cold start's state: 2e-8 .. 2e-6 of each variable's scale apart
+    (asserted 1e-5), C_D to 1e-6.  A residual of 1e-8 leaves that much open
+    because the slowest mode decays only about 40-fold per 5e4 steps; at
+    1e-6 the two end states are still 2e-3 apart."""
     from optimisation2forme_amd import warmstart as W
-    fields = W.solve_fields(W.candidates(8, seed=11), tol=0.25, max_steps=40000)
-    assert all(f[3] > 1000 for f in fields)     # developed fields (to 0.25 or 40000 steps)
-    net, losses, _ = W.train(fields, steps=600, batch=8192, lr=1e-3, seed=0)
+    prm = capi.make_params(viscosity=1.8e-5, Pr=0.71, cfl_number=0.5, inflow_velocity=0.3, venkat_k=5.0)
+    shape = {0: (0.0, 0.4), 15: (0.014, 0.53), 29: (0.015, 0.61)}   # naca_2D_shape_params_{k}.txt
+    cases = [dict(desc=H.chain_mesh(k), params=prm, m=shape[k][0], p=shape[k][1], t=0.12, mach=0.3, aoa=0.0)
+             for k in (15, 29)]
+    fields = W.converge(cases, 1e-6, 200000)
+    assert all(f[4] <= 1e-6 for f in fields)
+    net, losses, _ = W.train(fields, steps=800, batch=8192, lr=1e-3, seed=0)
     assert losses[-20:].mean() < 0.2 * losses[:5].mean()
-    rows = W.compare(net, W.candidates(2, seed=12), tol=0.4, max_steps=40000)
-    for r in rows:
-        print("cells %d cold %d warm %d steps" % (r["cells"], r["cold_steps"], r["warm_steps"]), flush=True)
-        assert r["cold_res"] <= 0.4 and r["warm_res"] <= 0.4
-    assert sum(r["warm_steps"] for r in rows) < 0.7 * sum(r["cold_steps"] for r in rows)
+    cold = capi.Solver(H.chain_mesh(0), prm)
+    warm = capi.Solver(H.chain_mesh(0), prm)
+    cold.init_freestream()
+    net.warm_start(warm, 0.0, 0.4, 0.12, 0.0)
+    n_cold, r_cold = cold.run_to_residual(1e-8, 300000, check_every=100)
+    n_warm, r_warm = warm.run_to_residual(1e-8, 300000, check_every=100)
+    Uc, Uw = cold.state(), warm.state()
+    diff = H.var_rel_err(Uw, Uc)
+    cold.residual()
+    warm.residual()
+    (clc, cdc), (clw, cdw) = cold.lift_drag(), warm.lift_drag()
+    print("cold %d steps (r %.2e), warm %d steps (r %.2e); end states %s apart; C_D %.10f / %.10f"
+          % (n_cold, r_cold, n_warm, r_warm, diff, cdc, cdw), flush=True)
+    assert r_cold <= 1e-8 and r_warm <= 1e-8
+    assert n_warm < n_cold
+    assert max(diff) <= 1e-5
+    assert abs(cdw - cdc) <= 1e-6 * abs(cdc)
