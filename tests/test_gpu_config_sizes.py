@@ -1,0 +1,223 @@
+"""The benchmark configurations at their own sizes, on the GPU.
+
+- NACA 2412 transonic, 199,680 triangles (Mach 0.85, AoA 2.5, Re 5000):
+  after 1000 steps from the cold start (a developed state, not the
+  branch-trivial freestream), the residual with the Venkatakrishnan and the
+  Barth-Jespersen limiter, the reference-form J^T x and the adjoint
+  right-hand side are bitwise the oracle's.
+- Differentiation check, 50,176 triangles at (m, p, t) = (0.03, 0.45,
+  0.11) with the transonic flow: the reference-form shape gradient equals
+  the oracle's, the forward-mode tangent gradient equals the Richardson
+  limit of the central differences, dR/dm from the tangent equals central
+  differences of the oracle's own NACA perturbation, and the exact adjoint
+  is the transpose of the forward JVP at this size.
+- 64 batched candidates (m ~ U[0, 0.06], p ~ U[0.2, 0.6], t ~ U[0.08,
+  0.15], 20k-120k triangles each, 4.5 M in all, Mach ~ U[0.5, 0.85], AoA ~
+  U[0, 4]): ten batched steps leave every member bitwise where ten steps of
+  its own single-mesh run do.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from optimisation2forme_amd import capi, mesher
+from oracle import oracle as O
+from tests import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _announce(request):
+    print("\n[gpu case] %s" % request.node.nodeid, flush=True)
+    sys.stdout.flush()
+    yield
+
+
+def same(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b))
+
+
+# --------------------------------------------- transonic, 200k triangles ---
+
+@pytest.fixture(scope="module")
+def developed_200k():
+    """The device's state 1000 steps after the cold start (Venkatakrishnan,
+    the reference's physics)."""
+    s = capi.Solver(H.big_mesh(), H.capi_params(H.transonic(limiter=0)))
+    s.init_freestream()
+    s.run_steps(1000)
+    U = s.state()
+    s.close()
+    assert np.all(np.isfinite(U))
+    return U
+
+
+@pytest.mark.parametrize("limiter", [0, 1])
+def test_transonic_200k_developed_residual_bit_identical(developed_200k, limiter):
+    flow = H.transonic(limiter=limiter)
+    s = capi.Solver(H.big_mesh(), H.capi_params(flow))
+    m = O.Mesh(H.big_mesh())
+    p = H.oracle_params(flow)
+    s.set_state(developed_200k)
+    m.set_state(developed_200k)
+    Rg, Ro = s.residual(), m.residual(p)
+    assert same(Rg, Ro)
+    fl_g, fr_g = s.faces()
+    fl_o, fr_o = m.faces()
+    assert same(fl_g, fl_o) and same(fr_g, fr_o)
+    assert same(s.gradients(), m.gradients())
+    # a developed state: the two face states of an edge differ wherever the
+    # flow has left the freestream (1000 steps of ~1e-7 s reach a few
+    # hundredths of a chord around the airfoil, where the wall-clustered
+    # mesh holds about a tenth of its edges); freestream gives none
+    e = m.edges()
+    inner = e["cr"] >= 0
+    jump = np.mean(np.abs(fl_o[0, inner] - fr_o[0, inner]) > 1e-12 * 1.225)
+    print("interior edges with a density jump: %.3f" % jump, flush=True)
+    assert jump > 0.05
+
+
+def test_transonic_200k_developed_jt_and_rhs_bit_identical(developed_200k):
+    flow = H.transonic(limiter=0)
+    s = capi.Solver(H.big_mesh(), H.capi_params(flow))
+    m = O.Mesh(H.big_mesh())
+    p = H.oracle_params(flow)
+    s.set_state(developed_200k)
+    m.set_state(developed_200k)
+    s.residual()
+    m.residual(p)
+    assert same(s.adjoint_rhs(0.1), m.adjoint_rhs(p, 0.1))
+    x = np.random.default_rng(21).standard_normal((4, m.nc))
+    assert same(s.apply_jt(x), m.apply_jt(p, x))
+
+
+# ------------------------------------ differentiation check, 50k triangles ---
+
+NAMES = ["pos_x", "c", "m", "p", "t"]
+SHAPE2 = dict(m=0.03, p=0.45, t=0.11)
+
+
+@pytest.fixture(scope="module")
+def diff_case():
+    desc = mesher.naca_mesh(m=SHAPE2["m"], p=SHAPE2["p"], t=SHAPE2["t"], n_around=256, n_radial=99,
+                            wall_spacing=1e-3, radius=20.0)
+    assert 50000 <= len(desc["cell_nv"]) <= 50400
+    flow = dict(H.transonic(limiter=0), heat_flux=1)
+    s = capi.Solver(desc, H.capi_params(flow))
+    s.init_freestream()
+    s.run_steps(200)
+    U = s.state()
+    s.close()
+    assert np.all(np.isfinite(U))
+    return desc, flow, U
+
+
+def test_diff_50k_reference_and_tangent_shape_gradients(diff_case):
+    desc, flow, U = diff_case
+    p = H.oracle_params(flow)
+    s = capi.Solver(desc, H.capi_params(flow))
+    s.set_state(U)
+    s.residual()
+    vals = [0.355, 1.0, SHAPE2["m"], SHAPE2["p"], SHAPE2["t"]]
+    lam = np.random.default_rng(6).standard_normal((4, s.nc)) * 1e-6
+    g6 = s.shape_gradient(NAMES, vals, ["m", "p"], lam, 0.1, eps=1e-6)
+    g5 = s.shape_gradient(NAMES, vals, ["m", "p"], lam, 0.1, eps=1e-5)
+    g_tan = s.shape_gradient_tangent(NAMES, vals, ["m", "p", "t"], lam, 0.1)
+    m = O.Mesh(desc)
+    m.set_state(U)
+    m.residual(p)
+    for k, nm in enumerate(["m", "p"]):
+        go = m.shape_gradient(p, lam, nm, m0=SHAPE2["m"], p0=SHAPE2["p"], t=SHAPE2["t"], beta=0.1)
+        rich = g6[k] - (g5[k] - g6[k]) / 99.0
+        print("%s: tangent %.12g fd(1e-6) %.12g fd(1e-5) %.12g richardson %.12g oracle %.12g"
+              % (nm, g_tan[k], g6[k], g5[k], rich, go), flush=True)
+        assert abs(g6[k] - go) <= 1e-12 * abs(go)
+        # limiter switches inside the larger step leave the differences an
+        # O(eps) kink error on top of c eps^2; the tangent is the limit
+        assert abs(g_tan[k] - rich) <= 1e-3 * abs(rich)
+        assert abs(g_tan[k] - g6[k]) <= 1e-3 * abs(g6[k])
+    assert g_tan[2] != 0.0
+
+
+def test_diff_50k_dR_dm_against_the_reference_perturbation(diff_case):
+    desc, flow, U = diff_case
+    p = H.oracle_params(flow)
+    s = capi.Solver(desc, H.capi_params(flow))
+    s.set_state(U)
+    vals = [0.355, 1.0, SHAPE2["m"], SHAPE2["p"], SHAPE2["t"]]
+    dxy = s.naca_displacement("m", NAMES, vals)
+    _, dR, dJ = s.residual_tangent(dxy, "reference", beta=0.1)
+    base = O.Mesh(desc)
+    quot = {}
+    for h in (1e-6, 1e-7):
+        out = []
+        for e in (h, -h):
+            mm = base.clone()
+            mm.perturb_shape("m", 0.355, 1.0, SHAPE2["m"], SHAPE2["p"], SHAPE2["t"], e)
+            mm.set_state(U)
+            out.append((mm.residual(p).copy(), mm.objective(p, 0.1)))
+        quot[h] = ((out[0][0] - out[1][0]) / (2 * h), (out[0][1] - out[1][1]) / (2 * h))
+    c1, c2 = quot[1e-6][0], quot[1e-7][0]
+    scale = np.maximum(np.max(np.abs(c1), axis=1, keepdims=True), 1e-300)
+    smooth = np.abs(c1 - c2) / scale < 1e-5
+    err = np.abs(dR - c1) / scale
+    print("dR/dm: smooth %.4f p99 %.3g max %.3g; dJ %.10g fd %.10g" %
+          (smooth.mean(), np.percentile(err[smooth], 99), err[smooth].max(), dJ, quot[1e-6][1]), flush=True)
+    assert smooth.mean() > 0.95
+    assert np.percentile(err[smooth], 99) < 1e-5
+    assert abs(dJ - quot[1e-6][1]) <= 1e-5 * abs(quot[1e-6][1]) + 2 * abs(quot[1e-6][1] - quot[1e-7][1])
+
+
+def test_diff_50k_exact_adjoint_is_the_transpose_of_the_jvp(diff_case):
+    desc, flow, U = diff_case
+    s = capi.Solver(desc, H.capi_params(flow))
+    s.set_state(U)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((4, s.nc))
+    y = rng.standard_normal((4, s.nc)) * np.abs(U) * 1e-3
+    jtx = s.apply_jt_exact(x)
+    _, jy = s.residual_jvp(y)
+    lhs, rhs = float(np.sum(jtx * y)), float(np.sum(x * jy))
+    mag = float(np.sum(np.abs(x * jy)))
+    print("<J^T x, y> %.15g  <x, J y> %.15g  (sum |x J y| %.3g)" % (lhs, rhs, mag), flush=True)
+    assert abs(lhs - rhs) <= 1e-13 * mag
+
+
+# ------------------------------------------------ 64 batched candidates ---
+
+def test_batch_of_64_candidates_bit_identical_to_single_runs():
+    import bench
+    cands = bench.batch_candidates()
+    assert len(cands) == 64
+    solvers, descs, prms = [], [], []
+    for c in cands:
+        desc = mesher.naca_mesh(m=c["m"], p=c["p"], t=c["t"], n_around=c["n_around"], n_radial=c["n_radial"],
+                                wall_spacing=1e-3, radius=20.0, aoa_deg=c["aoa"])
+        mu = H.re_viscosity(c["mach"])
+        prm = capi.make_params(viscosity=mu, Pr=0.72, cfl_number=0.5, inflow_velocity=c["mach"], venkat_k=5.0,
+                               limiter="venkat", interp_mode="mvc", heat_flux="fourier")
+        s = capi.Solver(desc, prm)
+        s.init_freestream()
+        solvers.append(s)
+        descs.append(desc)
+        prms.append(prm)
+    cells = sum(s.nc for s in solvers)
+    print("64 candidates, %d cells (%d .. %d)" % (cells, min(s.nc for s in solvers), max(s.nc for s in solvers)),
+          flush=True)
+    assert 4.0e6 < cells < 5.0e6
+    b = capi.Batch(solvers)
+    b.autotune(reps=1)
+    steps, _ = b.run(-1.0, 10, check_every=5)
+    assert np.all(steps == 10)
+    batched = [s.state() for s in solvers]
+    b.close()
+    for k, (desc, prm) in enumerate(zip(descs, prms)):
+        one = capi.Solver(desc, prm)
+        one.init_freestream()
+        one.run_steps(10)
+        assert same(one.state(), batched[k]), k
+        one.close()
+    for s in solvers:
+        s.close()
