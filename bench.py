@@ -290,6 +290,82 @@ def bench_warm_start(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_warm_start_scale(a):
+    """The learned warm start at the benchmark's training scale: solver
+    fields of --ws-train seeded NACA 4-digit candidates on O-meshes of 30k to
+    100k triangles (Mach 0.5-0.85, AoA 0-4 deg, Re 5000, Fourier heat flux),
+    each run from the cold start to the physical time --ws-field-time,
+    batched;
+    --ws-steps Adam steps of --ws-batch samples on the GPU; then
+    --ws-test held-out candidates on 200k-triangle meshes at Mach 0.8,
+    developed the same way, against the network's prediction (error of the
+    normalised state, per variable).  The global time step of the reference
+    scheme is set by the smallest cells (2e-5 .. 1e-5 chords at the trailing
+    edge: dt ~ 1e-8 .. 1e-9 s), so the fields are taken at a short physical
+    time, 2e-5 s by default (thousands of steps); these Re 5000 flows do not
+    reach a steady state on these meshes (the time to a converged state is
+    measured by --time-to-converged on a case that converges)."""
+    from optimisation2forme_amd import warmstart as W
+
+    def note(msg):
+        print("[warm-start-scale] %s" % msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    train_c = W.candidates_sized(a.ws_train, seed=2025, triangles=(30e3, 100e3))
+    fields = []
+    for k in range(0, len(train_c), 64):      # batches of 64 candidates
+        fields += W.develop(train_c[k:k + 64], a.ws_field_time, a.ws_max_steps)
+        note("fields %d / %d (%.0f s)" % (len(fields), len(train_c), time.perf_counter() - t0))
+    t_fields = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    raw, tgt = W.dataset(fields)
+    t_data = time.perf_counter() - t0
+    note("dataset %d samples (%.0f s)" % (raw.shape[0], t_data))
+    finite = np.all(np.isfinite(tgt), axis=1)
+    from optimisation2forme_amd import capi
+    net = capi.WarmStartNet(seed=0)
+    net.set_data(raw[finite], tgt[finite])
+    t0 = time.perf_counter()
+    losses = net.train(a.ws_steps, batch=a.ws_batch, lr=1e-3, seed=1)
+    t_train = time.perf_counter() - t0
+    note("trained %d steps (%.1f s), loss %.3g -> %.3g" % (a.ws_steps, t_train, losses[0], np.mean(losses[-50:])))
+    test_c = W.candidates_sized(a.ws_test, seed=4242, triangles=(2e5, 2e5), mach=(0.8, 0.8))
+    held = []
+    t0 = time.perf_counter()
+    for k in range(0, len(test_c), 16):
+        held += W.develop(test_c[k:k + 16], a.ws_field_time, a.ws_max_steps)
+        note("held-out fields %d / %d (%.0f s)" % (len(held), len(test_c), time.perf_counter() - t0))
+    rows = []
+    for c, desc, U, _, r in held:  # _ = steps to the field time
+        pred = net.forward(W.raw_inputs(desc, c))
+        target = W.normalise(U, c["mach"])
+        err = np.sqrt(np.mean((pred - target) ** 2, 0))
+        spread = np.sqrt(np.mean(target ** 2, 0))
+        rows.append(dict(cells=int(U.shape[1]), residual=float(r), steps=int(_), rmse=err.tolist(),
+                         rmse_over_rms=(err / np.maximum(spread, 1e-30)).tolist()))
+    note("held-out %d candidates evaluated" % len(rows))
+    n_param = 64 * 256 + 3 * 256 * 256 + 256 * 64
+    out = {
+        "metric": "surrogate training and held-out error at the benchmark's training scale",
+        "value": float(np.mean([np.mean(r["rmse_over_rms"]) for r in rows])),
+        "unit": "held-out rmse / rms of the normalised state (mean over variables and candidates)",
+        "higher_is_better": False, "n_gpus": 1, "dtype": "bf16 network (fp32 accumulate), f64 solver fields",
+        "data": "synthetic (solver-produced fields on generated NACA 4-digit meshes)",
+        "config": {"workload": "warm_start_naca4_30k_100k_fields", "train_candidates": len(fields),
+                   "train_samples": int(finite.sum()), "field_time_s": a.ws_field_time,
+                   "field_steps_mean": float(np.mean([f[3] for f in fields])),
+                   "train_steps": a.ws_steps, "batch": a.ws_batch, "lr": 1e-3, "network": "4 x 256 SiLU, "
+                   "inputs x, y, wall distance, m, p, t, Mach, AoA + 16 Fourier features",
+                   "test_candidates": len(rows), "test_triangles": 200000, "test_mach": 0.8, "re": 5000,
+                   "heat_flux": "fourier"},
+        "fields_s": t_fields, "dataset_s": t_data, "train_s": t_train,
+        "train_tflops": 6.0 * n_param * a.ws_batch * a.ws_steps / t_train / 1e12,
+        "loss_first_last": [float(losses[0]), float(np.mean(losses[-50:]))],
+        "held_out": rows,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def _golden(name):
     """A data file of tests/golden (the reference's shipped mesh, states and
     shape-parameter files), gunzipped to a temporary file if needed."""
@@ -511,6 +587,10 @@ def main():
     ap.add_argument("--ttc-tol", type=float, default=1e-8)
     ap.add_argument("--ttc-adjoint-tol", type=float, default=1e-8)
     ap.add_argument("--ttc-max-steps", type=int, default=400000)
+    ap.add_argument("--warm-start-scale", action="store_true",
+                    help="surrogate training at the benchmark's scale (--ws-train fields of 30k-100k triangles)")
+    ap.add_argument("--ws-field-time", type=float, default=2e-5,
+                    help="physical time (s) of the solver fields of --warm-start-scale")
     ap.add_argument("--ws-train", type=int, default=32)
     ap.add_argument("--ws-test", type=int, default=8)
     ap.add_argument("--ws-train-tol", type=float, default=0.2)
@@ -521,6 +601,9 @@ def main():
     a = ap.parse_args()
     if a.warm_start and a.gpus == 1:
         bench_warm_start(a)
+        return
+    if a.warm_start_scale and a.gpus == 1:
+        bench_warm_start_scale(a)
         return
     if a.time_to_converged and a.gpus == 1:
         bench_time_to_converged(a)

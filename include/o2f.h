@@ -374,6 +374,11 @@ int o2f_batch_autotune(o2f_batch *batch, int reps, int *shapes);
  * (may be NULL). */
 int o2f_batch_run(o2f_batch *batch, double tol, int max_steps, int check_every, int use_graph, int *steps,
                   double *res);
+/* Simulator::time_loop's while (t < final_time) (src/simulation.cpp:75-124)
+ * for every member at once, from t = 0, each member stopping at its own
+ * step (its own dt); at most max_steps steps.  steps[i], t[i] per member
+ * (may be NULL). */
+int o2f_batch_run_until(o2f_batch *batch, double final_time, int max_steps, int use_graph, int *steps, double *t);
 /* Elapsed milliseconds of nsteps batched steps (tol <= 0, no early stop). */
 int o2f_batch_time_steps(o2f_batch *batch, int nsteps, int use_graph, double *total_ms);
 

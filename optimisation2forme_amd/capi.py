@@ -43,7 +43,7 @@ EXPORTS = [
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
     "o2f_rk3_save_old", "o2f_rk3_substep", "o2f_bicgstab_set_restart_eps",
-    "o2f_hllc_flux_batch_noalloc",
+    "o2f_hllc_flux_batch_noalloc", "o2f_batch_run_until",
 ]
 
 
@@ -159,6 +159,7 @@ def lib():
             "o2f_batch_autotune": (C.c_int, [vp, C.c_int, I]),
             "o2f_batch_run": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
             "o2f_batch_time_steps": (C.c_int, [vp, C.c_int, C.c_int, D]),
+            "o2f_batch_run_until": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, I, D]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -581,6 +582,13 @@ class Batch:
         self._check(lib().o2f_batch_run(self.h, tol, max_steps, check_every, int(use_graph), _i(steps),
                                         _d(res)))
         return steps, res
+
+    def run_until(self, final_time, max_steps, use_graph=True):
+        """Every member's time loop to final_time: (steps, t) per member."""
+        n = len(self.solvers)
+        steps, t = np.zeros(n, np.int32), np.zeros(n)
+        self._check(lib().o2f_batch_run_until(self.h, final_time, max_steps, int(use_graph), _i(steps), _d(t)))
+        return steps, t
 
     def time_steps(self, nsteps, use_graph=True):
         ms = C.c_double()

@@ -98,7 +98,9 @@ int build_tiles(o2f_batch *b, int kind) {
 }
 
 // Member table from the contexts' current arrays and parameters.
-int upload_members(o2f_batch *b, double tol) {
+// final_t: every member's time loop stops once its t >= final_t
+// (while (t < final_time), src/simulation.cpp:75); HUGE_VAL for none.
+int upload_members(o2f_batch *b, double tol, double final_t = HUGE_VAL) {
     b->hM.resize(b->n());
     for (int m = 0; m < b->n(); ++m) {
         o2f_ctx *c = b->ctx[m];
@@ -107,7 +109,7 @@ int upload_members(o2f_batch *b, double tol) {
         A.G = c->geo[0].view();
         A.P = c->phys;
         A.P.gate = c->scal + kScalActive;
-        A.P.final_t = HUGE_VAL;
+        A.P.final_t = final_t;
         A.U = c->U;
         A.Uold = c->Uold;
         A.R = c->R;
@@ -275,13 +277,14 @@ int sync_members(o2f_batch *b) {
 }
 
 int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph, int *steps, double *res,
-        double *elapsed_ms) {
+        double *elapsed_ms, double final_t = HUGE_VAL) {
     if (max_steps < 0 || check_every < 1) return b->fail(O2F_ERR_ARG, "batch run: max_steps >= 0, check_every >= 1");
     int st;
     if ((st = sync_members(b))) return st;
-    if ((st = upload_members(b, tol))) return st;
+    if ((st = upload_members(b, tol, final_t))) return st;
     if ((st = reset_scalars(b))) return st;
     const bool check = tol > 0.0;
+    const bool poll = check || final_t < HUGE_VAL;  // members may stop before max_steps
     const int k = check_every;
     const int groups = max_steps / k, tail = max_steps % k;
     const auto plan = chunks_of(k, check), tail_plan = chunks_of(tail, check);
@@ -303,7 +306,7 @@ int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph,
     bool live = true;
     for (int g = 0; g < groups && live; ++g) {
         if ((st = enqueue_chunks(b, plan, use_graph))) return st;
-        if (check && (g + 1) % per_sync == 0) {
+        if (poll && (g + 1) % per_sync == 0) {
             live = any_active(b, st);
             if (st) return st;
         }
@@ -470,6 +473,18 @@ int o2f_batch_run(o2f_batch *b, double tol, int max_steps, int check_every, int 
                   double *res) {
     if (!b) return O2F_ERR_ARG;
     return run(b, tol, max_steps, check_every, use_graph, steps, res, nullptr);
+}
+
+int o2f_batch_run_until(o2f_batch *b, double final_time, int max_steps, int use_graph, int *steps, double *t) {
+    if (!b || !(final_time > 0.0)) return O2F_ERR_ARG;
+    int st = run(b, -1.0, max_steps, 50, use_graph, steps, nullptr, nullptr, final_time);
+    if (st || !t) return st;
+    for (int m = 0; m < b->n(); ++m) {
+        double tm = 0.0;
+        BCHK(b, hipMemcpy(&tm, b->ctx[m]->scal + kScalT, sizeof(double), hipMemcpyDeviceToHost));
+        t[m] = tm;
+    }
+    return O2F_OK;
 }
 
 int o2f_batch_time_steps(o2f_batch *b, int nsteps, int use_graph, double *total_ms) {

@@ -22,11 +22,14 @@ import math
 import numpy as np
 
 
-def naca4(m, p, t, xr):
-    """Mean line and half thickness at chord fraction xr (vectorised)."""
+def naca4(m, p, t, xr, closed_te=False):
+    """Mean line and half thickness at chord fraction xr (vectorised); with
+    closed_te the last thickness coefficient is -0.1036, which closes the
+    trailing edge (yt(1) = 0)."""
     xr = np.clip(xr, 0.0, 1.0)
+    a4 = -0.1036 if closed_te else -0.1015
     yt = 5.0 * t * (0.2969 * np.sqrt(xr) - 0.1260 * xr - 0.3516 * xr ** 2 + 0.2843 * xr ** 3
-                    - 0.1015 * xr ** 4)
+                    + a4 * xr ** 4)
     if m == 0.0:
         return np.zeros_like(xr), yt
     yc = np.where(xr <= p, m / (p * p) * (2.0 * p * xr - xr * xr),
@@ -34,21 +37,25 @@ def naca4(m, p, t, xr):
     return yc, yt
 
 
-def airfoil_contour(m, p, t, n_around, n_base=4, pos_x=0.355, chord=1.0):
+def airfoil_contour(m, p, t, n_around, n_base=4, pos_x=0.355, chord=1.0, closed_te=False):
     """Closed clockwise contour: lower surface TE->LE, upper LE->TE, then the
-    flat trailing-edge base back to the start (n_base segments)."""
+    flat trailing-edge base back to the start (n_base segments); with
+    closed_te the two surfaces meet at a sharp trailing edge (no base)."""
+    if closed_te:
+        n_base = 0
     n_side = (n_around - n_base) // 2
     if n_side < 4 or 2 * n_side + n_base != n_around:
         raise ValueError("n_around must be n_base + 2*k with k >= 4")
     beta = np.linspace(0.0, math.pi, n_side + 1)
     xr = 0.5 * (1.0 - np.cos(beta))            # 0 .. 1, clustered at LE and TE
-    yc, yt = naca4(m, p, t, xr)
+    yc, yt = naca4(m, p, t, xr, closed_te)
     lower = np.stack([xr[::-1], (yc - yt)[::-1]], 1)      # TE -> LE
     upper = np.stack([xr, yc + yt], 1)                    # LE -> TE
     pts = np.concatenate([lower[:-1], upper[:-1]])        # 2*n_side points, LE once
-    te_up, te_lo = upper[-1], lower[0]
-    base = [te_up + (te_lo - te_up) * (k / n_base) for k in range(n_base)]
-    pts = np.concatenate([pts, np.array(base)])
+    if n_base:
+        te_up, te_lo = upper[-1], lower[0]
+        base = [te_up + (te_lo - te_up) * (k / n_base) for k in range(n_base)]
+        pts = np.concatenate([pts, np.array(base)])
     pts[:, 0] = pos_x + pts[:, 0] * chord
     pts[:, 1] = pts[:, 1] * chord
     return pts
@@ -121,11 +128,11 @@ def march_layers(wall, d, centre, R, d_switch=0.25):
 
 
 def naca_mesh(m=0.0, p=0.4, t=0.12, n_around=128, n_radial=40, wall_spacing=2e-3, radius=20.0,
-              growth=None, aoa_deg=0.0, pos_x=0.355, chord=1.0):
+              growth=None, aoa_deg=0.0, pos_x=0.355, chord=1.0, closed_te=False):
     """Triangle O-mesh; returns the mesh description (the arrays of
     o2f_mesh_desc) plus metadata.  With `growth` given the far-field radius
     follows from it; otherwise the growth is solved to reach `radius`."""
-    wall = airfoil_contour(m, p, t, n_around, pos_x=pos_x, chord=chord)
+    wall = airfoil_contour(m, p, t, n_around, pos_x=pos_x, chord=chord, closed_te=closed_te)
     if aoa_deg:
         a = math.radians(aoa_deg)
         ca, sa = math.cos(a), math.sin(a)

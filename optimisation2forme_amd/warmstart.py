@@ -36,6 +36,45 @@ def candidates(n, seed, n_around=(56, 96), n_radial=(24, 40)):
     return out
 
 
+def candidates_sized(n, seed, triangles=(30e3, 100e3), mach=(0.5, 0.85)):
+    """Seeded shapes and flows as candidates() (Mach range given), each with
+    an O-mesh of a target size drawn uniformly from `triangles` (about 2.6
+    cells around per radial layer, as the batched workload's meshes)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m, p, t = float(rng.uniform(0, 0.06)), float(rng.uniform(0.2, 0.6)), float(rng.uniform(0.08, 0.15))
+        tri = rng.uniform(*triangles) if triangles[1] > triangles[0] else triangles[0]
+        n_around = max(16, int(round(np.sqrt(tri / 2 * 2.6) / 4)) * 4)
+        n_radial = int(round(tri / (2 * n_around))) + 1
+        out.append(dict(m=m, p=p, t=t, mach=float(rng.uniform(*mach)) if mach[1] > mach[0] else mach[0],
+                        aoa=float(rng.uniform(0, 4)), n_around=n_around, n_radial=n_radial))
+    return out
+
+
+def develop(cands, final_time, max_steps, wall_spacing=1e-3, use_graph=True):
+    """Each candidate's time loop from its freestream cold start to the
+    physical time final_time (at most max_steps steps), all in one batch;
+    returns (candidate, mesh, state, steps, residual)."""
+    solvers, descs = [], []
+    for c in cands:
+        desc, prm = build(c, wall_spacing=wall_spacing)
+        s = capi.Solver(desc, prm)
+        s.init_freestream()
+        solvers.append(s)
+        descs.append(desc)
+    b = capi.Batch(solvers)
+    steps, _ = b.run_until(final_time, max_steps, use_graph=use_graph)
+    out = []
+    for c, d, s, n in zip(cands, descs, solvers, steps):
+        s.residual()
+        out.append((c, d, s.state(), int(n), s.residual_norm()))
+    b.close()
+    for s in solvers:
+        s.close()
+    return out
+
+
 def build(c, wall_spacing=4e-3, radius=20.0):
     desc = mesher.naca_mesh(m=c["m"], p=c["p"], t=c["t"], n_around=c["n_around"], n_radial=c["n_radial"],
                             wall_spacing=wall_spacing, radius=radius, aoa_deg=c["aoa"])
@@ -58,8 +97,44 @@ def cell_centres(desc):
     return ctr[:, 0], ctr[:, 1]
 
 
-def wall_distance(desc, cx, cy):
-    """Distance of each point to the nearest wall edge segment."""
+def wall_distance(desc, cx, cy, k_near=8):
+    """Distance of each point to the nearest wall edge segment: the exact
+    segment distances to the wall edges at the k_near wall vertices nearest
+    to the point (a k-d tree over the wall vertices), which hold the nearest
+    segment on any mesh whose wall edges are short against the distance to
+    the opposite surface."""
+    from scipy.spatial import cKDTree
+    xy = np.asarray(desc["xy"], dtype=np.float64)
+    bv = np.asarray(desc["bnd_vtx"])
+    wall = bv[np.asarray(desc["bnd_tag"]) == ord("w")]
+    verts, inv = np.unique(wall.ravel(), return_inverse=True)
+    inv = inv.reshape(wall.shape)
+    # the (at most two) wall edges of each wall vertex
+    edges_of = [[] for _ in range(len(verts))]
+    for e, (i, j) in enumerate(inv):
+        edges_of[i].append(e)
+        edges_of[j].append(e)
+    width = max(len(x) for x in edges_of)
+    adj = np.array([x + [x[0]] * (width - len(x)) for x in edges_of])
+    P = np.stack([cx, cy], 1)
+    k = min(k_near, len(verts))
+    _, near = cKDTree(xy[verts]).query(P, k=k)
+    near = near.reshape(len(P), k)
+    a, b = xy[wall[:, 0]], xy[wall[:, 1]]
+    d = np.full(len(P), np.inf)
+    for j in range(k):
+        for w in range(width):
+            e = adj[near[:, j], w]
+            A, AB = a[e], b[e] - a[e]
+            l2 = np.maximum(np.sum(AB * AB, 1), 1e-300)
+            t = np.clip(np.sum((P - A) * AB, 1) / l2, 0.0, 1.0)
+            Q = A + t[:, None] * AB
+            d = np.minimum(d, np.sqrt(np.sum((Q - P) ** 2, 1)))
+    return d
+
+
+def wall_distance_brute(desc, cx, cy):
+    """The same by testing every wall edge (the reference for small meshes)."""
     xy = np.asarray(desc["xy"])
     bv = np.asarray(desc["bnd_vtx"])
     wall = bv[np.asarray(desc["bnd_tag"]) == ord("w")]

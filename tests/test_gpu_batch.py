@@ -162,3 +162,25 @@ def test_batch_members_stop_independently():
         assert np.array_equal(s.state(), ref.state()), k
         if steps[k] < 300:
             assert res[k] <= tol
+
+
+def test_batch_run_until_matches_each_members_own_time_loop():
+    """o2f_batch_run_until: every member stops at its own step once its t
+    reaches the final time, bitwise where its own o2f_run_until stops."""
+    descs = [mesher.naca_mesh(m=0.02, p=0.4, t=0.12, n_around=n, n_radial=r, wall_spacing=2e-3, radius=10.0)
+             for n, r in ((48, 20), (64, 30), (56, 24))]
+    flow = dict(H.transonic(limiter=0), heat_flux=1)
+    solvers = [capi.Solver(d, H.capi_params(flow)) for d in descs]
+    for s in solvers:
+        s.init_freestream()
+    final = 40.5 * solvers[1].dt()
+    b = capi.Batch(solvers)
+    steps, t = b.run_until(final, 10000)
+    b.close()
+    assert len(set(steps.tolist())) > 1          # the members stop at different steps
+    for s, d, n, tt in zip(solvers, descs, steps, t):
+        one = capi.Solver(d, H.capi_params(flow))
+        one.init_freestream()
+        n1, t1, _, _ = one.run_until(final)
+        assert n1 == n and t1 == tt and tt >= final
+        assert np.array_equal(one.state(), s.state())

@@ -71,3 +71,16 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_wall_distance_tree_equals_brute_force():
+    """warmstart.wall_distance (a k-d tree over the wall vertices, exact
+    distances to their edges) equals the distance to every wall edge, on an
+    O-mesh with a blunt and one with a sharp trailing edge and on the
+    reference's shipped quads."""
+    from optimisation2forme_amd import mesher, warmstart as W
+    from tests import helpers as H
+    for desc in (H.small_mesh(), mesher.naca_mesh(m=0.04, p=0.3, t=0.1, n_around=96, n_radial=30,
+                                                  wall_spacing=1e-3, closed_te=True), H.shipped_mesh()):
+        cx, cy = W.cell_centres(desc)
+        assert np.array_equal(W.wall_distance(desc, cx, cy), W.wall_distance_brute(desc, cx, cy))
