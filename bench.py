@@ -165,14 +165,26 @@ def measured_traffic(kernel):
 
 
 def copy_calibration():
+    """Calibration kernels of known traffic for the FETCH_SIZE / WRITE_SIZE
+    counters, each reading and writing COPY_BYTES: the runtime's device copy,
+    an elementwise kernel (ordinary per-lane loads and stores, as the stage
+    kernels issue them) and a gather of 128-byte blocks in a random order
+    (every block read once, like a cell gathering its neighbours' lines)."""
     import torch
     n = COPY_BYTES // 8
     x = torch.ones(n, dtype=torch.float64, device="cuda")
     y = torch.empty_like(x)
+    perm = torch.randperm(n // 16, device="cuda", generator=torch.Generator(device="cuda").manual_seed(0))
     for _ in range(5):
         y.copy_(x)
+    for _ in range(5):
+        torch.mul(x, 1.0, out=y)
+    xb = x.view(-1, 16)
+    for _ in range(5):
+        torch.index_select(xb, 0, perm, out=y.view(-1, 16))
     torch.cuda.synchronize()
-    print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "copies": 5}), flush=True)
+    print(json.dumps({"copy_calibration_bytes": COPY_BYTES, "copies": 5, "elementwise": 5, "block_gathers": 5}),
+          flush=True)
 
 
 def batch_candidates(n=64, seed=20240611):

@@ -54,11 +54,16 @@ def counter_per_dispatch(path_or_dir, counter):
     return {k: (len(v), sum(v) / len(v)) for k, v in acc.items()}
 
 
-def copy_kib(path_or_dir, counter):
-    """Mean counter value (KiB) of the device copy dispatches of the
-    calibration run: the largest-valued kernel is the copy."""
+def copy_kib(path_or_dir, counter, pattern="copyBuffer"):
+    """Mean counter value (KiB) of one calibration kernel of the
+    calibration run (`bench.py --copy-calibration`): the runtime's device
+    copy by default; "elementwise_kernel<4, at::native::AUnaryFunctor" is
+    the elementwise copy, "gather_kernel" the 128-byte block gather."""
     res = counter_per_dispatch(path_or_dir, counter)
-    name = max(res, key=lambda k: res[k][1])
+    hits = [k for k in res if pattern in k]
+    if not hits:
+        raise KeyError("no calibration kernel matching %r in %s" % (pattern, path_or_dir))
+    name = max(hits, key=lambda k: res[k][1])
     return name, res[name][1]
 
 
@@ -71,11 +76,21 @@ def traffic(fetch_dir, write_dir, cfetch_dir, cwrite_dir, copy_bytes):
     wscale = copy_bytes / (cw * 1024.0)
     out = {"copy_kernel": cf_name, "copy_bytes": copy_bytes, "copy_fetch_kib": cf, "copy_write_kib": cw,
            "fetch_scale": fscale, "write_scale": wscale, "kernels": {}}
+    # the other calibration kernels, where the run has them
+    for label, pat in (("elementwise", "AUnaryFunctor"), ("block_gather", "gather_kernel")):
+        try:
+            _, f = copy_kib(cfetch_dir, "FETCH_SIZE", pat)
+            _, w = copy_kib(cwrite_dir, "WRITE_SIZE", pat)
+            out[label + "_fetch_scale"] = copy_bytes / (f * 1024.0)
+            out[label + "_write_scale"] = copy_bytes / (w * 1024.0)
+        except KeyError:
+            pass
     for k in sorted(set(fetch) & set(write)):
         fb = fetch[k][1] * 1024.0 * fscale
         wb = write[k][1] * 1024.0 * wscale
+        fb1 = fetch[k][1] * 1024.0 * out.get("block_gather_fetch_scale", fscale)
         out["kernels"][k] = {"dispatches": fetch[k][0], "fetch_kib": fetch[k][1], "write_kib": write[k][1],
-                             "traffic_bytes": fb + wb}
+                             "traffic_bytes": fb + wb, "traffic_bytes_gather_scaled": fb1 + wb}
     return out
 
 
@@ -104,11 +119,16 @@ def main(argv):
         print("copy calibration: %s, %.0f bytes -> FETCH_SIZE %.1f KiB (x%.3f), WRITE_SIZE %.1f KiB (x%.3f)"
               % (out["copy_kernel"], out["copy_bytes"], out["copy_fetch_kib"], out["fetch_scale"],
                  out["copy_write_kib"], out["write_scale"]))
-        print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes / launch |")
-        print("|---|---|---|---|---|")
+        for label in ("elementwise", "block_gather"):
+            if label + "_fetch_scale" in out:
+                print("%s calibration: FETCH_SIZE x%.3f, WRITE_SIZE x%.3f"
+                      % (label, out[label + "_fetch_scale"], out[label + "_write_scale"]))
+        print("| kernel | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes / launch (copy scale) "
+              "| (block-gather scale) |")
+        print("|---|---|---|---|---|---|")
         for k, v in sorted(out["kernels"].items(), key=lambda kv: -kv[1]["traffic_bytes"]):
-            print("| %s | %d | %.1f | %.1f | %.0f |" % (k, v["dispatches"], v["fetch_kib"], v["write_kib"],
-                                                       v["traffic_bytes"]))
+            print("| %s | %d | %.1f | %.1f | %.0f | %.0f |" % (k, v["dispatches"], v["fetch_kib"], v["write_kib"],
+                                                             v["traffic_bytes"], v["traffic_bytes_gather_scaled"]))
         return 0
     print(__doc__, file=sys.stderr)
     return 2
