@@ -545,8 +545,18 @@ __device__ __forceinline__ void step_begin_body(const double *__restrict__ parts
                                                 double *__restrict__ hist, int advance_t) {
     __shared__ double sm[256];
     if (P.gate && *P.gate == 0.0) return;  // uniform: read before any thread writes it
+    // The max of the partials is exact and order-free (m starts at 1e-8, so
+    // a NaN partial is skipped wherever it comes): sixteen loads per round,
+    // indices past the end clamped to the last partial.
     double m = 1e-8;
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) m = smax(m, parts[b]);
+    constexpr int kRound = 16;
+    for (int b0 = 0; b0 < nparts; b0 += kRound * (int)blockDim.x) {
+        double v[kRound];
+#pragma unroll
+        for (int u = 0; u < kRound; ++u) v[u] = parts[min(b0 + u * (int)blockDim.x + (int)threadIdx.x, nparts - 1)];
+#pragma unroll
+        for (int u = 0; u < kRound; ++u) m = smax(m, v[u]);
+    }
     sm[threadIdx.x] = m;
     __syncthreads();
     for (int s = blockDim.x / 2; s > 0; s >>= 1) {
