@@ -90,6 +90,32 @@ def workload_params(limiter, heat_flux="fourier"):
                 limiter=0 if limiter == "venkat" else 1, interp_mode=1, heat_flux=HEAT_FLUX[heat_flux])
 
 
+def scrambled_mesh(desc, seed=4):
+    """The same mesh with its cells and vertices numbered at random: a mesh
+    numbered without locality, as a mesher may hand one over."""
+    rng = np.random.default_rng(seed)
+    nc, nv = len(desc["cell_nv"]), len(desc["xy"])
+    cperm = rng.permutation(nc)
+    vnew = rng.permutation(nv)
+    vold = np.empty(nv, np.int64)
+    vold[vnew] = np.arange(nv)
+    cv = np.asarray(desc["cell_vtx"])[cperm].copy()
+    cv[cv >= 0] = vnew[cv[cv >= 0]]
+    return dict(xy=np.asarray(desc["xy"])[vold], cell_nv=np.asarray(desc["cell_nv"])[cperm],
+                cell_vtx=cv.astype(np.int32), bnd_vtx=vnew[np.asarray(desc["bnd_vtx"])].astype(np.int32),
+                bnd_tag=np.asarray(desc["bnd_tag"]))
+
+
+def ordered_mesh(desc, order):
+    """generated: the mesher's layer-by-layer numbering; scrambled: a random
+    one; rcm / rcm_xcd: o2f_mesh_reorder applied to the random one."""
+    if order == "generated":
+        return desc
+    from optimisation2forme_amd import capi
+    scr = scrambled_mesh(desc)
+    return scr if order == "scrambled" else capi.reorder_mesh(scr, order)[0]
+
+
 def state_summary(U, gamma=1.4):
     """Physical-range check of a state [4][nc]: minimum density and pressure,
     fraction of cells with non-positive density or pressure."""
@@ -603,6 +629,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--limiter", default="venkat", choices=["venkat", "bj"])
+    ap.add_argument("--mesh-order", default="generated", choices=["generated", "scrambled", "rcm", "rcm_xcd"],
+                    help="numbering of the 200k mesh: the mesher's, a random one, or o2f_mesh_reorder of the "
+                         "random one")
     ap.add_argument("--heat-flux", default="fourier", choices=["fourier", "reference"],
                     help="fourier: q = -mu c_p / Pr grad T; reference: the reference's R times that, which "
                          "leaves the physical range on this case within 100 steps (same step time)")
@@ -681,7 +710,7 @@ def main():
     from optimisation2forme_amd import capi, mesher
 
     flow = workload_params(a.limiter, a.heat_flux)
-    desc = mesher.transonic_200k()
+    desc = ordered_mesh(mesher.transonic_200k(), a.mesh_order)
     kw = dict(flow)
     kw["cfl_number"] = kw.pop("cfl")
     kw["inflow_velocity"] = kw.pop("mach")
@@ -742,7 +771,8 @@ def main():
         "config": {"workload": "naca2412_transonic_200k_triangles", "cells": nc, "vertices": nv,
                    "edges": ne, "mach": 0.85, "aoa_deg": 2.5, "re": 5000, "cfl": 0.5,
                    "limiter": "venkatakrishnan K=5" if a.limiter == "venkat" else "barth_jespersen",
-                   "interp_mode": "mvc", "heat_flux": a.heat_flux, "parallelism": "single GPU"},
+                   "interp_mode": "mvc", "heat_flux": a.heat_flux, "mesh_order": a.mesh_order,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": sum(traffic) if all(t is not None for t in traffic) else None,

@@ -117,6 +117,26 @@ void o2f_msh2_free(o2f_mesh_desc *desc);
 int o2f_mesh_topology(const o2f_mesh_desc *mesh, int *n_edges, int *edge_vtx, int *edge_cells,
                       char *edge_bc, double *edge_geom, double *cell_geom, int *cell_edges);
 
+/* A renumbered copy of a mesh for the device's gathers, and the two
+ * permutations (new -> old): cell_perm[nc], vertex_perm[nv] (either may be
+ * NULL).  No counterpart in the reference, which keeps the mesher's order
+ * (Mesh::Mesh, src/mesh.cpp:9-111); its callers are those who load a mesh
+ * numbered without locality (a state of the renumbered mesh is
+ * U_new[q][i] = U_old[q][cell_perm[i]]).
+ *   O2F_REORDER_RCM      cells by reverse Cuthill-McKee over cells sharing a
+ *                        vertex, vertices in order of first use
+ *   O2F_REORDER_RCM_XCD  the same, then blocks of 64 cells from eight
+ *                        contiguous chunks interleaved (every eighth
+ *                        workgroup, one XCD, covers one contiguous part)
+ * Cell vertex order (orientation) and boundary elements are kept.  The
+ * renumbered mesh has other edge ids, so results on it differ from the
+ * original's in the last bits (the sums over a cell's edges run in
+ * ascending edge id).  Free *out with o2f_msh2_free. */
+#define O2F_REORDER_RCM 0
+#define O2F_REORDER_RCM_XCD 1
+int o2f_mesh_reorder(const o2f_mesh_desc *mesh, int method, o2f_mesh_desc *out, int *cell_perm,
+                     int *vertex_perm);
+
 /* ---- context -------------------------------------------------------- */
 
 /* Builds topology and geometry (Mesh::Mesh, src/mesh.cpp:112-248) on the

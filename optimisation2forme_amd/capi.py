@@ -34,7 +34,7 @@ EXPORTS = [
     "o2f_set_face_states", "o2f_hllc_flux_batch", "o2f_compute_dt", "o2f_rk3_step",
     "o2f_run_steps", "o2f_time_loop", "o2f_residual_norm", "o2f_lift_drag", "o2f_objective",
     "o2f_adjoint_rhs", "o2f_apply_jt", "o2f_bicgstab", "o2f_shape_gradient", "o2f_time_steps",
-    "o2f_mesh_topology", "o2f_run_until", "o2f_run_to_residual", "o2f_batch_create",
+    "o2f_mesh_topology", "o2f_mesh_reorder", "o2f_run_until", "o2f_run_to_residual", "o2f_batch_create",
     "o2f_batch_destroy", "o2f_batch_last_error", "o2f_batch_set_shapes", "o2f_batch_get_shapes",
     "o2f_batch_autotune", "o2f_batch_run", "o2f_batch_time_steps", "o2f_apply_jt_exact",
     "o2f_bicgstab_exact", "o2f_mlp_create", "o2f_mlp_destroy", "o2f_mlp_last_error", "o2f_mlp_num_params",
@@ -129,6 +129,7 @@ def lib():
                                                      C.c_int, D]),
             "o2f_time_steps": (C.c_int, [vp, C.c_int, D, D]),
             "o2f_mesh_topology": (C.c_int, [C.POINTER(MeshDesc), I, I, I, C.c_char_p, D, D, I]),
+            "o2f_mesh_reorder": (C.c_int, [C.POINTER(MeshDesc), C.c_int, C.POINTER(MeshDesc), I, I]),
             "o2f_run_until": (C.c_int, [vp, C.c_double, C.c_int, I, D, D, D]),
             "o2f_run_to_residual": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
             "o2f_apply_jt_exact": (C.c_int, [vp, D, D]),
@@ -224,13 +225,8 @@ def mesh_topology(desc):
             "edges": ce.reshape(nc, 4)}
 
 
-def read_msh2(path):
-    """o2f_msh2_read -> dict of numpy arrays (the Gmsh API element order)."""
-    L = lib()
-    d = MeshDesc()
-    st = L.o2f_msh2_read(path.encode(), C.byref(d))
-    if st != O2F_OK:
-        raise O2FError(st, L.o2f_global_error().decode())
+def _owned_desc(L, d):
+    """A library-owned mesh description -> dict of numpy arrays (then freed)."""
     try:
         nv, nc, nb = d.n_vertices, d.n_cells, d.n_bnd
         out = {
@@ -244,6 +240,34 @@ def read_msh2(path):
     finally:
         L.o2f_msh2_free(C.byref(d))
     return out
+
+
+def read_msh2(path):
+    """o2f_msh2_read -> dict of numpy arrays (the Gmsh API element order)."""
+    L = lib()
+    d = MeshDesc()
+    st = L.o2f_msh2_read(path.encode(), C.byref(d))
+    if st != O2F_OK:
+        raise O2FError(st, L.o2f_global_error().decode())
+    return _owned_desc(L, d)
+
+
+REORDER = {"rcm": 0, "rcm_xcd": 1}
+
+
+def reorder_mesh(desc, method="rcm"):
+    """o2f_mesh_reorder -> (renumbered mesh, cell_perm, vertex_perm), both
+    permutations new -> old: a state maps as U_new = U_old[:, cell_perm]."""
+    L = lib()
+    md, keep = _mesh_desc(desc)
+    out = MeshDesc()
+    nc, nv = md.n_cells, md.n_vertices
+    cperm = np.zeros(nc, np.int32)
+    vperm = np.zeros(nv, np.int32)
+    st = L.o2f_mesh_reorder(C.byref(md), REORDER.get(method, method), C.byref(out), _i(cperm), _i(vperm))
+    if st != O2F_OK:
+        raise O2FError(st, L.o2f_global_error().decode())
+    return _owned_desc(L, out), cperm, vperm
 
 
 class Solver:
