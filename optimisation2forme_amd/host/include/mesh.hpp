@@ -67,6 +67,12 @@ struct Mesh {
     Mesh &operator=(const Mesh &) = delete;
 
     o2f_ctx *ctx() const { return ctx_; }
+    // mesh_reorder != "none": this mesh's cell i is the mesh file's cell
+    // cell_perm[i] (empty: the file's numbering)
+    const std::vector<int> &cell_perm() const { return cell_perm_; }
+    // a state [4][n_cells] from the file's cell order to this mesh's, and back
+    void state_from_file_order(std::vector<double> &U) const;
+    void state_to_file_order(std::vector<double> &U) const;
     void set_params(const SimulationParams &params);  // hot reload
     // host cell state -> device, device -> host cell state
     void push_state();
@@ -80,6 +86,7 @@ struct Mesh {
 
    private:
     o2f_ctx *ctx_ = nullptr;
+    std::vector<int> cell_perm_;
 };
 
 }  // namespace navier_stokes

@@ -73,6 +73,9 @@ struct SimulationParams {
     // reference's loop body on the host, one dt reduction and one
     // step_SSP_RK3 (three rk3_substep calls) per step
     int device_time_loop = 1;
+    // "none" | "rcm" | "rcm_xcd": renumber the mesh for the device's gathers
+    // (o2f_mesh_reorder); state files keep the mesh file's cell order
+    std::string mesh_reorder = "none";
 
     // c_v = c_p / gamma, R = c_p - c_v (src/params.cpp:5-8)
     void finalize();

@@ -20,6 +20,20 @@ Mesh::Mesh(const SimulationParams &params) {
         o2f_mesh_desc *d;
         ~Free() { o2f_msh2_free(d); }
     } guard{&d};
+    if (params.mesh_reorder != "none") {
+        int method;
+        if (params.mesh_reorder == "rcm")
+            method = O2F_REORDER_RCM;
+        else if (params.mesh_reorder == "rcm_xcd")
+            method = O2F_REORDER_RCM_XCD;
+        else
+            throw std::invalid_argument("Unknown mesh_reorder: " + params.mesh_reorder);
+        o2f_mesh_desc r{};
+        cell_perm_.resize(d.n_cells);
+        check(o2f_mesh_reorder(&d, method, &r, cell_perm_.data(), nullptr));
+        o2f_msh2_free(&d);
+        d = r;  // freed by the guard
+    }
     if (params.verbosity > 0) std::printf("\nNombre de Noeuds : %d\n", d.n_vertices);
     int ne = 0;
     check(o2f_mesh_topology(&d, &ne, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
@@ -82,6 +96,24 @@ Mesh::Mesh(const SimulationParams &params) {
 }
 
 Mesh::~Mesh() { o2f_ctx_destroy(ctx_); }
+
+void Mesh::state_from_file_order(std::vector<double> &U) const {
+    if (cell_perm_.empty()) return;
+    const size_t nc = cell_perm_.size();
+    std::vector<double> out(U.size());
+    for (size_t q = 0; q < 4; ++q)
+        for (size_t i = 0; i < nc; ++i) out[q * nc + i] = U[q * nc + cell_perm_[i]];
+    U.swap(out);
+}
+
+void Mesh::state_to_file_order(std::vector<double> &U) const {
+    if (cell_perm_.empty()) return;
+    const size_t nc = cell_perm_.size();
+    std::vector<double> out(U.size());
+    for (size_t q = 0; q < 4; ++q)
+        for (size_t i = 0; i < nc; ++i) out[q * nc + cell_perm_[i]] = U[q * nc + i];
+    U.swap(out);
+}
 
 void Mesh::set_params(const SimulationParams &params) {
     const o2f_params op = to_o2f_params(params);

@@ -64,6 +64,12 @@ void loadCellStates(Mesh &mesh, const SimulationParams &params) {
     const std::string &path = params.state_load_path;
     if (path.empty() || path == "none") return;
     mesh.check(o2f_load_state_file(mesh.ctx(), path.c_str()));
+    if (!mesh.cell_perm().empty()) {  // the file is in the mesh file's cell order
+        std::vector<double> U(4 * mesh.cells.size());
+        mesh.check(o2f_get_state(mesh.ctx(), U.data()));
+        mesh.state_from_file_order(U);
+        mesh.check(o2f_set_state(mesh.ctx(), U.data()));
+    }
     mesh.pull_state();
 }
 
@@ -76,7 +82,19 @@ void saveCellStates(const Mesh &mesh, const SimulationParams &params) {
         if (!std::filesystem::create_directories(p.parent_path(), ec) && ec)
             throw std::runtime_error("Impossible de créer le dossier: " + p.parent_path().string());
     }
-    mesh.check(o2f_save_state_file(mesh.ctx(), path.c_str()));
+    if (mesh.cell_perm().empty()) {
+        mesh.check(o2f_save_state_file(mesh.ctx(), path.c_str()));
+        return;
+    }
+    // written in the mesh file's cell order, then the state put back
+    std::vector<double> U(4 * mesh.cells.size());
+    mesh.check(o2f_get_state(mesh.ctx(), U.data()));
+    std::vector<double> F(U);
+    mesh.state_to_file_order(F);
+    mesh.check(o2f_set_state(mesh.ctx(), F.data()));
+    const int st = o2f_save_state_file(mesh.ctx(), path.c_str());
+    mesh.check(o2f_set_state(mesh.ctx(), U.data()));
+    mesh.check(st);
 }
 
 void initialize_flow_field(Mesh &mesh, const SimulationParams &params) {

@@ -69,6 +69,7 @@ constexpr Key kKeys[] = {
     {"limiter", &P::limiter},
     {"max_steps", &P::max_steps},
     {"mesh_name", &P::mesh_name},
+    {"mesh_reorder", &P::mesh_reorder},
     {"noise_level", &P::noise_level},
     {"obstacle_center_x", &P::obstacle_center_x},
     {"obstacle_center_y", &P::obstacle_center_y},

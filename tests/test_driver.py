@@ -40,6 +40,7 @@ adjoint_operator = exact
 adjoint_rhs = exact
 adjoint_preconditioner = block_jacobi
 shape_gradient = tangent
+mesh_reorder = rcm_xcd
 state_load_path = a/b
 mesh_name = mesh_opti/naca_2D_shape_0.msh
 Lx = 2.5e0xyz
@@ -56,6 +57,7 @@ Ny = not_a_number
     assert int(kv["reload_config"]) == 1000 and kv["state_load_path"] == "a/b"
     assert kv["adjoint_operator"] == "exact" and kv["adjoint_rhs"] == "exact"
     assert kv["adjoint_preconditioner"] == "block_jacobi" and kv["shape_gradient"] == "tangent"
+    assert kv["mesh_reorder"] == "rcm_xcd"
     # finalize(): c_v = c_p / gamma, R = c_p - c_v (src/params.cpp:5-8)
     assert float(kv["cv"]) == 1005.0 / 1.4 and float(kv["R"]) == 1005.0 - 1005.0 / 1.4
     # numbers as operator>> reads them: the longest valid prefix, 0 when none
