@@ -569,7 +569,7 @@ def bench_adjoint(a):
     t0 = time.perf_counter()
     _, bicg_it, _, _ = s.bicgstab(50.0, tol=1e-300, chunk=300, max_total=300, check=False)
     bicg_us = 1e6 * (time.perf_counter() - t0) / max(1, bicg_it)
-    B = 72 * s.nc + 96 * s.ne          # per product: blocks / state / result, the survey's figure
+    B = 72 * s.nc + 96 * s.ne          # algorithmic bytes per product: blocks / state / result
     out = {"metric": "adjoint J^T x products", "unit": "ms per product", "higher_is_better": False,
            "n_gpus": 1, "dtype": "f64", "steps": n,
            "config": {"workload": "naca2412_transonic_200k_triangles", "cells": s.nc, "edges": s.ne},

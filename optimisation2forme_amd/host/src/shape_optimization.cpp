@@ -136,10 +136,10 @@ void print_preview(double J, const std::vector<std::string> &names, const std::v
                        "[preview] Gradient de forme (dJ/dθ) et sens pour diminuer J :\n";
     for (size_t k = 0; k < names.size(); ++k) {
         const double g = grad[k];
-        const char *advice = g > 0.0   ? "-> diminuer ce paramètre (pas négatif)"
+        const char *hint = g > 0.0   ? "-> diminuer ce paramètre (pas négatif)"
                              : g < 0.0 ? "-> augmenter ce paramètre (pas positif)"
                                        : "-> gradient ~ 0 (aucune direction claire)";
-        text += "  - " + names[k] + " : dJ/d" + names[k] + " = " + format("%.8f", g) + "    " + advice + "\n";
+        text += "  - " + names[k] + " : dJ/d" + names[k] + " = " + format("%.8f", g) + "    " + hint + "\n";
     }
     std::cout << text << std::flush;
 }

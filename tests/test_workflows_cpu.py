@@ -56,7 +56,7 @@ def test_raw_inputs_layout():
     assert np.allclose(raw[:, 3:], [c["m"], c["p"], c["t"], c["mach"], c["aoa"]], rtol=1e-6)
 
 
-def test_batch_candidates_cover_the_survey_ranges():
+def test_batch_candidates_cover_the_specified_ranges():
     cands = bench.batch_candidates()
     assert len(cands) == 64
     tri = [2 * c["n_around"] * (c["n_radial"] - 1) for c in cands]
