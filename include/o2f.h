@@ -465,11 +465,30 @@ int o2f_time_adjoint(o2f_ctx *ctx, int nprod, int exact, double *total_ms);
  * kernel trace reports it): kernel 0 the vertex gather, 1 the cell
  * reconstruction, 2 the vertex viscous gather + edge flux, 3 the flux sum
  * and viscous divergence in residual form (no stage update), 4 the whole
- * residual (those four launches in order, per "launch").  Every launch
- * recomputes the same outputs from the same inputs.  *avg_us receives the
- * microseconds per launch.  A measurement entry point of this library (no
- * counterpart in the reference). */
+ * residual (those four launches in order, per "launch"), 5 one fused tile
+ * stage (residual and stage update of every cell in one launch, written to
+ * a spare state buffer; O2F_ERR_ARG when the context does not tile).  Every
+ * launch recomputes the same outputs from the same inputs.  *avg_us
+ * receives the microseconds per launch.  A measurement entry point of this
+ * library (no counterpart in the reference). */
 int o2f_time_kernel(o2f_ctx *ctx, int kernel, int launches, double *avg_us);
+
+/* ---- fused tile stages (no counterpart in the reference) ------------ */
+
+/* The time-stepping entry points (o2f_run_steps, o2f_run_until,
+ * o2f_time_loop, o2f_time_steps) evaluate each SSP-RK3 stage in one launch
+ * over compact tiles of cells, one workgroup per tile, with the stage's
+ * intermediates in LDS (the mesh's rim entities formed by every tile that
+ * needs them).  Results are bit-identical to the four-kernel stages.
+ * cells: -1 (the default) sizes the tiles from the device's CU count and
+ * the kernel's occupancy, 0 uses the four-kernel stages, n > 0 caps a tile
+ * at n cells.  The tables are built on the next step. */
+int o2f_set_tile_cells(o2f_ctx *ctx, int cells);
+/* The tiling in use (built now if needed): *n_tiles 0 when the stages are
+ * the four-kernel ones (o2f_last_error then says why, unless cells was 0);
+ * the cell cap, the LDS bytes of one tile, the tiles one CU holds at once
+ * and the kernel's VGPRs.  Any pointer may be NULL. */
+int o2f_tile_info(o2f_ctx *ctx, int *n_tiles, int *cells, int *lds_bytes, int *tiles_per_cu, int *vgprs);
 
 #ifdef __cplusplus
 }

@@ -18,8 +18,9 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libo2f.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["host_mesh.cpp", "host_reorder.cpp", "host_tangent.cpp", "o2f_kernels.hip", "o2f_adjoint.hip",
-           "o2f_tangent.hip", "o2f_capi.cpp", "o2f_batch.cpp", "o2f_mlp.hip", "o2f_noalloc.hip"]
+SOURCES = ["host_mesh.cpp", "host_reorder.cpp", "host_tangent.cpp", "host_tiles.cpp", "o2f_kernels.hip",
+           "o2f_adjoint.hip", "o2f_tangent.hip", "o2f_tiles.hip", "o2f_capi.cpp", "o2f_batch.cpp", "o2f_mlp.hip",
+           "o2f_noalloc.hip"]
 
 # -ffp-contract=off: no fused multiply-add anywhere (host or device), so the
 # results round exactly like the reference's C++ expressions.  Packed FP32

@@ -43,7 +43,7 @@ EXPORTS = [
     "o2f_residual_tangent", "o2f_shape_gradient_tangent", "o2f_adjoint_rhs_exact",
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
     "o2f_rk3_save_old", "o2f_rk3_substep", "o2f_bicgstab_set_restart_eps",
-    "o2f_hllc_flux_batch_noalloc", "o2f_batch_run_until",
+    "o2f_hllc_flux_batch_noalloc", "o2f_batch_run_until", "o2f_set_tile_cells", "o2f_tile_info",
 ]
 
 
@@ -135,6 +135,8 @@ def lib():
             "o2f_apply_jt_exact": (C.c_int, [vp, D, D]),
             "o2f_time_adjoint": (C.c_int, [vp, C.c_int, C.c_int, D]),
             "o2f_time_kernel": (C.c_int, [vp, C.c_int, C.c_int, D]),
+            "o2f_set_tile_cells": (C.c_int, [vp, C.c_int]),
+            "o2f_tile_info": (C.c_int, [vp, I, I, I, I, I]),
             "o2f_mlp_create": (C.c_int, [C.c_ulonglong, C.POINTER(vp)]),
             "o2f_mlp_destroy": (None, [vp]),
             "o2f_mlp_last_error": (C.c_char_p, [vp]),
@@ -527,10 +529,25 @@ class Solver:
         self._check(lib().o2f_time_adjoint(self.h, nprod, int(exact), C.byref(ms)))
         return ms.value
 
+    def set_tile_cells(self, cells):
+        """Stages of the time-stepping entry points: -1 fused tile stages
+        sized automatically (the default), 0 the four-kernel stages, n > 0
+        fused tile stages of at most n cells per tile."""
+        self._check(lib().o2f_set_tile_cells(self.h, int(cells)))
+
+    def tile_info(self):
+        """The tiling in use: dict of tiles, cells (per-tile cap), lds_bytes,
+        tiles_per_cu, vgprs (tiles 0: the four-kernel stages)."""
+        v = [C.c_int() for _ in range(5)]
+        self._check(lib().o2f_tile_info(self.h, *[C.byref(x) for x in v]))
+        keys = ("tiles", "cells", "lds_bytes", "tiles_per_cu", "vgprs")
+        return {k: x.value for k, x in zip(keys, v)}
+
     def time_kernel(self, kernel, launches):
         """Microseconds per launch of one stage kernel (0 vertex gather,
         1 reconstruction, 2 viscous gather + edge flux, 3 flux sum in
-        residual form), launched back to back at the current state."""
+        residual form, 4 those four in order, 5 one fused tile stage),
+        launched back to back at the current state."""
         us = C.c_double()
         self._check(lib().o2f_time_kernel(self.h, int(kernel), int(launches), C.byref(us)))
         return us.value

@@ -80,19 +80,23 @@ const long kNone = -1;
 // geometry set g.  Four kernels: vertex gather, cell reconstruction,
 // viscous vertex gather, flux + update.
 // begin_step: the step's dt reduction rides in the vertex gather launch.
+// Uin: the state to evaluate instead of c->U (flags without kFlagRK).
 int run_stage(o2f_ctx *c, int g, int flags, const double *dtp, double c0, double c1, bool store_grad,
-              bool begin_step = false, double *hist = nullptr) {
+              bool begin_step = false, double *hist = nullptr, double *Uin = nullptr) {
     const Topo T = c->topo();
     const Geom G = c->geo[g].view();
     hipStream_t s = c->stream;
+    double *U = Uin ? Uin : c->U;
     if (begin_step)
-        HIPCHK(c, launch_step_gather(T, G, c->U, c->vU, c->phys, c->speed, c->hm.min_h, c->scal, hist, 1, s));
+        HIPCHK(c, launch_step_gather(T, G, U, c->vU, c->phys, c->speed, c->speed_nparts, c->hm.min_h, c->scal, hist,
+                                     1, s));
     else
-        HIPCHK(c, launch_vertex_state(T, G, c->U, c->vU, c->phys, s));
-    HIPCHK(c, launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, store_grad ? c->Gr : nullptr, c->phys, s));
+        HIPCHK(c, launch_vertex_state(T, G, U, c->vU, c->phys, s));
+    HIPCHK(c, launch_reconstruct(T, G, U, c->vU, c->fL, c->fR, c->tq, store_grad ? c->Gr : nullptr, c->phys, s));
     HIPCHK(c, launch_visc_and_flux(T, G, c->tq, c->vtq, c->fL, c->fR, c->Fe, c->phys, 1, s));
-    HIPCHK(c, launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, dtp, c0, c1,
-                                 flags | kFlagAdv | kFlagVisc, c->phys, s, c->speed));
+    HIPCHK(c, launch_flux_update(T, G, U, c->Uold, c->Fe, c->vtq, c->R, dtp, c0, c1, flags | kFlagAdv | kFlagVisc,
+                                 c->phys, s, c->speed));
+    if (flags & kFlagSpeed) c->speed_nparts = speed_parts(c->hm.nc);
     ++c->faces_version;
     if (store_grad) c->have_grad = true;
     return O2F_OK;
@@ -122,15 +126,166 @@ int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed, bool begin_step =
                      1.0 / 3.0, 2.0 / 3.0, last_outputs);
 }
 
+void release_tiles(o2f_ctx *c) {
+    for (void *p : {c->tile_mem, (void *)c->Ub, (void *)c->Uc})
+        if (p) (void)hipFree(p);
+    c->tile_mem = nullptr;
+    c->Ub = c->Uc = nullptr;
+    c->tv = TileView{};
+    c->tiles_ready = false;
+}
+
+// The tile tables of the current mesh and stencils on the device (built on
+// first use).  Returns false when the stages stay the four-kernel ones.
+// Automatic size: the fused kernel holds tiles resident on every CU at once
+// (blocks_per_cu of them, set by its VGPRs and its LDS), so the tiles are
+// made just large enough that one round covers the mesh: the rim each tile
+// recomputes shrinks as the tiles grow, and no CU waits on a second round.
+bool tiles_on(o2f_ctx *c) {
+    if (c->tile_cells == 0 || c->tiles_unusable) return false;
+    if (c->tiles_ready) return true;
+    release_tiles(c);
+    auto unusable = [&](const std::string &why) {
+        c->tiles_unusable = true;
+        c->tiles_why = why;
+        return false;
+    };
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+        return unusable("no device attributes");
+    const int nc = c->hm.nc;
+    TileSet ts;
+    std::string err;
+    int bpc = 0, vgprs = 0, target = c->tile_cells;
+    auto build = [&](int t) -> bool {
+        if (!build_tiles(c->hm, t, ts, err)) return false;
+        const hipError_t e = tile_stage_attributes(ts.lds_bytes(), &bpc, &vgprs);
+        if (e != hipSuccess) err = std::string("tile kernel attributes: ") + hipGetErrorString(e);
+        return e == hipSuccess && bpc > 0;
+    };
+    if (target > 0) {
+        if (!build(target)) return unusable(err.empty() ? "tile LDS too large" : err);
+    } else {
+        // largest resident count first; tiles of at least 96 cells
+        int resident = 0;
+        if (!build(192)) return unusable(err.empty() ? "tile LDS too large" : err);
+        resident = bpc;
+        for (int m = resident; m >= 1; --m) {
+            target = std::max(96, (int)std::ceil(nc / (0.985 * m * ncu)));
+            bool ok = build(target);
+            for (int it = 0; ok && it < 12 && (int)ts.head.size() > m * ncu; ++it)
+                ok = build(target = (int)std::ceil(target * 1.02));
+            if (ok && bpc >= m && (int)ts.head.size() <= m * ncu) break;
+            if (m == 1) return unusable("no tile size fits");
+        }
+    }
+    // one allocation: heads, cells, owned-cell slots, vertices, edges, stencils
+    size_t off = 0;
+    auto slot = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_head = slot(ts.head.size() * sizeof(TileHead)), o_cell = slot(ts.cell.size() * sizeof(TileCell)),
+                 o_own = slot(ts.own.size() * sizeof(TileOwn)), o_vg = slot(ts.vg.size() * sizeof(TileVertex)),
+                 o_edge = slot(ts.edge.size() * sizeof(int)), o_vo = slot(ts.vo.size() * sizeof(TileVertex)),
+                 o_ls = slot(ts.lsten.size() * sizeof(uint16_t) + 16);
+    char *m = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&m), off);
+    if (e == hipSuccess) c->tile_mem = m;
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpy(m + o, src, bytes, hipMemcpyHostToDevice);
+    };
+    put(o_head, ts.head.data(), ts.head.size() * sizeof(TileHead));
+    put(o_cell, ts.cell.data(), ts.cell.size() * sizeof(TileCell));
+    put(o_own, ts.own.data(), ts.own.size() * sizeof(TileOwn));
+    put(o_vg, ts.vg.data(), ts.vg.size() * sizeof(TileVertex));
+    put(o_edge, ts.edge.data(), ts.edge.size() * sizeof(int));
+    put(o_vo, ts.vo.data(), ts.vo.size() * sizeof(TileVertex));
+    put(o_ls, ts.lsten.data(), ts.lsten.size() * sizeof(uint16_t));
+    if (e == hipSuccess) e = dalloc(c->Ub, 4 * (size_t)nc);
+    if (e == hipSuccess) e = dalloc(c->Uc, 4 * (size_t)nc);
+    // one speed partial per tile: small tiles outnumber the four-lane blocks
+    const int nparts = std::max((int)ts.head.size(), speed_parts(nc));
+    if (e == hipSuccess && nparts > speed_parts(nc)) {
+        (void)hipFree(c->speed);
+        c->speed = nullptr;
+        c->speed_valid = false;
+        e = dalloc(c->speed, nparts);
+    }
+    if (e != hipSuccess) {
+        release_tiles(c);
+        return unusable(std::string("tile tables: ") + hipGetErrorString(e));
+    }
+    TileView &v = c->tv;
+    v.head = reinterpret_cast<const TileHead *>(m + o_head);
+    v.cell = reinterpret_cast<const TileCell *>(m + o_cell);
+    v.own = reinterpret_cast<const TileOwn *>(m + o_own);
+    v.vg = reinterpret_cast<const TileVertex *>(m + o_vg);
+    v.edge = reinterpret_cast<const int *>(m + o_edge);
+    v.vo = reinterpret_cast<const TileVertex *>(m + o_vo);
+    v.lsten = reinterpret_cast<const uint16_t *>(m + o_ls);
+    v.ntiles = (int)ts.head.size();
+    v.max_vo = ts.max_vo;
+    v.max_gd = ts.max_gd;
+    const int vert = std::max(4 * ts.max_vg, 5 * ts.max_vo);
+    v.lds_tq = (vert + 3) & ~3;
+    v.lds_fs = v.lds_tq + ((5 * ts.max_gd + 3) & ~3);
+    v.lds_bytes = sizeof(double) * (size_t)(v.lds_fs + 8 * ts.max_e);
+    c->tile_target = ts.target;
+    c->tile_bpc = bpc;
+    c->tile_vgprs = vgprs;
+    c->tiles_ready = true;
+    return true;
+}
+
+// One step on the tiles: dt, then three fused stages, U -> Ub -> Uc -> U
+// (each stage reads one buffer and writes another, so no tile overwrites a
+// state its neighbours still read).  last_outputs: the third stage runs on
+// the four kernels instead, in place (U <- Ub -> U), and leaves the
+// intermediates the adjoint and the force coefficients read.
+int tiled_step(o2f_ctx *c, bool last_outputs, double *hist) {
+    hipStream_t s = c->stream;
+    const Topo T = c->topo();
+    const Geom G = c->geo[0].view();
+    if (!c->speed_valid) {
+        HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
+        c->speed_nparts = speed_parts(c->hm.nc);
+    }
+    HIPCHK(c, launch_step_begin(c->speed, c->speed_nparts, c->phys, c->hm.min_h, c->scal, hist, 1, s));
+    const double *dtp = c->scal + kScalDt;
+    const TileView &A = c->tv;
+    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->U, c->Ub, c->Uold, c->R, dtp, 0.0, 1.0, kFlagRK | kFlagSaveOld,
+                                nullptr, s));
+    if (last_outputs) {
+        HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->U, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0, kFlagRK,
+                                    nullptr, s));
+        return run_stage(c, 0, kFlagRK | kFlagStoreR | kFlagSpeed, dtp, 1.0 / 3.0, 2.0 / 3.0, true);
+    }
+    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->Uc, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0, kFlagRK,
+                                nullptr, s));
+    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Uc, c->U, c->Uold, c->R, dtp, 1.0 / 3.0, 2.0 / 3.0,
+                                kFlagRK | kFlagSpeed, c->speed, s));
+    c->speed_nparts = A.ntiles;
+    return O2F_OK;
+}
+
 // One step of the time loop body: dt from the speed word, the three stages,
 // and (fused into the last stage) the speed of the new state for the next
 // step's dt.
 int one_step(o2f_ctx *c, bool last_outputs, double *hist) {
     hipStream_t s = c->stream;
-    if (!c->speed_valid) {
-        HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
+    int st;
+    if (tiles_on(c)) {
+        st = tiled_step(c, last_outputs, hist);
+    } else {
+        if (!c->speed_valid) {
+            HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
+            c->speed_nparts = speed_parts(c->hm.nc);
+        }
+        st = rk3_stages(c, last_outputs, true, true, hist);
     }
-    int st = rk3_stages(c, last_outputs, true, true, hist);
     c->speed_valid = (st == O2F_OK);
     return st;
 }
@@ -356,6 +511,7 @@ void o2f_ctx_destroy(o2f_ctx *c) {
     for (double *p : c->kv)
         if (p) (void)hipFree(p);
     for (auto &g : c->geo) g.release();
+    release_tiles(c);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -379,6 +535,7 @@ int o2f_set_params(o2f_ctx *c, const o2f_params *p) {
         HIPCHK(c, hipMemcpyAsync(c->scell, c->hm.scell.data(), c->hm.scell.size() * sizeof(int),
                                  hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, c->geo[0].fill(c->hm, c->stream));
+        release_tiles(c);  // the stress vertices' stencils changed
     }
     return O2F_OK;
 }
@@ -572,7 +729,8 @@ int o2f_hllc_flux_batch(const double *qL, const double *qR, const double *normal
 int o2f_compute_dt(o2f_ctx *c, double *dt) {
     if (!c || !dt) return O2F_ERR_ARG;
     HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, c->stream));
-    HIPCHK(c, launch_step_begin(c->speed, c->hm.nc, c->phys, c->hm.min_h, c->scal, nullptr, 0, c->stream));
+    HIPCHK(c, launch_step_begin(c->speed, speed_parts(c->hm.nc), c->phys, c->hm.min_h, c->scal, nullptr, 0,
+                                c->stream));
     HIPCHK(c, hipMemcpyAsync(dt, c->scal + kScalDt, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->speed_valid = false;
@@ -649,11 +807,15 @@ int o2f_run_until(o2f_ctx *c, double final_time, int max_steps, int *steps, doub
     c->phys.final_t = final_time;
     double h[kScalCount];
     int launched = 0;
+    // Any step may be the last: the four-kernel stages store the
+    // intermediates of every third stage; tiled steps store none, and the
+    // last one's are formed once the loop has stopped.
+    const bool tiled = tiles_on(c);
     for (;;) {
         int batch = 64;
         if (max_steps >= 0) batch = std::min(batch, max_steps - launched);
         if (batch <= 0) break;
-        for (int k = 0; k < batch && st == 0; ++k) st = one_step(c, true, nullptr);
+        for (int k = 0; k < batch && st == 0; ++k) st = one_step(c, !tiled, nullptr);
         launched += batch;
         if (st) break;
         hipError_t e = hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s);
@@ -669,6 +831,12 @@ int o2f_run_until(o2f_ctx *c, double final_time, int max_steps, int *steps, doub
     if (st) return st;
     HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
+    // the last step's third stage read its input from Uc: its residual,
+    // gradients, face states and stresses, as that stage's four kernels
+    // would have left them
+    if (tiled && h[kScalSteps] > 0.0 &&
+        (st = run_stage(c, 0, kFlagStoreR, nullptr, 0.0, 0.0, true, false, nullptr, c->Uc)))
+        return st;
     if (steps) *steps = static_cast<int>(h[kScalSteps]);
     if (t_inout) *t_inout = h[kScalT];
     if (t_last_start) *t_last_start = h[kScalTPrev];
@@ -1556,9 +1724,11 @@ int o2f_time_adjoint(o2f_ctx *c, int nprod, int exact, double *total_ms) {
 }
 
 int o2f_time_kernel(o2f_ctx *c, int kernel, int launches, double *avg_us) {
-    if (!c || kernel < 0 || kernel > 4 || launches <= 0 || !avg_us) return O2F_ERR_ARG;
+    if (!c || kernel < 0 || kernel > 5 || launches <= 0 || !avg_us) return O2F_ERR_ARG;
     int st = check_bc(c);
     if (st) return st;
+    if (kernel == 5 && !tiles_on(c))
+        return c->fail(O2F_ERR_ARG, "o2f_time_kernel: no tile stages (" + c->tiles_why + ")");
     // every intermediate the kernels read, at the current state
     if ((st = residual_on(c, 0))) return st;
     hipStream_t s = c->stream;
@@ -1572,6 +1742,9 @@ int o2f_time_kernel(o2f_ctx *c, int kernel, int launches, double *avg_us) {
             case 3:
                 return launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, nullptr, 0.0, 0.0,
                                           kFlagAdv | kFlagVisc | kFlagStoreR, c->phys, s);
+            case 5:  // a second stage's work: U_old read, the new state to Ub
+                return launch_tile_stage(T, G, c->phys, c->tv, c->U, c->Ub, c->Uold, c->R, c->scal + kScalDt,
+                                         3.0 / 4.0, 1.0 / 4.0, kFlagRK, nullptr, s);
             default: {  // the whole residual, its four launches in order
                 hipError_t e = launch_vertex_state(T, G, c->U, c->vU, c->phys, s);
                 if (!e) e = launch_reconstruct(T, G, c->U, c->vU, c->fL, c->fR, c->tq, nullptr, c->phys, s);
@@ -1597,6 +1770,28 @@ int o2f_time_kernel(o2f_ctx *c, int kernel, int launches, double *avg_us) {
     (void)hipEventDestroy(e1);
     *avg_us = 1e3 * ms / launches;
     ++c->faces_version;
+    return O2F_OK;
+}
+
+int o2f_set_tile_cells(o2f_ctx *c, int cells) {
+    if (!c || cells < -1) return O2F_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    release_tiles(c);
+    c->tile_cells = cells;
+    c->tiles_unusable = false;
+    c->tiles_why.clear();
+    return O2F_OK;
+}
+
+int o2f_tile_info(o2f_ctx *c, int *n_tiles, int *cells, int *lds_bytes, int *tiles_per_cu, int *vgprs) {
+    if (!c) return O2F_ERR_ARG;
+    const bool on = tiles_on(c);
+    if (!on && c->tile_cells != 0) c->err = "four-kernel stages: " + c->tiles_why;
+    if (n_tiles) *n_tiles = on ? c->tv.ntiles : 0;
+    if (cells) *cells = on ? c->tile_target : 0;
+    if (lds_bytes) *lds_bytes = on ? (int)c->tv.lds_bytes : 0;
+    if (tiles_per_cu) *tiles_per_cu = on ? c->tile_bpc : 0;
+    if (vgprs) *vgprs = on ? c->tile_vgprs : 0;
     return O2F_OK;
 }
 
@@ -1630,9 +1825,10 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
         for (int k = 0; k < nsteps; ++k) {
             if (!c->speed_valid) {
                 HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
+                c->speed_nparts = speed_parts(c->hm.nc);
                 c->speed_valid = true;
             }
-            HIPCHK(c, launch_step_begin(c->speed, c->hm.nc, c->phys, c->hm.min_h, c->scal, nullptr, 1, s));
+            HIPCHK(c, launch_step_begin(c->speed, c->speed_nparts, c->phys, c->hm.min_h, c->scal, nullptr, 1, s));
             for (int stg = 0; stg < 3; ++stg) {
                 const int fl = kFlagAdv | kFlagVisc | kFlagRK | (stg == 0 ? kFlagSaveOld : 0) |
                                (stg == 2 ? kFlagSpeed : 0);
@@ -1650,6 +1846,7 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
                                              cs[stg][1], fl, c->phys, s, c->speed));
                 HIPCHK(c, hipEventRecord(ev[q++], s));
             }
+            c->speed_nparts = speed_parts(c->hm.nc);
         }
         HIPCHK(c, hipStreamSynchronize(s));
         for (int j = 0; j < nk; ++j) kernel_ms[j] = 0.0;

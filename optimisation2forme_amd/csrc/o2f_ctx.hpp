@@ -10,6 +10,7 @@
 #include "host_mesh.hpp"
 #include "o2f.h"
 #include "o2f_kernels.hpp"
+#include "o2f_tiles.hpp"
 
 namespace o2f {
 
@@ -90,6 +91,18 @@ struct o2f_ctx {
     double *scal = nullptr, *partials = nullptr, *wall_out = nullptr;
     double *speed = nullptr;              // per-block max_c(|u|+c) partials of the next step's dt
     bool speed_valid = false;             // it holds the speed of the current U
+    int speed_nparts = 0;                 // partials the last speed fold wrote
+    // fused tile stages (host_tiles.hpp, o2f_tiles.hip), built on first use:
+    // tile_cells -1 sizes the tiles from the kernel's occupancy, 0 keeps the
+    // four-kernel stages, > 0 caps a tile at that many cells
+    int tile_cells = -1;
+    bool tiles_ready = false;     // tables built for the current mesh and stencils
+    bool tiles_unusable = false;  // the mesh cannot be tiled (tiles_why): four-kernel stages
+    std::string tiles_why;
+    o2f::TileView tv{};
+    void *tile_mem = nullptr;             // every table, one allocation
+    double *Ub = nullptr, *Uc = nullptr;  // outputs of the first two stages of a tiled step
+    int tile_target = 0, tile_bpc = 0, tile_vgprs = 0;
     bool have_grad = false;
     // adjoint
     double *blk = nullptr;

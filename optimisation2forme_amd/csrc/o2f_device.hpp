@@ -438,6 +438,117 @@ __device__ __forceinline__ void quad_transpose(double (&m)[4], int k) {
     m[3] = upper ? m[3] : t3;
 }
 
+// Cell -> vertex gather with precomputed stencils
+// (src/advection.cpp:26-131 for the state, src/diffusion.cpp:28-140 for the
+// viscous quantities): the value of vertex v, NV = 4 state variables with
+// the state BC, or NV = 5 viscous quantities with apply_diffusion_bc
+// (src/boundary.cpp:101-127).
+// SRC_AOS: the source is AoS (4 + trailing block), else SoA [q][nc].
+template <int NV, bool SRC_AOS>
+__device__ __forceinline__ double src_val(const double *__restrict__ src, int nc, int c, int q) {
+    return SRC_AOS ? (q < 4 ? src[4 * (long)c + q] : src[4 * (long)nc + c]) : src[(long)q * nc + c];
+}
+
+// The vertex's stencil is (kind, offset b into scell / sw, length n) and
+// bc_of() gives its boundary tag; store(out) receives the NV values.
+template <int NV, bool SRC_AOS, class BcOf, class Store>
+__device__ __forceinline__ void vertex_gather_vals(const Topo &T, const Geom &G, const double *__restrict__ src,
+                                                   const Phys &P, uint8_t kind, int b, int n, BcOf &&bc_of,
+                                                   Store &&store) {
+    const int nc = T.nc;
+    // The first kStencilSlots ids and weights in 16-byte loads (both arrays
+    // carry kStencilSlots entries of padding; entries past the stencil are
+    // never used), then the values of the slots k < n: two dependent load
+    // levels, and few address computations.  (Per-entry loads under k < n
+    // let the compiler finish each id's address arithmetic inside its
+    // branch, which waits for the ids one at a time.)
+    int cid[kStencilSlots];
+    double w[kStencilSlots], val[kStencilSlots][NV];
+    {
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        typedef double v2d __attribute__((ext_vector_type(2)));
+        typedef v4i v4i_a4 __attribute__((aligned(4)));
+        typedef v2d v2d_a8 __attribute__((aligned(8)));
+        const v4i_a4 *ip = reinterpret_cast<const v4i_a4 *>(T.scell + b);
+        const v2d_a8 *wp = reinterpret_cast<const v2d_a8 *>(G.sw + b);
+#pragma unroll
+        for (int h = 0; h < kStencilSlots / 4; ++h) {
+            const v4i t = ip[h];
+            cid[4 * h] = t.x;
+            cid[4 * h + 1] = t.y;
+            cid[4 * h + 2] = t.z;
+            cid[4 * h + 3] = t.w;
+        }
+#pragma unroll
+        for (int h = 0; h < kStencilSlots / 2; ++h) {
+            const v2d t = wp[h];
+            w[2 * h] = t.x;
+            w[2 * h + 1] = t.y;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kStencilSlots; ++k)
+        if (k < n) {
+            if (SRC_AOS) {
+                double t4[4];
+                ld4(src, cid[k], t4);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) val[k][q] = t4[q];
+                if (NV == 5) val[k][NV - 1] = src[4 * (long)nc + cid[k]];
+            } else {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) val[k][q] = src[(long)q * nc + cid[k]];
+            }
+        }
+    double out[NV];
+    if (kind == kStencilWsum) {
+        // w0*a0 + w1*a1 + ... left to right (src/advection.cpp:47-50, :101-104)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = w[0] * val[0][q];
+#pragma unroll
+        for (int k = 1; k < kStencilSlots; ++k)
+            if (k < n) {
+#pragma unroll
+                for (int q = 0; q < NV; ++q) out[q] = out[q] + w[k] * val[k][q];
+            }
+        for (int k = kStencilSlots; k < n; ++k) {
+            const int c = T.scell[b + k];
+            const double wk = G.sw[b + k];
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] = out[q] + wk * src_val<NV, SRC_AOS>(src, nc, c, q);
+        }
+    } else if (kind == kStencilAvg2) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = 0.5 * (val[0][q] + val[1][q]);
+    } else if (kind == kStencilCopy) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = val[0][q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) out[q] = 0.0;
+    }
+    const char bc = bc_of();
+    if (NV == 4) {
+        if (tagged(bc)) bc_state(P, bc, out[0], out[1], out[2], out[3]);
+    } else {
+        if (bc == 'f' || bc == 'w') {
+#pragma unroll
+            for (int q = 0; q < NV; ++q) out[q] = 0.0;
+        }
+    }
+    store(out);
+}
+
+
+// |u| + c of one state (src/simulation.cpp:84-95)
+__device__ __forceinline__ double wave_speed(double rho, double ru, double rv, double E, double gamma) {
+    const double u = ru / rho, v = rv / rho;
+    const double kinetic = 0.5 * (ru * ru + rv * rv) / rho;
+    const double p = (gamma - 1.0) * (E - kinetic);
+    const double cs = sqrt(gamma * p / rho);
+    return hypot(u, v) + cs;
+}
+
 // f(j, value of lane j) for j = 0, 1, 2, 3 in order
 template <class F>
 __device__ __forceinline__ void for_lanes(double v, F &&f) {

@@ -21,102 +21,17 @@ namespace o2f {
 // viscous quantities).  NV = 4 state variables with the state BC, or NV = 5
 // viscous quantities with apply_diffusion_bc (src/boundary.cpp:101-127).
 // ------------------------------------------------------------------------
-// SRC_AOS: the source is AoS (4 + trailing block), else SoA [q][nc].
 // The destination is always AoS (4 + trailing block).
-template <int NV, bool SRC_AOS>
-__device__ __forceinline__ double src_val(const double *__restrict__ src, int nc, int c, int q) {
-    return SRC_AOS ? (q < 4 ? src[4 * (long)c + q] : src[4 * (long)nc + c]) : src[(long)q * nc + c];
-}
-
 template <int NV, bool SRC_AOS>
 __device__ __forceinline__ void gather_vertex(const Topo &T, const Geom &G, const double *__restrict__ src,
                                               double *__restrict__ dst, const Phys &P, int v) {
-    const int nc = T.nc;
     const uint8_t kind = T.skind[v];
     const int b = T.sptr[v], n = T.sptr[v + 1] - b;
-    // The first kStencilSlots ids and weights in 16-byte loads (both arrays
-    // carry kStencilSlots entries of padding; entries past the stencil are
-    // never used), then the values of the slots k < n: two dependent load
-    // levels, and few address computations.  (Per-entry loads under k < n
-    // let the compiler finish each id's address arithmetic inside its
-    // branch, which waits for the ids one at a time.)
-    int cid[kStencilSlots];
-    double w[kStencilSlots], val[kStencilSlots][NV];
-    {
-        typedef int v4i __attribute__((ext_vector_type(4)));
-        typedef double v2d __attribute__((ext_vector_type(2)));
-        typedef v4i v4i_a4 __attribute__((aligned(4)));
-        typedef v2d v2d_a8 __attribute__((aligned(8)));
-        const v4i_a4 *ip = reinterpret_cast<const v4i_a4 *>(T.scell + b);
-        const v2d_a8 *wp = reinterpret_cast<const v2d_a8 *>(G.sw + b);
-#pragma unroll
-        for (int h = 0; h < kStencilSlots / 4; ++h) {
-            const v4i t = ip[h];
-            cid[4 * h] = t.x;
-            cid[4 * h + 1] = t.y;
-            cid[4 * h + 2] = t.z;
-            cid[4 * h + 3] = t.w;
-        }
-#pragma unroll
-        for (int h = 0; h < kStencilSlots / 2; ++h) {
-            const v2d t = wp[h];
-            w[2 * h] = t.x;
-            w[2 * h + 1] = t.y;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < kStencilSlots; ++k)
-        if (k < n) {
-            if (SRC_AOS) {
-                double t4[4];
-                ld4(src, cid[k], t4);
-#pragma unroll
-                for (int q = 0; q < 4; ++q) val[k][q] = t4[q];
-                if (NV == 5) val[k][NV - 1] = src[4 * (long)nc + cid[k]];
-            } else {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) val[k][q] = src[(long)q * nc + cid[k]];
-            }
-        }
-    double out[NV];
-    if (kind == kStencilWsum) {
-        // w0*a0 + w1*a1 + ... left to right (src/advection.cpp:47-50, :101-104)
-#pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = w[0] * val[0][q];
-#pragma unroll
-        for (int k = 1; k < kStencilSlots; ++k)
-            if (k < n) {
-#pragma unroll
-                for (int q = 0; q < NV; ++q) out[q] = out[q] + w[k] * val[k][q];
-            }
-        for (int k = kStencilSlots; k < n; ++k) {
-            const int c = T.scell[b + k];
-            const double wk = G.sw[b + k];
-#pragma unroll
-            for (int q = 0; q < NV; ++q) out[q] = out[q] + wk * src_val<NV, SRC_AOS>(src, nc, c, q);
-        }
-    } else if (kind == kStencilAvg2) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = 0.5 * (val[0][q] + val[1][q]);
-    } else if (kind == kStencilCopy) {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = val[0][q];
-    } else {
-#pragma unroll
-        for (int q = 0; q < NV; ++q) out[q] = 0.0;
-    }
-    const char bc = T.vbc[v];
-    if (NV == 4) {
-        if (tagged(bc)) bc_state(P, bc, out[0], out[1], out[2], out[3]);
-    } else {
-        if (bc == 'f' || bc == 'w') {
-#pragma unroll
-            for (int q = 0; q < NV; ++q) out[q] = 0.0;
-        }
-    }
-    double o4[4] = {out[0], out[1], out[2], out[3]};
-    st4(dst, v, o4);
-    if (NV == 5) dst[4 * (long)T.nv + v] = out[NV - 1];
+    vertex_gather_vals<NV, SRC_AOS>(T, G, src, P, kind, b, n, [&] { return T.vbc[v]; }, [&](const double(&out)[NV]) {
+        double o4[4] = {out[0], out[1], out[2], out[3]};
+        st4(dst, v, o4);
+        if (NV == 5) dst[4 * (long)T.nv + v] = out[NV - 1];
+    });
 }
 
 template <int NV, bool SRC_AOS>
@@ -363,15 +278,6 @@ __global__ __launch_bounds__(256) void k_cell_stress(Topo T, const double *__res
 // per-edge HLLC pass; each cell gathers its edges' fluxes (no scatter, no
 // atomics) and sums them in ascending edge order.
 // ------------------------------------------------------------------------
-// |u| + c of one state (src/simulation.cpp:84-95)
-__device__ __forceinline__ double wave_speed(double rho, double ru, double rv, double E, double gamma) {
-    const double u = ru / rho, v = rv / rho;
-    const double kinetic = 0.5 * (ru * ru + rv * rv) / rho;
-    const double p = (gamma - 1.0) * (E - kinetic);
-    const double cs = sqrt(gamma * p / rho);
-    return hypot(u, v) + cs;
-}
-
 // Block max of m (NaN ignored like std::max) stored as this block's partial;
 // the step-begin block takes the max of the partials (exact, order-free).
 __device__ __forceinline__ void speed_fold(double m, double *parts, int blk) {
@@ -1268,16 +1174,15 @@ hipError_t launch_speed_max(const double *U, int nc, const Phys &P, double *part
     return hipGetLastError();
 }
 hipError_t launch_step_gather(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
-                              const double *parts, double min_h, double *scal, double *hist, int advance_t,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_step_gather, dim3(nblk(T.nv) + 1), dim3(256), 0, s, T, G, U, vU, P, parts, speed_parts(T.nc),
-                       min_h, scal, hist, advance_t);
+                              const double *parts, int nparts, double min_h, double *scal, double *hist,
+                              int advance_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_step_gather, dim3(nblk(T.nv) + 1), dim3(256), 0, s, T, G, U, vU, P, parts, nparts, min_h,
+                       scal, hist, advance_t);
     return hipGetLastError();
 }
-hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double min_h, double *scal, double *hist,
-                             int advance_t, hipStream_t s) {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, parts, speed_parts(nc), P, min_h, scal, hist,
-                       advance_t);
+hipError_t launch_step_begin(const double *parts, int nparts, const Phys &P, double min_h, double *scal,
+                             double *hist, int advance_t, hipStream_t s) {
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(256), 0, s, parts, nparts, P, min_h, scal, hist, advance_t);
     return hipGetLastError();
 }
 // Blocks of a reduction over n elements (fixed, so the order is fixed).

@@ -222,13 +222,15 @@ hipError_t launch_visc_and_flux(const Topo &T, const Geom &G, const double *tq, 
 // Per-block max_c(|u|+c) partials (speed_parts(nc) of them).
 int speed_parts(int nc);
 hipError_t launch_speed_max(const double *U, int nc, const Phys &P, double *parts, hipStream_t s);
-// dt from the speed partials, time advance, gate (one block)
-hipError_t launch_step_begin(const double *parts, int nc, const Phys &P, double min_h, double *scal, double *hist,
-                             int advance_t, hipStream_t s);
+// dt from nparts speed partials, time advance, gate (one block).  The
+// partials come from launch_speed_max or the last stage of a step
+// (speed_parts(nc) of them), or from the last tile stage (one per tile).
+hipError_t launch_step_begin(const double *parts, int nparts, const Phys &P, double min_h, double *scal,
+                             double *hist, int advance_t, hipStream_t s);
 // The step begin and the first stage's vertex state gather in one launch.
 hipError_t launch_step_gather(const Topo &T, const Geom &G, const double *U, double *vU, const Phys &P,
-                              const double *parts, double min_h, double *scal, double *hist, int advance_t,
-                              hipStream_t s);
+                              const double *parts, int nparts, double min_h, double *scal, double *hist,
+                              int advance_t, hipStream_t s);
 hipError_t launch_dot2(const double *a, const double *b, const double *c, const double *d, long n,
                        double *partials, double *scal, int op, hipStream_t s, const double *gate = nullptr,
                        const BicgTest &test = BicgTest{});

@@ -1,0 +1,433 @@
+// Fused residual stage over compact cell tiles (tables: host_tiles.hpp).
+//
+// One workgroup runs one whole stage -- vertex states, Green-Gauss
+// gradients, limiters, face states, HLLC, cell and vertex stresses, the
+// convective and viscous sums and the Shu-Osher update -- for the cells of
+// one tile, with every intermediate in LDS.  The four-kernel stage
+// (o2f_kernels.hip) writes about 88 MB of intermediates to memory and reads
+// them back per stage on the 200k-triangle workload and spreads a cell over
+// four lanes; here each cell, vertex and edge is one lane and only the state
+// in, the state out and the read-only mesh cross the memory system.
+//
+// Every entity is formed with the arithmetic of the four-kernel stage (same
+// expressions, same summation orders), so the results are bit-identical to
+// it and to the CPU restatement.  A tile recomputes the entities on its rim
+// that its neighbours also form (about 1.5x the gradients at 192-cell
+// tiles): that is the price of keeping the rest on chip.
+//
+// The state is read from Uin and written to Uout (a different buffer): the
+// tiles of one launch read the old state of their rim cells while others
+// write their own cells.
+#include <hip/hip_runtime.h>
+
+#include "host_tiles.hpp"
+#include "o2f_device.hpp"
+#include "o2f_tiles.hpp"
+
+namespace o2f {
+
+namespace {
+
+// LDS rows of four doubles as two 16-byte accesses.
+__device__ __forceinline__ void lds_ld4(const double *p, double (&v)[4]) {
+    const double2 a = reinterpret_cast<const double2 *>(p)[0], b = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+}
+__device__ __forceinline__ void lds_st4(double *p, const double (&v)[4]) {
+    reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+}
+
+// One gradient-cell record (TileCell) in four 16-byte loads.
+struct Rec {
+    int c, info;
+    int edge[4], nbr[4];
+    int lv[8], le[4];
+};
+__device__ __forceinline__ Rec load_rec(const TileCell *p) {
+    const int4 *q = reinterpret_cast<const int4 *>(p);
+    const int4 w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3];
+    Rec r;
+    r.c = w0.x;
+    r.edge[0] = w0.y;
+    r.edge[1] = w0.z;
+    r.edge[2] = w0.w;
+    r.edge[3] = w1.x;
+    r.nbr[0] = w1.y;
+    r.nbr[1] = w1.z;
+    r.nbr[2] = w1.w;
+    r.nbr[3] = w2.x;
+    const unsigned lvw[4] = {(unsigned)w2.y, (unsigned)w2.z, (unsigned)w2.w, (unsigned)w3.x};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        r.lv[2 * j] = (int)(lvw[j] & 0xffffu);
+        r.lv[2 * j + 1] = (int)(lvw[j] >> 16);
+    }
+    const unsigned lew[2] = {(unsigned)w3.y, (unsigned)w3.z};
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        r.le[2 * j] = (int)(short)(lew[j] & 0xffffu);
+        r.le[2 * j + 1] = (int)(short)(lew[j] >> 16);
+    }
+    r.info = w3.w & 0xff;
+    return r;
+}
+
+// A vertex record in one 16-byte load: stencil offset, length, kind, tag and
+// local stencil offset.
+struct Vtx {
+    int b, n, ls;
+    uint8_t kind;
+    char bc;
+};
+__device__ __forceinline__ Vtx load_vtx(const TileVertex *p) {
+    const int4 w = *reinterpret_cast<const int4 *>(p);
+    return Vtx{w.y, w.z & 0xffff, w.w, (uint8_t)((w.z >> 16) & 0xff), (char)(w.z >> 24)};
+}
+
+// LDS layout (in doubles; every region a multiple of four):
+//   [0, vert)            state vertices (4 each), later stress vertices (4 each, then q_y of each)
+//   [vert, vert + 5 gd)  cell stresses of the gradient cells (4 each, then q_y of each)
+//   [.., + 8 e)          face states of the tile's edges: L (4) then R (4); the flux replaces L
+struct Lds {
+    double *vert, *tq, *fs;
+    int max_vo, max_gd;
+};
+
+// Gradient cell i (local): edge values + Green-Gauss (src/advection.cpp:136-188)
+// and the cell stress / heat flux (src/diffusion.cpp:223-241); for a face
+// cell also the neighbour min / max, the limiter and its face states on the
+// tile's edges with the boundary ghosts (src/advection.cpp:232-321,
+// src/boundary.cpp:48-80).  The arithmetic of reconstruct_body, one lane
+// per cell.
+__device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Topo &T, const Geom &G, const Phys &P,
+                                          const double *__restrict__ U, const Lds &L, int i) {
+    const int nc = T.nc, c = r.c;
+    const int nk = r.info >> 4;
+    double u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) u[q] = U[(long)q * nc + c];
+    const double vol = G.cvol[c];
+    double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
+    double dx[4] = {0.0, 0.0, 0.0, 0.0}, dy[4] = {0.0, 0.0, 0.0, 0.0};
+    int bcs[4] = {0, 0, 0, 0};
+    const double cx = face ? G.ccx[c] : 0.0, cy = face ? G.ccy[c] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= nk) break;
+        const int e = r.edge[j];
+        const double el = G.elen[e];
+        double nx = G.enx[e], ny = G.eny[e];
+        const char ebc = T.ebc[e];
+        bcs[j] = ebc;
+        if (face) {
+            dx[j] = G.emx[e] - cx;
+            dy[j] = G.emy[e] - cy;
+        }
+        double xa[4], xb[4], val[4];
+        lds_ld4(L.vert + 4 * r.lv[2 * j], xa);
+        lds_ld4(L.vert + 4 * r.lv[2 * j + 1], xb);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) val[q] = 0.5 * (xa[q] + xb[q]);
+        if (tagged(ebc)) bc_state(P, ebc, val[0], val[1], val[2], val[3]);
+        if ((r.info >> j) & 1) {
+            nx = -nx;
+            ny = -ny;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            gx[q] += val[q] * el * nx / vol;
+            gy[q] += val[q] * el * ny / vol;
+        }
+    }
+    double t5[5];
+    cell_stress_t(u, gx, gy, P.mu, P.gamma, P.kappa, t5);
+    {
+        const double t4[4] = {t5[0], t5[1], t5[2], t5[3]};
+        lds_st4(L.tq + 4 * i, t4);
+        L.tq[4 * L.max_gd + i] = t5[4];
+    }
+    if (!face) return;
+    double nbv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int nb = r.nbr[j];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) nbv[j][q] = (nb >= 0) ? U[(long)q * nc + nb] : u[q];
+    }
+    const double veps = (P.limiter == 1) ? 0.0 : venkat_eps(vol, P.venkat_k);
+    const double fsq[4] = {P.fs_rho, P.fs_ru, P.fs_rv, P.fs_E};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double mn = u[q], mx = u[q];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double v = nbv[j][q];
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+        double al = 1.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nk) break;
+            const double pred = u[q] + gx[q] * dx[j] + gy[q] * dy[j];
+            const double a = (P.limiter == 1) ? alpha_bj(u[q], pred, mn, mx)
+                                              : alpha_venkat_eps(u[q], pred, mn, mx, veps);
+            al = smin(al, a);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nk) break;
+            const int le = r.le[j];
+            if (le < 0) continue;
+            const double f = u[q] + al * (gx[q] * dx[j] + gy[q] * dy[j]);
+            double *row = L.fs + 8 * le;
+            if ((r.info >> j) & 1) {
+                row[q] = f;
+                const char bcj = (char)bcs[j];
+                if (tagged(bcj)) row[4 + q] = (bcj == 'i' || bcj == 'f') ? fsq[q] : (bcj == 'w' && (q == 1 || q == 2)) ? -f : f;
+            } else {
+                row[4 + q] = f;
+            }
+        }
+    }
+}
+
+// Stress vertex: the viscous gather of vertex v from the tile's cell
+// stresses (src/diffusion.cpp:28-140, apply_diffusion_bc), the arithmetic of
+// vertex_gather_vals<5, true>, with the stencil's cells given by their
+// local indices.
+__device__ __forceinline__ void tile_vertex_stress(const Vtx &r, const uint16_t *__restrict__ ls, const Geom &G,
+                                                   const Lds &L, int j) {
+    const uint8_t kind = r.kind;
+    const int b = r.b, n = r.n;
+    auto val = [&](int k, double (&t)[5]) {
+        const int li = ls[k];
+        double t4[4];
+        lds_ld4(L.tq + 4 * li, t4);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = t4[q];
+        t[4] = L.tq[4 * L.max_gd + li];
+    };
+    double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (kind == kStencilWsum) {
+        double t[5];
+        val(0, t);
+        const double w0 = G.sw[b];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) out[q] = w0 * t[q];
+        for (int k = 1; k < n; ++k) {
+            val(k, t);
+            const double wk = G.sw[b + k];
+#pragma unroll
+            for (int q = 0; q < 5; ++q) out[q] = out[q] + wk * t[q];
+        }
+    } else if (kind == kStencilAvg2) {
+        double t0[5], t1[5];
+        val(0, t0);
+        val(1, t1);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) out[q] = 0.5 * (t0[q] + t1[q]);
+    } else if (kind == kStencilCopy) {
+        val(0, out);
+    }
+    if (r.bc == 'f' || r.bc == 'w') {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) out[q] = 0.0;
+    }
+    const double o4[4] = {out[0], out[1], out[2], out[3]};
+    lds_st4(L.vert + 4 * j, o4);
+    L.vert[4 * L.max_vo + j] = out[4];
+}
+
+// Owned cell i: the convective sum of its edges' fluxes
+// (src/advection.cpp:433-464), the viscous divergence
+// (src/diffusion.cpp:145-201) and the Shu-Osher update
+// (src/simulation.cpp:159-167): the arithmetic of flux_update_body, one
+// lane per cell.  Returns |u| + c of the new state (kFlagSpeed), else 0.
+__device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, const Topo &T, const Geom &G,
+                                              const Phys &P, const double *__restrict__ U,
+                                              double *__restrict__ Uout, double *__restrict__ Uold,
+                                              double *__restrict__ R, const double *__restrict__ dtp, double c0,
+                                              double c1, int flags, const Lds &L) {
+    const int nc = T.nc, c = r.c;
+    const int nk = r.info >> 4;
+    const double vol = G.cvol[c];
+    double u[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) u[q] = U[(long)q * nc + c];
+    // one division each: u, v, 1 / V
+    const double uu = u[1] / u[0], vv = u[2] / u[0], invV = 1.0 / vol;
+    double rk[4] = {0.0, 0.0, 0.0, 0.0};
+    double w1[4] = {0.0, 0.0, 0.0, 0.0}, w2[4] = {0.0, 0.0, 0.0, 0.0}, w3[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= nk) break;
+        const int e = r.edge[j];
+        const double el = G.elen[e];
+        double nx = G.enx[e], ny = G.eny[e];
+        const char ebc = T.ebc[e];
+        const bool is_left = (r.info >> j) & 1;
+        double F[4];
+        lds_ld4(L.fs + 8 * r.le[j], F);
+        const double sign = is_left ? +1.0 : -1.0;
+        // convective term of this edge, src/advection.cpp:453-456, summed in slot order
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rk[q] += sign * F[q] * el;
+        // viscous terms of this edge, src/diffusion.cpp:183-200
+        if (is_left) {
+            nx = -nx;
+            ny = -ny;
+        }
+        double ta[4], tb[4], t[5];
+        const int la = o.lo[2 * j], lb = o.lo[2 * j + 1];
+        lds_ld4(L.vert + 4 * la, ta);
+        lds_ld4(L.vert + 4 * lb, tb);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = 0.5 * (ta[q] + tb[q]);
+        t[4] = 0.5 * (L.vert[4 * L.max_vo + la] + L.vert[4 * L.max_vo + lb]);
+        if (ebc == 'f' || ebc == 'w') {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) t[q] = 0.0;
+        }
+        const double txx = t[0], tyy = t[1], txy = t[2], qx = t[3], qy = t[4];
+        w1[j] = txx * el * nx / vol + txy * el * ny / vol;
+        w2[j] = txy * el * nx / vol + tyy * el * ny / vol;
+        w3[j] = (uu * txx + vv * txy - qx) * el * nx / vol + (uu * txy + vv * tyy - qy) * el * ny / vol;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rk[q] = rk[q] * -invV;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (j >= nk) break;
+        rk[1] = rk[1] + w1[j];
+        rk[2] = rk[2] + w2[j];
+        rk[3] = rk[3] + w3[j];
+    }
+    if (flags & kFlagStoreR) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) R[(long)q * nc + c] = rk[q];
+    }
+    const double dt = *dtp;
+    double unew[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        double old;
+        if (flags & kFlagSaveOld) {
+            old = u[q];
+            Uold[(long)q * nc + c] = u[q];
+        } else {
+            old = Uold[(long)q * nc + c];
+        }
+        unew[q] = c0 * old + c1 * (u[q] + dt * rk[q]);
+        Uout[(long)q * nc + c] = unew[q];
+    }
+    return (flags & kFlagSpeed) ? wave_speed(unew[0], unew[1], unew[2], unew[3], P.gamma) : 0.0;
+}
+
+// At most 128 VGPRs (four waves per SIMD): the kernel keeps every tile of
+// the mesh resident in one round, so occupancy is what hides its loads.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_stage(Topo T, Geom G, Phys P, TileView A,
+                                                    const double *__restrict__ U, double *__restrict__ Uout,
+                                                    double *__restrict__ Uold, double *__restrict__ R,
+                                                    const double *__restrict__ dtp, double c0, double c1, int flags,
+                                                    double *__restrict__ speed) {
+    extern __shared__ double4 lds_raw[];
+    if (P.gate && *P.gate == 0.0) return;  // uniform
+    const TileHead h = A.head[blockIdx.x];
+    double *lds = reinterpret_cast<double *>(lds_raw);
+    const Lds L{lds, lds + A.lds_tq, lds + A.lds_fs, A.max_vo, A.max_gd};
+    const int tid = threadIdx.x, nth = blockDim.x;
+
+    // 1. states of the tile's state vertices (the four-kernel stage's k_vertex_gather)
+    for (int i = tid; i < h.nVg; i += nth) {
+        const Vtx r = load_vtx(A.vg + h.vg0 + i);
+        vertex_gather_vals<4, false>(
+            T, G, U, P, r.kind, r.b, r.n, [&] { return r.bc; },
+            [&](const double(&out)[4]) { lds_st4(L.vert + 4 * i, out); });
+    }
+    __syncthreads();
+    // 2. gradients and stresses of every gradient cell; limiter and face states of the face cells
+    for (int i = tid; i < h.nGd; i += nth) {
+        const Rec r = load_rec(A.cell + h.cell0 + i);
+        tile_cell(r, i < h.nF, T, G, P, U, L, i);
+    }
+    __syncthreads();
+    // 3. HLLC once per tile edge (in place of its left face state), and the
+    //    stresses of the owned cells' vertices (over the state vertices, no
+    //    longer read)
+    for (int i = tid; i < h.nE + h.nVo; i += nth) {
+        if (i < h.nE) {
+            const int e = A.edge[h.e0 + i];
+            double qL[4], qR[4], f[4];
+            lds_ld4(L.fs + 8 * i, qL);
+            lds_ld4(L.fs + 8 * i + 4, qR);
+            hllc(qL, qR, G.enx[e], G.eny[e], P.gamma, f);
+            lds_st4(L.fs + 8 * i, f);
+        } else {
+            const int j = i - h.nE;
+            const Vtx r = load_vtx(A.vo + h.vo0 + j);
+            tile_vertex_stress(r, A.lsten + r.ls, G, L, j);
+        }
+    }
+    __syncthreads();
+    // 4. sums and stage update of the owned cells
+    double m = 1e-8;
+    for (int i = tid; i < h.nO; i += nth) {
+        const Rec r = load_rec(A.cell + h.cell0 + i);
+        const double s = tile_update(r, A.own[h.cell0 + i], T, G, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
+        if (flags & kFlagSpeed) m = smax(m, s);
+    }
+    if (flags & kFlagSpeed) {
+        // the tile's max of |u| + c (every value is >= 1e-8 and NaN-free,
+        // so the order of this tree does not matter)
+        __shared__ double wmax[4];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off));
+        if ((tid & 63) == 0) wmax[tid >> 6] = m;
+        __syncthreads();
+        if (tid == 0) {
+            double b = wmax[0];
+            for (int w = 1; w < (nth >> 6); ++w) b = smax(b, wmax[w]);
+            speed[blockIdx.x] = b;
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
+                             double *Uout, double *Uold, double *R, const double *dtp, double c0, double c1,
+                             int flags, double *speed, hipStream_t s) {
+    hipLaunchKernelGGL(k_tile_stage, dim3(A.ntiles), dim3(kTileThreads), A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R,
+                       dtp, c0, c1, flags, speed);
+    return hipGetLastError();
+}
+
+hipError_t tile_stage_attributes(size_t lds_bytes, int *blocks_per_cu, int *vgprs) {
+    // every context may launch up to the device's LDS per workgroup, less
+    // the kernel's static LDS; a failed query leaves no error behind
+    const void *f = reinterpret_cast<const void *>(&k_tile_stage);
+    int dev = 0, lds_max = 0;
+    hipFuncAttributes fa{};
+    *blocks_per_cu = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e == hipSuccess) e = hipFuncGetAttributes(&fa, f);
+    const int dyn_max = lds_max - (int)fa.sharedSizeBytes;
+    if (e == hipSuccess && lds_bytes <= (size_t)dyn_max) {
+        e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, kTileThreads, lds_bytes);
+    }
+    if (vgprs) *vgprs = fa.numRegs;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        *blocks_per_cu = 0;
+    }
+    return e;
+}
+
+}  // namespace o2f

@@ -1,0 +1,40 @@
+// Device view and launcher of the fused tile stage (o2f_tiles.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "host_tiles.hpp"
+#include "o2f_kernels.hpp"
+
+namespace o2f {
+
+constexpr int kTileThreads = 256;  // one workgroup per tile
+
+// A TileSet uploaded to the device, with its LDS layout (offsets in doubles).
+struct TileView {
+    const TileHead *head;
+    const TileCell *cell;
+    const TileOwn *own;
+    const TileVertex *vg, *vo;
+    const int *edge;
+    const uint16_t *lsten;
+    int ntiles;
+    int max_vo, max_gd;
+    int lds_tq, lds_fs;
+    size_t lds_bytes;
+};
+
+// One residual stage fused with its Shu-Osher update over every tile:
+// Uout = c0 Uold + c1 (U + dt R(U)) for each owned cell (flags as for
+// k_cell_flux_update; kFlagAdv and kFlagVisc are implied).  U and Uout must
+// be different buffers.  kFlagSpeed writes one max(|u| + c) partial per tile.
+hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
+                             double *Uout, double *Uold, double *R, const double *dtp, double c0, double c1,
+                             int flags, double *speed, hipStream_t s);
+// Allows lds_bytes of dynamic LDS for the stage kernel and reports how many
+// of its workgroups fit on one CU with that LDS, and its VGPRs.
+hipError_t tile_stage_attributes(size_t lds_bytes, int *blocks_per_cu, int *vgprs);
+
+}  // namespace o2f
