@@ -127,10 +127,11 @@ int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed, bool begin_step =
 }
 
 void release_tiles(o2f_ctx *c) {
-    for (void *p : {c->tile_mem, (void *)c->Ub, (void *)c->Uc})
+    for (void *p : {c->tile_mem, (void *)c->Ua, (void *)c->Ub, (void *)c->Uc})
         if (p) (void)hipFree(p);
     c->tile_mem = nullptr;
-    c->Ub = c->Uc = nullptr;
+    c->Ua = c->Ub = c->Uc = nullptr;
+    c->tile_aos = false;
     c->tv = TileView{};
     c->tiles_ready = false;
 }
@@ -204,6 +205,7 @@ bool tiles_on(o2f_ctx *c) {
     put(o_edge, ts.edge.data(), ts.edge.size() * sizeof(int));
     put(o_vo, ts.vo.data(), ts.vo.size() * sizeof(TileVertex));
     put(o_ls, ts.lsten.data(), ts.lsten.size() * sizeof(uint16_t));
+    if (e == hipSuccess) e = dalloc(c->Ua, 4 * (size_t)nc);
     if (e == hipSuccess) e = dalloc(c->Ub, 4 * (size_t)nc);
     if (e == hipSuccess) e = dalloc(c->Uc, 4 * (size_t)nc);
     // one speed partial per tile: small tiles outnumber the four-lane blocks
@@ -240,11 +242,29 @@ bool tiles_on(o2f_ctx *c) {
     return true;
 }
 
-// One step on the tiles: dt, then three fused stages, U -> Ub -> Uc -> U
+// The state of a tiled run lives in Ua (AoS) from the first tiled step on;
+// tiles_leave writes it back to c->U (SoA) before the entry point returns.
+int tiles_enter(o2f_ctx *c) {
+    if (!c->tile_aos) {
+        HIPCHK(c, launch_state_to_aos(c->U, c->hm.nc, c->Ua, c->stream));
+        c->tile_aos = true;
+    }
+    return O2F_OK;
+}
+int tiles_leave(o2f_ctx *c) {
+    if (c->tile_aos) {
+        c->tile_aos = false;
+        HIPCHK(c, launch_state_to_soa(c->Ua, c->hm.nc, c->U, c->stream));
+    }
+    return O2F_OK;
+}
+
+// One step on the tiles: dt, then three fused stages, Ua -> Ub -> Uc -> Ua
 // (each stage reads one buffer and writes another, so no tile overwrites a
-// state its neighbours still read).  last_outputs: the third stage runs on
-// the four kernels instead, in place (U <- Ub -> U), and leaves the
-// intermediates the adjoint and the force coefficients read.
+// state its neighbours still read).  last_outputs: the second stage writes
+// the C ABI's c->U, and the third stage runs on the four kernels, in place,
+// leaving the intermediates the adjoint and the force coefficients read
+// (and the state in c->U).
 int tiled_step(o2f_ctx *c, bool last_outputs, double *hist) {
     hipStream_t s = c->stream;
     const Topo T = c->topo();
@@ -253,19 +273,22 @@ int tiled_step(o2f_ctx *c, bool last_outputs, double *hist) {
         HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
         c->speed_nparts = speed_parts(c->hm.nc);
     }
+    int st = tiles_enter(c);
+    if (st) return st;
     HIPCHK(c, launch_step_begin(c->speed, c->speed_nparts, c->phys, c->hm.min_h, c->scal, hist, 1, s));
     const double *dtp = c->scal + kScalDt;
     const TileView &A = c->tv;
-    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->U, c->Ub, c->Uold, c->R, dtp, 0.0, 1.0, kFlagRK | kFlagSaveOld,
-                                nullptr, s));
+    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ua, c->Ub, true, c->Uold, c->R, dtp, 0.0, 1.0,
+                                kFlagRK | kFlagSaveOld, nullptr, s));
     if (last_outputs) {
-        HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->U, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0, kFlagRK,
-                                    nullptr, s));
+        HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->U, false, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0,
+                                    kFlagRK, nullptr, s));
+        c->tile_aos = false;
         return run_stage(c, 0, kFlagRK | kFlagStoreR | kFlagSpeed, dtp, 1.0 / 3.0, 2.0 / 3.0, true);
     }
-    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->Uc, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0, kFlagRK,
-                                nullptr, s));
-    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Uc, c->U, c->Uold, c->R, dtp, 1.0 / 3.0, 2.0 / 3.0,
+    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->Uc, true, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0,
+                                kFlagRK, nullptr, s));
+    HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Uc, c->Ua, true, c->Uold, c->R, dtp, 1.0 / 3.0, 2.0 / 3.0,
                                 kFlagRK | kFlagSpeed, c->speed, s));
     c->speed_nparts = A.ntiles;
     return O2F_OK;
@@ -778,8 +801,10 @@ int o2f_run_steps(o2f_ctx *c, int nsteps, double *dts, double *t_inout) {
     HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
     double *hist = nullptr;
     if (dts && nsteps > 0) HIPCHK(c, hipMalloc(reinterpret_cast<void **>(&hist), nsteps * sizeof(double)));
-    for (int s = 0; s < nsteps; ++s)
-        if ((st = one_step(c, s == nsteps - 1, hist ? hist + s : nullptr))) return st;
+    for (int s = 0; s < nsteps && st == 0; ++s) st = one_step(c, s == nsteps - 1, hist ? hist + s : nullptr);
+    const int st_leave = tiles_leave(c);
+    if (st) return st;
+    if (st_leave) return st_leave;
     if (hist) {
         HIPCHK(c, hipMemcpyAsync(dts, hist, nsteps * sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -828,15 +853,18 @@ int o2f_run_until(o2f_ctx *c, double final_time, int max_steps, int *steps, doub
     }
     c->phys = saved;
     c->speed_valid = false;  // gated steps may have skipped the speed fold
+    const int st_leave = tiles_leave(c);
     if (st) return st;
+    if (st_leave) return st_leave;
     HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     // the last step's third stage read its input from Uc: its residual,
     // gradients, face states and stresses, as that stage's four kernels
-    // would have left them
-    if (tiled && h[kScalSteps] > 0.0 &&
-        (st = run_stage(c, 0, kFlagStoreR, nullptr, 0.0, 0.0, true, false, nullptr, c->Uc)))
-        return st;
+    // would have left them (from an SoA copy of Uc in Ub)
+    if (tiled && h[kScalSteps] > 0.0) {
+        HIPCHK(c, launch_state_to_soa(c->Uc, c->hm.nc, c->Ub, s));
+        if ((st = run_stage(c, 0, kFlagStoreR, nullptr, 0.0, 0.0, true, false, nullptr, c->Ub))) return st;
+    }
     if (steps) *steps = static_cast<int>(h[kScalSteps]);
     if (t_inout) *t_inout = h[kScalT];
     if (t_last_start) *t_last_start = h[kScalTPrev];
@@ -1727,8 +1755,10 @@ int o2f_time_kernel(o2f_ctx *c, int kernel, int launches, double *avg_us) {
     if (!c || kernel < 0 || kernel > 5 || launches <= 0 || !avg_us) return O2F_ERR_ARG;
     int st = check_bc(c);
     if (st) return st;
-    if (kernel == 5 && !tiles_on(c))
-        return c->fail(O2F_ERR_ARG, "o2f_time_kernel: no tile stages (" + c->tiles_why + ")");
+    if (kernel == 5) {
+        if (!tiles_on(c)) return c->fail(O2F_ERR_ARG, "o2f_time_kernel: no tile stages (" + c->tiles_why + ")");
+        HIPCHK(c, launch_state_to_aos(c->U, c->hm.nc, c->Ua, c->stream));
+    }
     // every intermediate the kernels read, at the current state
     if ((st = residual_on(c, 0))) return st;
     hipStream_t s = c->stream;
@@ -1743,7 +1773,7 @@ int o2f_time_kernel(o2f_ctx *c, int kernel, int launches, double *avg_us) {
                 return launch_flux_update(T, G, c->U, c->Uold, c->Fe, c->vtq, c->R, nullptr, 0.0, 0.0,
                                           kFlagAdv | kFlagVisc | kFlagStoreR, c->phys, s);
             case 5:  // a second stage's work: U_old read, the new state to Ub
-                return launch_tile_stage(T, G, c->phys, c->tv, c->U, c->Ub, c->Uold, c->R, c->scal + kScalDt,
+                return launch_tile_stage(T, G, c->phys, c->tv, c->Ua, c->Ub, true, c->Uold, c->R, c->scal + kScalDt,
                                          3.0 / 4.0, 1.0 / 4.0, kFlagRK, nullptr, s);
             default: {  // the whole residual, its four launches in order
                 hipError_t e = launch_vertex_state(T, G, c->U, c->vU, c->phys, s);
@@ -1805,8 +1835,10 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipStreamSynchronize(s));
     HIPCHK(c, hipEventRecord(e0, s));
-    for (int k = 0; k < nsteps; ++k)
-        if ((st = one_step(c, k == nsteps - 1, nullptr))) return st;
+    for (int k = 0; k < nsteps && st == 0; ++k) st = one_step(c, k == nsteps - 1, nullptr);
+    const int st_leave = tiles_leave(c);
+    if (st) return st;
+    if (st_leave) return st_leave;
     HIPCHK(c, hipEventRecord(e1, s));
     HIPCHK(c, hipEventSynchronize(e1));
     float ms = 0.f;

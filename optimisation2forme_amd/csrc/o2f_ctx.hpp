@@ -100,8 +100,12 @@ struct o2f_ctx {
     bool tiles_unusable = false;  // the mesh cannot be tiled (tiles_why): four-kernel stages
     std::string tiles_why;
     o2f::TileView tv{};
-    void *tile_mem = nullptr;             // every table, one allocation
-    double *Ub = nullptr, *Uc = nullptr;  // outputs of the first two stages of a tiled step
+    void *tile_mem = nullptr;  // every table, one allocation
+    // AoS states of a tiled run: Ua the state (while tile_aos), Ub / Uc the
+    // outputs of a step's first two stages; c->U is brought up to date
+    // before any entry point returns
+    double *Ua = nullptr, *Ub = nullptr, *Uc = nullptr;
+    bool tile_aos = false;
     int tile_target = 0, tile_bpc = 0, tile_vgprs = 0;
     bool have_grad = false;
     // adjoint

@@ -15,10 +15,15 @@
 // that its neighbours also form (about 1.5x the gradients at 192-cell
 // tiles): that is the price of keeping the rest on chip.
 //
-// The state is read from Uin and written to Uout (a different buffer): the
+// The state is read from U and written to Uout (a different buffer): the
 // tiles of one launch read the old state of their rim cells while others
-// write their own cells.
+// write their own cells.  Between stages the state is AoS (four doubles,
+// one 32-byte run per cell): a cell's state is then one cache line where the
+// SoA layout of the C ABI spreads it over four, and the stage's time follows
+// the number of distinct lines its gathers touch.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "host_tiles.hpp"
 #include "o2f_device.hpp"
@@ -108,8 +113,7 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Topo &T
     const int nc = T.nc, c = r.c;
     const int nk = r.info >> 4;
     double u[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) u[q] = U[(long)q * nc + c];
+    ld4(U, c, u);
     const double vol = G.cvol[c];
     double gx[4] = {0.0, 0.0, 0.0, 0.0}, gy[4] = {0.0, 0.0, 0.0, 0.0};
     double dx[4] = {0.0, 0.0, 0.0, 0.0}, dy[4] = {0.0, 0.0, 0.0, 0.0};
@@ -155,8 +159,12 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Topo &T
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int nb = r.nbr[j];
+        if (nb >= 0) {
+            ld4(U, nb, nbv[j]);
+        } else {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) nbv[j][q] = (nb >= 0) ? U[(long)q * nc + nb] : u[q];
+            for (int q = 0; q < 4; ++q) nbv[j][q] = u[q];
+        }
     }
     const double veps = (P.limiter == 1) ? 0.0 : venkat_eps(vol, P.venkat_k);
     const double fsq[4] = {P.fs_rho, P.fs_ru, P.fs_rv, P.fs_E};
@@ -248,6 +256,7 @@ __device__ __forceinline__ void tile_vertex_stress(const Vtx &r, const uint16_t 
 // (src/diffusion.cpp:145-201) and the Shu-Osher update
 // (src/simulation.cpp:159-167): the arithmetic of flux_update_body, one
 // lane per cell.  Returns |u| + c of the new state (kFlagSpeed), else 0.
+template <bool OUT_AOS>
 __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, const Topo &T, const Geom &G,
                                               const Phys &P, const double *__restrict__ U,
                                               double *__restrict__ Uout, double *__restrict__ Uold,
@@ -257,8 +266,7 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
     const int nk = r.info >> 4;
     const double vol = G.cvol[c];
     double u[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) u[q] = U[(long)q * nc + c];
+    ld4(U, c, u);
     // one division each: u, v, 1 / V
     const double uu = u[1] / u[0], vv = u[2] / u[0], invV = 1.0 / vol;
     double rk[4] = {0.0, 0.0, 0.0, 0.0};
@@ -323,13 +331,15 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
             old = Uold[(long)q * nc + c];
         }
         unew[q] = c0 * old + c1 * (u[q] + dt * rk[q]);
-        Uout[(long)q * nc + c] = unew[q];
+        if (!OUT_AOS) Uout[(long)q * nc + c] = unew[q];
     }
+    if (OUT_AOS) st4(Uout, c, unew);
     return (flags & kFlagSpeed) ? wave_speed(unew[0], unew[1], unew[2], unew[3], P.gamma) : 0.0;
 }
 
 // At most 128 VGPRs (four waves per SIMD): the kernel keeps every tile of
 // the mesh resident in one round, so occupancy is what hides its loads.
+template <bool OUT_AOS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_stage(Topo T, Geom G, Phys P, TileView A,
                                                     const double *__restrict__ U, double *__restrict__ Uout,
                                                     double *__restrict__ Uold, double *__restrict__ R,
@@ -345,7 +355,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // 1. states of the tile's state vertices (the four-kernel stage's k_vertex_gather)
     for (int i = tid; i < h.nVg; i += nth) {
         const Vtx r = load_vtx(A.vg + h.vg0 + i);
-        vertex_gather_vals<4, false>(
+        vertex_gather_vals<4, true>(
             T, G, U, P, r.kind, r.b, r.n, [&] { return r.bc; },
             [&](const double(&out)[4]) { lds_st4(L.vert + 4 * i, out); });
     }
@@ -378,7 +388,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     double m = 1e-8;
     for (int i = tid; i < h.nO; i += nth) {
         const Rec r = load_rec(A.cell + h.cell0 + i);
-        const double s = tile_update(r, A.own[h.cell0 + i], T, G, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
+        const double s =
+            tile_update<OUT_AOS>(r, A.own[h.cell0 + i], T, G, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
         if (flags & kFlagSpeed) m = smax(m, s);
     }
     if (flags & kFlagSpeed) {
@@ -397,20 +408,51 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
 }
 
+__global__ __launch_bounds__(256) void k_state_to_aos(const double *__restrict__ U, int nc, double *__restrict__ A) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    const double u[4] = {U[c], U[(long)nc + c], U[2 * (long)nc + c], U[3 * (long)nc + c]};
+    st4(A, c, u);
+}
+
+__global__ __launch_bounds__(256) void k_state_to_soa(const double *__restrict__ A, int nc, double *__restrict__ U) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nc) return;
+    double u[4];
+    ld4(A, c, u);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) U[(long)q * nc + c] = u[q];
+}
+
 }  // namespace
 
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
-                             double *Uout, double *Uold, double *R, const double *dtp, double c0, double c1,
-                             int flags, double *speed, hipStream_t s) {
-    hipLaunchKernelGGL(k_tile_stage, dim3(A.ntiles), dim3(kTileThreads), A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R,
-                       dtp, c0, c1, flags, speed);
+                             double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
+                             double c1, int flags, double *speed, hipStream_t s) {
+    if (out_aos)
+        hipLaunchKernelGGL(k_tile_stage<true>, dim3(A.ntiles), dim3(kTileThreads), A.lds_bytes, s, T, G, P, A, U,
+                           Uout, Uold, R, dtp, c0, c1, flags, speed);
+    else
+        hipLaunchKernelGGL(k_tile_stage<false>, dim3(A.ntiles), dim3(kTileThreads), A.lds_bytes, s, T, G, P, A, U,
+                           Uout, Uold, R, dtp, c0, c1, flags, speed);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_to_aos(const double *U, int nc, double *A, hipStream_t s) {
+    hipLaunchKernelGGL(k_state_to_aos, dim3((nc + 255) / 256), dim3(256), 0, s, U, nc, A);
+    return hipGetLastError();
+}
+
+hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s) {
+    hipLaunchKernelGGL(k_state_to_soa, dim3((nc + 255) / 256), dim3(256), 0, s, A, nc, U);
     return hipGetLastError();
 }
 
 hipError_t tile_stage_attributes(size_t lds_bytes, int *blocks_per_cu, int *vgprs) {
     // every context may launch up to the device's LDS per workgroup, less
     // the kernel's static LDS; a failed query leaves no error behind
-    const void *f = reinterpret_cast<const void *>(&k_tile_stage);
+    const void *f = reinterpret_cast<const void *>(&k_tile_stage<true>);
+    const void *g = reinterpret_cast<const void *>(&k_tile_stage<false>);
     int dev = 0, lds_max = 0;
     hipFuncAttributes fa{};
     *blocks_per_cu = 0;
@@ -420,7 +462,11 @@ hipError_t tile_stage_attributes(size_t lds_bytes, int *blocks_per_cu, int *vgpr
     const int dyn_max = lds_max - (int)fa.sharedSizeBytes;
     if (e == hipSuccess && lds_bytes <= (size_t)dyn_max) {
         e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+        if (e == hipSuccess) e = hipFuncSetAttribute(g, hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+        int bg = 0;
         if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, kTileThreads, lds_bytes);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bg, g, kTileThreads, lds_bytes);
+        if (e == hipSuccess) *blocks_per_cu = std::min(*blocks_per_cu, bg);
     }
     if (vgprs) *vgprs = fa.numRegs;
     if (e != hipSuccess) {

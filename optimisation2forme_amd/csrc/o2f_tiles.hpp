@@ -28,11 +28,16 @@ struct TileView {
 
 // One residual stage fused with its Shu-Osher update over every tile:
 // Uout = c0 Uold + c1 (U + dt R(U)) for each owned cell (flags as for
-// k_cell_flux_update; kFlagAdv and kFlagVisc are implied).  U and Uout must
-// be different buffers.  kFlagSpeed writes one max(|u| + c) partial per tile.
+// k_cell_flux_update; kFlagAdv and kFlagVisc are implied).  U is AoS (four
+// doubles per cell); Uout is AoS (out_aos) or SoA [q][nc]; Uold and R are
+// SoA.  U and Uout must be different buffers.  kFlagSpeed writes one
+// max(|u| + c) partial per tile.
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
-                             double *Uout, double *Uold, double *R, const double *dtp, double c0, double c1,
-                             int flags, double *speed, hipStream_t s);
+                             double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
+                             double c1, int flags, double *speed, hipStream_t s);
+// The state between the C ABI's SoA [q][nc] and the stages' AoS.
+hipError_t launch_state_to_aos(const double *U, int nc, double *A, hipStream_t s);
+hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s);
 // Allows lds_bytes of dynamic LDS for the stage kernel and reports how many
 // of its workgroups fit on one CU with that LDS, and its VGPRs.
 hipError_t tile_stage_attributes(size_t lds_bytes, int *blocks_per_cu, int *vgprs);
