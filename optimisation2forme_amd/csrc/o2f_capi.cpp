@@ -127,10 +127,10 @@ int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed, bool begin_step =
 }
 
 void release_tiles(o2f_ctx *c) {
-    for (void *p : {c->tile_mem, (void *)c->Ua, (void *)c->Ub, (void *)c->Uc})
+    for (void *p : {c->tile_mem, (void *)c->Ua, (void *)c->Ub, (void *)c->Uc, (void *)c->epk})
         if (p) (void)hipFree(p);
     c->tile_mem = nullptr;
-    c->Ua = c->Ub = c->Uc = nullptr;
+    c->Ua = c->Ub = c->Uc = c->epk = nullptr;
     c->tile_aos = false;
     c->tv = TileView{};
     c->tiles_ready = false;
@@ -205,6 +205,8 @@ bool tiles_on(o2f_ctx *c) {
     put(o_edge, ts.edge.data(), ts.edge.size() * sizeof(int));
     put(o_vo, ts.vo.data(), ts.vo.size() * sizeof(TileVertex));
     put(o_ls, ts.lsten.data(), ts.lsten.size() * sizeof(uint16_t));
+    if (e == hipSuccess) e = dalloc(c->epk, 8 * (size_t)c->hm.ne);
+    if (e == hipSuccess) e = launch_pack_edges(c->geo[0].view(), c->ebc, c->hm.ne, c->epk, c->stream);
     if (e == hipSuccess) e = dalloc(c->Ua, 4 * (size_t)nc);
     if (e == hipSuccess) e = dalloc(c->Ub, 4 * (size_t)nc);
     if (e == hipSuccess) e = dalloc(c->Uc, 4 * (size_t)nc);
@@ -228,6 +230,7 @@ bool tiles_on(o2f_ctx *c) {
     v.edge = reinterpret_cast<const int *>(m + o_edge);
     v.vo = reinterpret_cast<const TileVertex *>(m + o_vo);
     v.lsten = reinterpret_cast<const uint16_t *>(m + o_ls);
+    v.epk = c->epk;
     v.ntiles = (int)ts.head.size();
     v.max_vo = ts.max_vo;
     v.max_gd = ts.max_gd;

@@ -106,6 +106,7 @@ struct o2f_ctx {
     // before any entry point returns
     double *Ua = nullptr, *Ub = nullptr, *Uc = nullptr;
     bool tile_aos = false;
+    double *epk = nullptr;  // packed edge geometry of geo[0] for the tile stages
     int tile_target = 0, tile_bpc = 0, tile_vgprs = 0;
     bool have_grad = false;
     // adjoint

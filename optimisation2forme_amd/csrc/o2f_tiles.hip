@@ -108,9 +108,9 @@ struct Lds {
 // tile's edges with the boundary ghosts (src/advection.cpp:232-321,
 // src/boundary.cpp:48-80).  The arithmetic of reconstruct_body, one lane
 // per cell.
-__device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Topo &T, const Geom &G, const Phys &P,
-                                          const double *__restrict__ U, const Lds &L, int i) {
-    const int nc = T.nc, c = r.c;
+__device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Geom &G, const double2 *__restrict__ EP,
+                                          const Phys &P, int nc, const double *__restrict__ U, const Lds &L, int i) {
+    const int c = r.c;
     const int nk = r.info >> 4;
     double u[4];
     ld4(U, c, u);
@@ -122,14 +122,16 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Topo &T
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (j >= nk) break;
-        const int e = r.edge[j];
-        const double el = G.elen[e];
-        double nx = G.enx[e], ny = G.eny[e];
-        const char ebc = T.ebc[e];
+        const double2 *pe = EP + 4 * (long)r.edge[j];
+        const double2 n2 = pe[0], lb = pe[1];
+        const double el = lb.x;
+        double nx = n2.x, ny = n2.y;
+        const char ebc = (char)__double2loint(lb.y);
         bcs[j] = ebc;
         if (face) {
-            dx[j] = G.emx[e] - cx;
-            dy[j] = G.emy[e] - cy;
+            const double2 m2 = pe[2];
+            dx[j] = m2.x - cx;
+            dy[j] = m2.y - cy;
         }
         double xa[4], xb[4], val[4];
         lds_ld4(L.vert + 4 * r.lv[2 * j], xa);
@@ -258,7 +260,8 @@ __device__ __forceinline__ void tile_vertex_stress(const Vtx &r, const uint16_t 
 // lane per cell.  Returns |u| + c of the new state (kFlagSpeed), else 0.
 template <bool OUT_AOS>
 __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, const Topo &T, const Geom &G,
-                                              const Phys &P, const double *__restrict__ U,
+                                              const double2 *__restrict__ EP, const Phys &P,
+                                              const double *__restrict__ U,
                                               double *__restrict__ Uout, double *__restrict__ Uold,
                                               double *__restrict__ R, const double *__restrict__ dtp, double c0,
                                               double c1, int flags, const Lds &L) {
@@ -274,10 +277,11 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         if (j >= nk) break;
-        const int e = r.edge[j];
-        const double el = G.elen[e];
-        double nx = G.enx[e], ny = G.eny[e];
-        const char ebc = T.ebc[e];
+        const double2 *pe = EP + 4 * (long)r.edge[j];
+        const double2 n2 = pe[0], lt = pe[1];
+        const double el = lt.x;
+        double nx = n2.x, ny = n2.y;
+        const char ebc = (char)__double2loint(lt.y);
         const bool is_left = (r.info >> j) & 1;
         double F[4];
         lds_ld4(L.fs + 8 * r.le[j], F);
@@ -347,10 +351,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                     double *__restrict__ speed) {
     extern __shared__ double4 lds_raw[];
     if (P.gate && *P.gate == 0.0) return;  // uniform
-    const TileHead h = A.head[blockIdx.x];
+    // Workgroups go to the eight XCDs in turn; workgroup b runs tile
+    // x q + min(x, r) + b / 8 (x = b mod 8, ntiles = 8 q + r), so each XCD
+    // takes one contiguous run of the tile order, whose neighbouring tiles
+    // share rim cells, vertices and edges in that XCD's L2.
+    const int nt = A.ntiles, xcd = blockIdx.x & 7, q8 = nt >> 3, r8 = nt & 7;
+    const int tile = xcd * q8 + min(xcd, r8) + (int)(blockIdx.x >> 3);
+    const TileHead h = A.head[tile];
     double *lds = reinterpret_cast<double *>(lds_raw);
     const Lds L{lds, lds + A.lds_tq, lds + A.lds_fs, A.max_vo, A.max_gd};
     const int tid = threadIdx.x, nth = blockDim.x;
+    const double2 *__restrict__ EP = reinterpret_cast<const double2 *>(A.epk);
 
     // 1. states of the tile's state vertices (the four-kernel stage's k_vertex_gather)
     for (int i = tid; i < h.nVg; i += nth) {
@@ -363,7 +374,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // 2. gradients and stresses of every gradient cell; limiter and face states of the face cells
     for (int i = tid; i < h.nGd; i += nth) {
         const Rec r = load_rec(A.cell + h.cell0 + i);
-        tile_cell(r, i < h.nF, T, G, P, U, L, i);
+        tile_cell(r, i < h.nF, G, EP, P, T.nc, U, L, i);
     }
     __syncthreads();
     // 3. HLLC once per tile edge (in place of its left face state), and the
@@ -371,11 +382,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     //    longer read)
     for (int i = tid; i < h.nE + h.nVo; i += nth) {
         if (i < h.nE) {
-            const int e = A.edge[h.e0 + i];
+            const double2 n2 = EP[4 * (long)A.edge[h.e0 + i]];
             double qL[4], qR[4], f[4];
             lds_ld4(L.fs + 8 * i, qL);
             lds_ld4(L.fs + 8 * i + 4, qR);
-            hllc(qL, qR, G.enx[e], G.eny[e], P.gamma, f);
+            hllc(qL, qR, n2.x, n2.y, P.gamma, f);
             lds_st4(L.fs + 8 * i, f);
         } else {
             const int j = i - h.nE;
@@ -389,7 +400,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int i = tid; i < h.nO; i += nth) {
         const Rec r = load_rec(A.cell + h.cell0 + i);
         const double s =
-            tile_update<OUT_AOS>(r, A.own[h.cell0 + i], T, G, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
+            tile_update<OUT_AOS>(r, A.own[h.cell0 + i], T, G, EP, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
         if (flags & kFlagSpeed) m = smax(m, s);
     }
     if (flags & kFlagSpeed) {
@@ -403,9 +414,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         if (tid == 0) {
             double b = wmax[0];
             for (int w = 1; w < (nth >> 6); ++w) b = smax(b, wmax[w]);
-            speed[blockIdx.x] = b;
+            speed[tile] = b;
         }
     }
+}
+
+// Packed edge geometry: (n_x, n_y), (L, tag), (mid_x, mid_y), pad -- the
+// values of the geometry arrays, three 16-byte loads per edge instead of six
+// scattered words.
+__global__ __launch_bounds__(256) void k_pack_edges(Geom G, const char *__restrict__ ebc, int ne,
+                                                    double *__restrict__ epk) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    double2 *p = reinterpret_cast<double2 *>(epk) + 4 * (long)e;
+    p[0] = make_double2(G.enx[e], G.eny[e]);
+    p[1] = make_double2(G.elen[e], __hiloint2double(0, (int)ebc[e]));
+    p[2] = make_double2(G.emx[e], G.emy[e]);
+    p[3] = make_double2(0.0, 0.0);
 }
 
 __global__ __launch_bounds__(256) void k_state_to_aos(const double *__restrict__ U, int nc, double *__restrict__ A) {
@@ -435,6 +460,11 @@ hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const 
     else
         hipLaunchKernelGGL(k_tile_stage<false>, dim3(A.ntiles), dim3(kTileThreads), A.lds_bytes, s, T, G, P, A, U,
                            Uout, Uold, R, dtp, c0, c1, flags, speed);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_edges(const Geom &G, const char *ebc, int ne, double *epk, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_edges, dim3((ne + 255) / 256), dim3(256), 0, s, G, ebc, ne, epk);
     return hipGetLastError();
 }
 

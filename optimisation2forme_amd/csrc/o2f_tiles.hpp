@@ -20,6 +20,7 @@ struct TileView {
     const TileVertex *vg, *vo;
     const int *edge;
     const uint16_t *lsten;
+    const double *epk;  // packed edge geometry (launch_pack_edges), 8 doubles per edge
     int ntiles;
     int max_vo, max_gd;
     int lds_tq, lds_fs;
@@ -35,6 +36,9 @@ struct TileView {
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
                              double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
                              double c1, int flags, double *speed, hipStream_t s);
+// Packed edge geometry of the tile stage: per edge (n_x, n_y, L, tag bits,
+// mid_x, mid_y, 0, 0), 8 doubles, from the geometry set and the edge tags.
+hipError_t launch_pack_edges(const Geom &G, const char *ebc, int ne, double *epk, hipStream_t s);
 // The state between the C ABI's SoA [q][nc] and the stages' AoS.
 hipError_t launch_state_to_aos(const double *U, int nc, double *A, hipStream_t s);
 hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s);
