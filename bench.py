@@ -10,17 +10,23 @@ state starts uniform).  One step = dt reduction + 3 x (residual + stage
 update), all on the device.  W untimed steps, then K timed steps between
 device synchronisations.  Prints one JSON line:
   value     = N_vertices * K / seconds (vertex-steps/s, one GPU)
-  roofline  = the residual's algorithmic bytes B_residual = 64 N_c + G
-              (state in, residual out, geometry once; intermediates
-              excluded) over the summed mean durations of its four kernels
-              (each timed over back-to-back launches with HIP events on the
+  roofline  = the fused tile stage (o2f::k_tile_stage, one launch per
+              SSP-RK3 stage: residual + stage update over every cell tile,
+              intermediates in LDS): its algorithmic bytes per launch
+              B_stage = B_step / 3 = (256 N_c + 3 G) / 3 (state in and out,
+              U_old, geometry once; intermediates excluded) over its mean
+              duration (back-to-back launches between HIP events on the
               solver's stream), against 8 TB/s HBM3E
+  four_kernel_residual = the same for the residual's four-kernel form
+              (o2f_residual and the adjoint's linearisation use it):
+              B_residual = 64 N_c + G over its four kernels' summed durations
   cpu_baseline = the CPU restatement of the reference (oracle/liboracle.so,
               OpenMP over the host cores) on a bounded sample of the same
               workload.
-roofline.traffic is the residual's HBM bytes (its four kernels) from the
+roofline.traffic is the tile stage's HBM bytes per launch from the
 committed FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_transonic200k.json,
-copy-calibrated; null when a kernel is not in it).
+copy-calibrated; null when the kernel is not in it).  --tile-cells 0 times
+the four-kernel stages instead (the fused stage is bitwise the same).
 
     python bench.py --copy-calibration
 
@@ -72,6 +78,7 @@ COPY_BYTES = 512 * 1024 * 1024
 KERNEL_NAMES = {"vertex_gather": "o2f::k_vertex_gather<4, false>", "cell_reconstruct": "o2f::k_cell_reconstruct",
                 "viscous_gather_edge_flux": "o2f::k_viscous_gather_edge_flux",
                 "flux_update": "o2f::k_cell_flux_update"}
+TILE_KERNEL = "o2f::k_tile_stage<true>"
 METRIC = "vertex-steps/s"
 
 
@@ -636,6 +643,9 @@ def main():
                     help="fourier: q = -mu c_p / Pr grad T; reference: the reference's R times that, which "
                          "leaves the physical range on this case within 100 steps (same step time)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tile-cells", type=int, default=-1,
+                    help="stages of the timed steps: -1 fused tile stages sized automatically, 0 the four-kernel "
+                         "stages, n > 0 tiles of at most n cells")
     ap.add_argument("--gpu-warmup-ms", type=float, default=50.0,
                     help="untimed back-to-back residual evaluations on the initial state (which they leave "
                          "unchanged) before the warm-up steps, so that the timed steps start with the device "
@@ -716,6 +726,8 @@ def main():
     kw["inflow_velocity"] = kw.pop("mach")
     s = capi.Solver(desc, capi.make_params(**kw))
     nv, ne, nc = s.nv, s.ne, s.nc
+    s.set_tile_cells(a.tile_cells)
+    tiling = s.tile_info()
     s.init_freestream()
     # device warm-up: from idle, the first ~100 steps (~17 ms) of a fresh run
     # take up to 9 % longer than the steady rate, whatever the state; residual
@@ -744,16 +756,35 @@ def main():
     # the residual as a whole: its four launches in order, back to back
     res_b2b_us = s.time_kernel(4, launches)
     kb = kernel_bytes(desc, nv, ne, nc)
-    # roofline of the residual path (the hot kernel sequence): its
-    # algorithmic bytes B_residual = 64 N_c + G over the summed durations of
-    # its four kernels
+    # the residual's four-kernel form: its algorithmic bytes B_residual =
+    # 64 N_c + G over the summed durations of its four kernels
     res_bytes = residual_algorithmic_bytes(nv, ne, nc)
     res_us = sum(rep_us.values())
-    achieved = res_bytes / (res_us * 1e-6) / 1e9
-    traffic = [measured_traffic(KERNEL_NAMES[n]) for n in names]
+    res_achieved = res_bytes / (res_us * 1e-6) / 1e9
+    res_traffic = [measured_traffic(KERNEL_NAMES[n]) for n in names]
     dom = max(names, key=lambda n: rep_us[n])
     step_bytes = step_algorithmic_bytes(nv, ne, nc)
     ms_step = total_ms / a.steps
+    four_kernel = {"kernel": "residual = " + " + ".join(KERNEL_NAMES[n] for n in names),
+                   "bytes_per_launch": res_bytes, "avg_launch_us": res_us, "achieved": res_achieved,
+                   "frac": res_achieved / HBM_PEAK_GBS,
+                   "traffic": sum(res_traffic) if all(t is not None for t in res_traffic) else None,
+                   "dominant_kernel": KERNEL_NAMES[dom], "dominant_share_of_time": rep_us[dom] / res_us,
+                   "residual_back_to_back_us": res_b2b_us}
+    if tiling["tiles"] > 0:
+        # the fused stage (the kernel the timed steps run three times per
+        # step): B_stage = B_step / 3 over its duration in second-stage form
+        stage_bytes = step_bytes / 3.0
+        stage_us = s.time_kernel(5, launches)
+        achieved = stage_bytes / (stage_us * 1e-6) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(TILE_KERNEL),
+                    "kernel": TILE_KERNEL + " (residual + SSP-RK3 stage update, one launch per stage)",
+                    "bytes_per_launch": stage_bytes, "avg_launch_us": stage_us,
+                    "timing": "HIP events around back-to-back launches on the solver's stream"}
+    else:
+        roofline = dict(four_kernel, bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",
+                        timing="HIP events around back-to-back launches of each kernel on the solver's stream")
 
     out = {
         "metric": METRIC,
@@ -773,14 +804,9 @@ def main():
                    "limiter": "venkatakrishnan K=5" if a.limiter == "venkat" else "barth_jespersen",
                    "interp_mode": "mvc", "heat_flux": a.heat_flux, "mesh_order": a.mesh_order,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": sum(traffic) if all(t is not None for t in traffic) else None,
-                     "kernel": "residual = " + " + ".join(KERNEL_NAMES[n] for n in names),
-                     "bytes_per_launch": res_bytes, "avg_launch_us": res_us,
-                     "dominant_kernel": KERNEL_NAMES[dom], "dominant_share_of_time": rep_us[dom] / res_us,
-                     "residual_back_to_back_us": res_b2b_us,
-                     "timing": "HIP events around back-to-back launches of each kernel on the solver's stream"},
+        "roofline": roofline,
+        "tiling": tiling,
+        "four_kernel_residual": four_kernel,
         "kernels_avg_us": rep_us,
         "kernels_event_bracketed_us": avg_us,
         "kernels_compulsory_bytes": kb,
