@@ -386,19 +386,20 @@ def test_freestream_init_matches():
 
 def test_time_kernel_leaves_the_state():
     """o2f_time_kernel (bench.py's per-kernel timing) recomputes the stage
-    kernels' outputs at the current state: the state, the residual and the
-    face states afterwards are those before; bad arguments are refused."""
+    kernels' outputs at the current state (kernel 5, the fused tile stage,
+    writes a spare state buffer): the state, the residual and the face
+    states afterwards are those before; bad arguments are refused."""
     s, m, p = pair(H.shipped_mesh(), H.SHIPPED, H.shipped_flow())
     U0 = s.state()
     R0 = s.residual()
     f0 = s.faces()
-    for k in range(5):
+    for k in range(6):
         assert s.time_kernel(k, 5) > 0.0
     assert same(s.state(), U0)
     assert same(s.residual(), R0)
     f1 = s.faces()
     assert same(f1[0], f0[0]) and same(f1[1], f0[1])
-    for k, n in ((5, 5), (-1, 5), (0, 0)):
+    for k, n in ((6, 5), (-1, 5), (0, 0)):
         with pytest.raises(RuntimeError):
             s.time_kernel(k, n)
 
