@@ -159,9 +159,10 @@ bool tiles_on(o2f_ctx *c) {
     TileSet ts;
     std::string err;
     int bpc = 0, vgprs = 0, target = c->tile_cells;
+    const int nk_max = *std::max_element(c->hm.cnv.begin(), c->hm.cnv.end());
     auto build = [&](int t) -> bool {
         if (!build_tiles(c->hm, t, ts, err)) return false;
-        const hipError_t e = tile_stage_attributes(ts.lds_bytes(), &bpc, &vgprs);
+        const hipError_t e = tile_stage_attributes(ts.lds_bytes(), nk_max, &bpc, &vgprs);
         if (e != hipSuccess) err = std::string("tile kernel attributes: ") + hipGetErrorString(e);
         return e == hipSuccess && bpc > 0;
     };
@@ -232,6 +233,7 @@ bool tiles_on(o2f_ctx *c) {
     v.lsten = reinterpret_cast<const uint16_t *>(m + o_ls);
     v.epk = c->epk;
     v.ntiles = (int)ts.head.size();
+    v.nk_max = nk_max;
     v.max_vo = ts.max_vo;
     v.max_gd = ts.max_gd;
     const int vert = std::max(4 * ts.max_vg, 5 * ts.max_vo);

@@ -108,6 +108,7 @@ struct Lds {
 // tile's edges with the boundary ghosts (src/advection.cpp:232-321,
 // src/boundary.cpp:48-80).  The arithmetic of reconstruct_body, one lane
 // per cell.
+template <int NK>
 __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Geom &G, const double2 *__restrict__ EP,
                                           const Phys &P, int nc, const double *__restrict__ U, const Lds &L, int i) {
     const int c = r.c;
@@ -120,7 +121,7 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Geom &G
     int bcs[4] = {0, 0, 0, 0};
     const double cx = face ? G.ccx[c] : 0.0, cy = face ? G.ccy[c] : 0.0;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NK; ++j) {
         if (j >= nk) break;
         const double2 *pe = EP + 4 * (long)r.edge[j];
         const double2 n2 = pe[0], lb = pe[1];
@@ -159,7 +160,7 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Geom &G
     if (!face) return;
     double nbv[4][4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NK; ++j) {
         const int nb = r.nbr[j];
         if (nb >= 0) {
             ld4(U, nb, nbv[j]);
@@ -174,14 +175,14 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Geom &G
     for (int q = 0; q < 4; ++q) {
         double mn = u[q], mx = u[q];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NK; ++j) {
             const double v = nbv[j][q];
             if (v < mn) mn = v;
             if (v > mx) mx = v;
         }
         double al = 1.0;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NK; ++j) {
             if (j >= nk) break;
             const double pred = u[q] + gx[q] * dx[j] + gy[q] * dy[j];
             const double a = (P.limiter == 1) ? alpha_bj(u[q], pred, mn, mx)
@@ -189,7 +190,7 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Geom &G
             al = smin(al, a);
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < NK; ++j) {
             if (j >= nk) break;
             const int le = r.le[j];
             if (le < 0) continue;
@@ -258,7 +259,7 @@ __device__ __forceinline__ void tile_vertex_stress(const Vtx &r, const uint16_t 
 // (src/diffusion.cpp:145-201) and the Shu-Osher update
 // (src/simulation.cpp:159-167): the arithmetic of flux_update_body, one
 // lane per cell.  Returns |u| + c of the new state (kFlagSpeed), else 0.
-template <bool OUT_AOS>
+template <bool OUT_AOS, int NK>
 __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, const Topo &T, const Geom &G,
                                               const double2 *__restrict__ EP, const Phys &P,
                                               const double *__restrict__ U,
@@ -275,7 +276,7 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
     double rk[4] = {0.0, 0.0, 0.0, 0.0};
     double w1[4] = {0.0, 0.0, 0.0, 0.0}, w2[4] = {0.0, 0.0, 0.0, 0.0}, w3[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NK; ++j) {
         if (j >= nk) break;
         const double2 *pe = EP + 4 * (long)r.edge[j];
         const double2 n2 = pe[0], lt = pe[1];
@@ -313,7 +314,7 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
 #pragma unroll
     for (int q = 0; q < 4; ++q) rk[q] = rk[q] * -invV;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < NK; ++j) {
         if (j >= nk) break;
         rk[1] = rk[1] + w1[j];
         rk[2] = rk[2] + w2[j];
@@ -343,7 +344,7 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
 
 // At most 128 VGPRs (four waves per SIMD): the kernel keeps every tile of
 // the mesh resident in one round, so occupancy is what hides its loads.
-template <bool OUT_AOS>
+template <bool OUT_AOS, int NK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_stage(Topo T, Geom G, Phys P, TileView A,
                                                     const double *__restrict__ U, double *__restrict__ Uout,
                                                     double *__restrict__ Uold, double *__restrict__ R,
@@ -374,7 +375,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // 2. gradients and stresses of every gradient cell; limiter and face states of the face cells
     for (int i = tid; i < h.nGd; i += nth) {
         const Rec r = load_rec(A.cell + h.cell0 + i);
-        tile_cell(r, i < h.nF, G, EP, P, T.nc, U, L, i);
+        tile_cell<NK>(r, i < h.nF, G, EP, P, T.nc, U, L, i);
     }
     __syncthreads();
     // 3. HLLC once per tile edge (in place of its left face state), and the
@@ -400,7 +401,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int i = tid; i < h.nO; i += nth) {
         const Rec r = load_rec(A.cell + h.cell0 + i);
         const double s =
-            tile_update<OUT_AOS>(r, A.own[h.cell0 + i], T, G, EP, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
+            tile_update<OUT_AOS, NK>(r, A.own[h.cell0 + i], T, G, EP, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
         if (flags & kFlagSpeed) m = smax(m, s);
     }
     if (flags & kFlagSpeed) {
@@ -451,15 +452,30 @@ __global__ __launch_bounds__(256) void k_state_to_soa(const double *__restrict__
 
 }  // namespace
 
+template <bool OUT_AOS, int NK>
+const void *tile_kernel() {
+    return reinterpret_cast<const void *>(&k_tile_stage<OUT_AOS, NK>);
+}
+
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
                              double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
                              double c1, int flags, double *speed, hipStream_t s) {
-    if (out_aos)
-        hipLaunchKernelGGL(k_tile_stage<true>, dim3(A.ntiles), dim3(kTileThreads), A.lds_bytes, s, T, G, P, A, U,
-                           Uout, Uold, R, dtp, c0, c1, flags, speed);
-    else
-        hipLaunchKernelGGL(k_tile_stage<false>, dim3(A.ntiles), dim3(kTileThreads), A.lds_bytes, s, T, G, P, A, U,
-                           Uout, Uold, R, dtp, c0, c1, flags, speed);
+    const dim3 grid(A.ntiles), block(kTileThreads);
+    if (A.nk_max <= 3) {
+        if (out_aos)
+            hipLaunchKernelGGL((k_tile_stage<true, 3>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R, dtp,
+                               c0, c1, flags, speed);
+        else
+            hipLaunchKernelGGL((k_tile_stage<false, 3>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R,
+                               dtp, c0, c1, flags, speed);
+    } else {
+        if (out_aos)
+            hipLaunchKernelGGL((k_tile_stage<true, 4>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R, dtp,
+                               c0, c1, flags, speed);
+        else
+            hipLaunchKernelGGL((k_tile_stage<false, 4>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R,
+                               dtp, c0, c1, flags, speed);
+    }
     return hipGetLastError();
 }
 
@@ -478,31 +494,36 @@ hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t tile_stage_attributes(size_t lds_bytes, int *blocks_per_cu, int *vgprs) {
+hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_cu, int *vgprs) {
     // every context may launch up to the device's LDS per workgroup, less
     // the kernel's static LDS; a failed query leaves no error behind
-    const void *f = reinterpret_cast<const void *>(&k_tile_stage<true>);
-    const void *g = reinterpret_cast<const void *>(&k_tile_stage<false>);
+    const void *f[2] = {nk_max <= 3 ? tile_kernel<true, 3>() : tile_kernel<true, 4>(),
+                        nk_max <= 3 ? tile_kernel<false, 3>() : tile_kernel<false, 4>()};
     int dev = 0, lds_max = 0;
-    hipFuncAttributes fa{};
     *blocks_per_cu = 0;
+    if (vgprs) *vgprs = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-    if (e == hipSuccess) e = hipFuncGetAttributes(&fa, f);
-    const int dyn_max = lds_max - (int)fa.sharedSizeBytes;
-    if (e == hipSuccess && lds_bytes <= (size_t)dyn_max) {
-        e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
-        if (e == hipSuccess) e = hipFuncSetAttribute(g, hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
-        int bg = 0;
-        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, f, kTileThreads, lds_bytes);
-        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&bg, g, kTileThreads, lds_bytes);
-        if (e == hipSuccess) *blocks_per_cu = std::min(*blocks_per_cu, bg);
+    int bpc = 1 << 30;
+    for (const void *k : f) {
+        hipFuncAttributes fa{};
+        if (e == hipSuccess) e = hipFuncGetAttributes(&fa, k);
+        const int dyn_max = lds_max - (int)fa.sharedSizeBytes;
+        if (e != hipSuccess || lds_bytes > (size_t)dyn_max) {
+            bpc = 0;
+            break;
+        }
+        if (vgprs) *vgprs = std::max(*vgprs, fa.numRegs);
+        int b = 0;
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, kTileThreads, lds_bytes);
+        bpc = std::min(bpc, b);
     }
-    if (vgprs) *vgprs = fa.numRegs;
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        *blocks_per_cu = 0;
+        bpc = 0;
     }
+    *blocks_per_cu = bpc;
     return e;
 }
 

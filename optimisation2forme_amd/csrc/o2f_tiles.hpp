@@ -22,6 +22,7 @@ struct TileView {
     const uint16_t *lsten;
     const double *epk;  // packed edge geometry (launch_pack_edges), 8 doubles per edge
     int ntiles;
+    int nk_max;  // most edges of a cell: 3 (triangles only) selects the three-slot kernels
     int max_vo, max_gd;
     int lds_tq, lds_fs;
     size_t lds_bytes;
@@ -42,8 +43,9 @@ hipError_t launch_pack_edges(const Geom &G, const char *ebc, int ne, double *epk
 // The state between the C ABI's SoA [q][nc] and the stages' AoS.
 hipError_t launch_state_to_aos(const double *U, int nc, double *A, hipStream_t s);
 hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s);
-// Allows lds_bytes of dynamic LDS for the stage kernel and reports how many
-// of its workgroups fit on one CU with that LDS, and its VGPRs.
-hipError_t tile_stage_attributes(size_t lds_bytes, int *blocks_per_cu, int *vgprs);
+// Allows the device's full LDS for the stage kernels of cells of at most
+// nk_max edges and reports how many of their workgroups fit on one CU with
+// lds_bytes each (0: they do not fit), and their VGPRs.
+hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_cu, int *vgprs);
 
 }  // namespace o2f
