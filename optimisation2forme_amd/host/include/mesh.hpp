@@ -86,6 +86,7 @@ struct Mesh {
 
    private:
     o2f_ctx *ctx_ = nullptr;
+    int tile_cells_ = -1;  // the tile setting last given to the context
     std::vector<int> cell_perm_;
 };
 

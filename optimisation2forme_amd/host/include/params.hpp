@@ -76,6 +76,10 @@ struct SimulationParams {
     // "none" | "rcm" | "rcm_xcd": renumber the mesh for the device's gathers
     // (o2f_mesh_reorder); state files keep the mesh file's cell order
     std::string mesh_reorder = "none";
+    // stages of the time loop (o2f_set_tile_cells): -1 fused tile stages
+    // sized automatically, 0 the four-kernel stages, n > 0 tiles of at most
+    // n cells (results bitwise the same)
+    int tile_cells = -1;
 
     // c_v = c_p / gamma, R = c_p - c_v (src/params.cpp:5-8)
     void finalize();

@@ -44,6 +44,8 @@ Mesh::Mesh(const SimulationParams &params) {
     check(o2f_mesh_topology(&d, &ne, ev.data(), ec.data(), eb.data(), eg.data(), cg.data(), ce.data()));
     const o2f_params op = to_o2f_params(params);
     check(o2f_ctx_create(&d, &op, &ctx_));
+    check(o2f_set_tile_cells(ctx_, params.tile_cells));
+    tile_cells_ = params.tile_cells;
 
     vertices.resize(nv);
     x_min = y_min = std::numeric_limits<double>::infinity();
@@ -118,6 +120,10 @@ void Mesh::state_to_file_order(std::vector<double> &U) const {
 void Mesh::set_params(const SimulationParams &params) {
     const o2f_params op = to_o2f_params(params);
     check(o2f_set_params(ctx_, &op));
+    if (params.tile_cells != tile_cells_) {  // the tiles are rebuilt only on a change
+        check(o2f_set_tile_cells(ctx_, params.tile_cells));
+        tile_cells_ = params.tile_cells;
+    }
 }
 
 void Mesh::push_state() {

@@ -87,6 +87,7 @@ constexpr Key kKeys[] = {
     {"shape_param_file", &P::shape_param_file},
     {"state_load_path", &P::state_load_path},
     {"state_save_path", &P::state_save_path},
+    {"tile_cells", &P::tile_cells},
     {"venkat_K", &P::venkat_K},
     {"verbosity", &P::verbosity},
     {"viscosity", &P::viscosity},
