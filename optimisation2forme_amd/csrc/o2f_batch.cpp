@@ -493,9 +493,11 @@ int o2f_batch_time_steps(o2f_batch *b, int nsteps, int use_graph, double *total_
     return run(b, -1.0, nsteps, k, use_graph, nullptr, nullptr, total_ms);
 }
 
-int o2f_run_to_residual(o2f_ctx *c, double tol, int max_steps, int check_every, int use_graph, int *steps,
-                        double *res) {
-    if (!c) return O2F_ERR_ARG;
+}  // extern "C"
+
+// o2f_run_to_residual on the batched four-kernel stages (a batch of one).
+int run_to_residual_batch(o2f_ctx *c, double tol, int max_steps, int check_every, int use_graph, int *steps,
+                          double *res) {
     o2f_batch *b = nullptr;
     o2f_ctx *const one[1] = {c};
     int st = o2f_batch_create(one, 1, &b);
@@ -505,5 +507,3 @@ int o2f_run_to_residual(o2f_ctx *c, double tol, int max_steps, int check_every, 
     o2f_batch_destroy(b);
     return st;
 }
-
-}  // extern "C"

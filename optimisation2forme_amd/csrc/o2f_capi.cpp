@@ -127,10 +127,10 @@ int rk3_stages(o2f_ctx *c, bool last_outputs, bool next_speed, bool begin_step =
 }
 
 void release_tiles(o2f_ctx *c) {
-    for (void *p : {c->tile_mem, (void *)c->Ua, (void *)c->Ub, (void *)c->Uc, (void *)c->epk})
+    for (void *p : {c->tile_mem, (void *)c->Ua, (void *)c->Ub, (void *)c->Uc, (void *)c->epk, (void *)c->rparts})
         if (p) (void)hipFree(p);
     c->tile_mem = nullptr;
-    c->Ua = c->Ub = c->Uc = c->epk = nullptr;
+    c->Ua = c->Ub = c->Uc = c->epk = c->rparts = nullptr;
     c->tile_aos = false;
     c->tv = TileView{};
     c->tiles_ready = false;
@@ -270,7 +270,9 @@ int tiles_leave(o2f_ctx *c) {
 // the C ABI's c->U, and the third stage runs on the four kernels, in place,
 // leaving the intermediates the adjoint and the force coefficients read
 // (and the state in c->U).
-int tiled_step(o2f_ctx *c, bool last_outputs, double *hist) {
+// check: the first stage also stores R_rho of the step's start state and
+// the density-residual check follows it (o2f_run_to_residual).
+int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check = false, double res_tol = -1.0) {
     hipStream_t s = c->stream;
     const Topo T = c->topo();
     const Geom G = c->geo[0].view();
@@ -284,7 +286,12 @@ int tiled_step(o2f_ctx *c, bool last_outputs, double *hist) {
     const double *dtp = c->scal + kScalDt;
     const TileView &A = c->tv;
     HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ua, c->Ub, true, c->Uold, c->R, dtp, 0.0, 1.0,
-                                kFlagRK | kFlagSaveOld, nullptr, s));
+                                kFlagRK | kFlagSaveOld | (check ? kFlagStoreR : 0), nullptr, s));
+    if (check) {
+        const double a_inf = std::sqrt(c->prm.gamma * c->prm.p_ref / c->prm.rho_ref);
+        const double res_scale = c->prm.rho_ref * (c->prm.inflow_velocity * a_inf) / 1.0;
+        HIPCHK(c, launch_residual_check(c->R, c->hm.nc, c->rparts, res_scale, res_tol, c->scal, c->phys.gate, s));
+    }
     if (last_outputs) {
         HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->U, false, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0,
                                     kFlagRK, nullptr, s));
@@ -882,6 +889,65 @@ int o2f_time_loop(o2f_ctx *c, double final_time, int max_steps, int *steps, doub
     int st = o2f_run_until(c, final_time, max_steps, steps, &tt, nullptr, nullptr);
     if (st == O2F_OK && t) *t = tt;
     return st;
+}
+
+// Tiled steps to a density-residual tolerance: the batch path's semantics
+// (steps 0, k, 2k, ... checked at their start state; the next step's start
+// stops the loop; steps and r from the device) on the tile stages, one host
+// poll per 64 steps; the last stage's intermediates formed after the loop.
+int o2f_run_to_residual(o2f_ctx *c, double tol, int max_steps, int check_every, int use_graph, int *steps,
+                        double *res) {
+    if (!c) return O2F_ERR_ARG;
+    if (max_steps < 0 || check_every < 1) return c->fail(O2F_ERR_ARG, "batch run: max_steps >= 0, check_every >= 1");
+    int st = check_bc(c);
+    if (st) return st;
+    if (!tiles_on(c)) return run_to_residual_batch(c, tol, max_steps, check_every, use_graph, steps, res);
+    hipStream_t s = c->stream;
+    const long nrp = (c->hm.nc + kResChunk - 1) / kResChunk;
+    if (!c->rparts) HIPCHK(c, dalloc(c->rparts, nrp));
+    double init[kScalCount] = {};
+    init[kScalActive] = 1.0;
+    init[kScalRes] = -1.0;
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &init[kScalT], sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemcpyAsync(c->scal + kScalActive, &init[kScalActive], (kScalRes - kScalActive + 1) * sizeof(double),
+                             hipMemcpyHostToDevice, s));
+    const Phys saved = c->phys;
+    c->phys.gate = c->scal + kScalActive;
+    c->phys.final_t = HUGE_VAL;
+    c->speed_valid = false;
+    const double res_tol = tol > 0.0 ? tol : -1.0;
+    double h[kScalCount];
+    for (int launched = 0; launched < max_steps && st == 0;) {
+        const int n = std::min(64, max_steps - launched);
+        for (int k = 0; k < n && st == 0; ++k, ++launched) {
+            // after a step the speed partials are the new state's (c->U is
+            // stale while the state lives in Ua)
+            st = tiled_step(c, false, nullptr, tol > 0.0 && launched % check_every == 0, res_tol);
+            c->speed_valid = (st == O2F_OK);
+        }
+        if (st) break;
+        hipError_t e = hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) st = c->fail(O2F_ERR_HIP, std::string("o2f_run_to_residual: ") + hipGetErrorString(e));
+        else if (h[kScalActive] == 0.0) break;
+    }
+    c->phys = saved;
+    c->speed_valid = false;
+    const int st_leave = tiles_leave(c);
+    if (st) return st;
+    if (st_leave) return st_leave;
+    HIPCHK(c, hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    // the last step's third stage read Uc: its face states, stresses and
+    // vertex values as those four kernels leave them (R keeps the last
+    // checked step's, as on the batched stages)
+    if (h[kScalSteps] > 0.0) {
+        HIPCHK(c, launch_state_to_soa(c->Uc, c->hm.nc, c->Ub, s));
+        if ((st = run_stage(c, 0, 0, nullptr, 0.0, 0.0, false, false, nullptr, c->Ub))) return st;
+    }
+    if (steps) *steps = static_cast<int>(h[kScalSteps]);
+    if (res) *res = h[kScalRes];
+    return O2F_OK;
 }
 
 int o2f_residual_norm(o2f_ctx *c, double *rnorm) {

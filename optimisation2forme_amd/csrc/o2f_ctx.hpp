@@ -107,6 +107,7 @@ struct o2f_ctx {
     double *Ua = nullptr, *Ub = nullptr, *Uc = nullptr;
     bool tile_aos = false;
     double *epk = nullptr;  // packed edge geometry of geo[0] for the tile stages
+    double *rparts = nullptr;  // density-residual partials of a tiled o2f_run_to_residual
     int tile_target = 0, tile_bpc = 0, tile_vgprs = 0;
     bool have_grad = false;
     // adjoint
@@ -154,3 +155,6 @@ struct o2f_ctx {
 
 // apply_boundary_conditions' check (src/boundary.cpp:84-92) for context c.
 int o2f_ctx_check_bc(o2f_ctx *c);
+// o2f_run_to_residual on the batched four-kernel stages (o2f_batch.cpp).
+int run_to_residual_batch(o2f_ctx *c, double tol, int max_steps, int check_every, int use_graph, int *steps,
+                          double *res);

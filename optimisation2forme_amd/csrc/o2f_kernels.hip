@@ -1084,33 +1084,31 @@ __global__ __launch_bounds__(256) void kb_flux_update(const Member *__restrict__
 
 // Sum of R_rho^2 over cells [chunk kResChunk, +kResChunk): thread t adds
 // cells t, t + 256, ... in order, then a fixed tree.
-__global__ __launch_bounds__(256) void kb_residual_partial(const Member *__restrict__ M,
-                                                           const int2 *__restrict__ tiles, int ntiles) {
+// Sum of R_rho^2 over cells [chunk kResChunk, +kResChunk) in a fixed
+// 256-wide tree: partial `chunk` of the density residual.
+__device__ __forceinline__ void residual_partial_body(const double *__restrict__ R, int nc, int chunk,
+                                                      double *__restrict__ rparts) {
     __shared__ double sm[256];
-    each_tile(tiles, ntiles, [&](int m, int chunk) {
-        const Member &A = M[m];
-        if (*A.P.gate == 0.0) return;  // uniform per tile
-        const int nc = A.T.nc, c0 = chunk * kResChunk, c1 = min(nc, c0 + kResChunk);
-        double acc = 0.0;
-        for (int c = c0 + threadIdx.x; c < c1; c += 256) acc += A.R[c] * A.R[c];
-        sm[threadIdx.x] = acc;
+    const int c0 = chunk * kResChunk, c1 = min(nc, c0 + kResChunk);
+    double acc = 0.0;
+    for (int c = c0 + threadIdx.x; c < c1; c += 256) acc += R[c] * R[c];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sm[threadIdx.x] += sm[threadIdx.x + st];
         __syncthreads();
-        for (int st = 128; st > 0; st >>= 1) {
-            if (threadIdx.x < st) sm[threadIdx.x] += sm[threadIdx.x + st];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) A.rparts[chunk] = sm[0];
-        __syncthreads();
-    });
+    }
+    if (threadIdx.x == 0) rparts[chunk] = sm[0];
+    __syncthreads();
 }
 
-// r = sqrt(mean_c R_rho^2) / (rho_inf U_inf / c), one block per member.
-__global__ __launch_bounds__(256) void kb_residual_final(const Member *__restrict__ M) {
+// r = sqrt(mean_c R_rho^2) / (rho_inf U_inf / c) from the partials in order
+// (one block); sets the converged flag once r <= tol.
+__device__ __forceinline__ void residual_final_body(const double *__restrict__ rparts, int nrparts, int nc,
+                                                    double res_scale, double tol, double *__restrict__ scal) {
     __shared__ double sm[256];
-    const Member &A = M[blockIdx.x];
-    if (*A.P.gate == 0.0) return;
     double acc = 0.0;
-    for (int k = threadIdx.x; k < A.nrparts; k += 256) acc += A.rparts[k];
+    for (int k = threadIdx.x; k < nrparts; k += 256) acc += rparts[k];
     sm[threadIdx.x] = acc;
     __syncthreads();
     for (int st = 128; st > 0; st >>= 1) {
@@ -1118,10 +1116,38 @@ __global__ __launch_bounds__(256) void kb_residual_final(const Member *__restric
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const double r = sqrt(sm[0] / A.T.nc) / A.res_scale;
-        A.scal[kScalRes] = r;
-        if (r <= A.res_tol) A.scal[kScalConv] = 1.0;
+        const double r = sqrt(sm[0] / nc) / res_scale;
+        scal[kScalRes] = r;
+        if (r <= tol) scal[kScalConv] = 1.0;
     }
+}
+
+__global__ __launch_bounds__(256) void kb_residual_partial(const Member *__restrict__ M,
+                                                           const int2 *__restrict__ tiles, int ntiles) {
+    each_tile(tiles, ntiles, [&](int m, int chunk) {
+        const Member &A = M[m];
+        if (*A.P.gate == 0.0) return;  // uniform per tile
+        residual_partial_body(A.R, A.T.nc, chunk, A.rparts);
+    });
+}
+
+__global__ __launch_bounds__(256) void kb_residual_final(const Member *__restrict__ M) {
+    const Member &A = M[blockIdx.x];
+    if (*A.P.gate == 0.0) return;
+    residual_final_body(A.rparts, A.nrparts, A.T.nc, A.res_scale, A.res_tol, A.scal);
+}
+
+// The same for one mesh (its gate, one block per chunk, then one block).
+__global__ __launch_bounds__(256) void k_residual_partial(const double *__restrict__ R, int nc,
+                                                          double *__restrict__ rparts, const double *gate) {
+    if (gate && *gate == 0.0) return;
+    residual_partial_body(R, nc, blockIdx.x, rparts);
+}
+__global__ __launch_bounds__(256) void k_residual_final(const double *__restrict__ rparts, int nrparts, int nc,
+                                                        double res_scale, double tol, double *__restrict__ scal,
+                                                        const double *gate) {
+    if (gate && *gate == 0.0) return;
+    residual_final_body(rparts, nrparts, nc, res_scale, tol, scal);
 }
 
 __global__ __launch_bounds__(64) void kb_collect(const Member *__restrict__ M, int n, double *__restrict__ flags) {
@@ -1320,6 +1346,15 @@ hipError_t launch_b_residual(const Member *M, const int2 *chunk_tiles, int nchun
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(kb_residual_final, dim3(nmembers), dim3(256), 0, s, M);
+    return hipGetLastError();
+}
+hipError_t launch_residual_check(const double *R, int nc, double *rparts, double res_scale, double tol,
+                                 double *scal, const double *gate, hipStream_t s) {
+    const int nparts = (nc + kResChunk - 1) / kResChunk;
+    hipLaunchKernelGGL(k_residual_partial, dim3((unsigned)std::max(1, nparts)), dim3(256), 0, s, R, nc, rparts, gate);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(256), 0, s, rparts, nparts, nc, res_scale, tol, scal, gate);
     return hipGetLastError();
 }
 hipError_t launch_b_collect(const Member *M, int nmembers, double *flags, hipStream_t s) {

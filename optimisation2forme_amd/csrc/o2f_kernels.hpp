@@ -142,6 +142,12 @@ hipError_t launch_b_flux_update(const Member *M, const int2 *tiles, int ntiles, 
 // fixed chunks of kResChunk cells, then one block per member sums them in
 // order, stores r in scal[kScalRes] and sets scal[kScalConv] when r <= tol.
 hipError_t launch_b_residual(const Member *M, const int2 *chunk_tiles, int nchunks, int nmembers, hipStream_t s);
+// The same density-residual check for one mesh: the R_rho stored by the
+// last StoreR stage, partials over kResChunk cells (rparts, ceil(nc /
+// kResChunk) of them), r into scal[kScalRes], scal[kScalConv] = 1 when
+// r <= tol; gated on *gate when given.
+hipError_t launch_residual_check(const double *R, int nc, double *rparts, double res_scale, double tol,
+                                 double *scal, const double *gate, hipStream_t s);
 // flags[i] = member i still active
 hipError_t launch_b_collect(const Member *M, int nmembers, double *flags, hipStream_t s);
 

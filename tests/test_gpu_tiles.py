@@ -161,3 +161,48 @@ def test_tiled_steps_match_the_oracle_10k(small_developed):
     dts_o = m.run_steps(p, 4)
     assert np.allclose(dts_g, dts_o, rtol=1e-14, atol=0)
     assert max(H.var_rel_err(s.state(), m.state())) < 1e-10
+
+
+@pytest.mark.parametrize("check_every", [1, 7])
+def test_tiled_run_to_residual_bitwise_batched_stages(small_developed, check_every):
+    """o2f_run_to_residual on tiles against the batched four-kernel stages
+    (o2f_set_tile_cells(0)): the same stopping step and residual, state and
+    last-stage face states, checking every step or every seventh."""
+    flow = small_flow()
+    # the residual of step 49's start state (checked whether every step or
+    # every seventh is): a tolerance some step up to 49 reaches
+    s0 = capi.Solver(H.small_mesh(), H.capi_params(flow))
+    s0.set_tile_cells(0)
+    s0.set_state(small_developed)
+    n0, r49 = s0.run_to_residual(1e-300, 50, check_every=1)
+    s0.close()
+    assert n0 == 50 and r49 > 0
+    tol = r49
+    res = []
+    for cells in (-1, 0):
+        s = capi.Solver(H.small_mesh(), H.capi_params(flow))
+        s.set_tile_cells(cells)
+        s.set_state(small_developed)
+        out = s.run_to_residual(tol, 400, check_every=check_every)
+        res.append((out, s.state(), s.faces(), s.tile_info()["tiles"]))
+        s.close()
+    (oa, Ua, fa, ta), (ob, Ub, fb, tb) = res
+    assert ta > 0 and tb == 0
+    assert oa == ob and 0 < oa[0] <= 50 and oa[1] <= tol
+    assert np.all(np.isfinite(Ua)) and same(Ua, Ub)
+    assert same(fa[0], fb[0]) and same(fa[1], fb[1])
+
+
+def test_tiled_run_to_residual_without_tolerance(small_developed):
+    """With no tolerance every step runs: the same end state either way."""
+    flow = small_flow()
+    states = []
+    for cells in (-1, 0):
+        s = capi.Solver(H.small_mesh(), H.capi_params(flow))
+        s.set_tile_cells(cells)
+        s.set_state(small_developed)
+        n, r = s.run_to_residual(0.0, 9)
+        assert n == 9 and r == -1.0
+        states.append(s.state())
+        s.close()
+    assert same(states[0], states[1])
