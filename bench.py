@@ -315,10 +315,7 @@ def bench_batch(a):
         "speedup_vs_sequential": seq_ms / batch_ms,
         "autotuned_shapes": dict(zip(capi.Batch.KINDS, shapes)),
         "setup_s": setup_s,
-        "gpu_warmup": "%d untimed residual evaluations (%.0f ms) on the initial state before the %d warm-up "
-                      "steps; the state is unchanged by them" % (spin, a.gpu_warmup_ms, a.warmup),
-        "state_finite": state["finite"],
-        "state_after_timed_steps": state,
+        "state_finite": finite,
     }
     print(json.dumps(out), flush=True)
 
