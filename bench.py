@@ -39,8 +39,10 @@ measures the batched-candidate workload instead: 64 NACA 4-digit shapes
 (m ~ U[0, 0.06], p ~ U[0.2, 0.6], t ~ U[0.08, 0.15], seeded), each with its
 own O-mesh of U[20k, 120k] triangles and its own flow (Mach ~ U[0.5, 0.85],
 AoA ~ U[0, 4] deg, Re 5000, Fourier heat flux), advanced together by one
-launch per stage kernel (autotuned shapes, graph replay), against the same
-64 meshes solved one after another.
+fused tile-stage launch per SSP-RK3 stage over every member's tiles (graph
+replay; --tile-cells as for one mesh), against the same 64 meshes solved
+one after another and against the four-kernel batched stages (autotuned
+launch shapes).
 
     python bench.py --adjoint
 
@@ -284,7 +286,17 @@ def bench_batch(a):
         seq_graph += one.time_steps(K, use_graph=not a.no_graph)
         one.close()
     b = capi.Batch(solvers)
+    # the four-kernel batched stages: autotuned launch shapes, timed beside
     shapes = b.autotune(reps=3)
+    b.set_tile_cells(0)
+    for s in solvers:
+        s.init_freestream()
+    b.time_steps(2, use_graph=not a.no_graph)
+    for s in solvers:
+        s.init_freestream()
+    four_kernel_ms = b.time_steps(K, use_graph=not a.no_graph)
+    b.set_tile_cells(a.tile_cells)
+    tiling = b.tile_info()
     for s in solvers:
         s.init_freestream()
     b.time_steps(2, use_graph=not a.no_graph)
@@ -313,7 +325,9 @@ def bench_batch(a):
         "sequential": {"direct_ms_per_step": seq_direct / K, "graph_ms_per_step": seq_graph / K,
                        "value": total_vs / (seq_ms * 1e-3)},
         "speedup_vs_sequential": seq_ms / batch_ms,
-        "autotuned_shapes": dict(zip(capi.Batch.KINDS, shapes)),
+        "tiling": tiling,
+        "four_kernel_batch": {"ms_per_step": four_kernel_ms / K, "value": total_vs / (four_kernel_ms * 1e-3),
+                              "autotuned_shapes": dict(zip(capi.Batch.KINDS, shapes))},
         "setup_s": setup_s,
         "state_finite": finite,
     }

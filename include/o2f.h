@@ -401,6 +401,22 @@ int o2f_batch_run(o2f_batch *batch, double tol, int max_steps, int check_every, 
 int o2f_batch_run_until(o2f_batch *batch, double final_time, int max_steps, int use_graph, int *steps, double *t);
 /* Elapsed milliseconds of nsteps batched steps (tol <= 0, no early stop). */
 int o2f_batch_time_steps(o2f_batch *batch, int nsteps, int use_graph, double *total_ms);
+/* The batch's steps on fused tile stages (no counterpart in the reference):
+ * one launch per SSP-RK3 stage over the cell tiles of every member, the
+ * members' tile tables concatenated (a tile as in o2f_set_tile_cells).
+ * cells: -1 (the default) the largest tiles of which one CU holds as many
+ * as the kernel's registers allow, 0 the four-kernel batched stages (whose
+ * launch shapes o2f_batch_set_shapes sets), n > 0 at most n cells per tile.
+ * A batch with a member that cannot be tiled runs the four-kernel stages.
+ * Every member's results are bitwise the same either way; after a run the
+ * members hold the intermediates of their last stage as on the four-kernel
+ * stages. */
+int o2f_batch_set_tile_cells(o2f_batch *batch, int cells);
+/* The tiling in use (built now if needed): *n_tiles 0 on the four-kernel
+ * stages (o2f_batch_last_error says why), else the tiles over all members,
+ * the cell cap, the LDS bytes of one tile, the tiles one CU holds at once
+ * and the kernel's VGPRs. */
+int o2f_batch_tile_info(o2f_batch *batch, int *n_tiles, int *cells, int *lds_bytes, int *tiles_per_cu, int *vgprs);
 
 /* ---- learned warm start ---------------------------------------------- */
 

@@ -44,6 +44,7 @@ EXPORTS = [
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
     "o2f_rk3_save_old", "o2f_rk3_substep", "o2f_bicgstab_set_restart_eps",
     "o2f_hllc_flux_batch_noalloc", "o2f_batch_run_until", "o2f_set_tile_cells", "o2f_tile_info",
+    "o2f_batch_set_tile_cells", "o2f_batch_tile_info",
 ]
 
 
@@ -162,6 +163,8 @@ def lib():
             "o2f_batch_autotune": (C.c_int, [vp, C.c_int, I]),
             "o2f_batch_run": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, C.c_int, I, D]),
             "o2f_batch_time_steps": (C.c_int, [vp, C.c_int, C.c_int, D]),
+            "o2f_batch_set_tile_cells": (C.c_int, [vp, C.c_int]),
+            "o2f_batch_tile_info": (C.c_int, [vp, I, I, I, I, I]),
             "o2f_batch_run_until": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, I, D]),
         }
         for name, (res, args) in sig.items():
@@ -635,6 +638,20 @@ class Batch:
         ms = C.c_double()
         self._check(lib().o2f_batch_time_steps(self.h, nsteps, int(use_graph), C.byref(ms)))
         return ms.value
+
+    def set_tile_cells(self, cells):
+        """Stages of the batch's steps: -1 fused tile stages over every
+        member's tiles, sized automatically (the default), 0 the four-kernel
+        batched stages, n > 0 tiles of at most n cells."""
+        self._check(lib().o2f_batch_set_tile_cells(self.h, int(cells)))
+
+    def tile_info(self):
+        """The tiling in use: dict of tiles (over all members), cells (per-tile
+        cap), lds_bytes, tiles_per_cu, vgprs (tiles 0: the four-kernel stages)."""
+        v = [C.c_int() for _ in range(5)]
+        self._check(lib().o2f_batch_tile_info(self.h, *[C.byref(x) for x in v]))
+        keys = ("tiles", "cells", "lds_bytes", "tiles_per_cu", "vgprs")
+        return {k: x.value for k, x in zip(keys, v)}
 
 
 def _f(a):

@@ -3,6 +3,7 @@
 #include "host_tiles.hpp"
 
 #include <algorithm>
+#include <cstring>
 #include <queue>
 #include <tuple>
 #include <unordered_map>
@@ -15,15 +16,15 @@ bool tagged_bc(char b) { return b == 'i' || b == 'o' || b == 'f' || b == 'w'; }
 
 // Cells of vertex v's interpolation stencil that its gather reads
 // (stencil kinds of compute_stencils).
-int stencil_used(const HostMesh &hm, int v) {
-    const int n = hm.vptr[v + 1] - hm.vptr[v];
-    switch (hm.skind[v]) {
+int stencil_used(int kind, int n) {
+    switch (kind) {
         case kStencilWsum: return n;
         case kStencilAvg2: return 2;
         case kStencilCopy: return 1;
         default: return 0;
     }
 }
+int stencil_used(const HostMesh &hm, int v) { return stencil_used(hm.skind[v], hm.vptr[v + 1] - hm.vptr[v]); }
 
 // Greedy growth of compact tiles (see build_tiles).
 std::vector<int> grow_tiles(const HostMesh &hm, int target, int &ntiles) {
@@ -86,9 +87,14 @@ void uniq(std::vector<int> &v) {
 
 }  // namespace
 
-size_t TileSet::lds_bytes() const {
-    const size_t vert = std::max<size_t>(32 * (size_t)max_vg, 40 * (size_t)max_vo);
-    return vert + 40 * (size_t)max_gd + 64 * (size_t)max_e;
+size_t TileSet::lds_layout(int *tq, int *fs) const {
+    // four doubles per state vertex, later five per stress vertex; five per
+    // gradient cell; eight per edge
+    const int vert = std::max(4 * max_vg, 5 * max_vo);
+    const int q = (vert + 3) & ~3, f = q + ((5 * max_gd + 3) & ~3);
+    if (tq) *tq = q;
+    if (fs) *fs = f;
+    return sizeof(double) * ((size_t)f + 8 * (size_t)max_e);
 }
 
 bool build_tiles(const HostMesh &hm, int target, TileSet &ts, std::string &err) {
@@ -164,6 +170,7 @@ bool build_tiles(const HostMesh &hm, int target, TileSet &ts, std::string &err) 
         h.nE = (int)E.size();
         h.vo0 = (int)ts.vo.size();
         h.nVo = (int)Vo.size();
+        h.slot = (int)ts.head.size();
         for (size_t i = 0; i < Gd.size(); ++i) {
             const int c = Gd[i];
             TileCell r{};
@@ -225,6 +232,7 @@ bool check_tiles(const HostMesh &hm, const TileSet &ts, std::string &err) {
             h.vg0 < 0 || h.vg0 + h.nVg > (int)ts.vg.size() || h.e0 < 0 || h.e0 + h.nE > (int)ts.edge.size() ||
             h.vo0 < 0 || h.vo0 + h.nVo > (int)ts.vo.size() || (int)ts.own.size() != (int)ts.cell.size())
             return fail(t, "counts or offsets out of range");
+        if (h.member != 0 || h.slot != t) return fail(t, "member or slot differs");
         if (h.nGd > ts.max_gd || h.nVg > ts.max_vg || h.nVo > ts.max_vo || h.nE > ts.max_e || h.nO > ts.max_o)
             return fail(t, "a count exceeds the LDS maxima");
         const TileCell *C = &ts.cell[h.cell0];
@@ -313,6 +321,83 @@ bool check_tiles(const HostMesh &hm, const TileSet &ts, std::string &err) {
     }
     for (int c = 0; c < hm.nc; ++c)
         if (owner[c] < 0) return fail(-1, "cell " + std::to_string(c) + " owned by no tile");
+    return true;
+}
+
+void concat_tiles(const std::vector<TileSet> &sets, TileSet &all) {
+    all = TileSet();
+    for (int m = 0; m < (int)sets.size(); ++m) {
+        const TileSet &ts = sets[m];
+        const int cb = (int)all.cell.size(), vgb = (int)all.vg.size(), eb = (int)all.edge.size(),
+                  vob = (int)all.vo.size(), lsb = (int)all.lsten.size();
+        for (TileHead h : ts.head) {
+            h.cell0 += cb;
+            h.vg0 += vgb;
+            h.e0 += eb;
+            h.vo0 += vob;
+            h.member = m;
+            all.head.push_back(h);
+        }
+        all.cell.insert(all.cell.end(), ts.cell.begin(), ts.cell.end());
+        all.own.insert(all.own.end(), ts.own.begin(), ts.own.end());
+        all.vg.insert(all.vg.end(), ts.vg.begin(), ts.vg.end());
+        all.edge.insert(all.edge.end(), ts.edge.begin(), ts.edge.end());
+        for (TileVertex r : ts.vo) {
+            r.ls += lsb;
+            all.vo.push_back(r);
+        }
+        all.lsten.insert(all.lsten.end(), ts.lsten.begin(), ts.lsten.end());
+        all.target = std::max(all.target, ts.target);
+        all.max_vg = std::max(all.max_vg, ts.max_vg);
+        all.max_vo = std::max(all.max_vo, ts.max_vo);
+        all.max_gd = std::max(all.max_gd, ts.max_gd);
+        all.max_e = std::max(all.max_e, ts.max_e);
+        all.max_o = std::max(all.max_o, ts.max_o);
+        all.sum_gd += ts.sum_gd;
+        all.sum_f += ts.sum_f;
+        all.sum_vg += ts.sum_vg;
+        all.sum_vo += ts.sum_vo;
+        all.sum_e += ts.sum_e;
+    }
+}
+
+bool check_concat(const std::vector<TileSet> &sets, const TileSet &all, std::string &err) {
+    auto fail = [&](int k, const std::string &what) {
+        err = "batch tile " + std::to_string(k) + ": " + what;
+        return false;
+    };
+    auto same = [](const void *a, const void *b, size_t bytes) { return bytes == 0 || std::memcmp(a, b, bytes) == 0; };
+    size_t k = 0;
+    for (int m = 0; m < (int)sets.size(); ++m) {
+        const TileSet &ts = sets[m];
+        if (ts.max_vg > all.max_vg || ts.max_vo > all.max_vo || ts.max_gd > all.max_gd || ts.max_e > all.max_e)
+            return fail((int)k, "LDS maxima below a member's");
+        for (int t = 0; t < (int)ts.head.size(); ++t, ++k) {
+            if (k >= all.head.size()) return fail((int)k, "missing");
+            const TileHead &a = all.head[k], &h = ts.head[t];
+            if (a.member != m || a.slot != t || a.nO != h.nO || a.nF != h.nF || a.nGd != h.nGd ||
+                a.nVg != h.nVg || a.nE != h.nE || a.nVo != h.nVo)
+                return fail((int)k, "head differs from its member's tile");
+            if (a.cell0 < 0 || a.cell0 + a.nGd > (int)all.cell.size() || a.vg0 < 0 ||
+                a.vg0 + a.nVg > (int)all.vg.size() || a.e0 < 0 || a.e0 + a.nE > (int)all.edge.size() ||
+                a.vo0 < 0 || a.vo0 + a.nVo > (int)all.vo.size() || all.own.size() != all.cell.size())
+                return fail((int)k, "offsets out of range");
+            if (!same(&all.cell[a.cell0], &ts.cell[h.cell0], h.nGd * sizeof(TileCell)) ||
+                !same(&all.own[a.cell0], &ts.own[h.cell0], h.nGd * sizeof(TileOwn)) ||
+                !same(&all.vg[a.vg0], &ts.vg[h.vg0], h.nVg * sizeof(TileVertex)) ||
+                !same(&all.edge[a.e0], &ts.edge[h.e0], h.nE * sizeof(int)))
+                return fail((int)k, "records differ from its member's tile");
+            for (int i = 0; i < h.nVo; ++i) {
+                const TileVertex &x = all.vo[a.vo0 + i], &y = ts.vo[h.vo0 + i];
+                if (x.v != y.v || x.b != y.b || x.nkb != y.nkb) return fail((int)k, "stress vertex differs");
+                const int n = stencil_used((y.nkb >> 16) & 0xff, y.nkb & 0xffff);
+                if (x.ls < 0 || x.ls + n > (int)all.lsten.size() ||
+                    !same(&all.lsten[x.ls], &ts.lsten[y.ls], n * sizeof(uint16_t)))
+                    return fail((int)k, "local stencil differs");
+            }
+        }
+    }
+    if (k != all.head.size()) return fail((int)k, "extra tiles");
     return true;
 }
 

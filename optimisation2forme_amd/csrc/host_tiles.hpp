@@ -62,7 +62,8 @@ struct TileHead {
     int vg0, nVg;            // state vertices
     int e0, nE;              // edges (global ids)
     int vo0, nVo;            // stress vertices
-    int pad[6];
+    int member, slot;        // the mesh of a batch it belongs to, and its index among that mesh's tiles
+    int pad[4];
 };
 static_assert(sizeof(TileHead) == 64, "TileHead is one 64-byte record");
 
@@ -79,8 +80,12 @@ struct TileSet {
     int max_vg = 0, max_vo = 0, max_gd = 0, max_e = 0, max_o = 0;
     // sums over the tiles (work with the rim counted as often as it is formed)
     long sum_gd = 0, sum_f = 0, sum_vg = 0, sum_vo = 0, sum_e = 0;
-    // LDS bytes one tile needs: state or stress vertices, cell stresses, face states
-    size_t lds_bytes() const;
+    // LDS layout of one tile sized for the largest counts, in doubles: the
+    // state / stress vertices from 0, the cell stresses from *tq, the face
+    // states from *fs (each region a multiple of four doubles, so rows stay
+    // 16-byte aligned); returns the bytes
+    size_t lds_layout(int *tq, int *fs) const;
+    size_t lds_bytes() const { return lds_layout(nullptr, nullptr); }
 };
 
 // Cells grouped into compact tiles of at most `target` cells: a tile grows
@@ -96,5 +101,15 @@ bool build_tiles(const HostMesh &hm, int target, TileSet &ts, std::string &err);
 // reads, with local indices that point at the right global entity.
 // Returns false with a description of the first violation.
 bool check_tiles(const HostMesh &hm, const TileSet &ts, std::string &err);
+
+// The tiles of several meshes (a batch) as one set: member m's tiles follow
+// member m - 1's, each head naming its member and its slot among that
+// member's tiles, its offsets moved past the earlier members' records and
+// its stress vertices' local stencils moved with them; the maxima are those
+// over every member.  Global ids inside the records stay the member's own.
+void concat_tiles(const std::vector<TileSet> &sets, TileSet &all);
+// Every tile of the concatenation against its member's own tile: same
+// counts, same records, same local stencils, member and slot in the head.
+bool check_concat(const std::vector<TileSet> &sets, const TileSet &all, std::string &err);
 
 }  // namespace o2f

@@ -10,15 +10,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <map>
+#include <thread>
 #include <utility>
 #include <string>
 #include <vector>
 
+#include "host_tiles.hpp"
 #include "o2f.h"
 #include "o2f_ctx.hpp"
 #include "o2f_kernels.hpp"
+#include "o2f_tiles.hpp"
 
 using namespace o2f;
 
@@ -60,6 +64,17 @@ struct o2f_batch {
     std::vector<double> scal_init;                // host source of the per-run scalar reset
     // captured step chunks, keyed by (steps, check on the first step)
     std::map<std::pair<int, bool>, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    // fused tile stages over every member's tiles (tile_cells: -1 automatic,
+    // 0 the four-kernel stages, n > 0 at most n cells per tile)
+    int tile_cells = -1;
+    bool tiles_ready = false, tiles_unusable = false;
+    std::string tiles_why;
+    TileView tv{};
+    void *tile_mem = nullptr;
+    std::vector<double *> epk, Us;  // per member: packed edges; three AoS states (3 per member)
+    std::vector<int> mtiles;        // tiles of each member
+    std::vector<long> mversion;     // each member's stencil_version the tiles were built on
+    int tile_target = 0, tile_bpc = 0, tile_vgprs = 0;
 
     int n() const { return static_cast<int>(ctx.size()); }
     int fail(int code, const std::string &m) {
@@ -82,6 +97,155 @@ struct o2f_batch {
     } while (0)
 
 namespace {
+
+void release_tiles(o2f_batch *b) {
+    if (b->tile_mem) (void)hipFree(b->tile_mem);
+    b->tile_mem = nullptr;
+    for (double *p : b->epk) (void)hipFree(p);
+    for (double *p : b->Us) (void)hipFree(p);
+    b->epk.clear();
+    b->Us.clear();
+    b->mtiles.clear();
+    b->mversion.clear();
+    b->tv = TileView{};
+    b->tiles_ready = false;
+}
+
+// Whether the steps run on the fused tile stages (built on first use).  One
+// launch covers the tiles of every member, in many rounds of workgroups, so
+// the automatic size is the largest whose LDS lets a CU hold as many tiles
+// as the kernel's registers allow (about 200 cells); members' tables are
+// concatenated.  Any member that cannot be tiled keeps the whole batch on
+// the four-kernel stages (tiles_why says why).
+bool tiles_on(o2f_batch *b) {
+    if (b->tile_cells == 0 || b->tiles_unusable) return false;
+    if (b->tiles_ready) {
+        bool current = true;
+        for (int m = 0; m < b->n(); ++m) current = current && b->mversion[m] == b->ctx[m]->stencil_version;
+        if (current) return true;
+    }
+    b->drop_graph();
+    release_tiles(b);
+    auto unusable = [&](const std::string &why) {
+        release_tiles(b);
+        b->tiles_unusable = true;
+        b->tiles_why = why;
+        return false;
+    };
+    int nk_max = 0;
+    for (o2f_ctx *c : b->ctx) nk_max = std::max(nk_max, *std::max_element(c->hm.cnv.begin(), c->hm.cnv.end()));
+    std::vector<TileSet> sets(b->n());
+    TileSet all;
+    std::string err;
+    int bpc = 0, vgprs = 0, most = 0;
+    // tiles one CU can hold at the kernel's registers (a tiny LDS)
+    if (tile_stage_attributes(1024, nk_max, &most, nullptr) != hipSuccess || most < 1)
+        return unusable("tile kernel attributes");
+    // the members' tables are independent: built on up to 16 host threads
+    const int nthreads = std::max(1, std::min({16, b->n(), (int)std::thread::hardware_concurrency()}));
+    auto build = [&](int t) -> bool {
+        std::vector<std::string> errs(b->n());
+        std::vector<char> ok(b->n(), 0);
+        std::atomic<int> next{0};
+        auto work = [&] {
+            for (int m; (m = next++) < b->n();) ok[m] = build_tiles(b->ctx[m]->hm, t, sets[m], errs[m]);
+        };
+        std::vector<std::thread> pool;
+        for (int i = 1; i < nthreads; ++i) pool.emplace_back(work);
+        work();
+        for (std::thread &th : pool) th.join();
+        for (int m = 0; m < b->n(); ++m)
+            if (!ok[m]) {
+                err = errs[m];
+                return false;
+            }
+        concat_tiles(sets, all);
+        if (!check_concat(sets, all, err)) return false;
+        const hipError_t e = tile_stage_attributes(all.lds_bytes(), nk_max, &bpc, &vgprs);
+        if (e != hipSuccess) err = std::string("tile kernel attributes: ") + hipGetErrorString(e);
+        return e == hipSuccess && bpc > 0;
+    };
+    int target = b->tile_cells;
+    if (target > 0) {
+        if (!build(target)) return unusable(err.empty() ? "tile LDS too large" : err);
+    } else {
+        constexpr int kFirst = 208, kMin = 64;
+        bool ok = false;
+        for (target = kFirst; target >= kMin; target = target * 24 / 25) {
+            ok = build(target);
+            if (!ok && !err.empty()) return unusable(err);
+            if (ok && bpc >= most) break;
+        }
+        if (!ok) return unusable("no tile size fits");
+    }
+    // one allocation for the tables
+    size_t off = 0;
+    auto slot = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_head = slot(all.head.size() * sizeof(TileHead)), o_cell = slot(all.cell.size() * sizeof(TileCell)),
+                 o_own = slot(all.own.size() * sizeof(TileOwn)), o_vg = slot(all.vg.size() * sizeof(TileVertex)),
+                 o_edge = slot(all.edge.size() * sizeof(int)), o_vo = slot(all.vo.size() * sizeof(TileVertex)),
+                 o_ls = slot(all.lsten.size() * sizeof(uint16_t) + 16);
+    char *mem = nullptr;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&mem), off);
+    if (e == hipSuccess) b->tile_mem = mem;
+    auto put = [&](size_t o, const void *src, size_t bytes) {
+        if (e == hipSuccess && bytes) e = hipMemcpy(mem + o, src, bytes, hipMemcpyHostToDevice);
+    };
+    put(o_head, all.head.data(), all.head.size() * sizeof(TileHead));
+    put(o_cell, all.cell.data(), all.cell.size() * sizeof(TileCell));
+    put(o_own, all.own.data(), all.own.size() * sizeof(TileOwn));
+    put(o_vg, all.vg.data(), all.vg.size() * sizeof(TileVertex));
+    put(o_edge, all.edge.data(), all.edge.size() * sizeof(int));
+    put(o_vo, all.vo.data(), all.vo.size() * sizeof(TileVertex));
+    put(o_ls, all.lsten.data(), all.lsten.size() * sizeof(uint16_t));
+    for (int m = 0; m < b->n() && e == hipSuccess; ++m) {
+        o2f_ctx *c = b->ctx[m];
+        const int nc = c->hm.nc;
+        b->mtiles.push_back((int)sets[m].head.size());
+        b->mversion.push_back(c->stencil_version);
+        double *p = nullptr;
+        e = dalloc(p, 8 * (size_t)c->hm.ne);
+        b->epk.push_back(p);
+        if (e == hipSuccess) e = launch_pack_edges(c->geo[0].view(), c->ebc, c->hm.ne, p, b->stream);
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) {
+            p = nullptr;
+            e = dalloc(p, 4 * (size_t)nc);
+            b->Us.push_back(p);
+        }
+        // one speed partial per tile of the member
+        if (e == hipSuccess && b->mtiles[m] > blocks_of(kKFlux, c->hm, kMinBlock)) {
+            (void)hipFree(b->speed[m]);
+            b->speed[m] = nullptr;
+            e = dalloc(b->speed[m], b->mtiles[m]);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return unusable(std::string("tile tables: ") + hipGetErrorString(e));
+    TileView &v = b->tv;
+    v.head = reinterpret_cast<const TileHead *>(mem + o_head);
+    v.cell = reinterpret_cast<const TileCell *>(mem + o_cell);
+    v.own = reinterpret_cast<const TileOwn *>(mem + o_own);
+    v.vg = reinterpret_cast<const TileVertex *>(mem + o_vg);
+    v.edge = reinterpret_cast<const int *>(mem + o_edge);
+    v.vo = reinterpret_cast<const TileVertex *>(mem + o_vo);
+    v.lsten = reinterpret_cast<const uint16_t *>(mem + o_ls);
+    v.epk = nullptr;  // per member (Member::epk)
+    v.ntiles = (int)all.head.size();
+    v.nk_max = nk_max;
+    v.max_vo = all.max_vo;
+    v.max_gd = all.max_gd;
+    v.lds_bytes = all.lds_layout(&v.lds_tq, &v.lds_fs);
+    b->tile_target = target;
+    b->tile_bpc = bpc;
+    b->tile_vgprs = vgprs;
+    b->tiles_ready = true;
+    b->drop_graph();
+    return true;
+}
 
 int build_tiles(o2f_batch *b, int kind) {
     std::vector<int2> t;
@@ -128,6 +292,13 @@ int upload_members(o2f_batch *b, double tol, double final_t = HUGE_VAL) {
         A.res_tol = tol > 0.0 ? tol : -1.0;
         A.nparts = static_cast<int>(blocks_of(kKFlux, c->hm, b->shape[kKFlux].block));
         A.nrparts = static_cast<int>(ceil_div(c->hm.nc, kResChunk));
+        A.epk = nullptr;
+        A.Us[0] = A.Us[1] = A.Us[2] = nullptr;
+        if (b->tiles_ready) {
+            A.nparts = b->mtiles[m];
+            A.epk = b->epk[m];
+            for (int k = 0; k < 3; ++k) A.Us[k] = b->Us[3 * m + k];
+        }
     }
     BCHK(b, hipMemcpyAsync(b->dM, b->hM.data(), b->hM.size() * sizeof(Member), hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));
@@ -174,13 +345,18 @@ int enqueue_stage(o2f_batch *b, int flags, double c0, double c1, hipEvent_t *ev 
     return O2F_OK;
 }
 
+// One step: every member's dt, then the three stages (on the tiles, the
+// state cycles Us[0] -> Us[1] -> Us[2] -> Us[0]).
 int enqueue_step(o2f_batch *b, bool check) {
     int st;
     BCHK(b, launch_b_step_begin(b->dM, b->n(), b->stream));
     for (int stg = 0; stg < 3; ++stg) {
         const int flags = kFlagRK | (stg == 0 ? kFlagSaveOld : 0) | (stg == 0 && check ? kFlagStoreR : 0) |
                           (stg == 2 ? kFlagSpeed : 0);
-        if ((st = enqueue_stage(b, flags, kC0[stg], kC1[stg]))) return st;
+        if (b->tiles_ready)
+            BCHK(b, launch_b_tile_stage(b->dM, b->tv, stg, kC0[stg], kC1[stg], flags, b->stream));
+        else if ((st = enqueue_stage(b, flags, kC0[stg], kC1[stg])))
+            return st;
         if (stg == 0 && check) BCHK(b, launch_b_residual(b->dM, b->rtiles, b->nrtiles, b->n(), b->stream));
     }
     return O2F_OK;
@@ -281,6 +457,7 @@ int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph,
     if (max_steps < 0 || check_every < 1) return b->fail(O2F_ERR_ARG, "batch run: max_steps >= 0, check_every >= 1");
     int st;
     if ((st = sync_members(b))) return st;
+    const bool tiled = tiles_on(b);
     if ((st = upload_members(b, tol, final_t))) return st;
     if ((st = reset_scalars(b))) return st;
     const bool check = tol > 0.0;
@@ -294,13 +471,19 @@ int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph,
             for (const auto &c : *p)
                 if ((st = get_graph(b, c.first, c.second, &x))) return st;
     }
+    if (tiled)  // the tile stages' AoS state
+        for (int m = 0; m < b->n(); ++m)
+            BCHK(b, launch_state_to_aos(b->ctx[m]->U, b->ctx[m]->hm.nc, b->Us[3 * m], b->stream));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (elapsed_ms) {
         BCHK(b, hipEventCreate(&e0));
         BCHK(b, hipEventCreate(&e1));
         BCHK(b, hipEventRecord(e0, b->stream));
     }
-    BCHK(b, launch_b_speed_max(b->dM, b->tiles[kKFlux], b->ntiles[kKFlux], b->shape[kKFlux], b->stream));
+    if (tiled)
+        BCHK(b, launch_b_tile_speed(b->dM, b->tv, b->stream));
+    else
+        BCHK(b, launch_b_speed_max(b->dM, b->tiles[kKFlux], b->ntiles[kKFlux], b->shape[kKFlux], b->stream));
     // one host check per ~64 steps: finished members cost only early-exit launches meanwhile
     const int per_sync = std::max(1, 64 / k);
     bool live = true;
@@ -321,6 +504,14 @@ int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph,
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
+    // the state back to the contexts; the last stage's input (Us[2]) as SoA
+    // in Us[1] for its intermediates below
+    if (tiled)
+        for (int m = 0; m < b->n(); ++m) {
+            const int nc = b->ctx[m]->hm.nc;
+            BCHK(b, launch_state_to_soa(b->Us[3 * m], nc, b->ctx[m]->U, b->stream));
+            BCHK(b, launch_state_to_soa(b->Us[3 * m + 2], nc, b->Us[3 * m + 1], b->stream));
+        }
     BCHK(b, hipStreamSynchronize(b->stream));
     for (int m = 0; m < b->n(); ++m) {
         o2f_ctx *c = b->ctx[m];
@@ -330,6 +521,13 @@ int run(o2f_batch *b, double tol, int max_steps, int check_every, int use_graph,
         if (res) res[m] = h[kScalRes];
         c->speed_valid = false;  // the batch kept its own speed partials
         ++c->faces_version;
+        // the tile stages store no intermediates: those of the member's last
+        // third stage, formed from its input as the four kernels form them
+        // (R keeps the last checked step's, as on the four-kernel stages)
+        if (tiled && h[kScalSteps] > 0.0) {
+            if ((st = o2f_ctx_form_outputs(c, b->Us[3 * m + 1]))) return b->fail(st, c->err);
+            BCHK(b, hipStreamSynchronize(c->stream));
+        }
     }
     return O2F_OK;
 }
@@ -376,6 +574,7 @@ void o2f_batch_destroy(o2f_batch *b) {
     if (!b) return;
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     b->drop_graph();
+    release_tiles(b);
     for (int k = 0; k < kKinds; ++k)
         if (b->tiles[k]) (void)hipFree(b->tiles[k]);
     for (double *p : b->speed) (void)hipFree(p);
@@ -406,6 +605,32 @@ int o2f_batch_set_shapes(o2f_batch *b, const int *shapes) {
             b->drop_graph();
         }
     }
+    return O2F_OK;
+}
+
+int o2f_batch_set_tile_cells(o2f_batch *b, int cells) {
+    if (!b || cells < -1) return O2F_ERR_ARG;
+    if (cells != b->tile_cells || b->tiles_unusable) {
+        if (b->stream) (void)hipStreamSynchronize(b->stream);
+        b->drop_graph();
+        release_tiles(b);
+        b->tiles_unusable = false;
+        b->tiles_why.clear();
+        b->tile_cells = cells;
+    }
+    return O2F_OK;
+}
+
+int o2f_batch_tile_info(o2f_batch *b, int *n_tiles, int *cells, int *lds_bytes, int *tiles_per_cu, int *vgprs) {
+    if (!b) return O2F_ERR_ARG;
+    b->tiles_why.clear();
+    const bool on = tiles_on(b);
+    if (!on && b->tile_cells != 0) b->err = "four-kernel stages: " + b->tiles_why;
+    if (n_tiles) *n_tiles = on ? b->tv.ntiles : 0;
+    if (cells) *cells = on ? b->tile_target : 0;
+    if (lds_bytes) *lds_bytes = on ? (int)b->tv.lds_bytes : 0;
+    if (tiles_per_cu) *tiles_per_cu = on ? b->tile_bpc : 0;
+    if (vgprs) *vgprs = on ? b->tile_vgprs : 0;
     return O2F_OK;
 }
 
@@ -502,7 +727,8 @@ int run_to_residual_batch(o2f_ctx *c, double tol, int max_steps, int check_every
     o2f_ctx *const one[1] = {c};
     int st = o2f_batch_create(one, 1, &b);
     if (st) return c->fail(st, "o2f_run_to_residual: batch set-up failed");
-    st = o2f_batch_run(b, tol, max_steps, check_every, use_graph, steps, res);
+    st = o2f_batch_set_tile_cells(b, 0);  // the context's own stages are the four kernels
+    if (st == O2F_OK) st = o2f_batch_run(b, tol, max_steps, check_every, use_graph, steps, res);
     if (st) c->fail(st, b->err);
     o2f_batch_destroy(b);
     return st;

@@ -246,10 +246,7 @@ bool tiles_on(o2f_ctx *c) {
     v.nk_max = nk_max;
     v.max_vo = ts.max_vo;
     v.max_gd = ts.max_gd;
-    const int vert = std::max(4 * ts.max_vg, 5 * ts.max_vo);
-    v.lds_tq = (vert + 3) & ~3;
-    v.lds_fs = v.lds_tq + ((5 * ts.max_gd + 3) & ~3);
-    v.lds_bytes = sizeof(double) * (size_t)(v.lds_fs + 8 * ts.max_e);
+    v.lds_bytes = ts.lds_layout(&v.lds_tq, &v.lds_fs);
     c->tile_target = ts.target;
     c->tile_bpc = bpc;
     c->tile_vgprs = vgprs;
@@ -344,6 +341,9 @@ std::string fmt_double(double v) {
 }  // namespace
 
 int o2f_ctx_check_bc(o2f_ctx *c) { return check_bc(c); }
+int o2f_ctx_form_outputs(o2f_ctx *c, double *Uin) {
+    return run_stage(c, 0, 0, nullptr, 0.0, 0.0, false, false, nullptr, Uin);
+}
 
 extern "C" {
 
@@ -581,6 +581,7 @@ int o2f_set_params(o2f_ctx *c, const o2f_params *p) {
                                  hipMemcpyHostToDevice, c->stream));
         HIPCHK(c, c->geo[0].fill(c->hm, c->stream));
         release_tiles(c);  // the stress vertices' stencils changed
+        ++c->stencil_version;
     }
     return O2F_OK;
 }

@@ -109,6 +109,7 @@ struct o2f_ctx {
     double *epk = nullptr;  // packed edge geometry of geo[0] for the tile stages
     double *rparts = nullptr;  // density-residual partials of a tiled o2f_run_to_residual
     int tile_target = 0, tile_bpc = 0, tile_vgprs = 0;
+    long stencil_version = 0;  // counts changes of the interpolation stencils (tiles built on them are stale)
     bool have_grad = false;
     // adjoint
     double *blk = nullptr;
@@ -155,6 +156,10 @@ struct o2f_ctx {
 
 // apply_boundary_conditions' check (src/boundary.cpp:84-92) for context c.
 int o2f_ctx_check_bc(o2f_ctx *c);
+// The four kernels of a residual evaluation of the SoA state Uin on the
+// context's stream, leaving the intermediates a stage of the four-kernel
+// path leaves (face states, stresses, vertex values; R and U untouched).
+int o2f_ctx_form_outputs(o2f_ctx *c, double *Uin);
 // o2f_run_to_residual on the batched four-kernel stages (o2f_batch.cpp).
 int run_to_residual_batch(o2f_ctx *c, double tol, int max_steps, int check_every, int use_graph, int *steps,
                           double *res);

@@ -115,8 +115,12 @@ struct Member {
     double min_h;
     double res_scale;  // rho_inf U_inf / chord
     double res_tol;    // stop once r <= res_tol (r from the last checked step)
-    int nparts;        // speed partials in use (flux-update grid of the member)
+    int nparts;        // speed partials in use (flux-update grid of the member, or its tiles)
     int nrparts;       // residual partials
+    // fused tile stages: packed edge geometry and the three AoS state
+    // buffers a step cycles through (stage s reads Us[s], writes Us[s+1 mod 3])
+    const double *epk;
+    double *Us[3];
 };
 
 constexpr int kResChunk = 2048;  // cells per residual partial (fixed: results do not depend on launch shapes)

@@ -342,27 +342,35 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
     return (flags & kFlagSpeed) ? wave_speed(unew[0], unew[1], unew[2], unew[3], P.gamma) : 0.0;
 }
 
-// At most 128 VGPRs (four waves per SIMD): the kernel keeps every tile of
-// the mesh resident in one round, so occupancy is what hides its loads.
+// The largest of every lane's m (all >= 1e-8 and NaN-free, so the order
+// of this tree does not matter) into *dst, by thread 0.
+__device__ __forceinline__ void block_max_to(double m, double *dst) {
+    __shared__ double wmax[kTileThreads / 64];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = smax(b, wmax[w]);
+        *dst = b;
+    }
+}
+
+// One stage over the tile with head h, on the mesh (T, G, EP) with the
+// physics P: reads U, writes Uout (and Uold / R as the flags say) and, with
+// kFlagSpeed, the tile's max(|u| + c) into *speed_slot.
 template <bool OUT_AOS, int NK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_stage(Topo T, Geom G, Phys P, TileView A,
-                                                    const double *__restrict__ U, double *__restrict__ Uout,
-                                                    double *__restrict__ Uold, double *__restrict__ R,
-                                                    const double *__restrict__ dtp, double c0, double c1, int flags,
-                                                    double *__restrict__ speed) {
+__device__ __forceinline__ void stage_tile(const Topo &T, const Geom &G, const Phys &P, const TileView &A,
+                                           const TileHead &h, const double2 *__restrict__ EP,
+                                           const double *__restrict__ U, double *__restrict__ Uout,
+                                           double *__restrict__ Uold, double *__restrict__ R,
+                                           const double *__restrict__ dtp, double c0, double c1, int flags,
+                                           double *__restrict__ speed_slot) {
     extern __shared__ double4 lds_raw[];
-    if (P.gate && *P.gate == 0.0) return;  // uniform
-    // Workgroups go to the eight XCDs in turn; workgroup b runs tile
-    // x q + min(x, r) + b / 8 (x = b mod 8, ntiles = 8 q + r), so each XCD
-    // takes one contiguous run of the tile order, whose neighbouring tiles
-    // share rim cells, vertices and edges in that XCD's L2.
-    const int nt = A.ntiles, xcd = blockIdx.x & 7, q8 = nt >> 3, r8 = nt & 7;
-    const int tile = xcd * q8 + min(xcd, r8) + (int)(blockIdx.x >> 3);
-    const TileHead h = A.head[tile];
     double *lds = reinterpret_cast<double *>(lds_raw);
     const Lds L{lds, lds + A.lds_tq, lds + A.lds_fs, A.max_vo, A.max_gd};
     const int tid = threadIdx.x, nth = blockDim.x;
-    const double2 *__restrict__ EP = reinterpret_cast<const double2 *>(A.epk);
 
     // 1. states of the tile's state vertices (the four-kernel stage's k_vertex_gather)
     for (int i = tid; i < h.nVg; i += nth) {
@@ -404,20 +412,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             tile_update<OUT_AOS, NK>(r, A.own[h.cell0 + i], T, G, EP, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
         if (flags & kFlagSpeed) m = smax(m, s);
     }
-    if (flags & kFlagSpeed) {
-        // the tile's max of |u| + c (every value is >= 1e-8 and NaN-free,
-        // so the order of this tree does not matter)
-        __shared__ double wmax[4];
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off));
-        if ((tid & 63) == 0) wmax[tid >> 6] = m;
-        __syncthreads();
-        if (tid == 0) {
-            double b = wmax[0];
-            for (int w = 1; w < (nth >> 6); ++w) b = smax(b, wmax[w]);
-            speed[tile] = b;
-        }
+    if (flags & kFlagSpeed) block_max_to(m, speed_slot);
+}
+
+// Workgroups go to the eight XCDs in turn; workgroup b runs tile
+// x q + min(x, r) + b / 8 (x = b mod 8, ntiles = 8 q + r), so each XCD
+// takes one contiguous run of the tile order, whose neighbouring tiles
+// share rim cells, vertices and edges in that XCD's L2.
+__device__ __forceinline__ int xcd_tile(int nt) {
+    const int xcd = blockIdx.x & 7, q8 = nt >> 3, r8 = nt & 7;
+    return xcd * q8 + min(xcd, r8) + (int)(blockIdx.x >> 3);
+}
+
+// At most 128 VGPRs (four waves per SIMD): the kernel keeps every tile of
+// the mesh resident in one round, so occupancy is what hides its loads.
+template <bool OUT_AOS, int NK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_tile_stage(Topo T, Geom G, Phys P, TileView A,
+                                                    const double *__restrict__ U, double *__restrict__ Uout,
+                                                    double *__restrict__ Uold, double *__restrict__ R,
+                                                    const double *__restrict__ dtp, double c0, double c1, int flags,
+                                                    double *__restrict__ speed) {
+    if (P.gate && *P.gate == 0.0) return;  // uniform
+    const int tile = xcd_tile(A.ntiles);
+    const TileHead h = A.head[tile];
+    stage_tile<OUT_AOS, NK>(T, G, P, A, h, reinterpret_cast<const double2 *>(A.epk), U, Uout, Uold, R, dtp, c0, c1,
+                            flags, speed + tile);
+}
+
+// The same stage over the tiles of every member of a batch (tables
+// concatenated, each head naming its member): member B's tiles read its
+// state buffer B.Us[stage] and write B.Us[stage + 1 mod 3]; a member whose
+// gate is closed skips its tiles.
+template <int NK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void kb_tile_stage(
+    const Member *__restrict__ M, TileView A, int stage, double c0, double c1, int flags) {
+    const TileHead h = A.head[xcd_tile(A.ntiles)];
+    const Member &B = M[h.member];
+    if (*B.P.gate == 0.0) return;  // uniform
+    const Topo T = B.T;
+    const Geom G = B.G;
+    const Phys P = B.P;
+    stage_tile<true, NK>(T, G, P, A, h, reinterpret_cast<const double2 *>(B.epk), B.Us[stage],
+                         B.Us[stage == 2 ? 0 : stage + 1], B.Uold, B.R, B.scal + kScalDt, c0, c1, flags,
+                         B.speed + h.slot);
+}
+
+// Speed partials of the state B.U (SoA) over the owned cells of every tile,
+// in the slots the last stage's kFlagSpeed fills: the first dt of a batched
+// tiled run.
+__global__ __launch_bounds__(256) void kb_tile_speed(const Member *__restrict__ M, TileView A) {
+    const TileHead h = A.head[blockIdx.x];
+    const Member &B = M[h.member];
+    const int nc = B.T.nc;
+    const double *__restrict__ U = B.U;
+    double m = 1e-8;
+    for (int i = threadIdx.x; i < h.nO; i += blockDim.x) {
+        const int c = A.cell[h.cell0 + i].c;
+        m = smax(m, wave_speed(U[c], U[nc + c], U[2 * nc + c], U[3 * nc + c], B.P.gamma));
     }
+    block_max_to(m, B.speed + h.slot);
 }
 
 // Packed edge geometry: (n_x, n_y), (L, tag), (mid_x, mid_y), pad -- the
@@ -452,9 +505,17 @@ __global__ __launch_bounds__(256) void k_state_to_soa(const double *__restrict__
 
 }  // namespace
 
-template <bool OUT_AOS, int NK>
-const void *tile_kernel() {
-    return reinterpret_cast<const void *>(&k_tile_stage<OUT_AOS, NK>);
+// The stage kernels of cells of at most nk_max edges.
+void tile_kernels(int nk_max, const void *(&f)[3]) {
+    if (nk_max <= 3) {
+        f[0] = reinterpret_cast<const void *>(&k_tile_stage<true, 3>);
+        f[1] = reinterpret_cast<const void *>(&k_tile_stage<false, 3>);
+        f[2] = reinterpret_cast<const void *>(&kb_tile_stage<3>);
+    } else {
+        f[0] = reinterpret_cast<const void *>(&k_tile_stage<true, 4>);
+        f[1] = reinterpret_cast<const void *>(&k_tile_stage<false, 4>);
+        f[2] = reinterpret_cast<const void *>(&kb_tile_stage<4>);
+    }
 }
 
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
@@ -479,6 +540,21 @@ hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const 
     return hipGetLastError();
 }
 
+hipError_t launch_b_tile_stage(const Member *M, const TileView &A, int stage, double c0, double c1, int flags,
+                               hipStream_t s) {
+    const dim3 grid(A.ntiles), block(kTileThreads);
+    if (A.nk_max <= 3)
+        hipLaunchKernelGGL((kb_tile_stage<3>), grid, block, A.lds_bytes, s, M, A, stage, c0, c1, flags);
+    else
+        hipLaunchKernelGGL((kb_tile_stage<4>), grid, block, A.lds_bytes, s, M, A, stage, c0, c1, flags);
+    return hipGetLastError();
+}
+
+hipError_t launch_b_tile_speed(const Member *M, const TileView &A, hipStream_t s) {
+    hipLaunchKernelGGL(kb_tile_speed, dim3(A.ntiles), dim3(kTileThreads), 0, s, M, A);
+    return hipGetLastError();
+}
+
 hipError_t launch_pack_edges(const Geom &G, const char *ebc, int ne, double *epk, hipStream_t s) {
     hipLaunchKernelGGL(k_pack_edges, dim3((ne + 255) / 256), dim3(256), 0, s, G, ebc, ne, epk);
     return hipGetLastError();
@@ -497,8 +573,8 @@ hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s
 hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_cu, int *vgprs) {
     // every context may launch up to the device's LDS per workgroup, less
     // the kernel's static LDS; a failed query leaves no error behind
-    const void *f[2] = {nk_max <= 3 ? tile_kernel<true, 3>() : tile_kernel<true, 4>(),
-                        nk_max <= 3 ? tile_kernel<false, 3>() : tile_kernel<false, 4>()};
+    const void *f[3];
+    tile_kernels(nk_max, f);
     int dev = 0, lds_max = 0;
     *blocks_per_cu = 0;
     if (vgprs) *vgprs = 0;

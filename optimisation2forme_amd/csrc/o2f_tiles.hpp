@@ -37,6 +37,15 @@ struct TileView {
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
                              double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
                              double c1, int flags, double *speed, hipStream_t s);
+// One stage over the concatenated tiles of a batch's members (each head
+// names its member): member m's tiles read M[m].Us[stage] and write
+// M[m].Us[stage + 1 mod 3] (AoS), with M[m]'s Uold, R, dt and speed
+// partials (one per tile of the member, in its slot) as launch_tile_stage.
+hipError_t launch_b_tile_stage(const Member *M, const TileView &A, int stage, double c0, double c1, int flags,
+                               hipStream_t s);
+// The speed partials of every member's SoA state M[m].U, one per tile in
+// the slots the stages' kFlagSpeed fills.
+hipError_t launch_b_tile_speed(const Member *M, const TileView &A, hipStream_t s);
 // Packed edge geometry of the tile stage: per edge (n_x, n_y, L, tag bits,
 // mid_x, mid_y, 0, 0), 8 doubles, from the geometry set and the edge tags.
 hipError_t launch_pack_edges(const Geom &G, const char *ebc, int ne, double *epk, hipStream_t s);

@@ -53,11 +53,14 @@ def solvers(limiter=0):
     return out
 
 
-def single_states(nsteps, limiter=0):
+def single_states(nsteps, limiter=0, faces=False):
+    """Each candidate alone, on the four-kernel stages: its state after
+    nsteps steps (and its last stage's face states)."""
     out = []
     for s in solvers(limiter):
+        s.set_tile_cells(0)
         s.run_steps(nsteps)
-        out.append(s.state())
+        out.append((s.state(), s.faces()) if faces else s.state())
     return out
 
 
@@ -71,16 +74,66 @@ SHAPES = {
 @pytest.mark.parametrize("shape", sorted(SHAPES))
 @pytest.mark.parametrize("use_graph", [True, False])
 def test_batch_members_bit_identical_to_single_runs(shape, use_graph):
+    """The four-kernel batched stages at several launch shapes."""
     n = 23  # two graph replays of 10 steps and a tail of 3
     ref = single_states(n)
     ss = solvers()
     b = capi.Batch(ss)
+    b.set_tile_cells(0)
+    assert b.tile_info()["tiles"] == 0
     b.set_shapes(SHAPES[shape])
     assert b.shapes() == SHAPES[shape]
     steps, res = b.run(-1.0, n, check_every=10, use_graph=use_graph)
     assert list(steps) == [n] * len(ss)
     for k, s in enumerate(ss):
         assert np.array_equal(s.state(), ref[k]), (shape, use_graph, k)
+    b.close()
+
+
+@pytest.mark.parametrize("cells", [-1, 40, 300])
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_tiled_batch_bit_identical_to_single_runs(cells, use_graph):
+    """The fused tile stages over every member's tiles (the members' tables
+    concatenated, one launch per stage): every member's state and the face
+    states its last stage leaves equal its own four-kernel run, at the
+    automatic tile size and at small and large caps."""
+    n = 23
+    ref = single_states(n, faces=True)
+    ss = solvers()
+    b = capi.Batch(ss)
+    b.set_tile_cells(cells)
+    info = b.tile_info()
+    nc = [len(candidate(k)[0]["cell_nv"]) for k in range(len(CANDIDATES))]
+    assert info["tiles"] >= sum(-(-c // info["cells"]) for c in nc), info
+    assert info["tiles_per_cu"] >= 1 and 0 < info["lds_bytes"] <= 160 * 1024, info
+    if cells > 0:
+        assert info["cells"] == cells
+    steps, _ = b.run(-1.0, n, check_every=10, use_graph=use_graph)
+    assert list(steps) == [n] * len(ss)
+    for k, s in enumerate(ss):
+        U, (fl, fr) = ref[k]
+        assert np.array_equal(s.state(), U), (cells, use_graph, k)
+        got_l, got_r = s.faces()
+        assert np.array_equal(got_l, fl) and np.array_equal(got_r, fr), (cells, use_graph, k)
+    b.close()
+
+
+def test_tiled_batch_follows_a_change_of_stencils():
+    """A member's interpolation mode changed between two batch runs: the
+    batch rebuilds its tiles on the new stencils (bitwise the four-kernel
+    stages on the new mode)."""
+    ss = solvers()
+    b = capi.Batch(ss)
+    b.run(-1.0, 3, check_every=3, use_graph=False)
+    desc, flow = candidate(1)
+    flow = dict(flow, interp_mode=0)
+    ss[1].set_params(H.capi_params(flow))
+    ref = capi.Solver(desc, H.capi_params(flow))
+    ref.set_tile_cells(0)
+    ref.set_state(ss[1].state())
+    b.run(-1.0, 4, check_every=4, use_graph=False)
+    ref.run_steps(4)
+    assert np.array_equal(ss[1].state(), ref.state())
     b.close()
 
 

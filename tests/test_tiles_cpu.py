@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "optimisation2forme_amd", "csrc")
 
 PLANTED = ["vertex_slot", "edge_slot", "stencil_cell", "owned_twice", "face_cell_dropped", "vertex_stencil",
-           "stress_slot"]
+           "stress_slot", "tile_slot", "concat_member", "concat_cell_offset", "concat_stencil_offset"]
 
 
 @pytest.fixture(scope="module")
@@ -54,6 +54,8 @@ def run(exe, desc, d, sizes):
             planted[w[1]] = w[2]
         elif w[0] == "untagged":
             untagged = w[1]
+        elif w[0] == "concat":
+            assert w[1] == "ok", line
     return tiles, planted, untagged
 
 

@@ -21,6 +21,12 @@
 //   untagged <refused|accepted>
 //
 // builds tiles for the same mesh with one boundary edge's tag removed.
+// The first two sizes also form a two-member batch (concat_tiles):
+//
+//   concat <ok|FAIL>
+//
+// followed by planted slips in the concatenation (a head naming the wrong
+// member, a record offset or a local stencil offset one off).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -94,6 +100,44 @@ void planted(const HostMesh &hm, const TileSet &good) {
         o.lo[0] = (uint16_t)((o.lo[0] + 1) % h.nVo);
         report("stress_slot", ts);
     }
+    {
+        TileSet ts = good;  // a tile's slot (its speed partial) that of the next tile
+        ts.head[t].slot += 1;
+        report("tile_slot", ts);
+    }
+}
+
+// Two tile sets of the mesh as the members of a batch: the concatenation
+// must check, and each planted slip in it must be caught.
+void concat(const TileSet &a, const TileSet &b) {
+    const std::vector<TileSet> sets = {a, b};
+    TileSet all;
+    concat_tiles(sets, all);
+    std::string err;
+    std::printf("concat %s\n", check_concat(sets, all, err) ? "ok" : "FAIL");
+    auto report = [&](const char *kind, const TileSet &bad) {
+        std::printf("planted %s %s\n", kind, check_concat(sets, bad, err) ? "missed" : "detected");
+    };
+    const int k = (int)a.head.size();  // the second member's first tile
+    {
+        TileSet bad = all;  // the second member's tile named as the first member's
+        bad.head[k + 1].member = 0;
+        report("concat_member", bad);
+    }
+    {
+        TileSet bad = all;  // a record offset one short
+        bad.head[k + 1].cell0 -= 1;
+        report("concat_cell_offset", bad);
+    }
+    {
+        TileSet bad = all;  // a local stencil offset off by one (a stencil of two or more cells)
+        for (size_t i = b.vo.size() / 2; i < b.vo.size(); ++i)
+            if ((b.vo[i].nkb & 0xffff) >= 2 && ((b.vo[i].nkb >> 16) & 0xff) == kStencilWsum) {
+                bad.vo[a.vo.size() + i].ls += 1;
+                break;
+            }
+        report("concat_stencil_offset", bad);
+    }
 }
 
 }  // namespace
@@ -120,6 +164,7 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "%s\n", err.c_str());
         return 1;
     }
+    std::vector<TileSet> built;
     for (int a = 2; a < argc; ++a) {
         const int target = std::atoi(argv[a]);
         TileSet ts;
@@ -132,7 +177,9 @@ int main(int argc, char **argv) {
         std::printf("tiles %d ok %d %ld %d %.4f\n", target, (int)ts.head.size(), owned, ts.max_o,
                     (double)ts.sum_gd / hm.nc);
         if (a == 2) planted(hm, ts);
+        if (built.size() < 2) built.push_back(ts);
     }
+    if (built.size() == 2) concat(built[0], built[1]);
     // one boundary edge without a tag
     for (int e = 0; e < hm.ne; ++e)
         if (hm.ecr[e] < 0) {
