@@ -308,6 +308,11 @@ def bench_batch(a):
     finite = all(bool(np.isfinite(s.state()).all()) for s in solvers)
     seq_ms = min(seq_direct, seq_graph)
     total_vs = float(nv.sum()) * K
+    # B_step of every member (state once per stage, geometry once per stage)
+    # over the event-timed step of the batch: three tile-stage launches
+    # (all but ~0.2 % of the step) and the members' dt block
+    b_step = float(sum(step_algorithmic_bytes(s.nv, s.ne, s.nc) for s in solvers))
+    achieved = b_step / (batch_ms / K * 1e-3) / 1e9
     out = {
         "metric": "vertex-steps/s (batched shape candidates)",
         "value": total_vs / (wall),
@@ -325,6 +330,10 @@ def bench_batch(a):
         "sequential": {"direct_ms_per_step": seq_direct / K, "graph_ms_per_step": seq_graph / K,
                        "value": total_vs / (seq_ms * 1e-3)},
         "speedup_vs_sequential": seq_ms / batch_ms,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "basis": "sum over members of B_step = 256 N_c + 3 G (%.1f MB) over the event-timed step"
+                              % (b_step / 1e6)},
         "tiling": tiling,
         "four_kernel_batch": {"ms_per_step": four_kernel_ms / K, "value": total_vs / (four_kernel_ms * 1e-3),
                               "autotuned_shapes": dict(zip(capi.Batch.KINDS, shapes))},
