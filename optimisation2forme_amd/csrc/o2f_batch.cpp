@@ -623,7 +623,6 @@ int o2f_batch_set_tile_cells(o2f_batch *b, int cells) {
 
 int o2f_batch_tile_info(o2f_batch *b, int *n_tiles, int *cells, int *lds_bytes, int *tiles_per_cu, int *vgprs) {
     if (!b) return O2F_ERR_ARG;
-    b->tiles_why.clear();
     const bool on = tiles_on(b);
     if (!on && b->tile_cells != 0) b->err = "four-kernel stages: " + b->tiles_why;
     if (n_tiles) *n_tiles = on ? b->tv.ntiles : 0;

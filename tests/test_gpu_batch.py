@@ -118,6 +118,53 @@ def test_tiled_batch_bit_identical_to_single_runs(cells, use_graph):
     b.close()
 
 
+def test_tiled_batch_with_quads_and_triangles():
+    """The reference's shipped quad mesh (its own flow and state) in one
+    batch with two triangle candidates: the four-slot batched tile stage
+    over all three, each member bitwise its own four-kernel run."""
+    n = 9
+    members = [(H.shipped_mesh(), H.SHIPPED, H.shipped_flow())]
+    for k in (0, 2):
+        desc, flow = candidate(k)
+        members.append((desc, flow, None))
+    ref, ss = [], []
+    for desc, flow, U in members:
+        for out in (ref, ss):
+            s = capi.Solver(desc, H.capi_params(flow))
+            if U is None:
+                s.init_freestream()
+            else:
+                s.set_state(U)
+            out.append(s)
+    for s in ref:
+        s.set_tile_cells(0)
+        s.run_steps(n)
+    b = capi.Batch(ss)
+    assert b.tile_info()["tiles"] > 0
+    b.run(-1.0, n, check_every=4, use_graph=True)
+    for k, (s, r) in enumerate(zip(ss, ref)):
+        assert np.array_equal(s.state(), r.state()), k
+        assert np.array_equal(s.faces()[0], r.faces()[0]) and np.array_equal(s.faces()[1], r.faces()[1]), k
+    b.close()
+
+
+def test_tiled_batch_falls_back_when_tiles_do_not_fit():
+    """A cap whose tiles need more LDS than a workgroup has: the batch runs
+    the four-kernel batched stages (tile_info reports none, with the reason)
+    and the members are unchanged."""
+    n = 5
+    ref = single_states(n)
+    ss = solvers()
+    b = capi.Batch(ss)
+    b.set_tile_cells(100000)
+    assert b.tile_info()["tiles"] == 0
+    assert "four-kernel" in capi.lib().o2f_batch_last_error(b.h).decode()
+    b.run(-1.0, n, check_every=5, use_graph=False)
+    for k, s in enumerate(ss):
+        assert np.array_equal(s.state(), ref[k]), k
+    b.close()
+
+
 def test_tiled_batch_follows_a_change_of_stencils():
     """A member's interpolation mode changed between two batch runs: the
     batch rebuilds its tiles on the new stencils (bitwise the four-kernel
