@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <exception>
 #include <map>
 #include <thread>
 #include <utility>
@@ -151,7 +152,10 @@ bool tiles_on(o2f_batch *b) {
             for (int m; (m = next++) < b->n();) ok[m] = build_tiles(b->ctx[m]->hm, t, sets[m], errs[m]);
         };
         std::vector<std::thread> pool;
-        for (int i = 1; i < nthreads; ++i) pool.emplace_back(work);
+        try {
+            for (int i = 1; i < nthreads; ++i) pool.emplace_back(work);
+        } catch (const std::exception &) {  // no more threads: the rest on this one
+        }
         work();
         for (std::thread &th : pool) th.join();
         for (int m = 0; m < b->n(); ++m)
