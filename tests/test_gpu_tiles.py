@@ -127,12 +127,13 @@ def test_tiled_run_until_bitwise(small_developed, max_steps):
     assert na == nb and la == lb
 
 
-def test_tiled_steps_bitwise_four_kernel_steps_200k_developed():
+@pytest.mark.parametrize("limiter", [0, 1])
+def test_tiled_steps_bitwise_four_kernel_steps_200k_developed(limiter):
     """The benchmark mesh (NACA 2412, 199,680 triangles, Mach 0.85) after
     300 steps from the cold start: 20 more steps on tiles and on the four
-    kernels, bit for bit; and the tiling is the automatic one the benchmark
-    uses."""
-    flow = dict(H.transonic(limiter=0), heat_flux=1)
+    kernels, bit for bit, with either limiter; and the tiling is the
+    automatic one the benchmark uses."""
+    flow = dict(H.transonic(limiter=limiter), heat_flux=1)
     s = capi.Solver(H.big_mesh(), H.capi_params(flow))
     s.set_tile_cells(0)
     s.init_freestream()
