@@ -42,7 +42,8 @@ AoA ~ U[0, 4] deg, Re 5000, Fourier heat flux), advanced together by one
 fused tile-stage launch per SSP-RK3 stage over every member's tiles (graph
 replay; --tile-cells as for one mesh), against the same 64 meshes solved
 one after another and against the four-kernel batched stages (autotuned
-launch shapes).
+launch shapes); then every member's adjoint solve (100 BiCGSTAB
+iterations each) one after another and together (o2f_batch_bicgstab).
 
     python bench.py --adjoint
 
@@ -306,6 +307,20 @@ def bench_batch(a):
     batch_ms = b.time_steps(K, use_graph=not a.no_graph)
     wall = time.perf_counter() - t1
     finite = all(bool(np.isfinite(s.state()).all()) for s in solvers)
+    # the members' adjoint solves at the end state, a fixed 100 BiCGSTAB
+    # iterations each (tolerance out of reach): one after another, then
+    # together (o2f_batch_bicgstab)
+    adj_it = 100
+    adj_kw = dict(tol=1e-300, chunk=adj_it, max_total=adj_it)
+    for s in solvers[:2]:
+        s.bicgstab(0.1, check=False, **adj_kw)   # work arrays and blocks allocated
+    t1 = time.perf_counter()
+    adj_seq = [s.bicgstab(0.1, check=False, **adj_kw) for s in solvers]
+    adj_seq_s = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    lams, adj_its, _, _ = b.bicgstab(0.1, **adj_kw)
+    adj_batch_s = time.perf_counter() - t1
+    adj_same = all(np.array_equal(a[0], l) and a[1] == i for a, l, i in zip(adj_seq, lams, adj_its))
     seq_ms = min(seq_direct, seq_graph)
     total_vs = float(nv.sum()) * K
     # B_step of every member (state once per stage, geometry once per stage)
@@ -335,6 +350,10 @@ def bench_batch(a):
                      "basis": "sum over members of B_step = 256 N_c + 3 G (%.1f MB) over the event-timed step"
                               % (b_step / 1e6)},
         "tiling": tiling,
+        "adjoint": {"bicgstab_iterations_each": adj_it, "operator": "reference form (FD blocks)",
+                    "sequential_s": adj_seq_s, "batch_s": adj_batch_s, "speedup": adj_seq_s / adj_batch_s,
+                    "member_iterations_per_s": len(solvers) * adj_it / adj_batch_s,
+                    "bitwise_same_as_sequential": bool(adj_same)},
         "four_kernel_batch": {"ms_per_step": four_kernel_ms / K, "value": total_vs / (four_kernel_ms * 1e-3),
                               "autotuned_shapes": dict(zip(capi.Batch.KINDS, shapes))},
         "setup_s": setup_s,

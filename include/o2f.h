@@ -412,6 +412,19 @@ int o2f_batch_time_steps(o2f_batch *batch, int nsteps, int use_graph, double *to
  * members hold the intermediates of their last stage as on the four-kernel
  * stages. */
 int o2f_batch_set_tile_cells(o2f_batch *batch, int cells);
+/* The adjoint solve of every member (no counterpart in the reference, which
+ * solves one shape per process, run_opti.sh:211-254): each member's
+ * o2f_bicgstab (reference-form operator, eps as there) or, with exact = 1,
+ * its o2f_bicgstab_exact (no options), the members' solves driven from
+ * several host threads at once, each on the member's own stream, so that
+ * the solves of meshes too small to fill the GPU overlap on it.  Each
+ * member's lambda, iterations and relative residual are bitwise those of its
+ * own call.  lambdas[i] (4 x nc of member i, host; the array or an entry
+ * may be NULL), iters, relres and status (per member: O2F_OK,
+ * O2F_ERR_NOCONV, O2F_ERR_DIVERGED; may be NULL).  Returns O2F_OK unless
+ * an argument or a device call failed. */
+int o2f_batch_bicgstab(o2f_batch *batch, double beta, double eps, double tol, int chunk, int max_total, int exact,
+                       double *const *lambdas, int *iters, double *relres, int *status);
 /* The tiling in use (built now if needed): *n_tiles 0 on the four-kernel
  * stages (o2f_batch_last_error says why), else the tiles over all members,
  * the cell cap, the LDS bytes of one tile, the tiles one CU holds at once

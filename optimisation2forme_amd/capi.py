@@ -44,7 +44,7 @@ EXPORTS = [
     "o2f_adjoint_block_diagonal", "o2f_bicgstab_options", "o2f_mlp_set_gemm_variant", "o2f_residual_jvp",
     "o2f_rk3_save_old", "o2f_rk3_substep", "o2f_bicgstab_set_restart_eps",
     "o2f_hllc_flux_batch_noalloc", "o2f_batch_run_until", "o2f_set_tile_cells", "o2f_tile_info",
-    "o2f_batch_set_tile_cells", "o2f_batch_tile_info",
+    "o2f_batch_set_tile_cells", "o2f_batch_tile_info", "o2f_batch_bicgstab",
 ]
 
 
@@ -165,6 +165,8 @@ def lib():
             "o2f_batch_time_steps": (C.c_int, [vp, C.c_int, C.c_int, D]),
             "o2f_batch_set_tile_cells": (C.c_int, [vp, C.c_int]),
             "o2f_batch_tile_info": (C.c_int, [vp, I, I, I, I, I]),
+            "o2f_batch_bicgstab": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                             C.POINTER(D), I, D, I]),
             "o2f_batch_run_until": (C.c_int, [vp, C.c_double, C.c_int, C.c_int, I, D]),
         }
         for name, (res, args) in sig.items():
@@ -644,6 +646,17 @@ class Batch:
         member's tiles, sized automatically (the default), 0 the four-kernel
         batched stages, n > 0 tiles of at most n cells."""
         self._check(lib().o2f_batch_set_tile_cells(self.h, int(cells)))
+
+    def bicgstab(self, beta, eps=1e-6, tol=1e-6, chunk=500, max_total=100000, exact=False):
+        """Every member's adjoint solve (Solver.bicgstab, or bicgstab_exact),
+        several at once: (lambdas, iters, relres, status) per member."""
+        lams = [np.zeros((4, s.nc)) for s in self.solvers]
+        ptrs = (D * len(lams))(*[_d(l) for l in lams])
+        n = len(self.solvers)
+        it, rr, code = np.zeros(n, np.int32), np.zeros(n), np.zeros(n, np.int32)
+        self._check(lib().o2f_batch_bicgstab(self.h, beta, eps, tol, chunk, max_total, int(bool(exact)), ptrs,
+                                             _i(it), _d(rr), _i(code)))
+        return lams, it, rr, code
 
     def tile_info(self):
         """The tiling in use: dict of tiles (over all members), cells (per-tile

@@ -30,6 +30,11 @@ using namespace o2f;
 namespace {
 
 enum { kKVertex = 0, kKRecon = 1, kKViscEdge = 2, kKFlux = 3, kKinds = 4 };
+// host threads of o2f_batch_bicgstab: each drives one member's solve at a
+// time (its loop waits on its stream once per group of iterations); eight
+// keep the GPU's queues busy (measured: 4 threads 2.56x, 8 threads 2.74x,
+// 16 threads 2.72x the sequential solves of the 64-candidate batch)
+constexpr int kAdjointThreads = 8;
 constexpr int kMinBlock = 64;
 
 long ceil_div(long a, long b) { return (a + b - 1) / b; }
@@ -634,6 +639,52 @@ int o2f_batch_tile_info(o2f_batch *b, int *n_tiles, int *cells, int *lds_bytes, 
     if (lds_bytes) *lds_bytes = on ? (int)b->tv.lds_bytes : 0;
     if (tiles_per_cu) *tiles_per_cu = on ? b->tile_bpc : 0;
     if (vgprs) *vgprs = on ? b->tile_vgprs : 0;
+    return O2F_OK;
+}
+
+int o2f_batch_bicgstab(o2f_batch *b, double beta, double eps, double tol, int chunk, int max_total, int exact,
+                       double *const *lambdas, int *iters, double *relres, int *status) {
+    if (!b || chunk <= 0 || max_total < 0 || (exact != 0 && exact != 1)) return O2F_ERR_ARG;
+    int st;
+    if ((st = sync_members(b))) return st;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    // allocations first, on this thread; then each member's own solve on
+    // its own stream, several members at once
+    for (o2f_ctx *c : b->ctx)
+        if ((st = o2f_ctx_adjoint_prepare(c, exact))) return b->fail(st, c->err);
+    std::vector<std::vector<double>> scratch(b->n());
+    std::vector<int> it(b->n(), 0), code(b->n(), O2F_OK);
+    std::vector<double> rr(b->n(), 0.0);
+    std::atomic<int> next{0};
+    auto work = [&] {
+        for (int m; (m = next++) < b->n();) {
+            o2f_ctx *c = b->ctx[m];
+            double *lam = lambdas ? lambdas[m] : nullptr;
+            if (!lam) {
+                scratch[m].assign(4 * (size_t)c->hm.nc, 0.0);
+                lam = scratch[m].data();
+            }
+            code[m] = exact ? o2f_bicgstab_exact(c, beta, tol, chunk, max_total, lam, &it[m], &rr[m], 0)
+                            : o2f_bicgstab(c, beta, eps, tol, chunk, max_total, lam, &it[m], &rr[m]);
+        }
+    };
+    const int nthreads = std::max(1, std::min({kAdjointThreads, b->n(), (int)std::thread::hardware_concurrency()}));
+    std::vector<std::thread> pool;
+    try {
+        for (int i = 1; i < nthreads; ++i) pool.emplace_back(work);
+    } catch (const std::exception &) {  // no more threads: the rest on this one
+    }
+    work();
+    for (std::thread &th : pool) th.join();
+    for (int m = 0; m < b->n(); ++m) {
+        if (iters) iters[m] = it[m];
+        if (relres) relres[m] = rr[m];
+        if (status) status[m] = code[m];
+    }
+    // a member whose solve failed on the device (not a convergence
+    // outcome) fails the call
+    for (int m = 0; m < b->n(); ++m)
+        if (code[m] == O2F_ERR_HIP || code[m] == O2F_ERR_ARG) return b->fail(code[m], b->ctx[m]->err);
     return O2F_OK;
 }
 

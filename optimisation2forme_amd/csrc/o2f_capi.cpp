@@ -1978,3 +1978,9 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
 }
 
 }  // extern "C"
+
+// The device memory a BiCGSTAB call allocates on first use (work vectors,
+// FD blocks; for the exact operator its tables and work arrays), allocated
+// now: o2f_batch_bicgstab prepares every member on one thread before the
+// members' solves run concurrently.
+int o2f_ctx_adjoint_prepare(o2f_ctx *c, int exact) { return exact ? ensure_adjoint(c) : ensure_work(c); }

@@ -160,6 +160,9 @@ int o2f_ctx_check_bc(o2f_ctx *c);
 // context's stream, leaving the intermediates a stage of the four-kernel
 // path leaves (face states, stresses, vertex values; R and U untouched).
 int o2f_ctx_form_outputs(o2f_ctx *c, double *Uin);
+// The device allocations of an adjoint solve (exact: the exact operator's
+// too), made now rather than inside the solve.
+int o2f_ctx_adjoint_prepare(o2f_ctx *c, int exact);
 // o2f_run_to_residual on the batched four-kernel stages (o2f_batch.cpp).
 int run_to_residual_batch(o2f_ctx *c, double tol, int max_steps, int check_every, int use_graph, int *steps,
                           double *res);

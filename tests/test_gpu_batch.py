@@ -184,6 +184,25 @@ def test_tiled_batch_follows_a_change_of_stencils():
     b.close()
 
 
+@pytest.mark.parametrize("exact", [False, True])
+def test_batch_adjoint_solves_bitwise_each_members_own(exact):
+    """o2f_batch_bicgstab: the members' adjoint solves driven together from
+    several host threads, each on its member's stream; every member's
+    lambda, iteration count, residual and status equal its own solve's."""
+    ss = solvers()
+    b = capi.Batch(ss)
+    b.run(-1.0, 30, check_every=10)
+    kw = dict(tol=1e-8, chunk=40, max_total=120)
+    lams, it, rr, code = b.bicgstab(0.1, exact=exact, **kw)
+    assert np.all(it > 0)
+    for k, s in enumerate(ss):
+        one = s.bicgstab_exact(0.1, check=False, **kw) if exact else s.bicgstab(0.1, check=False, **kw)
+        lam, i, r, st = one
+        assert np.all(np.isfinite(lam))
+        assert np.array_equal(lam, lams[k]) and i == it[k] and r == rr[k] and st == code[k], (exact, k)
+    b.close()
+
+
 def test_batch_barth_jespersen_and_consecutive_runs():
     ref = single_states(30, limiter=1)
     ss = solvers(limiter=1)
