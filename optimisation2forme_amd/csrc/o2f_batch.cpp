@@ -248,6 +248,7 @@ bool tiles_on(o2f_batch *b) {
     v.max_vo = all.max_vo;
     v.max_gd = all.max_gd;
     v.lds_bytes = all.lds_layout(&v.lds_tq, &v.lds_fs);
+    v.quad = 0;  // many rounds of tiles: the 256-thread kernel
     b->tile_target = target;
     b->tile_bpc = bpc;
     b->tile_vgprs = vgprs;

@@ -158,19 +158,20 @@ bool tiles_on(o2f_ctx *c) {
     const int nc = c->hm.nc;
     TileSet ts;
     std::string err;
-    int bpc = 0, vgprs = 0, target = c->tile_cells;
+    int bpc = 0, vgprs = 0, qpc = 0, target = c->tile_cells;
     const int nk_max = *std::max_element(c->hm.cnv.begin(), c->hm.cnv.end());
     auto build = [&](int t) -> bool {
         if (!build_tiles(c->hm, t, ts, err)) return false;
-        const hipError_t e = tile_stage_attributes(ts.lds_bytes(), nk_max, &bpc, &vgprs);
+        const hipError_t e = tile_stage_attributes(ts.lds_bytes(), nk_max, &bpc, &vgprs, &qpc);
         if (e != hipSuccess) err = std::string("tile kernel attributes: ") + hipGetErrorString(e);
         return e == hipSuccess && bpc > 0;
     };
-    // the smallest tiles (of at least kMinCells) of which every CU holds at
+    // the smallest tiles (of at least kMinCells; kMinQuadCells for one tile
+    // per CU, which runs the four-lane kernels) of which every CU holds at
     // most m at once
-    constexpr int kMinCells = 96, kSmallCells = 128;
+    constexpr int kMinCells = 96, kMinQuadCells = 32, kSmallCells = 128;
     auto fits = [&](int m) -> bool {
-        target = std::max(kMinCells, (int)std::ceil(nc / (0.985 * m * ncu)));
+        target = std::max(m == 1 ? kMinQuadCells : kMinCells, (int)std::ceil(nc / (0.985 * m * ncu)));
         bool ok = build(target);
         for (int it = 0; ok && it < 12 && (int)ts.head.size() > m * ncu; ++it)
             ok = build(target = (int)std::ceil(target * 1.02));
@@ -184,8 +185,9 @@ bool tiles_on(o2f_ctx *c) {
         // A small mesh: the fewest tiles per CU whose tiles stay within
         // kSmallCells (a stage then takes about one tile's time, and larger
         // tiles recompute less rim; measured on 10k triangles and the
-        // reference's 25k quads).  Otherwise: as many resident tiles per
-        // CU as the kernel allows, which hides the phases' load latency.
+        // reference's 25k quads); with one per CU, every CU gets one (the
+        // four-lane kernels).  Otherwise: as many resident tiles per CU as
+        // the kernel allows, which hides the phases' load latency.
         bool done = false;
         for (int m = 1; m <= resident && !done; ++m)
             if (nc / (0.985 * m * ncu) <= kSmallCells) done = fits(m) && target <= kSmallCells;
@@ -247,6 +249,9 @@ bool tiles_on(o2f_ctx *c) {
     v.max_vo = ts.max_vo;
     v.max_gd = ts.max_gd;
     v.lds_bytes = ts.lds_layout(&v.lds_tq, &v.lds_fs);
+    // tiles that run one per CU (a small mesh): the four-lane kernels, whose
+    // sixteen waves share one tile's cells
+    v.quad = (qpc >= 1 && v.ntiles <= ncu) ? 1 : 0;
     c->tile_target = ts.target;
     c->tile_bpc = bpc;
     c->tile_vgprs = vgprs;

@@ -342,10 +342,236 @@ __device__ __forceinline__ double tile_update(const Rec &r, const TileOwn &o, co
     return (flags & kFlagSpeed) ? wave_speed(unew[0], unew[1], unew[2], unew[3], P.gamma) : 0.0;
 }
 
+// The same two cell phases with a cell's work spread over a group of four
+// lanes (k = 0..3), for tiles that run one per CU: a lane then carries a
+// quarter of a cell's dependent arithmetic.  The lanes first own edge slot
+// k (its loads, edge value and Green-Gauss numerators), then conservative
+// variable k; the arithmetic is that of the four-kernel stage's
+// reconstruct_body and flux_update_body, with the tile's local vertex,
+// edge and cell indices, so every value is the one-lane form's bit for bit.
+// All four lanes of a group run together (the cross-lane reads need them).
+__device__ __forceinline__ int sel4(const int (&a)[4], int k) {
+    return (k == 0) ? a[0] : (k == 1) ? a[1] : (k == 2) ? a[2] : a[3];
+}
+__device__ __forceinline__ double sel4(const double (&a)[4], int k) {
+    return (k == 0) ? a[0] : (k == 1) ? a[1] : (k == 2) ? a[2] : a[3];
+}
+
+template <int NK>
+__device__ __forceinline__ void tile_cell_quad(const Rec &r, bool face, const Geom &G,
+                                               const double2 *__restrict__ EP, const Phys &P,
+                                               const double *__restrict__ U, const Lds &L, int i, int k) {
+    const int c = r.c;
+    const unsigned info = (unsigned)r.info;
+    const int nk = info >> 4;
+    const bool has_edge = k < nk;
+    const bool is_left = (info >> k) & 1u;
+    double u[4];
+    ld4(U, c, u);
+    const double vol = G.cvol[c], cx = G.ccx[c], cy = G.ccy[c];
+    // this lane's edge slot
+    double el = 0.0, dx = 0.0, dy = 0.0, tx[4] = {0.0, 0.0, 0.0, 0.0}, ty[4] = {0.0, 0.0, 0.0, 0.0};
+    int ebc = 0;
+    if (has_edge) {
+        const int e = sel4(r.edge, k);
+        const double2 *pe = EP + 4 * (long)e;
+        const double2 n2 = pe[0], lb = pe[1], m2 = pe[2];
+        el = lb.x;
+        ebc = (char)__double2loint(lb.y);
+        dx = m2.x - cx;
+        dy = m2.y - cy;
+        const int la = (k == 0) ? r.lv[0] : (k == 1) ? r.lv[2] : (k == 2) ? r.lv[4] : r.lv[6];
+        const int lb2 = (k == 0) ? r.lv[1] : (k == 1) ? r.lv[3] : (k == 2) ? r.lv[5] : r.lv[7];
+        double xa[4], xb[4], val[4];
+        lds_ld4(L.vert + 4 * la, xa);
+        lds_ld4(L.vert + 4 * lb2, xb);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) val[q] = 0.5 * (xa[q] + xb[q]);
+        if (tagged((char)ebc)) bc_state(P, (char)ebc, val[0], val[1], val[2], val[3]);
+        double nx = n2.x, ny = n2.y;
+        if (is_left) {
+            nx = -nx;
+            ny = -ny;
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            tx[q] = val[q] * el * nx;
+            ty[q] = val[q] * el * ny;
+        }
+    }
+    // lane k: variable k's terms of the edges in slot order, divided by the
+    // volume and summed from +0.0
+    quad_transpose(tx, k);
+    quad_transpose(ty, k);
+    double gxk = 0.0, gyk = 0.0;
+#pragma unroll
+    for (int j = 0; j < NK; ++j) {
+        if (j >= nk) break;
+        gxk += tx[j] / vol;
+        gyk += ty[j] / vol;
+    }
+    if (face) {
+        const double uk = sel4(u, k);
+        // neighbour min / max of variable k in neighbour-list order
+        double mn = uk, mx = uk;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            const int nb = r.nbr[j];
+            const double v = (nb >= 0) ? U[4 * (long)nb + k] : uk;
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+        const double dxs[4] = {from_lane<0>(dx), from_lane<1>(dx), from_lane<2>(dx), from_lane<3>(dx)};
+        const double dys[4] = {from_lane<0>(dy), from_lane<1>(dy), from_lane<2>(dy), from_lane<3>(dy)};
+        const int bcs[4] = {from_lane_i<0>(ebc), from_lane_i<1>(ebc), from_lane_i<2>(ebc), from_lane_i<3>(ebc)};
+        const double veps = (P.limiter == 1) ? 0.0 : venkat_eps(vol, P.venkat_k);
+        double al = 1.0;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            if (j >= nk) break;
+            const double pred = uk + gxk * dxs[j] + gyk * dys[j];
+            const double a = (P.limiter == 1) ? alpha_bj(uk, pred, mn, mx) : alpha_venkat_eps(uk, pred, mn, mx, veps);
+            al = smin(al, a);
+        }
+        const double fsk = (k == 0) ? P.fs_rho : (k == 1) ? P.fs_ru : (k == 2) ? P.fs_rv : P.fs_E;
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            if (j >= nk) break;
+            const int le = r.le[j];
+            if (le < 0) continue;
+            const double f = uk + al * (gxk * dxs[j] + gyk * dys[j]);
+            double *row = L.fs + 8 * le;
+            if ((info >> j) & 1u) {
+                row[k] = f;
+                const char bcj = (char)bcs[j];
+                if (tagged(bcj)) row[4 + k] = (bcj == 'i' || bcj == 'f') ? fsk : (bcj == 'w' && (k == 1 || k == 2)) ? -f : f;
+            } else {
+                row[4 + k] = f;
+            }
+        }
+    }
+    // viscous stress and heat flux (src/diffusion.cpp:227-240), the
+    // divisions spread over the four lanes as in reconstruct_body
+    const double gx0 = from_lane<0>(gxk), gy0 = from_lane<0>(gyk);
+    const double gx3 = from_lane<3>(gxk), gy3 = from_lane<3>(gyk);
+    const double px = quad_perm<0x99>(gxk), py = quad_perm<0x99>(gyk);
+    const double mu = P.mu, gamma = P.gamma, kappa = P.kappa;
+    const double rho = u[0], E = u[3];
+    const double qn = (k == 0)   ? u[1]
+                      : (k == 1) ? u[2]
+                      : (k == 2) ? (gx3 * rho - E * gx0)
+                                 : (gy3 * rho - E * gy0);
+    const double qv = qn / ((k < 2) ? rho : rho * rho);
+    const double uu = from_lane<0>(qv), vvel = from_lane<1>(qv), ex = from_lane<2>(qv), ey = from_lane<3>(qv);
+    const bool xk = (k == 0 || k == 3);
+    const double num = (xk ? px : py) - ((k & 1) ? vvel : uu) * (xk ? gx0 : gy0);
+    const double d = num / rho;
+    const double u_x = from_lane<0>(d), v_y = from_lane<1>(d), u_y = from_lane<2>(d), v_x = from_lane<3>(d);
+    const double tk = (k & 1) ? (uu * u_y + vvel * v_y) : (uu * u_x + vvel * v_x);
+    const double Tk = (gamma - 1.0) * (((k & 1) ? ey : ex) - tk);
+    const double T_x = from_lane<0>(Tk), T_y = from_lane<1>(Tk);
+    if (k != 0) return;
+    const double t4[4] = {2 * mu * u_x - 2.0 / 3.0 * mu * (u_x + v_y), 2 * mu * v_y - 2.0 / 3.0 * mu * (u_x + v_y),
+                          mu * (u_y + v_x), -kappa * T_x};
+    lds_st4(L.tq + 4 * i, t4);
+    L.tq[4 * L.max_gd + i] = -kappa * T_y;
+}
+
+// Owned cell i by its quad (flux_update_body's arithmetic); returns |u| + c
+// of the new state (kFlagSpeed), else 0, in every lane of the group.
+template <bool OUT_AOS, int NK>
+__device__ __forceinline__ double tile_update_quad(const Rec &r, const TileOwn &o, const Topo &T, const Geom &G,
+                                                   const double2 *__restrict__ EP, const Phys &P,
+                                                   const double *__restrict__ U, double *__restrict__ Uout,
+                                                   double *__restrict__ Uold, double *__restrict__ R,
+                                                   const double *__restrict__ dtp, double c0, double c1, int flags,
+                                                   const Lds &L, int k) {
+    const int nc = T.nc, c = r.c;
+    const unsigned info = (unsigned)r.info;
+    const int nk = info >> 4;
+    const bool has_edge = k < nk;
+    const bool is_left = (info >> k) & 1u;
+    const double vol = G.cvol[c];
+    double u[4];
+    ld4(U, c, u);
+    double adv[4] = {0.0, 0.0, 0.0, 0.0}, w1 = 0.0, w2 = 0.0, w3 = 0.0;
+    // one division for three quotients: lanes 0 / 1 / 2 form u, v, 1 / V
+    const double qv = ((k == 0) ? u[1] : (k == 1) ? u[2] : 1.0) / ((k < 2) ? u[0] : vol);
+    const double uu = from_lane<0>(qv), vv = from_lane<1>(qv), invV = from_lane<2>(qv);
+    if (has_edge) {
+        const int e = sel4(r.edge, k);
+        const double2 *pe = EP + 4 * (long)e;
+        const double2 n2 = pe[0], lt = pe[1];
+        const double el = lt.x;
+        const char ebc = (char)__double2loint(lt.y);
+        const int le = (k == 0) ? r.le[0] : (k == 1) ? r.le[1] : (k == 2) ? r.le[2] : r.le[3];
+        double F[4];
+        lds_ld4(L.fs + 8 * le, F);
+        // convective term of this edge, src/advection.cpp:453-456
+        const double sign = is_left ? +1.0 : -1.0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) adv[q] = sign * F[q] * el;
+        // viscous terms of this edge, src/diffusion.cpp:183-200
+        double nx = n2.x, ny = n2.y;
+        if (is_left) {
+            nx = -nx;
+            ny = -ny;
+        }
+        const int la = (k == 0) ? o.lo[0] : (k == 1) ? o.lo[2] : (k == 2) ? o.lo[4] : o.lo[6];
+        const int lb = (k == 0) ? o.lo[1] : (k == 1) ? o.lo[3] : (k == 2) ? o.lo[5] : o.lo[7];
+        double ta[4], tb[4], t[5];
+        lds_ld4(L.vert + 4 * la, ta);
+        lds_ld4(L.vert + 4 * lb, tb);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) t[q] = 0.5 * (ta[q] + tb[q]);
+        t[4] = 0.5 * (L.vert[4 * L.max_vo + la] + L.vert[4 * L.max_vo + lb]);
+        if (ebc == 'f' || ebc == 'w') {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) t[q] = 0.0;
+        }
+        const double txx = t[0], tyy = t[1], txy = t[2], qx = t[3], qy = t[4];
+        w1 = txx * el * nx / vol + txy * el * ny / vol;
+        w2 = txy * el * nx / vol + tyy * el * ny / vol;
+        w3 = (uu * txx + vv * txy - qx) * el * nx / vol + (uu * txy + vv * tyy - qy) * el * ny / vol;
+    }
+    // lane k: variable k's terms in slot order (+0.0 from lanes without an
+    // edge; the sum starts at +0.0, so it is never -0.0)
+    quad_transpose(adv, k);
+    double rk = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) rk += adv[j];
+    rk = rk * -invV;
+    double wv[4] = {0.0, w1, w2, w3};
+    quad_transpose(wv, k);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const double tt = rk + wv[j];
+        rk = (k > 0 && j < nk) ? tt : rk;
+    }
+    if (flags & kFlagStoreR) R[(long)k * nc + c] = rk;
+    const double dt = *dtp;
+    const double uk = sel4(u, k);
+    double old;
+    if (flags & kFlagSaveOld) {
+        old = uk;
+        Uold[(long)k * nc + c] = uk;
+    } else {
+        old = Uold[(long)k * nc + c];
+    }
+    const double unew = c0 * old + c1 * (uk + dt * rk);
+    if (OUT_AOS)
+        Uout[4 * (long)c + k] = unew;
+    else
+        Uout[(long)k * nc + c] = unew;
+    return (flags & kFlagSpeed)
+               ? wave_speed(from_lane<0>(unew), from_lane<1>(unew), from_lane<2>(unew), from_lane<3>(unew), P.gamma)
+               : 0.0;
+}
+
 // The largest of every lane's m (all >= 1e-8 and NaN-free, so the order
 // of this tree does not matter) into *dst, by thread 0.
 __device__ __forceinline__ void block_max_to(double m, double *dst) {
-    __shared__ double wmax[kTileThreads / 64];
+    __shared__ double wmax[kTileQuadThreads / 64];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
@@ -360,7 +586,7 @@ __device__ __forceinline__ void block_max_to(double m, double *dst) {
 // One stage over the tile with head h, on the mesh (T, G, EP) with the
 // physics P: reads U, writes Uout (and Uold / R as the flags say) and, with
 // kFlagSpeed, the tile's max(|u| + c) into *speed_slot.
-template <bool OUT_AOS, int NK>
+template <bool OUT_AOS, int NK, bool QUAD>
 __device__ __forceinline__ void stage_tile(const Topo &T, const Geom &G, const Phys &P, const TileView &A,
                                            const TileHead &h, const double2 *__restrict__ EP,
                                            const double *__restrict__ U, double *__restrict__ Uout,
@@ -381,9 +607,16 @@ __device__ __forceinline__ void stage_tile(const Topo &T, const Geom &G, const P
     }
     __syncthreads();
     // 2. gradients and stresses of every gradient cell; limiter and face states of the face cells
-    for (int i = tid; i < h.nGd; i += nth) {
-        const Rec r = load_rec(A.cell + h.cell0 + i);
-        tile_cell<NK>(r, i < h.nF, G, EP, P, T.nc, U, L, i);
+    if (QUAD) {
+        for (int i = tid >> 2; i < h.nGd; i += nth >> 2) {
+            const Rec r = load_rec(A.cell + h.cell0 + i);
+            tile_cell_quad<NK>(r, i < h.nF, G, EP, P, U, L, i, tid & 3);
+        }
+    } else {
+        for (int i = tid; i < h.nGd; i += nth) {
+            const Rec r = load_rec(A.cell + h.cell0 + i);
+            tile_cell<NK>(r, i < h.nF, G, EP, P, T.nc, U, L, i);
+        }
     }
     __syncthreads();
     // 3. HLLC once per tile edge (in place of its left face state), and the
@@ -406,11 +639,20 @@ __device__ __forceinline__ void stage_tile(const Topo &T, const Geom &G, const P
     __syncthreads();
     // 4. sums and stage update of the owned cells
     double m = 1e-8;
-    for (int i = tid; i < h.nO; i += nth) {
-        const Rec r = load_rec(A.cell + h.cell0 + i);
-        const double s =
-            tile_update<OUT_AOS, NK>(r, A.own[h.cell0 + i], T, G, EP, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
-        if (flags & kFlagSpeed) m = smax(m, s);
+    if (QUAD) {
+        for (int i = tid >> 2; i < h.nO; i += nth >> 2) {
+            const Rec r = load_rec(A.cell + h.cell0 + i);
+            const double s = tile_update_quad<OUT_AOS, NK>(r, A.own[h.cell0 + i], T, G, EP, P, U, Uout, Uold, R,
+                                                           dtp, c0, c1, flags, L, tid & 3);
+            if (flags & kFlagSpeed) m = smax(m, s);
+        }
+    } else {
+        for (int i = tid; i < h.nO; i += nth) {
+            const Rec r = load_rec(A.cell + h.cell0 + i);
+            const double s =
+                tile_update<OUT_AOS, NK>(r, A.own[h.cell0 + i], T, G, EP, P, U, Uout, Uold, R, dtp, c0, c1, flags, L);
+            if (flags & kFlagSpeed) m = smax(m, s);
+        }
     }
     if (flags & kFlagSpeed) block_max_to(m, speed_slot);
 }
@@ -435,8 +677,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (P.gate && *P.gate == 0.0) return;  // uniform
     const int tile = xcd_tile(A.ntiles);
     const TileHead h = A.head[tile];
-    stage_tile<OUT_AOS, NK>(T, G, P, A, h, reinterpret_cast<const double2 *>(A.epk), U, Uout, Uold, R, dtp, c0, c1,
-                            flags, speed + tile);
+    stage_tile<OUT_AOS, NK, false>(T, G, P, A, h, reinterpret_cast<const double2 *>(A.epk), U, Uout, Uold, R, dtp, c0,
+                                   c1, flags, speed + tile);
+}
+
+// The same stage for meshes whose tiles run one per CU: 1,024 threads per
+// tile, the cell phases four lanes per cell (tile_cell_quad,
+// tile_update_quad), so that a CU's sixteen waves share the tile's cells.
+template <bool OUT_AOS, int NK>
+__global__ __launch_bounds__(kTileQuadThreads) void k_tile_stage_q(Topo T, Geom G, Phys P, TileView A,
+                                                                   const double *__restrict__ U,
+                                                                   double *__restrict__ Uout,
+                                                                   double *__restrict__ Uold, double *__restrict__ R,
+                                                                   const double *__restrict__ dtp, double c0,
+                                                                   double c1, int flags,
+                                                                   double *__restrict__ speed) {
+    if (P.gate && *P.gate == 0.0) return;  // uniform
+    const int tile = xcd_tile(A.ntiles);
+    const TileHead h = A.head[tile];
+    stage_tile<OUT_AOS, NK, true>(T, G, P, A, h, reinterpret_cast<const double2 *>(A.epk), U, Uout, Uold, R, dtp, c0,
+                                  c1, flags, speed + tile);
 }
 
 // The same stage over the tiles of every member of a batch (tables
@@ -452,7 +712,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const Topo T = B.T;
     const Geom G = B.G;
     const Phys P = B.P;
-    stage_tile<true, NK>(T, G, P, A, h, reinterpret_cast<const double2 *>(B.epk), B.Us[stage],
+    stage_tile<true, NK, false>(T, G, P, A, h, reinterpret_cast<const double2 *>(B.epk), B.Us[stage],
                          B.Us[stage == 2 ? 0 : stage + 1], B.Uold, B.R, B.scal + kScalDt, c0, c1, flags,
                          B.speed + h.slot);
 }
@@ -505,16 +765,21 @@ __global__ __launch_bounds__(256) void k_state_to_soa(const double *__restrict__
 
 }  // namespace
 
-// The stage kernels of cells of at most nk_max edges.
-void tile_kernels(int nk_max, const void *(&f)[3]) {
+// The stage kernels of cells of at most nk_max edges: the 256-thread ones
+// (occupancy counted on them) and the 1,024-thread four-lane ones.
+void tile_kernels(int nk_max, const void *(&f)[3], const void *(&fq)[2]) {
     if (nk_max <= 3) {
         f[0] = reinterpret_cast<const void *>(&k_tile_stage<true, 3>);
         f[1] = reinterpret_cast<const void *>(&k_tile_stage<false, 3>);
         f[2] = reinterpret_cast<const void *>(&kb_tile_stage<3>);
+        fq[0] = reinterpret_cast<const void *>(&k_tile_stage_q<true, 3>);
+        fq[1] = reinterpret_cast<const void *>(&k_tile_stage_q<false, 3>);
     } else {
         f[0] = reinterpret_cast<const void *>(&k_tile_stage<true, 4>);
         f[1] = reinterpret_cast<const void *>(&k_tile_stage<false, 4>);
         f[2] = reinterpret_cast<const void *>(&kb_tile_stage<4>);
+        fq[0] = reinterpret_cast<const void *>(&k_tile_stage_q<true, 4>);
+        fq[1] = reinterpret_cast<const void *>(&k_tile_stage_q<false, 4>);
     }
 }
 
@@ -522,6 +787,25 @@ hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const 
                              double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
                              double c1, int flags, double *speed, hipStream_t s) {
     const dim3 grid(A.ntiles), block(kTileThreads);
+    if (A.quad) {
+        const dim3 qblock(kTileQuadThreads);
+        if (A.nk_max <= 3) {
+            if (out_aos)
+                hipLaunchKernelGGL((k_tile_stage_q<true, 3>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout, Uold,
+                                   R, dtp, c0, c1, flags, speed);
+            else
+                hipLaunchKernelGGL((k_tile_stage_q<false, 3>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout,
+                                   Uold, R, dtp, c0, c1, flags, speed);
+        } else {
+            if (out_aos)
+                hipLaunchKernelGGL((k_tile_stage_q<true, 4>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout, Uold,
+                                   R, dtp, c0, c1, flags, speed);
+            else
+                hipLaunchKernelGGL((k_tile_stage_q<false, 4>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout,
+                                   Uold, R, dtp, c0, c1, flags, speed);
+        }
+        return hipGetLastError();
+    }
     if (A.nk_max <= 3) {
         if (out_aos)
             hipLaunchKernelGGL((k_tile_stage<true, 3>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R, dtp,
@@ -570,11 +854,12 @@ hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_cu, int *vgprs) {
+hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_cu, int *vgprs, int *quad_per_cu) {
     // every context may launch up to the device's LDS per workgroup, less
     // the kernel's static LDS; a failed query leaves no error behind
-    const void *f[3];
-    tile_kernels(nk_max, f);
+    const void *f[3], *fq[2];
+    tile_kernels(nk_max, f, fq);
+    if (quad_per_cu) *quad_per_cu = 0;
     int dev = 0, lds_max = 0;
     *blocks_per_cu = 0;
     if (vgprs) *vgprs = 0;
@@ -595,11 +880,28 @@ hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_c
         if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, kTileThreads, lds_bytes);
         bpc = std::min(bpc, b);
     }
+    int qpc = bpc > 0 ? 1 << 30 : 0;
+    for (const void *k : fq) {
+        if (e != hipSuccess || qpc == 0) break;
+        hipFuncAttributes fa{};
+        e = hipFuncGetAttributes(&fa, k);
+        const int dyn_max = lds_max - (int)fa.sharedSizeBytes;
+        if (e != hipSuccess || lds_bytes > (size_t)dyn_max) {
+            qpc = 0;
+            break;
+        }
+        int b = 0;
+        e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, dyn_max);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, kTileQuadThreads, lds_bytes);
+        qpc = std::min(qpc, b);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         bpc = 0;
+        qpc = 0;
     }
     *blocks_per_cu = bpc;
+    if (quad_per_cu) *quad_per_cu = qpc;
     return e;
 }
 

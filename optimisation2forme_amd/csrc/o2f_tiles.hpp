@@ -10,7 +10,8 @@
 
 namespace o2f {
 
-constexpr int kTileThreads = 256;  // one workgroup per tile
+constexpr int kTileThreads = 256;       // one workgroup per tile
+constexpr int kTileQuadThreads = 1024;  // the four-lane form, for tiles that run one per CU
 
 // A TileSet uploaded to the device, with its LDS layout (offsets in doubles).
 struct TileView {
@@ -26,6 +27,7 @@ struct TileView {
     int max_vo, max_gd;
     int lds_tq, lds_fs;
     size_t lds_bytes;
+    int quad;  // 1: the 1,024-thread four-lane kernels (tiles one per CU)
 };
 
 // One residual stage fused with its Shu-Osher update over every tile:
@@ -53,8 +55,11 @@ hipError_t launch_pack_edges(const Geom &G, const char *ebc, int ne, double *epk
 hipError_t launch_state_to_aos(const double *U, int nc, double *A, hipStream_t s);
 hipError_t launch_state_to_soa(const double *A, int nc, double *U, hipStream_t s);
 // Allows the device's full LDS for the stage kernels of cells of at most
-// nk_max edges and reports how many of their workgroups fit on one CU with
-// lds_bytes each (0: they do not fit), and their VGPRs.
-hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_cu, int *vgprs);
+// nk_max edges and reports how many of their 256-thread workgroups fit on
+// one CU with lds_bytes each (0: they do not fit), their VGPRs and how many
+// workgroups of the 1,024-thread four-lane form fit (quad_per_cu, may be
+// NULL).
+hipError_t tile_stage_attributes(size_t lds_bytes, int nk_max, int *blocks_per_cu, int *vgprs,
+                                 int *quad_per_cu = nullptr);
 
 }  // namespace o2f
