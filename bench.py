@@ -287,16 +287,12 @@ def bench_batch(a):
         seq_graph += one.time_steps(K, use_graph=not a.no_graph)
         one.close()
     b = capi.Batch(solvers)
-    # the four-kernel batched stages: autotuned launch shapes, timed beside
-    shapes = b.autotune(reps=3)
-    b.set_tile_cells(0)
-    for s in solvers:
-        s.init_freestream()
-    b.time_steps(2, use_graph=not a.no_graph)
-    for s in solvers:
-        s.init_freestream()
-    four_kernel_ms = b.time_steps(K, use_graph=not a.no_graph)
     b.set_tile_cells(a.tile_cells)
+    # launch shapes of the four-kernel stages and, on automatic tile stages,
+    # the tile size: timed, the fastest kept
+    for s in solvers:
+        s.init_freestream()
+    shapes = b.autotune(reps=3)
     tiling = b.tile_info()
     for s in solvers:
         s.init_freestream()
@@ -307,7 +303,15 @@ def bench_batch(a):
     batch_ms = b.time_steps(K, use_graph=not a.no_graph)
     wall = time.perf_counter() - t1
     finite = all(bool(np.isfinite(s.state()).all()) for s in solvers)
-    # the members' adjoint solves at the end state, a fixed 100 BiCGSTAB
+    # the four-kernel batched stages (autotuned shapes) beside
+    b.set_tile_cells(0)
+    for s in solvers:
+        s.init_freestream()
+    b.time_steps(2, use_graph=not a.no_graph)
+    for s in solvers:
+        s.init_freestream()
+    four_kernel_ms = b.time_steps(K, use_graph=not a.no_graph)
+    # the members' adjoint solves at the last run's end state, a fixed 100 BiCGSTAB
     # iterations each (tolerance out of reach): one after another, then
     # together (o2f_batch_bicgstab)
     adj_it = 100

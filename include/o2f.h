@@ -388,7 +388,11 @@ const char *o2f_batch_last_error(const o2f_batch *batch);
 int o2f_batch_set_shapes(o2f_batch *batch, const int *shapes);
 int o2f_batch_get_shapes(const o2f_batch *batch, int *shapes);
 /* Times every shape for each kernel on residual-only passes (the state is
- * not changed) and keeps the fastest; shapes (8, may be NULL) out. */
+ * not changed) and keeps the fastest; shapes (8, may be NULL) out.  On the
+ * automatic tile stages (o2f_batch_set_tile_cells -1) it also times the
+ * rule's tile size and 90 % and 80 % of it on first-stage launches from the
+ * members' states (into scratch buffers) and keeps the fastest for the
+ * batch's runs (o2f_batch_tile_info reports it). */
 int o2f_batch_autotune(o2f_batch *batch, int reps, int *shapes);
 /* o2f_run_to_residual for every member at once: steps[i], res[i] per member
  * (may be NULL). */

@@ -73,6 +73,7 @@ struct o2f_batch {
     // fused tile stages over every member's tiles (tile_cells: -1 automatic,
     // 0 the four-kernel stages, n > 0 at most n cells per tile)
     int tile_cells = -1;
+    int tile_tuned = 0;  // the automatic size o2f_batch_autotune timed fastest (0: the rule's)
     bool tiles_ready = false, tiles_unusable = false;
     std::string tiles_why;
     TileView tv{};
@@ -174,7 +175,7 @@ bool tiles_on(o2f_batch *b) {
         if (e != hipSuccess) err = std::string("tile kernel attributes: ") + hipGetErrorString(e);
         return e == hipSuccess && bpc > 0;
     };
-    int target = b->tile_cells;
+    int target = (b->tile_cells < 0 && b->tile_tuned > 0) ? b->tile_tuned : b->tile_cells;
     if (target > 0) {
         if (!build(target)) return unusable(err.empty() ? "tile LDS too large" : err);
     } else {
@@ -627,6 +628,7 @@ int o2f_batch_set_tile_cells(o2f_batch *b, int cells) {
         b->tiles_unusable = false;
         b->tiles_why.clear();
         b->tile_cells = cells;
+        b->tile_tuned = 0;
     }
     return O2F_OK;
 }
@@ -740,6 +742,51 @@ int o2f_batch_autotune(o2f_batch *b, int reps, int *shapes) {
             }
         b->shape[kind] = best;
         if ((st = build_tiles(b, kind))) return st;
+    }
+    // Automatic tile stages: the rule's size and two smaller ones (more
+    // tiles per round, less LDS), each timed on first-stage launches from
+    // the members' current states (written to the stage's scratch buffers
+    // and U_old only); the fastest is kept.
+    if (b->tile_cells < 0 && !b->tiles_unusable) {
+        BCHK(b, hipStreamSynchronize(b->stream));
+        b->drop_graph();
+        release_tiles(b);
+        b->tile_tuned = 0;
+        if (tiles_on(b)) {
+            const int t0 = b->tile_target;
+            const int cand[3] = {t0, t0 * 9 / 10, t0 * 4 / 5};
+            int best_t = t0;
+            float best_ms = -1.f;
+            for (int t : cand) {
+                b->drop_graph();
+                release_tiles(b);
+                b->tile_tuned = t;
+                if (!tiles_on(b)) {
+                    b->tiles_unusable = false;  // this size only: the others may fit
+                    b->tiles_why.clear();
+                    continue;
+                }
+                if ((st = upload_members(b, -1.0))) return st;
+                for (int m = 0; m < b->n(); ++m)
+                    BCHK(b, launch_state_to_aos(b->ctx[m]->U, b->ctx[m]->hm.nc, b->Us[3 * m], b->stream));
+                const int flags = kFlagRK | kFlagSaveOld;
+                BCHK(b, launch_b_tile_stage(b->dM, b->tv, 0, 0.0, 1.0, flags, b->stream));  // warm-up
+                BCHK(b, hipEventRecord(ev[0], b->stream));
+                for (int r = 0; r < reps; ++r) BCHK(b, launch_b_tile_stage(b->dM, b->tv, 0, 0.0, 1.0, flags, b->stream));
+                BCHK(b, hipEventRecord(ev[1], b->stream));
+                BCHK(b, hipEventSynchronize(ev[1]));
+                float ms = 0.f;
+                BCHK(b, hipEventElapsedTime(&ms, ev[0], ev[1]));
+                if (best_ms < 0.f || ms < best_ms) {
+                    best_ms = ms;
+                    best_t = t;
+                }
+            }
+            b->drop_graph();
+            release_tiles(b);
+            b->tile_tuned = best_t;
+            (void)tiles_on(b);
+        }
     }
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);

@@ -214,6 +214,28 @@ def test_batch_barth_jespersen_and_consecutive_runs():
         assert np.array_equal(s.state(), ref[k]), k
 
 
+def test_tile_size_autotuner_changes_only_speed():
+    """o2f_batch_autotune on automatic tile stages times three tile sizes
+    and keeps one: the state is left alone and the run afterwards is
+    bitwise the members' own four-kernel runs."""
+    n = 12
+    ref = single_states(n)
+    ss = solvers()
+    b = capi.Batch(ss)
+    before = b.tile_info()
+    U0 = [s.state() for s in ss]
+    b.autotune(reps=2)
+    after = b.tile_info()
+    assert after["tiles"] > 0 and after["cells"] in (before["cells"], before["cells"] * 9 // 10,
+                                                     before["cells"] * 4 // 5), (before, after)
+    for k, s in enumerate(ss):
+        assert np.array_equal(s.state(), U0[k]), k
+    b.run(-1.0, n, check_every=6, use_graph=True)
+    for k, s in enumerate(ss):
+        assert np.array_equal(s.state(), ref[k]), k
+    b.close()
+
+
 def test_autotuner_changes_only_speed():
     ref = single_states(15)
     ss = solvers()
