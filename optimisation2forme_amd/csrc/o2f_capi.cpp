@@ -294,11 +294,14 @@ int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check = false, 
     }
     int st = tiles_enter(c);
     if (st) return st;
-    HIPCHK(c, launch_step_begin(c->speed, c->speed_nparts, c->phys, c->hm.min_h, c->scal, hist, 1, s));
-    const double *dtp = c->scal + kScalDt;
     const TileView &A = c->tv;
+    // the four-lane stages start the step themselves (no one-block launch)
+    const StepBegin B{c->speed, c->speed_nparts, c->hm.min_h, c->scal, hist};
+    const int begin = A.quad ? kFlagBegin : 0, advance = A.quad ? kFlagAdvance : 0;
+    if (!A.quad) HIPCHK(c, launch_step_begin(c->speed, c->speed_nparts, c->phys, c->hm.min_h, c->scal, hist, 1, s));
+    const double *dtp = c->scal + kScalDt;
     HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ua, c->Ub, true, c->Uold, c->R, dtp, 0.0, 1.0,
-                                kFlagRK | kFlagSaveOld | (check ? kFlagStoreR : 0), nullptr, s));
+                                kFlagRK | kFlagSaveOld | (check ? kFlagStoreR : 0) | begin, nullptr, s, B));
     if (check) {
         const double a_inf = std::sqrt(c->prm.gamma * c->prm.p_ref / c->prm.rho_ref);
         const double res_scale = c->prm.rho_ref * (c->prm.inflow_velocity * a_inf) / 1.0;
@@ -306,12 +309,12 @@ int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check = false, 
     }
     if (last_outputs) {
         HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->U, false, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0,
-                                    kFlagRK, nullptr, s));
+                                    kFlagRK | advance, nullptr, s, B));
         c->tile_aos = false;
         return run_stage(c, 0, kFlagRK | kFlagStoreR | kFlagSpeed, dtp, 1.0 / 3.0, 2.0 / 3.0, true);
     }
     HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->Uc, true, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0,
-                                kFlagRK, nullptr, s));
+                                kFlagRK | advance, nullptr, s, B));
     HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Uc, c->Ua, true, c->Uold, c->R, dtp, 1.0 / 3.0, 2.0 / 3.0,
                                 kFlagRK | kFlagSpeed, c->speed, s));
     c->speed_nparts = A.ntiles;

@@ -22,6 +22,8 @@ enum : int {
     kFlagRK = 8,        // stage update U = c0 Uold + c1 (U + dt R)
     kFlagSaveOld = 16,  // first stage: Uold = U before the update
     kFlagSpeed = 32,    // after the update: max_c(|u|+c) of the new state into the speed word
+    kFlagBegin = 64,    // tile stage: the step's start (dt, the time loop's stop test) in every workgroup
+    kFlagAdvance = 128, // tile stage: workgroup 0 advances the time bookkeeping by the step's dt
 };
 
 // Slots of the device scalar block.

@@ -691,8 +691,42 @@ __global__ __launch_bounds__(kTileQuadThreads) void k_tile_stage_q(Topo T, Geom 
                                                                    double *__restrict__ Uold, double *__restrict__ R,
                                                                    const double *__restrict__ dtp, double c0,
                                                                    double c1, int flags,
-                                                                   double *__restrict__ speed) {
+                                                                   double *__restrict__ speed, StepBegin B) {
     if (P.gate && *P.gate == 0.0) return;  // uniform
+    __shared__ double s_dt;
+    if (flags & kFlagBegin) {
+        // the step's start, as k_step_begin forms it: the max of the speed
+        // partials (exact and order-free), the stop test on the time or the
+        // residual flag (neither changes during this launch), dt
+        double m = 1e-8;
+        for (int i = threadIdx.x; i < B.nparts; i += blockDim.x) m = smax(m, B.parts[i]);
+        __shared__ double s_max;
+        block_max_to(m, &s_max);
+        __syncthreads();
+        const bool stop = P.gate && (!(B.scal[kScalT] < P.final_t) || B.scal[kScalConv] != 0.0);
+        if (stop) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) B.scal[kScalActive] = 0.0;
+            return;  // uniform
+        }
+        const double dt_adv = P.cfl * B.min_h / s_max;
+        const double dt_diff = 0.5 * (B.min_h * B.min_h) / P.mu;
+        const double dt = smin(dt_adv, dt_diff);
+        if (threadIdx.x == 0) {
+            s_dt = dt;
+            if (blockIdx.x == 0) {
+                B.scal[kScalDt] = dt;
+                if (B.hist) *B.hist = dt;
+            }
+        }
+        __syncthreads();
+        dtp = &s_dt;
+    }
+    if ((flags & kFlagAdvance) && blockIdx.x == 0 && threadIdx.x == 0) {
+        // t is read by the next step's start only
+        B.scal[kScalTPrev] = B.scal[kScalT];
+        B.scal[kScalT] = B.scal[kScalT] + B.scal[kScalDt];
+        if (P.gate) B.scal[kScalSteps] = B.scal[kScalSteps] + 1.0;
+    }
     const int tile = xcd_tile(A.ntiles);
     const TileHead h = A.head[tile];
     stage_tile<OUT_AOS, NK, true>(T, G, P, A, h, reinterpret_cast<const double2 *>(A.epk), U, Uout, Uold, R, dtp, c0,
@@ -785,24 +819,25 @@ void tile_kernels(int nk_max, const void *(&f)[3], const void *(&fq)[2]) {
 
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
                              double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
-                             double c1, int flags, double *speed, hipStream_t s) {
+                             double c1, int flags, double *speed, hipStream_t s, const StepBegin &B) {
     const dim3 grid(A.ntiles), block(kTileThreads);
+    if (!A.quad && (flags & (kFlagBegin | kFlagAdvance))) return hipErrorInvalidValue;
     if (A.quad) {
         const dim3 qblock(kTileQuadThreads);
         if (A.nk_max <= 3) {
             if (out_aos)
                 hipLaunchKernelGGL((k_tile_stage_q<true, 3>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout, Uold,
-                                   R, dtp, c0, c1, flags, speed);
+                                   R, dtp, c0, c1, flags, speed, B);
             else
                 hipLaunchKernelGGL((k_tile_stage_q<false, 3>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout,
-                                   Uold, R, dtp, c0, c1, flags, speed);
+                                   Uold, R, dtp, c0, c1, flags, speed, B);
         } else {
             if (out_aos)
                 hipLaunchKernelGGL((k_tile_stage_q<true, 4>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout, Uold,
-                                   R, dtp, c0, c1, flags, speed);
+                                   R, dtp, c0, c1, flags, speed, B);
             else
                 hipLaunchKernelGGL((k_tile_stage_q<false, 4>), grid, qblock, A.lds_bytes, s, T, G, P, A, U, Uout,
-                                   Uold, R, dtp, c0, c1, flags, speed);
+                                   Uold, R, dtp, c0, c1, flags, speed, B);
         }
         return hipGetLastError();
     }

@@ -30,15 +30,32 @@ struct TileView {
     int quad;  // 1: the 1,024-thread four-lane kernels (tiles one per CU)
 };
 
+// The start of a step as a tile stage forms it (kFlagBegin): the speed
+// partials of the step's start state, min_h, the scalar block and the dt
+// history slot (may be NULL).
+struct StepBegin {
+    const double *parts;
+    int nparts;
+    double min_h;
+    double *scal;
+    double *hist;
+};
+
 // One residual stage fused with its Shu-Osher update over every tile:
 // Uout = c0 Uold + c1 (U + dt R(U)) for each owned cell (flags as for
 // k_cell_flux_update; kFlagAdv and kFlagVisc are implied).  U is AoS (four
 // doubles per cell); Uout is AoS (out_aos) or SoA [q][nc]; Uold and R are
 // SoA.  U and Uout must be different buffers.  kFlagSpeed writes one
 // max(|u| + c) partial per tile.
+// With the four-lane kernels (A.quad) the first stage may also start the
+// step (kFlagBegin: every workgroup forms the step's dt from B's partials
+// as k_step_begin does, workgroup 0 stores it, and the time loop's stop
+// test closes the gate instead of the stage) and the second stage advance
+// the time (kFlagAdvance: t_prev = t, t += dt, steps + 1 under a gate), in
+// place of the one-block k_step_begin launch.
 hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const TileView &A, const double *U,
                              double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
-                             double c1, int flags, double *speed, hipStream_t s);
+                             double c1, int flags, double *speed, hipStream_t s, const StepBegin &B = StepBegin{});
 // One stage over the concatenated tiles of a batch's members (each head
 // names its member): member m's tiles read M[m].Us[stage] and write
 // M[m].Us[stage + 1 mod 3] (AoS), with M[m]'s Uold, R, dt and speed
