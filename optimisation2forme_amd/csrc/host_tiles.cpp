@@ -79,6 +79,31 @@ TileVertex vertex_record(const HostMesh &hm, int v, int ls) {
     return TileVertex{v, b, n | (int)hm.skind[v] << 16 | (int)(uint8_t)hm.vbc[v] << 24, ls};
 }
 
+TileGather gather_record(const HostMesh &hm, int v) {
+    const TileVertex r = vertex_record(hm, v, -1);
+    const int n = r.nkb & 0xffff;
+    TileGather g{};
+    for (int k = 0; k < kTileSlots; ++k) g.cid[k] = k < n ? hm.scell[r.b + k] : -1;
+    g.b = r.b;
+    g.nkb = r.nkb;
+    return g;
+}
+
+// lsten: the vertex's local stencil (its used entries).
+TileStress stress_record(const HostMesh &hm, int v, int ls, const uint16_t *lsten) {
+    const TileVertex r = vertex_record(hm, v, ls);
+    const int used = stencil_used(hm, v);
+    TileStress s{};
+    for (int k = 0; k < kTileSlots; ++k) {
+        s.w[k] = k < used ? hm.sw[r.b + k] : 0.0;
+        s.lc[k] = k < used ? lsten[k] : (uint16_t)0xffff;
+    }
+    s.nkb = r.nkb;
+    s.b = r.b;
+    s.ls = ls;
+    return s;
+}
+
 // Sorted unique ids of a list.
 void uniq(std::vector<int> &v) {
     std::sort(v.begin(), v.end());
@@ -193,12 +218,21 @@ bool build_tiles(const HostMesh &hm, int target, TileSet &ts, std::string &err) 
             ts.cell.push_back(r);
             ts.own.push_back(o);
         }
-        for (int v : Vg) ts.vg.push_back(vertex_record(hm, v, -1));
+        for (int v : Vg) {
+            ts.vg.push_back(vertex_record(hm, v, -1));
+            ts.vgf.push_back(gather_record(hm, v));
+        }
         ts.edge.insert(ts.edge.end(), E.begin(), E.end());
+        for (int e : E) {
+            ts.en.push_back(hm.enx[e]);
+            ts.en.push_back(hm.eny[e]);
+        }
         for (int v : Vo) {
-            ts.vo.push_back(vertex_record(hm, v, (int)ts.lsten.size()));
+            const int ls = (int)ts.lsten.size();
+            ts.vo.push_back(vertex_record(hm, v, ls));
             const int b = hm.vptr[v], n = stencil_used(hm, v);
             for (int k = 0; k < n; ++k) ts.lsten.push_back((uint16_t)lg[hm.scell[b + k]]);
+            ts.vof.push_back(stress_record(hm, v, ls, ts.lsten.data() + ls));
         }
         ts.head.push_back(h);
         ts.max_o = std::max(ts.max_o, h.nO);
@@ -226,6 +260,9 @@ bool check_tiles(const HostMesh &hm, const TileSet &ts, std::string &err) {
     };
     std::vector<int> owner(hm.nc, -1);
     const int ntiles = (int)ts.head.size();
+    if (ts.vgf.size() != ts.vg.size() || ts.vof.size() != ts.vo.size() || ts.en.size() != 2 * ts.edge.size())
+        return fail(-1, "read records do not match the vertex and edge lists");
+    auto same = [](const void *a, const void *b, size_t bytes) { return std::memcmp(a, b, bytes) == 0; };
     for (int t = 0; t < ntiles; ++t) {
         const TileHead &h = ts.head[t];
         if (h.nO < 1 || h.nO > h.nF || h.nF > h.nGd || h.cell0 < 0 || h.cell0 + h.nGd > (int)ts.cell.size() ||
@@ -244,10 +281,19 @@ bool check_tiles(const HostMesh &hm, const TileSet &ts, std::string &err) {
             const TileVertex want = vertex_record(hm, r.v, r.ls);
             return r.v >= 0 && r.v < hm.nv && r.b == want.b && r.nkb == want.nkb;
         };
-        for (int i = 0; i < h.nVg; ++i)
+        for (int i = 0; i < h.nVg; ++i) {
             if (!vertex_ok(Vg[i]) || Vg[i].ls != -1) return fail(t, "state vertex record differs");
+            const TileGather want = gather_record(hm, Vg[i].v);
+            if (!same(&want, &ts.vgf[h.vg0 + i], sizeof want)) return fail(t, "state vertex gather record differs");
+        }
         for (int i = 0; i < h.nVo; ++i)
             if (!vertex_ok(Vo[i])) return fail(t, "stress vertex record differs");
+        for (int i = 0; i < h.nE; ++i) {
+            const int e = ts.edge[h.e0 + i];
+            if (e < 0 || e >= hm.ne || !same(&ts.en[2 * (h.e0 + i)], &hm.enx[e], sizeof(double)) ||
+                !same(&ts.en[2 * (h.e0 + i) + 1], &hm.eny[e], sizeof(double)))
+                return fail(t, "edge normal differs");
+        }
         std::unordered_map<int, int> gd_of, e_of;
         for (int i = 0; i < h.nGd; ++i) {
             const int c = C[i].c;
@@ -310,6 +356,8 @@ bool check_tiles(const HostMesh &hm, const TileSet &ts, std::string &err) {
                 const int li = ts.lsten[off + k];
                 if (li >= h.nGd || C[li].c != hm.scell[b + k]) return fail(t, "stencil cell not local");
             }
+            const TileStress want = stress_record(hm, v, off, ts.lsten.data() + off);
+            if (!same(&want, &ts.vof[h.vo0 + i], sizeof want)) return fail(t, "stress vertex read record differs");
         }
         for (int i = 0; i < h.nO; ++i) {
             const int c = C[i].c, nk = C[i].info >> 4;
@@ -341,10 +389,16 @@ void concat_tiles(const std::vector<TileSet> &sets, TileSet &all) {
         all.cell.insert(all.cell.end(), ts.cell.begin(), ts.cell.end());
         all.own.insert(all.own.end(), ts.own.begin(), ts.own.end());
         all.vg.insert(all.vg.end(), ts.vg.begin(), ts.vg.end());
+        all.vgf.insert(all.vgf.end(), ts.vgf.begin(), ts.vgf.end());
         all.edge.insert(all.edge.end(), ts.edge.begin(), ts.edge.end());
+        all.en.insert(all.en.end(), ts.en.begin(), ts.en.end());
         for (TileVertex r : ts.vo) {
             r.ls += lsb;
             all.vo.push_back(r);
+        }
+        for (TileStress r : ts.vof) {
+            r.ls += lsb;
+            all.vof.push_back(r);
         }
         all.lsten.insert(all.lsten.end(), ts.lsten.begin(), ts.lsten.end());
         all.target = std::max(all.target, ts.target);
@@ -382,14 +436,23 @@ bool check_concat(const std::vector<TileSet> &sets, const TileSet &all, std::str
                 a.vg0 + a.nVg > (int)all.vg.size() || a.e0 < 0 || a.e0 + a.nE > (int)all.edge.size() ||
                 a.vo0 < 0 || a.vo0 + a.nVo > (int)all.vo.size() || all.own.size() != all.cell.size())
                 return fail((int)k, "offsets out of range");
+            if (all.vgf.size() != all.vg.size() || all.vof.size() != all.vo.size() ||
+                all.en.size() != 2 * all.edge.size())
+                return fail((int)k, "read records do not match the vertex and edge lists");
             if (!same(&all.cell[a.cell0], &ts.cell[h.cell0], h.nGd * sizeof(TileCell)) ||
                 !same(&all.own[a.cell0], &ts.own[h.cell0], h.nGd * sizeof(TileOwn)) ||
                 !same(&all.vg[a.vg0], &ts.vg[h.vg0], h.nVg * sizeof(TileVertex)) ||
-                !same(&all.edge[a.e0], &ts.edge[h.e0], h.nE * sizeof(int)))
+                !same(&all.vgf[a.vg0], &ts.vgf[h.vg0], h.nVg * sizeof(TileGather)) ||
+                !same(&all.edge[a.e0], &ts.edge[h.e0], h.nE * sizeof(int)) ||
+                !same(&all.en[2 * a.e0], &ts.en[2 * h.e0], 2 * h.nE * sizeof(double)))
                 return fail((int)k, "records differ from its member's tile");
             for (int i = 0; i < h.nVo; ++i) {
                 const TileVertex &x = all.vo[a.vo0 + i], &y = ts.vo[h.vo0 + i];
                 if (x.v != y.v || x.b != y.b || x.nkb != y.nkb) return fail((int)k, "stress vertex differs");
+                TileStress fy = ts.vof[h.vo0 + i];
+                fy.ls = x.ls;
+                if (!same(&all.vof[a.vo0 + i], &fy, sizeof fy))
+                    return fail((int)k, "stress vertex read record differs");
                 const int n = stencil_used((y.nkb >> 16) & 0xff, y.nkb & 0xffff);
                 if (x.ls < 0 || x.ls + n > (int)all.lsten.size() ||
                     !same(&all.lsten[x.ls], &ts.lsten[y.ls], n * sizeof(uint16_t)))

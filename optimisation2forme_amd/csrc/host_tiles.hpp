@@ -56,6 +56,32 @@ struct TileVertex {
 };
 static_assert(sizeof(TileVertex) == 16, "TileVertex is one 16-byte record");
 
+// Stencil entries a vertex record carries (longer stencils read the rest
+// from the mesh's stencil arrays).
+constexpr int kTileSlots = 8;
+
+// A state vertex as the stage reads it: its first kTileSlots stencil cells
+// (global ids, -1 past the stencil), so the cells' states load right after
+// the record; the stencil offset b (weights, longer stencils) and nkb as in
+// TileVertex.
+struct TileGather {
+    int cid[kTileSlots];
+    int b, nkb;
+    int pad[2];
+};
+static_assert(sizeof(TileGather) == 48, "TileGather is three 16-byte words");
+
+// A stress vertex as the stage reads it: its first kTileSlots stencil
+// weights and local gradient-cell indices (0 / 0xffff past the stencil's
+// used entries), nkb, the stencil offset b and its local stencil offset ls
+// (entries past kTileSlots).
+struct TileStress {
+    double w[kTileSlots];
+    uint16_t lc[kTileSlots];
+    int nkb, b, ls, pad;
+};
+static_assert(sizeof(TileStress) == 96, "TileStress is six 16-byte words");
+
 // Offsets and counts of one tile in the concatenated tables.
 struct TileHead {
     int cell0, nO, nF, nGd;  // TileCell records [cell0, cell0 + nGd); TileOwn records [cell0, cell0 + nO)
@@ -76,6 +102,12 @@ struct TileSet {
     std::vector<int> edge;        // global edge ids
     std::vector<TileVertex> vo;   // stress vertices
     std::vector<uint16_t> lsten;  // local Gd index of each stencil cell, in stencil order
+    // what the stage reads in place of vg / vo / edge (one entry per entry of
+    // those): state-vertex gather records, stress-vertex records, and each
+    // tile edge's unit normal (n_x, n_y)
+    std::vector<TileGather> vgf;
+    std::vector<TileStress> vof;
+    std::vector<double> en;
     // largest per-tile counts (LDS is sized for them)
     int max_vg = 0, max_vo = 0, max_gd = 0, max_e = 0, max_o = 0;
     // sums over the tiles (work with the rim counted as often as it is formed)

@@ -196,8 +196,8 @@ bool tiles_on(o2f_batch *b) {
         return o;
     };
     const size_t o_head = slot(all.head.size() * sizeof(TileHead)), o_cell = slot(all.cell.size() * sizeof(TileCell)),
-                 o_own = slot(all.own.size() * sizeof(TileOwn)), o_vg = slot(all.vg.size() * sizeof(TileVertex)),
-                 o_edge = slot(all.edge.size() * sizeof(int)), o_vo = slot(all.vo.size() * sizeof(TileVertex)),
+                 o_own = slot(all.own.size() * sizeof(TileOwn)), o_vg = slot(all.vgf.size() * sizeof(TileGather)),
+                 o_edge = slot(all.en.size() * sizeof(double)), o_vo = slot(all.vof.size() * sizeof(TileStress)),
                  o_ls = slot(all.lsten.size() * sizeof(uint16_t) + 16);
     char *mem = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&mem), off);
@@ -208,9 +208,9 @@ bool tiles_on(o2f_batch *b) {
     put(o_head, all.head.data(), all.head.size() * sizeof(TileHead));
     put(o_cell, all.cell.data(), all.cell.size() * sizeof(TileCell));
     put(o_own, all.own.data(), all.own.size() * sizeof(TileOwn));
-    put(o_vg, all.vg.data(), all.vg.size() * sizeof(TileVertex));
-    put(o_edge, all.edge.data(), all.edge.size() * sizeof(int));
-    put(o_vo, all.vo.data(), all.vo.size() * sizeof(TileVertex));
+    put(o_vg, all.vgf.data(), all.vgf.size() * sizeof(TileGather));
+    put(o_edge, all.en.data(), all.en.size() * sizeof(double));
+    put(o_vo, all.vof.data(), all.vof.size() * sizeof(TileStress));
     put(o_ls, all.lsten.data(), all.lsten.size() * sizeof(uint16_t));
     for (int m = 0; m < b->n() && e == hipSuccess; ++m) {
         o2f_ctx *c = b->ctx[m];
@@ -239,9 +239,9 @@ bool tiles_on(o2f_batch *b) {
     v.head = reinterpret_cast<const TileHead *>(mem + o_head);
     v.cell = reinterpret_cast<const TileCell *>(mem + o_cell);
     v.own = reinterpret_cast<const TileOwn *>(mem + o_own);
-    v.vg = reinterpret_cast<const TileVertex *>(mem + o_vg);
-    v.edge = reinterpret_cast<const int *>(mem + o_edge);
-    v.vo = reinterpret_cast<const TileVertex *>(mem + o_vo);
+    v.vg = reinterpret_cast<const TileGather *>(mem + o_vg);
+    v.en = reinterpret_cast<const double2 *>(mem + o_edge);
+    v.vo = reinterpret_cast<const TileStress *>(mem + o_vo);
     v.lsten = reinterpret_cast<const uint16_t *>(mem + o_ls);
     v.epk = nullptr;  // per member (Member::epk)
     v.ntiles = (int)all.head.size();

@@ -202,8 +202,8 @@ bool tiles_on(o2f_ctx *c) {
         return o;
     };
     const size_t o_head = slot(ts.head.size() * sizeof(TileHead)), o_cell = slot(ts.cell.size() * sizeof(TileCell)),
-                 o_own = slot(ts.own.size() * sizeof(TileOwn)), o_vg = slot(ts.vg.size() * sizeof(TileVertex)),
-                 o_edge = slot(ts.edge.size() * sizeof(int)), o_vo = slot(ts.vo.size() * sizeof(TileVertex)),
+                 o_own = slot(ts.own.size() * sizeof(TileOwn)), o_vg = slot(ts.vgf.size() * sizeof(TileGather)),
+                 o_edge = slot(ts.en.size() * sizeof(double)), o_vo = slot(ts.vof.size() * sizeof(TileStress)),
                  o_ls = slot(ts.lsten.size() * sizeof(uint16_t) + 16);
     char *m = nullptr;
     hipError_t e = hipMalloc(reinterpret_cast<void **>(&m), off);
@@ -214,9 +214,9 @@ bool tiles_on(o2f_ctx *c) {
     put(o_head, ts.head.data(), ts.head.size() * sizeof(TileHead));
     put(o_cell, ts.cell.data(), ts.cell.size() * sizeof(TileCell));
     put(o_own, ts.own.data(), ts.own.size() * sizeof(TileOwn));
-    put(o_vg, ts.vg.data(), ts.vg.size() * sizeof(TileVertex));
-    put(o_edge, ts.edge.data(), ts.edge.size() * sizeof(int));
-    put(o_vo, ts.vo.data(), ts.vo.size() * sizeof(TileVertex));
+    put(o_vg, ts.vgf.data(), ts.vgf.size() * sizeof(TileGather));
+    put(o_edge, ts.en.data(), ts.en.size() * sizeof(double));
+    put(o_vo, ts.vof.data(), ts.vof.size() * sizeof(TileStress));
     put(o_ls, ts.lsten.data(), ts.lsten.size() * sizeof(uint16_t));
     if (e == hipSuccess) e = dalloc(c->epk, 8 * (size_t)c->hm.ne);
     if (e == hipSuccess) e = launch_pack_edges(c->geo[0].view(), c->ebc, c->hm.ne, c->epk, c->stream);
@@ -239,9 +239,9 @@ bool tiles_on(o2f_ctx *c) {
     v.head = reinterpret_cast<const TileHead *>(m + o_head);
     v.cell = reinterpret_cast<const TileCell *>(m + o_cell);
     v.own = reinterpret_cast<const TileOwn *>(m + o_own);
-    v.vg = reinterpret_cast<const TileVertex *>(m + o_vg);
-    v.edge = reinterpret_cast<const int *>(m + o_edge);
-    v.vo = reinterpret_cast<const TileVertex *>(m + o_vo);
+    v.vg = reinterpret_cast<const TileGather *>(m + o_vg);
+    v.en = reinterpret_cast<const double2 *>(m + o_edge);
+    v.vo = reinterpret_cast<const TileStress *>(m + o_vo);
     v.lsten = reinterpret_cast<const uint16_t *>(m + o_ls);
     v.epk = c->epk;
     v.ntiles = (int)ts.head.size();

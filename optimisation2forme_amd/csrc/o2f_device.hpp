@@ -449,36 +449,20 @@ __device__ __forceinline__ double src_val(const double *__restrict__ src, int nc
     return SRC_AOS ? (q < 4 ? src[4 * (long)c + q] : src[4 * (long)nc + c]) : src[(long)q * nc + c];
 }
 
-// The vertex's stencil is (kind, offset b into scell / sw, length n) and
-// bc_of() gives its boundary tag; store(out) receives the NV values.
+// The vertex's stencil is (kind, offset b into scell / sw, length n), its
+// first kStencilSlots cell ids are cid (entries past the stencil are never
+// used) and bc_of() gives its boundary tag; store(out) receives the NV
+// values.  The weights load beside the values.
 template <int NV, bool SRC_AOS, class BcOf, class Store>
-__device__ __forceinline__ void vertex_gather_vals(const Topo &T, const Geom &G, const double *__restrict__ src,
-                                                   const Phys &P, uint8_t kind, int b, int n, BcOf &&bc_of,
-                                                   Store &&store) {
+__device__ __forceinline__ void vertex_gather_ids(const Topo &T, const Geom &G, const double *__restrict__ src,
+                                                  const Phys &P, uint8_t kind, int b, int n,
+                                                  const int (&cid)[kStencilSlots], BcOf &&bc_of, Store &&store) {
     const int nc = T.nc;
-    // The first kStencilSlots ids and weights in 16-byte loads (both arrays
-    // carry kStencilSlots entries of padding; entries past the stencil are
-    // never used), then the values of the slots k < n: two dependent load
-    // levels, and few address computations.  (Per-entry loads under k < n
-    // let the compiler finish each id's address arithmetic inside its
-    // branch, which waits for the ids one at a time.)
-    int cid[kStencilSlots];
     double w[kStencilSlots], val[kStencilSlots][NV];
     {
-        typedef int v4i __attribute__((ext_vector_type(4)));
         typedef double v2d __attribute__((ext_vector_type(2)));
-        typedef v4i v4i_a4 __attribute__((aligned(4)));
         typedef v2d v2d_a8 __attribute__((aligned(8)));
-        const v4i_a4 *ip = reinterpret_cast<const v4i_a4 *>(T.scell + b);
         const v2d_a8 *wp = reinterpret_cast<const v2d_a8 *>(G.sw + b);
-#pragma unroll
-        for (int h = 0; h < kStencilSlots / 4; ++h) {
-            const v4i t = ip[h];
-            cid[4 * h] = t.x;
-            cid[4 * h + 1] = t.y;
-            cid[4 * h + 2] = t.z;
-            cid[4 * h + 3] = t.w;
-        }
 #pragma unroll
         for (int h = 0; h < kStencilSlots / 2; ++h) {
             const v2d t = wp[h];
@@ -537,6 +521,31 @@ __device__ __forceinline__ void vertex_gather_vals(const Topo &T, const Geom &G,
         }
     }
     store(out);
+}
+
+// The same with the ids read from the stencil arrays: the first
+// kStencilSlots ids and weights in 16-byte loads (both arrays carry
+// kStencilSlots entries of padding), then the values of the slots k < n:
+// two dependent load levels, and few address computations.  (Per-entry
+// loads under k < n let the compiler finish each id's address arithmetic
+// inside its branch, which waits for the ids one at a time.)
+template <int NV, bool SRC_AOS, class BcOf, class Store>
+__device__ __forceinline__ void vertex_gather_vals(const Topo &T, const Geom &G, const double *__restrict__ src,
+                                                   const Phys &P, uint8_t kind, int b, int n, BcOf &&bc_of,
+                                                   Store &&store) {
+    int cid[kStencilSlots];
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    typedef v4i v4i_a4 __attribute__((aligned(4)));
+    const v4i_a4 *ip = reinterpret_cast<const v4i_a4 *>(T.scell + b);
+#pragma unroll
+    for (int h = 0; h < kStencilSlots / 4; ++h) {
+        const v4i t = ip[h];
+        cid[4 * h] = t.x;
+        cid[4 * h + 1] = t.y;
+        cid[4 * h + 2] = t.z;
+        cid[4 * h + 3] = t.w;
+    }
+    vertex_gather_ids<NV, SRC_AOS>(T, G, src, P, kind, b, n, cid, bc_of, store);
 }
 
 

@@ -81,16 +81,68 @@ __device__ __forceinline__ Rec load_rec(const TileCell *p) {
     return r;
 }
 
-// A vertex record in one 16-byte load: stencil offset, length, kind, tag and
-// local stencil offset.
-struct Vtx {
-    int b, n, ls;
+static_assert(kTileSlots == kStencilSlots, "a vertex record holds the gathers' stencil slots");
+
+// A state vertex's record in three 16-byte loads: its first stencil cells,
+// the stencil offset, length, kind and tag.
+struct Gat {
+    int cid[kStencilSlots];
+    int b, n;
     uint8_t kind;
     char bc;
 };
-__device__ __forceinline__ Vtx load_vtx(const TileVertex *p) {
-    const int4 w = *reinterpret_cast<const int4 *>(p);
-    return Vtx{w.y, w.z & 0xffff, w.w, (uint8_t)((w.z >> 16) & 0xff), (char)(w.z >> 24)};
+__device__ __forceinline__ Gat load_gat(const TileGather *p) {
+    const int4 *q = reinterpret_cast<const int4 *>(p);
+    const int4 w0 = q[0], w1 = q[1], w2 = q[2];
+    Gat g;
+    g.cid[0] = w0.x;
+    g.cid[1] = w0.y;
+    g.cid[2] = w0.z;
+    g.cid[3] = w0.w;
+    g.cid[4] = w1.x;
+    g.cid[5] = w1.y;
+    g.cid[6] = w1.z;
+    g.cid[7] = w1.w;
+    g.b = w2.x;
+    g.n = w2.y & 0xffff;
+    g.kind = (uint8_t)((w2.y >> 16) & 0xff);
+    g.bc = (char)(w2.y >> 24);
+    return g;
+}
+
+// A stress vertex's record in six 16-byte loads: its first stencil weights
+// and local cells, then length, kind, tag, stencil offset and local stencil
+// offset.
+struct Str {
+    double w[kStencilSlots];
+    int lc[kStencilSlots];
+    int n, b, ls;
+    uint8_t kind;
+    char bc;
+};
+__device__ __forceinline__ Str load_str(const TileStress *p) {
+    const double2 *d = reinterpret_cast<const double2 *>(p);
+    Str r;
+#pragma unroll
+    for (int h = 0; h < kStencilSlots / 2; ++h) {
+        const double2 t = d[h];
+        r.w[2 * h] = t.x;
+        r.w[2 * h + 1] = t.y;
+    }
+    const int4 *q = reinterpret_cast<const int4 *>(p->lc);
+    const int4 l = q[0], m = q[1];
+    const unsigned lw[4] = {(unsigned)l.x, (unsigned)l.y, (unsigned)l.z, (unsigned)l.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        r.lc[2 * j] = (int)(lw[j] & 0xffffu);
+        r.lc[2 * j + 1] = (int)(lw[j] >> 16);
+    }
+    r.n = m.x & 0xffff;
+    r.kind = (uint8_t)((m.x >> 16) & 0xff);
+    r.bc = (char)(m.x >> 24);
+    r.b = m.y;
+    r.ls = m.z;
+    return r;
 }
 
 // LDS layout (in doubles; every region a multiple of four):
@@ -210,40 +262,45 @@ __device__ __forceinline__ void tile_cell(const Rec &r, bool face, const Geom &G
 // Stress vertex: the viscous gather of vertex v from the tile's cell
 // stresses (src/diffusion.cpp:28-140, apply_diffusion_bc), the arithmetic of
 // vertex_gather_vals<5, true>, with the stencil's cells given by their
-// local indices.
-__device__ __forceinline__ void tile_vertex_stress(const Vtx &r, const uint16_t *__restrict__ ls, const Geom &G,
+// local indices and the first weights in the record (a longer stencil reads
+// the rest from lsten and the stencil weights).
+__device__ __forceinline__ void tile_vertex_stress(const Str &r, const uint16_t *__restrict__ lsten, const Geom &G,
                                                    const Lds &L, int j) {
-    const uint8_t kind = r.kind;
-    const int b = r.b, n = r.n;
-    auto val = [&](int k, double (&t)[5]) {
-        const int li = ls[k];
+    auto val = [&](int li, double (&t)[5]) {
         double t4[4];
         lds_ld4(L.tq + 4 * li, t4);
 #pragma unroll
         for (int q = 0; q < 4; ++q) t[q] = t4[q];
         t[4] = L.tq[4 * L.max_gd + li];
     };
+    const int n = r.n;
     double out[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    if (kind == kStencilWsum) {
+    if (r.kind == kStencilWsum) {
         double t[5];
-        val(0, t);
-        const double w0 = G.sw[b];
+        val(r.lc[0], t);
 #pragma unroll
-        for (int q = 0; q < 5; ++q) out[q] = w0 * t[q];
-        for (int k = 1; k < n; ++k) {
-            val(k, t);
-            const double wk = G.sw[b + k];
+        for (int q = 0; q < 5; ++q) out[q] = r.w[0] * t[q];
+#pragma unroll
+        for (int k = 1; k < kStencilSlots; ++k)
+            if (k < n) {
+                val(r.lc[k], t);
+#pragma unroll
+                for (int q = 0; q < 5; ++q) out[q] = out[q] + r.w[k] * t[q];
+            }
+        for (int k = kStencilSlots; k < n; ++k) {
+            val(lsten[r.ls + k], t);
+            const double wk = G.sw[r.b + k];
 #pragma unroll
             for (int q = 0; q < 5; ++q) out[q] = out[q] + wk * t[q];
         }
-    } else if (kind == kStencilAvg2) {
+    } else if (r.kind == kStencilAvg2) {
         double t0[5], t1[5];
-        val(0, t0);
-        val(1, t1);
+        val(r.lc[0], t0);
+        val(r.lc[1], t1);
 #pragma unroll
         for (int q = 0; q < 5; ++q) out[q] = 0.5 * (t0[q] + t1[q]);
-    } else if (kind == kStencilCopy) {
-        val(0, out);
+    } else if (r.kind == kStencilCopy) {
+        val(r.lc[0], out);
     }
     if (r.bc == 'f' || r.bc == 'w') {
 #pragma unroll
@@ -600,9 +657,9 @@ __device__ __forceinline__ void stage_tile(const Topo &T, const Geom &G, const P
 
     // 1. states of the tile's state vertices (the four-kernel stage's k_vertex_gather)
     for (int i = tid; i < h.nVg; i += nth) {
-        const Vtx r = load_vtx(A.vg + h.vg0 + i);
-        vertex_gather_vals<4, true>(
-            T, G, U, P, r.kind, r.b, r.n, [&] { return r.bc; },
+        const Gat r = load_gat(A.vg + h.vg0 + i);
+        vertex_gather_ids<4, true>(
+            T, G, U, P, r.kind, r.b, r.n, r.cid, [&] { return r.bc; },
             [&](const double(&out)[4]) { lds_st4(L.vert + 4 * i, out); });
     }
     __syncthreads();
@@ -624,7 +681,7 @@ __device__ __forceinline__ void stage_tile(const Topo &T, const Geom &G, const P
     //    longer read)
     for (int i = tid; i < h.nE + h.nVo; i += nth) {
         if (i < h.nE) {
-            const double2 n2 = EP[4 * (long)A.edge[h.e0 + i]];
+            const double2 n2 = A.en[h.e0 + i];
             double qL[4], qR[4], f[4];
             lds_ld4(L.fs + 8 * i, qL);
             lds_ld4(L.fs + 8 * i + 4, qR);
@@ -632,8 +689,7 @@ __device__ __forceinline__ void stage_tile(const Topo &T, const Geom &G, const P
             lds_st4(L.fs + 8 * i, f);
         } else {
             const int j = i - h.nE;
-            const Vtx r = load_vtx(A.vo + h.vo0 + j);
-            tile_vertex_stress(r, A.lsten + r.ls, G, L, j);
+            tile_vertex_stress(load_str(A.vo + h.vo0 + j), A.lsten, G, L, j);
         }
     }
     __syncthreads();

@@ -18,8 +18,9 @@ struct TileView {
     const TileHead *head;
     const TileCell *cell;
     const TileOwn *own;
-    const TileVertex *vg, *vo;
-    const int *edge;
+    const TileGather *vg;  // state vertices
+    const TileStress *vo;  // stress vertices
+    const double2 *en;     // unit normal of each tile edge
     const uint16_t *lsten;
     const double *epk;  // packed edge geometry (launch_pack_edges), 8 doubles per edge
     int ntiles;

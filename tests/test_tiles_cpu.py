@@ -20,7 +20,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "optimisation2forme_amd", "csrc")
 
 PLANTED = ["vertex_slot", "edge_slot", "stencil_cell", "owned_twice", "face_cell_dropped", "vertex_stencil",
-           "stress_slot", "tile_slot", "concat_member", "concat_cell_offset", "concat_stencil_offset"]
+           "stress_slot", "tile_slot", "gather_cell", "stress_weight", "edge_normal", "concat_member",
+           "concat_cell_offset", "concat_stencil_offset"]
 
 
 @pytest.fixture(scope="module")

@@ -15,7 +15,8 @@
 // where each mistake changes one table entry the way a plausible slip in
 // the builder would (a wrong local vertex, a missing edge, a stencil cell
 // pointing elsewhere, a vertex record with another vertex's stencil, a
-// cell owned twice, a face neighbour left out) and
+// cell owned twice, a face neighbour left out, a tile's slot, the read
+// records of a state or stress vertex, an edge normal) and
 // check_tiles must reject it.  A last line
 //
 //   untagged <refused|accepted>
@@ -27,10 +28,12 @@
 //
 // followed by planted slips in the concatenation (a head naming the wrong
 // member, a record offset or a local stencil offset one off).
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "host_mesh.hpp"
@@ -104,6 +107,23 @@ void planted(const HostMesh &hm, const TileSet &good) {
         TileSet ts = good;  // a tile's slot (its speed partial) that of the next tile
         ts.head[t].slot += 1;
         report("tile_slot", ts);
+    }
+    {
+        TileSet ts = good;  // a state vertex's read record with its next stencil cell first
+        TileGather &g = ts.vgf[h.vg0];
+        std::swap(g.cid[0], g.cid[1]);
+        report("gather_cell", ts);
+    }
+    {
+        TileSet ts = good;  // a stress vertex's first weight rounded one unit up
+        double &w = ts.vof[h.vo0].w[0];
+        w = std::nextafter(w, 2.0);
+        report("stress_weight", ts);
+    }
+    {
+        TileSet ts = good;  // a tile edge's normal with its components swapped
+        std::swap(ts.en[2 * h.e0], ts.en[2 * h.e0 + 1]);
+        report("edge_normal", ts);
     }
 }
 
