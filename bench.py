@@ -45,6 +45,13 @@ one after another and against the four-kernel batched stages (autotuned
 launch shapes); then every member's adjoint solve (100 BiCGSTAB
 iterations each) one after another and together (o2f_batch_bicgstab).
 
+    python bench.py --shipped
+
+times stepping on the reference's own case (its shipped 25,274-quad mesh and
+state, mesh_opti/config.txt's Mach 0.3 flow): the tile stages the automatic
+choice gives it (one tile per CU, the four-lane form) and the four-kernel
+stages beside.
+
     python bench.py --adjoint
 
 times J^T x products (reference-form operator and exact adjoint) on the
@@ -673,6 +680,51 @@ def bench_shape_gradient(a):
     print(json.dumps(out), flush=True)
 
 
+def bench_shipped(a):
+    """Stepping throughput on the reference's own case: its shipped
+    25,274-quad mesh (shape 0) from its shipped state naca_2D_flow_1, with
+    mesh_opti/config.txt's flow (Mach 0.3, mu 1.8e-5, Pr 0.71, CFL 0.5, the
+    reference heat flux) and the Venkatakrishnan constant K = 5, FP64: the
+    tile stages the automatic choice gives this mesh (one tile per CU, the
+    four-lane form) against the four-kernel stages."""
+    from optimisation2forme_amd import capi
+    desc = capi.read_msh2(_golden("naca_2D_shape_0.msh.gz"))
+    prm = capi.make_params(viscosity=1.8e-5, Pr=0.71, cfl_number=0.5, inflow_velocity=0.3, venkat_k=5.0,
+                           limiter="venkat", interp_mode="ref", heat_flux="reference")
+    K = a.steps
+    out = {}
+    for cells in (a.tile_cells, 0):
+        s = capi.Solver(desc, prm)
+        s.set_tile_cells(cells)
+        s.load_state_file(_golden("naca_2D_flow_1.gz"))
+        s.run_steps(a.warmup)
+        ms, _ = s.time_steps(K)
+        info = s.tile_info()
+        stage_us = s.time_kernel(5, 600) if info["tiles"] > 0 else None
+        out[cells] = (ms, info, stage_us, s.nv, s.ne, s.nc, bool(np.isfinite(s.state()).all()))
+        s.close()
+    ms, info, stage_us, nv, ne, nc, finite = out[a.tile_cells]
+    b_stage = step_algorithmic_bytes(nv, ne, nc, k=4) / 3.0
+    line = {
+        "metric": "vertex-steps/s (the reference's shipped case)", "value": nv * K / (ms * 1e-3),
+        "unit": "vertex-steps/s", "n_gpus": 1, "steps": K, "warmup": a.warmup, "ms_per_step": ms / K,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "the reference's shipped mesh naca_2D_shape_0.msh and state naca_2D_flow_1",
+        "config": {"workload": "shipped_naca0012_quads_25k_mach0.3", "cells": nc, "vertices": nv, "edges": ne,
+                   "mach": 0.3, "viscosity": 1.8e-5, "Pr": 0.71, "cfl": 0.5, "limiter": "venkatakrishnan K=5",
+                   "interp_mode": "ref", "heat_flux": "reference", "parallelism": "single GPU"},
+        "tiling": info,
+        "roofline": None if stage_us is None else {
+            "bound": "hbm", "kernel": "o2f::k_tile_stage_q (four-lane tile stage, one tile per CU)",
+            "bytes_per_launch": b_stage, "avg_launch_us": stage_us,
+            "achieved": b_stage / (stage_us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": b_stage / (stage_us * 1e-6) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+        "four_kernel": {"ms_per_step": out[0][0] / K, "value": nv * K / (out[0][0] * 1e-3)},
+        "state_finite": finite,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -698,6 +750,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the 512 MiB device copy used to calibrate the HBM counters")
     ap.add_argument("--batch", action="store_true", help="the 64-candidate batched workload")
+    ap.add_argument("--shipped", action="store_true",
+                    help="stepping throughput on the reference's shipped 25k-quad case (its mesh, state and flow)")
     ap.add_argument("--warm-start", action="store_true", help="the learned warm-start comparison")
     ap.add_argument("--adjoint", action="store_true", help="J^T x product timings (reference form, exact)")
     ap.add_argument("--shape-gradient", action="store_true",
@@ -743,6 +797,11 @@ def main():
         return
     if a.copy_calibration:
         copy_calibration()
+        return
+    if a.shipped and a.gpus == 1:
+        if a.steps == ap.get_default("steps"):
+            a.steps = 2000
+        bench_shipped(a)
         return
     if a.batch and a.gpus == 1:
         if a.steps == ap.get_default("steps"):
