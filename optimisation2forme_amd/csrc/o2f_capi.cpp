@@ -284,7 +284,12 @@ int tiles_leave(o2f_ctx *c) {
 // (and the state in c->U).
 // check: the first stage also stores R_rho of the step's start state and
 // the density-residual check follows it (o2f_run_to_residual).
-int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check = false, double res_tol = -1.0) {
+// chain (256-thread tiles; a loop's steps after its first): kChainUse starts
+// the step from the speed max the previous step's last stage folded, in
+// place of the one-block step start; kChainForm has this step's last stage
+// fold it (the same max of the same partials).
+enum : int { kChainUse = 1, kChainForm = 2 };
+int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check, double res_tol, int chain) {
     hipStream_t s = c->stream;
     const Topo T = c->topo();
     const Geom G = c->geo[0].view();
@@ -297,8 +302,11 @@ int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check = false, 
     const TileView &A = c->tv;
     // the four-lane stages start the step themselves (no one-block launch)
     const StepBegin B{c->speed, c->speed_nparts, c->hm.min_h, c->scal, hist};
-    const int begin = A.quad ? kFlagBegin : 0, advance = A.quad ? kFlagAdvance : 0;
-    if (!A.quad) HIPCHK(c, launch_step_begin(c->speed, c->speed_nparts, c->phys, c->hm.min_h, c->scal, hist, 1, s));
+    const bool use = !A.quad && (chain & kChainUse), form = !A.quad && !last_outputs && (chain & kChainForm);
+    const int begin = A.quad ? kFlagBegin : (use ? kFlagBeginDt : 0);
+    const int advance = (A.quad || use) ? kFlagAdvance : 0;
+    if (!A.quad && !use)
+        HIPCHK(c, launch_step_begin(c->speed, c->speed_nparts, c->phys, c->hm.min_h, c->scal, hist, 1, s));
     const double *dtp = c->scal + kScalDt;
     HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ua, c->Ub, true, c->Uold, c->R, dtp, 0.0, 1.0,
                                 kFlagRK | kFlagSaveOld | (check ? kFlagStoreR : 0) | begin, nullptr, s, B));
@@ -314,9 +322,9 @@ int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check = false, 
         return run_stage(c, 0, kFlagRK | kFlagStoreR | kFlagSpeed, dtp, 1.0 / 3.0, 2.0 / 3.0, true);
     }
     HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Ub, c->Uc, true, c->Uold, c->R, dtp, 3.0 / 4.0, 1.0 / 4.0,
-                                kFlagRK | advance, nullptr, s, B));
+                                kFlagRK | advance | (form ? kFlagSpeedReset : 0), nullptr, s, B));
     HIPCHK(c, launch_tile_stage(T, G, c->phys, A, c->Uc, c->Ua, true, c->Uold, c->R, dtp, 1.0 / 3.0, 2.0 / 3.0,
-                                kFlagRK | kFlagSpeed, c->speed, s));
+                                kFlagRK | kFlagSpeed | (form ? kFlagSpeedMax : 0), c->speed, s, B));
     c->speed_nparts = A.ntiles;
     return O2F_OK;
 }
@@ -324,11 +332,11 @@ int tiled_step(o2f_ctx *c, bool last_outputs, double *hist, bool check = false, 
 // One step of the time loop body: dt from the speed word, the three stages,
 // and (fused into the last stage) the speed of the new state for the next
 // step's dt.
-int one_step(o2f_ctx *c, bool last_outputs, double *hist) {
+int one_step(o2f_ctx *c, bool last_outputs, double *hist, int chain = 0) {
     hipStream_t s = c->stream;
     int st;
     if (tiles_on(c)) {
-        st = tiled_step(c, last_outputs, hist);
+        st = tiled_step(c, last_outputs, hist, false, -1.0, chain);
     } else {
         if (!c->speed_valid) {
             HIPCHK(c, launch_speed_max(c->U, c->hm.nc, c->phys, c->speed, s));
@@ -832,7 +840,8 @@ int o2f_run_steps(o2f_ctx *c, int nsteps, double *dts, double *t_inout) {
     HIPCHK(c, hipMemcpyAsync(c->scal + kScalT, &t0, sizeof(double), hipMemcpyHostToDevice, c->stream));
     double *hist = nullptr;
     if (dts && nsteps > 0) HIPCHK(c, hipMalloc(reinterpret_cast<void **>(&hist), nsteps * sizeof(double)));
-    for (int s = 0; s < nsteps && st == 0; ++s) st = one_step(c, s == nsteps - 1, hist ? hist + s : nullptr);
+    for (int s = 0; s < nsteps && st == 0; ++s)
+        st = one_step(c, s == nsteps - 1, hist ? hist + s : nullptr, (s > 0 ? kChainUse : 0) | kChainForm);
     const int st_leave = tiles_leave(c);
     if (st) return st;
     if (st_leave) return st_leave;
@@ -871,7 +880,8 @@ int o2f_run_until(o2f_ctx *c, double final_time, int max_steps, int *steps, doub
         int batch = 64;
         if (max_steps >= 0) batch = std::min(batch, max_steps - launched);
         if (batch <= 0) break;
-        for (int k = 0; k < batch && st == 0; ++k) st = one_step(c, !tiled, nullptr);
+        for (int k = 0; k < batch && st == 0; ++k)
+            st = one_step(c, !tiled, nullptr, (launched + k > 0 ? kChainUse : 0) | kChainForm);
         launched += batch;
         if (st) break;
         hipError_t e = hipMemcpyAsync(h, c->scal, sizeof(h), hipMemcpyDeviceToHost, s);
@@ -941,7 +951,8 @@ int o2f_run_to_residual(o2f_ctx *c, double tol, int max_steps, int check_every, 
         for (int k = 0; k < n && st == 0; ++k, ++launched) {
             // after a step the speed partials are the new state's (c->U is
             // stale while the state lives in Ua)
-            st = tiled_step(c, false, nullptr, tol > 0.0 && launched % check_every == 0, res_tol);
+            st = tiled_step(c, false, nullptr, tol > 0.0 && launched % check_every == 0, res_tol,
+                            (launched > 0 ? kChainUse : 0) | kChainForm);
             c->speed_valid = (st == O2F_OK);
         }
         if (st) break;
@@ -1925,7 +1936,8 @@ int o2f_time_steps(o2f_ctx *c, int nsteps, double *total_ms, double *kernel_ms) 
     HIPCHK(c, hipEventCreate(&e1));
     HIPCHK(c, hipStreamSynchronize(s));
     HIPCHK(c, hipEventRecord(e0, s));
-    for (int k = 0; k < nsteps && st == 0; ++k) st = one_step(c, k == nsteps - 1, nullptr);
+    for (int k = 0; k < nsteps && st == 0; ++k)
+        st = one_step(c, k == nsteps - 1, nullptr, (k > 0 ? kChainUse : 0) | kChainForm);
     const int st_leave = tiles_leave(c);
     if (st) return st;
     if (st_leave) return st_leave;

@@ -24,6 +24,9 @@ enum : int {
     kFlagSpeed = 32,    // after the update: max_c(|u|+c) of the new state into the speed word
     kFlagBegin = 64,    // tile stage: the step's start (dt, the time loop's stop test) in every workgroup
     kFlagAdvance = 128, // tile stage: workgroup 0 advances the time bookkeeping by the step's dt
+    kFlagBeginDt = 256,     // tile stage: the step's start from the max(|u|+c) the previous step folded
+    kFlagSpeedReset = 512,  // tile stage: workgroup 0 starts the fold of the step's max(|u|+c)
+    kFlagSpeedMax = 1024,   // tile stage: each tile's speed partial folded into that max (atomic)
 };
 
 // Slots of the device scalar block.
@@ -46,6 +49,7 @@ enum : int {
     kScalGo = 15,      // BiCGSTAB loop: 1 run the next iteration, 0 stop, 2 restart first (host)
     kScalIter = 16,    // BiCGSTAB iterations taken in the current chunk
     kScalRhoAlt = 17,  // BiCGSTAB: rho of odd iterations of a group (kScalRho: even)
+    kScalSpeedMax = 18,  // tile stages: max(1e-8, max_c(|u|+c)) of the new state, folded by the last stage
     kScalCount = 20,
 };
 

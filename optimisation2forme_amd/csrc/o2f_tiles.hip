@@ -729,12 +729,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                     const double *__restrict__ U, double *__restrict__ Uout,
                                                     double *__restrict__ Uold, double *__restrict__ R,
                                                     const double *__restrict__ dtp, double c0, double c1, int flags,
-                                                    double *__restrict__ speed) {
+                                                    double *__restrict__ speed, StepBegin B) {
     if (P.gate && *P.gate == 0.0) return;  // uniform
+    __shared__ double s_dt;
+    if (flags & kFlagBeginDt) {
+        // the step's start from the max(|u| + c) the previous step's last
+        // stage folded, as k_step_begin forms it (the stop test's inputs and
+        // the max do not change during this launch)
+        if (P.gate && (!(B.scal[kScalT] < P.final_t) || B.scal[kScalConv] != 0.0)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) B.scal[kScalActive] = 0.0;
+            return;  // uniform
+        }
+        if (threadIdx.x == 0) {
+            const double max_wave = B.scal[kScalSpeedMax];
+            const double dt_adv = P.cfl * B.min_h / max_wave;
+            const double dt_diff = 0.5 * (B.min_h * B.min_h) / P.mu;
+            const double dt = smin(dt_adv, dt_diff);
+            s_dt = dt;
+            if (blockIdx.x == 0) {
+                B.scal[kScalDt] = dt;
+                if (B.hist) *B.hist = dt;
+            }
+        }
+        __syncthreads();
+        dtp = &s_dt;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (flags & kFlagAdvance) {
+            // t is read by the next step's start only
+            B.scal[kScalTPrev] = B.scal[kScalT];
+            B.scal[kScalT] = B.scal[kScalT] + B.scal[kScalDt];
+            if (P.gate) B.scal[kScalSteps] = B.scal[kScalSteps] + 1.0;
+        }
+        // the fold's start value, as k_step_begin's (read by the next step)
+        if (flags & kFlagSpeedReset) B.scal[kScalSpeedMax] = 1e-8;
+    }
     const int tile = xcd_tile(A.ntiles);
     const TileHead h = A.head[tile];
     stage_tile<OUT_AOS, NK, false>(T, G, P, A, h, reinterpret_cast<const double2 *>(A.epk), U, Uout, Uold, R, dtp, c0,
                                    c1, flags, speed + tile);
+    if ((flags & kFlagSpeedMax) && threadIdx.x == 0) {
+        // the tile's partial into the step's max: positive doubles order as
+        // their bit patterns, so one 64-bit atomic max is exact and
+        // order-free; smax(1e-8, .) skips a NaN partial as k_step_begin does
+        const double m = smax(1e-8, speed[tile]);
+        __hip_atomic_fetch_max(reinterpret_cast<unsigned long long *>(B.scal + kScalSpeedMax),
+                               static_cast<unsigned long long>(__double_as_longlong(m)), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // The same stage for meshes whose tiles run one per CU: 1,024 threads per
@@ -877,7 +919,10 @@ hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const 
                              double *Uout, bool out_aos, double *Uold, double *R, const double *dtp, double c0,
                              double c1, int flags, double *speed, hipStream_t s, const StepBegin &B) {
     const dim3 grid(A.ntiles), block(kTileThreads);
-    if (!A.quad && (flags & (kFlagBegin | kFlagAdvance))) return hipErrorInvalidValue;
+    constexpr int one_lane_only = kFlagBeginDt | kFlagSpeedReset | kFlagSpeedMax;
+    if (A.quad ? (flags & one_lane_only) != 0 : (flags & kFlagBegin) != 0) return hipErrorInvalidValue;
+    if ((flags & (kFlagBegin | kFlagAdvance | one_lane_only)) && !B.scal) return hipErrorInvalidValue;
+    if ((flags & kFlagSpeedMax) && (!(flags & kFlagSpeed) || !speed)) return hipErrorInvalidValue;
     if (A.quad) {
         const dim3 qblock(kTileQuadThreads);
         if (A.nk_max <= 3) {
@@ -900,17 +945,17 @@ hipError_t launch_tile_stage(const Topo &T, const Geom &G, const Phys &P, const 
     if (A.nk_max <= 3) {
         if (out_aos)
             hipLaunchKernelGGL((k_tile_stage<true, 3>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R, dtp,
-                               c0, c1, flags, speed);
+                               c0, c1, flags, speed, B);
         else
             hipLaunchKernelGGL((k_tile_stage<false, 3>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R,
-                               dtp, c0, c1, flags, speed);
+                               dtp, c0, c1, flags, speed, B);
     } else {
         if (out_aos)
             hipLaunchKernelGGL((k_tile_stage<true, 4>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R, dtp,
-                               c0, c1, flags, speed);
+                               c0, c1, flags, speed, B);
         else
             hipLaunchKernelGGL((k_tile_stage<false, 4>), grid, block, A.lds_bytes, s, T, G, P, A, U, Uout, Uold, R,
-                               dtp, c0, c1, flags, speed);
+                               dtp, c0, c1, flags, speed, B);
     }
     return hipGetLastError();
 }

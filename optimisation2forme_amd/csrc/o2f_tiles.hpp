@@ -48,6 +48,12 @@ struct StepBegin {
 // doubles per cell); Uout is AoS (out_aos) or SoA [q][nc]; Uold and R are
 // SoA.  U and Uout must be different buffers.  kFlagSpeed writes one
 // max(|u| + c) partial per tile.
+// With the 256-thread kernels a step may carry the next step's speed max:
+// its second stage starts the fold (kFlagSpeedReset), its last stage folds
+// every tile's partial into scal[kScalSpeedMax] (kFlagSpeedMax, with
+// kFlagSpeed), and the next step's first stage starts that step from it
+// (kFlagBeginDt: the stop test, dt as k_step_begin forms it), its second
+// stage advancing the time (kFlagAdvance).
 // With the four-lane kernels (A.quad) the first stage may also start the
 // step (kFlagBegin: every workgroup forms the step's dt from B's partials
 // as k_step_begin does, workgroup 0 stores it, and the time loop's stop

@@ -103,14 +103,19 @@ def test_oversized_tiles_fall_back_to_the_four_kernels(small_developed):
     assert_same_run(a, b)
 
 
+@pytest.mark.parametrize("tiles", [-1, 37])
 @pytest.mark.parametrize("max_steps", [-1, 7])
-def test_tiled_run_until_bitwise(small_developed, max_steps):
+def test_tiled_run_until_bitwise(small_developed, max_steps, tiles):
     """o2f_run_until on tiles: the same stopping step, times and state, and
     the last stage's intermediates (formed after the loop from that stage's
-    input) equal the four-kernel loop's, which stores them every step."""
+    input) equal the four-kernel loop's, which stores them every step.
+    tiles -1: one tile per CU (the four-lane stages, which start each step
+    themselves); 37 cells: more tiles than CUs (the 256-thread stages, each
+    step after the loop's first started from the speed max the previous
+    step's last stage folded, the gated steps after the stop included)."""
     flow = small_flow()
     res = []
-    for cells in (-1, 0):
+    for cells in (tiles, 0):
         s = capi.Solver(H.small_mesh(), H.capi_params(flow))
         s.set_tile_cells(cells)
         s.set_state(small_developed)
@@ -120,6 +125,7 @@ def test_tiled_run_until_bitwise(small_developed, max_steps):
         s.close()
     (ra, Ua, fa, na, la, ta), (rb, Ub, fb, nb, lb, tb) = res
     assert ta > 0 and tb == 0
+    assert tiles == -1 or ta > 256  # more tiles than the MI355X's 256 CUs
     assert ra == rb
     assert ra[0] == 7 if max_steps == 7 else ra[0] >= 10
     assert np.all(np.isfinite(Ua))
@@ -164,11 +170,13 @@ def test_tiled_steps_match_the_oracle_10k(small_developed):
     assert max(H.var_rel_err(s.state(), m.state())) < 1e-10
 
 
+@pytest.mark.parametrize("tiles", [-1, 37])
 @pytest.mark.parametrize("check_every", [1, 7])
-def test_tiled_run_to_residual_bitwise_batched_stages(small_developed, check_every):
+def test_tiled_run_to_residual_bitwise_batched_stages(small_developed, check_every, tiles):
     """o2f_run_to_residual on tiles against the batched four-kernel stages
     (o2f_set_tile_cells(0)): the same stopping step and residual, state and
-    last-stage face states, checking every step or every seventh."""
+    last-stage face states, checking every step or every seventh, on the
+    four-lane stages (tiles -1) and the 256-thread ones (37 cells)."""
     flow = small_flow()
     # the residual of step 49's start state (checked whether every step or
     # every seventh is): a tolerance some step up to 49 reaches
@@ -180,7 +188,7 @@ def test_tiled_run_to_residual_bitwise_batched_stages(small_developed, check_eve
     assert n0 == 50 and r49 > 0
     tol = r49
     res = []
-    for cells in (-1, 0):
+    for cells in (tiles, 0):
         s = capi.Solver(H.small_mesh(), H.capi_params(flow))
         s.set_tile_cells(cells)
         s.set_state(small_developed)
@@ -189,6 +197,7 @@ def test_tiled_run_to_residual_bitwise_batched_stages(small_developed, check_eve
         s.close()
     (oa, Ua, fa, ta), (ob, Ub, fb, tb) = res
     assert ta > 0 and tb == 0
+    assert tiles == -1 or ta > 256  # more tiles than the MI355X's 256 CUs
     assert oa == ob and 0 < oa[0] <= 50 and oa[1] <= tol
     assert np.all(np.isfinite(Ua)) and same(Ua, Ub)
     assert same(fa[0], fb[0]) and same(fa[1], fb[1])
