@@ -133,6 +133,25 @@ def test_tiled_run_until_bitwise(small_developed, max_steps, tiles):
     assert na == nb and la == lb
 
 
+@pytest.mark.parametrize("tiles", [-1, 37])
+def test_tiled_steps_with_a_nan_cell_bitwise_four_kernel_steps(small_developed, tiles):
+    """A state with one NaN cell (a blown-up solve): every dt reduction skips
+    the NaN speeds (the one-block step start, the four-lane stages' start,
+    and the 256-thread stages' carried 64-bit atomic max alike), so the dts
+    stay finite and equal, and the NaN spreads over the same cells with the
+    finite values bit for bit the same."""
+    flow = small_flow()
+    U = np.array(small_developed, copy=True)
+    nc = U.shape[-1]
+    U[..., nc // 2] = np.nan
+    a = run(H.small_mesh(), flow, U, 4, tiles)
+    b = run(H.small_mesh(), flow, U, 4, 0)
+    assert a["info"]["tiles"] > 0 and (tiles == -1 or a["info"]["tiles"] > 256)
+    assert np.all(np.isfinite(a["dts"])) and same(a["dts"], b["dts"]) and a["t"] == b["t"]
+    assert np.isnan(a["U"]).any() and np.isfinite(a["U"]).any()
+    assert np.array_equal(a["U"], b["U"], equal_nan=True)
+
+
 @pytest.mark.parametrize("limiter", [0, 1])
 def test_tiled_steps_bitwise_four_kernel_steps_200k_developed(limiter):
     """The benchmark mesh (NACA 2412, 199,680 triangles, Mach 0.85) after
